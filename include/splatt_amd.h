@@ -1,0 +1,69 @@
+/*
+ * splatt_amd.h -- extensions of the SPLATT API that only make sense with a
+ * GPU underneath: device-resident operands, precision control, the algorithmic
+ * cost model used for bandwidth figures, and error text. None of these has a
+ * counterpart in the reference; everything in splatt.h works without them.
+ */
+#ifndef SPLATT_AMD_H
+#define SPLATT_AMD_H
+
+#include <stddef.h>
+#include "splatt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPLATT_AMD_F64 0
+#define SPLATT_AMD_F32 1
+
+/* Text of the last device-side failure. */
+char const * splatt_amd_last_error(void);
+
+/* Arithmetic type of MTTKRP inside splatt_mttkrp() and splatt_cpd():
+ * SPLATT_AMD_F64 (default, the reference's val_t) or SPLATT_AMD_F32 (FP32
+ * values, factors and accumulation; Grams, Cholesky and fit stay FP64). */
+splatt_error_type splatt_amd_set_mttkrp_precision(int precision);
+
+/* MTTKRP with operands already in GPU memory. d_mats[m] are device pointers
+ * to row-major dims[m] x ncolumns arrays of `precision` type; d_out
+ * (dims[mode] x ncolumns) is overwritten. The CSF must come from this
+ * library (its GPU copy is used). stream is a hipStream_t or NULL for the
+ * library stream; the call returns without synchronising. */
+splatt_error_type splatt_amd_mttkrp_dev(
+    splatt_idx_t const mode,
+    splatt_idx_t const ncolumns,
+    splatt_csf const * const tensors,
+    void ** d_mats,
+    void * d_out,
+    int precision,
+    double const * const options,
+    void * stream);
+
+/* Algorithmic bytes and flops of one MTTKRP of `mode` on the tensor the
+ * options' CSF map selects (4-byte indices, value size of `precision`). */
+splatt_error_type splatt_amd_mttkrp_cost(
+    splatt_idx_t const mode,
+    splatt_idx_t const ncolumns,
+    splatt_csf const * const tensors,
+    int precision,
+    double const * const options,
+    double * bytes,
+    double * flops);
+
+/* Device time (ms) of the main kernel of the last MTTKRP, measured with HIP
+ * events on the stream it ran on, when timing was enabled; <0 otherwise. */
+void splatt_amd_mttkrp_timing(int enable);
+double splatt_amd_mttkrp_last_ms(void);
+
+/* Device memory helpers for callers that hold device-resident factors. */
+void * splatt_amd_dev_alloc(size_t bytes);
+void splatt_amd_dev_free(void * ptr);
+splatt_error_type splatt_amd_h2d(void * dst, void const * src, size_t bytes);
+splatt_error_type splatt_amd_d2h(void * dst, void const * src, size_t bytes);
+splatt_error_type splatt_amd_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,213 @@
+/*
+ * oracle_cpd.c -- TEST INFRASTRUCTURE (the oracle; see oracle.h).
+ * CPD-ALS restated on the CPU in FP64, with textbook dense kernels in place
+ * of BLAS/LAPACK (syrk, potrf, potrs).
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* src/util.c:15-40: magnitude 3*rand()/RAND_MAX, then sign from a second
+ * rand() (even -> negative). */
+void or_fill_rand(double * vals, uint64_t n)
+{
+  for(uint64_t i=0; i < n; ++i) {
+    double v = 3.0 * ((double) rand() / (double) RAND_MAX);
+    if(rand() % 2 == 0) {
+      v *= -1;
+    }
+    vals[i] = v;
+  }
+}
+
+/* A^T A of a row-major I x R matrix (src/matrix.c:171-205 computes the
+ * upper triangle with syrk); both triangles are filled here. */
+void or_mat_aTa(double const * A, uint64_t I, uint64_t R, double * out)
+{
+  for(uint64_t x=0; x < R * R; ++x) {
+    out[x] = 0;
+  }
+  for(uint64_t i=0; i < I; ++i) {
+    double const * row = A + i * R;
+    for(uint64_t a=0; a < R; ++a) {
+      for(uint64_t b=a; b < R; ++b) {
+        out[a * R + b] += row[a] * row[b];
+      }
+    }
+  }
+  for(uint64_t a=0; a < R; ++a) {
+    for(uint64_t b=0; b < a; ++b) {
+      out[a * R + b] = out[b * R + a];
+    }
+  }
+}
+
+/* Cholesky G = L L^T reading the upper triangle of row-major G and writing
+ * L into its lower triangle (src/matrix.c:104-124 calls potrf('L') on the
+ * column-major view, which is the same factor). Returns 0, or the 1-based
+ * column where a non-positive pivot appeared (LAPACK's info). */
+int or_cholesky(double * G, uint64_t R)
+{
+  for(uint64_t j=0; j < R; ++j) {
+    double d = G[j * R + j];
+    for(uint64_t k=0; k < j; ++k) {
+      d -= G[j * R + k] * G[j * R + k];
+    }
+    if(!(d > 0)) {
+      return (int) j + 1;
+    }
+    d = sqrt(d);
+    G[j * R + j] = d;
+    for(uint64_t i=j+1; i < R; ++i) {
+      double s = G[j * R + i];   /* upper (j,i) == lower (i,j) of G */
+      for(uint64_t k=0; k < j; ++k) {
+        s -= G[i * R + k] * G[j * R + k];
+      }
+      G[i * R + j] = s / d;
+    }
+  }
+  return 0;
+}
+
+/* potrs, src/matrix.c:127-151: each row b of B becomes G^{-1} b. */
+void or_cholesky_solve(double const * L, uint64_t R, double * B,
+    uint64_t nrows)
+{
+  for(uint64_t n=0; n < nrows; ++n) {
+    double * b = B + n * R;
+    for(uint64_t i=0; i < R; ++i) {
+      double s = b[i];
+      for(uint64_t k=0; k < i; ++k) {
+        s -= L[i * R + k] * b[k];
+      }
+      b[i] = s / L[i * R + i];
+    }
+    for(uint64_t i=R; i-- != 0; ) {
+      double s = b[i];
+      for(uint64_t k=i+1; k < R; ++k) {
+        s -= L[k * R + i] * b[k];
+      }
+      b[i] = s / L[i * R + i];
+    }
+  }
+}
+
+/* src/matrix.c:37-96: lambda_j = ||A(:,j)||_2, columns divided by it. */
+void or_mat_normalize(double * A, uint64_t I, uint64_t R, double * lambda)
+{
+  for(uint64_t r=0; r < R; ++r) {
+    lambda[r] = 0;
+  }
+  for(uint64_t i=0; i < I; ++i) {
+    for(uint64_t r=0; r < R; ++r) {
+      lambda[r] += A[i * R + r] * A[i * R + r];
+    }
+  }
+  for(uint64_t r=0; r < R; ++r) {
+    lambda[r] = sqrt(lambda[r]);
+  }
+  for(uint64_t i=0; i < I; ++i) {
+    for(uint64_t r=0; r < R; ++r) {
+      A[i * R + r] /= lambda[r];
+    }
+  }
+}
+
+int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
+    uint64_t nmodes, uint64_t rank, double ** factors, double * lambda,
+    uint64_t max_its, double tol, double * fits, int nthreads)
+{
+  uint64_t const R = rank;
+  or_csf const * c0 = csfs[0];
+  uint64_t maxdim = 0;
+  for(uint64_t m=0; m < nmodes; ++m) {
+    if(c0->dims[m] > maxdim) maxdim = c0->dims[m];
+  }
+  double * aTa[OR_MAX_NMODES];
+  for(uint64_t m=0; m < nmodes; ++m) {
+    aTa[m] = calloc(R * R, sizeof(double));
+  }
+  double * gram = malloc(R * R * sizeof(double));
+  double * mttkrp = malloc(maxdim * R * sizeof(double));
+  double * tmp = malloc(R * sizeof(double));
+
+  /* src/cpd/cpd.c:179-204 */
+  for(uint64_t m=0; m < nmodes; ++m) {
+    or_mat_normalize(factors[m], c0->dims[m], R, lambda);
+  }
+  for(uint64_t r=0; r < R; ++r) {
+    lambda[r] = 1.;
+  }
+  for(uint64_t m=1; m < nmodes; ++m) {
+    or_mat_aTa(factors[m], c0->dims[m], R, aTa[m]);
+  }
+  double const ttnormsq = or_csf_frobsq(c0);
+  double olderr = 1.;
+  int its = 0;
+  for(uint64_t it=0; it < max_its; ++it) {
+    for(uint64_t m=0; m < nmodes; ++m) {
+      uint64_t const I = c0->dims[m];
+      or_mttkrp_csf(csfs[mode_csf[m]], factors, R, m, mttkrp, nthreads);
+      /* src/matrix.c:311-345 then src/cpd/admm.c:217-235 */
+      for(uint64_t x=0; x < R * R; ++x) {
+        gram[x] = 1.;
+      }
+      for(uint64_t mm=0; mm < nmodes; ++mm) {
+        if(mm == m) continue;
+        for(uint64_t x=0; x < R * R; ++x) {
+          gram[x] *= aTa[mm][x];
+        }
+      }
+      or_cholesky(gram, R);
+      memcpy(factors[m], mttkrp, I * R * sizeof(double));
+      or_cholesky_solve(gram, R, factors[m], I);
+      or_mat_normalize(factors[m], I, R, lambda);
+      or_mat_aTa(factors[m], I, R, aTa[m]);
+    }
+    /* src/cpd/cpd.c:453-487 */
+    double norm = 0;
+    for(uint64_t i=0; i < R; ++i) {
+      for(uint64_t j=i; j < R; ++j) {
+        double h = 1.;
+        for(uint64_t m=0; m < nmodes; ++m) {
+          h *= aTa[m][i * R + j];
+        }
+        norm += (i == j) ? h * lambda[i] * lambda[i]
+                         : 2. * h * lambda[i] * lambda[j];
+      }
+    }
+    norm = fabs(norm);
+    /* src/cpd/cpd.c:490-531 with the last mode's MTTKRP */
+    uint64_t const last = nmodes - 1;
+    double inner = 0;
+    for(uint64_t r=0; r < R; ++r) {
+      double s = 0;
+      for(uint64_t i=0; i < c0->dims[last]; ++i) {
+        s += factors[last][i * R + r] * mttkrp[i * R + r];
+      }
+      inner += s * lambda[r];
+    }
+    double const err = (ttnormsq + norm - 2 * inner) / ttnormsq;
+    fits[it] = 1. - err;
+    its = (int) it + 1;
+    if(it > 0 && fabs(olderr - err) < tol) {
+      break;
+    }
+    olderr = err;
+  }
+  /* src/cpd/cpd.c:313-332 */
+  for(uint64_t m=0; m < nmodes; ++m) {
+    or_mat_normalize(factors[m], c0->dims[m], R, tmp);
+    for(uint64_t r=0; r < R; ++r) {
+      lambda[r] *= tmp[r];
+    }
+  }
+  for(uint64_t m=0; m < nmodes; ++m) {
+    free(aTa[m]);
+  }
+  free(gram);
+  free(mttkrp);
+  free(tmp);
+  return its;
+}

@@ -1,0 +1,114 @@
+/*
+ * spd.h -- internal C ABI between the C host library and the HIP (gfx950)
+ * device code. Plain C types only: opaque handles, pointers, sizes, int
+ * status. Not part of the public splatt.h surface.
+ *
+ * Conventions
+ *   - Every function returns SPD_OK (0) or a negative SPD_E* code; the text
+ *     of the last error is available from spd_last_error().
+ *   - Device memory is owned by the handle that allocated it.
+ *   - All work is issued on the library's stream (spd_stream()) unless a
+ *     stream argument is given; functions that return host results
+ *     synchronise that stream before returning.
+ */
+#ifndef SPD_ABI_H
+#define SPD_ABI_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPD_OK          0
+#define SPD_EHIP       -1   /* HIP runtime error */
+#define SPD_ENOMEM     -2   /* device allocation failed */
+#define SPD_EINPUT     -3   /* bad input (index out of range, sizes) */
+#define SPD_ENODEV     -4   /* no usable GPU */
+#define SPD_ENUMERIC   -5   /* non-SPD Gram matrix, NaN, ... */
+
+#define SPD_MAX_MODES 8
+
+#define SPD_F64 0
+#define SPD_F32 1
+
+const char * spd_last_error(void);
+int spd_init(void);
+void * spd_stream(void);   /* hipStream_t of the library */
+int spd_sync(void);
+
+int spd_malloc(void ** ptr, size_t bytes);
+int spd_free(void * ptr);
+int spd_h2d(void * dst, void const * src, size_t bytes);
+int spd_d2h(void * dst, void const * src, size_t bytes);
+int spd_d2d(void * dst, void const * src, size_t bytes);
+int spd_zero(void * ptr, size_t bytes);   /* fill kernel, graph-safe */
+
+/* ---- coordinate tensor resident on the device ------------------------- */
+typedef struct spd_coo spd_coo;
+
+/* Upload nnz coordinates (inds[m][n], 0-based, < dims[m]) and values. */
+int spd_coo_upload(int nmodes, uint64_t nnz, uint64_t const * dims,
+    uint64_t const * const * inds, double const * vals, spd_coo ** out);
+void spd_coo_free(spd_coo * coo);
+
+/* ---- CSF --------------------------------------------------------------- */
+typedef struct spd_csf spd_csf;
+
+/* Build one CSF from a device COO with level order perm[0..nmodes-1]
+ * (perm[l] = mode stored at level l). tile_dims[m] > 1 requests dense
+ * tiling of mode m (reference DENSETILE); all ones means untiled. The
+ * untiled structure is always built (the kernels use it); a tiled one is
+ * kept as well for the host mirror. */
+int spd_csf_build(spd_coo const * coo, int const * perm,
+    uint64_t const * tile_dims, spd_csf ** out);
+void spd_csf_free(spd_csf * csf);
+
+/* Shape of the host mirror: number of tiles, then per tile and level the
+ * node count (ntiles x nmodes, tile-major). */
+uint64_t spd_csf_ntiles(spd_csf const * csf);
+int spd_csf_tile_nfibs(spd_csf const * csf, uint64_t * nfibs);
+
+/* Copy the host mirror of tile t into host arrays sized from
+ * spd_csf_tile_nfibs: fptr[l] (nfibs[l]+1 entries, l < nmodes-1),
+ * fids[l] (nfibs[l] entries; fids[0] may be NULL to skip), vals (nfibs[N-1]).
+ */
+int spd_csf_export_tile(spd_csf const * csf, uint64_t t, uint64_t ** fptr,
+    uint64_t ** fids, double * vals);
+
+/* sum of squared values, FP64 accumulation */
+int spd_csf_frobsq(spd_csf const * csf, double * out);
+
+/* ---- MTTKRP ------------------------------------------------------------ */
+
+/* out (dims[mode] x R, row-major, device) = MTTKRP of mode `mode` using this
+ * CSF. mats[m] are device row-major dims[m] x R arrays of type dtype
+ * (SPD_F64 / SPD_F32); mats[mode] is not read. root_rows_are_positions
+ * selects the reference's untiled root-row convention. stream may be NULL
+ * (library stream). */
+int spd_mttkrp(spd_csf * csf, int mode, uint64_t R, void * const * mats,
+    void * out, int dtype, int root_rows_are_positions, void * stream);
+
+/* Bytes and flops of one MTTKRP by the algorithmic model: compulsory reads of
+ * values, indices, pointers and input factors plus one write of the output,
+ * with 4-byte indices and sizeof(dtype) values. */
+int spd_mttkrp_model(spd_csf const * csf, int mode, uint64_t R, int dtype,
+    double * bytes, double * flops);
+
+/* Record HIP events around the main MTTKRP kernel (on the stream it runs
+ * on) when enabled; spd_mttkrp_last_kernel_ms() waits for and returns the
+ * last such time in ms, or <0 if none was recorded. */
+int spd_mttkrp_timing(int enable);
+double spd_mttkrp_last_kernel_ms(void);
+
+/* ---- dense / CPD ------------------------------------------------------- */
+
+/* Convert FP64 device array to FP32 and back. */
+int spd_f64_to_f32(double const * src, float * dst, uint64_t n, void * stream);
+int spd_f32_to_f64(float const * src, double * dst, uint64_t n, void * stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,66 @@
+// spd_internal.hpp -- shared helpers for the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "spd.h"
+
+// Records an error message for spd_last_error().
+void spd_set_error(int code, const char * fmt, ...);
+
+#define SPD_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if(e_ != hipSuccess) {                                                   \
+      spd_set_error(SPD_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,     \
+          hipGetErrorString(e_));                                            \
+      return SPD_EHIP;                                                       \
+    }                                                                        \
+  } while(0)
+
+#define SPD_TRY(call)                                                        \
+  do {                                                                       \
+    int r_ = (call);                                                         \
+    if(r_ != SPD_OK) {                                                       \
+      return r_;                                                             \
+    }                                                                        \
+  } while(0)
+
+#define SPD_LAUNCH_CHECK()                                                   \
+  do {                                                                       \
+    hipError_t e_ = hipGetLastError();                                       \
+    if(e_ != hipSuccess) {                                                   \
+      spd_set_error(SPD_EHIP, "%s:%d launch: %s", __FILE__, __LINE__,        \
+          hipGetErrorString(e_));                                            \
+      return SPD_EHIP;                                                       \
+    }                                                                        \
+  } while(0)
+
+static inline hipStream_t spd_hs(void * s)
+{
+  return s ? (hipStream_t) s : (hipStream_t) spd_stream();
+}
+
+static inline uint64_t spd_cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Exclusive prefix sum of n uint32 values (in place allowed: in == out).
+// Returns the total in *total (host) if total != NULL (synchronises).
+int spd_scan_u32(uint32_t const * in, uint32_t * out, uint64_t n,
+    uint32_t * total, hipStream_t st);
+
+// Scratch device memory owned by the library, grown on demand. Slot k keeps
+// its buffer until the next request for a larger size.
+int spd_scratch(int slot, size_t bytes, void ** ptr);
+
+// Zero fill with a kernel on stream st.
+int spd_zero_async(void * ptr, size_t bytes, hipStream_t st);
+
+// Sort (key, value) pairs by the low `bits` bits of key, stable. Ping-pongs
+// between (k0,v0) and (k1,v1); on return *which tells where the result is
+// (0: k0/v0, 1: k1/v1).
+int spd_radix_sort(uint64_t * k0, uint32_t * v0, uint64_t * k1, uint32_t * v1,
+    uint64_t n, int bits, int * which, hipStream_t st);
+
+// Device-wide wavefront helpers (64 lanes on gfx950).
+__device__ __forceinline__ int spd_lane() { return __lane_id(); }

@@ -1,0 +1,157 @@
+"""GPU MTTKRP parity against the oracle's brute-force COO MTTKRP (long
+double accumulation, the reference's own known-answer procedure,
+tests/mttkrp_test.c:36-88).
+
+Tolerances (normwise relative, ||got - ref||_F / ||ref||_F):
+  FP64 path 1e-12, FP32 path 1e-5  -- as stated for this path.
+Covered: every mode of every allocation scheme (root, internal and leaf
+kernels), 3- to 8-mode tensors, tiled tensors, ranks that exercise every
+lane layout (1, 3, 8, 16, 32, 64, 100), Zipf-skewed tensors whose heavy
+slices span many work items, the device-pointer entry point, and at a
+larger size the mode-independence of <X, [[A_1..A_N]]> = sum(M_m * A_m).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import DATASETS, tensor_path
+import oracle_py as orc
+import pysplatt as ps
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - np.asarray(b, np.float64))
+                 / max(np.linalg.norm(np.asarray(b, np.float64)), 1e-300))
+
+
+def _set_precision(p):
+    assert ps.lib().splatt_amd_set_mttkrp_precision(p) == ps.SUCCESS
+
+
+@pytest.fixture(autouse=True)
+def _reset_precision():
+    yield
+    ps.lib().splatt_amd_set_mttkrp_precision(ps.F64)
+
+
+@pytest.mark.parametrize("name", DATASETS)
+@pytest.mark.parametrize("kind", [ps.CSF_ONEMODE, ps.CSF_TWOMODE, ps.CSF_ALLMODE])
+def test_fixtures_fp64(gpu_available, name, kind):
+    print("case", name, kind, flush=True)
+    coo = orc.Coo.read(tensor_path(name))
+    s = ps.CsfSet.convert(coo.inds(), coo.vals(), ps.Opts(csf_alloc=kind))
+    mats = [ps.synth_matrix(d, 16, 100 + m) for m, d in enumerate(coo.dims)]
+    for mode in range(coo.nmodes):
+        assert rel(s.mttkrp(mode, mats), orc.mttkrp_coo(coo, mats, mode)) <= TOL64
+
+
+@pytest.mark.parametrize("name", ["med.tns", "med4.tns", "med5.tns"])
+def test_fixtures_fp32(gpu_available, name):
+    print("case fp32", name, flush=True)
+    coo = orc.Coo.read(tensor_path(name))
+    s = ps.CsfSet.convert(coo.inds(), coo.vals(), ps.Opts(csf_alloc=ps.CSF_ONEMODE))
+    mats = [ps.synth_matrix(d, 32, 200 + m).astype(np.float32) for m, d in enumerate(coo.dims)]
+    _set_precision(ps.F32)
+    for mode in range(coo.nmodes):
+        got = s.mttkrp(mode, [m.astype(np.float64) for m in mats])
+        assert rel(got, orc.mttkrp_coo(coo, mats, mode, f32=True)) <= TOL32
+
+
+@pytest.mark.parametrize("R", [1, 3, 8, 16, 32, 64, 100])
+@pytest.mark.parametrize("prec", [ps.F64, ps.F32])
+def test_ranks_and_layouts(gpu_available, R, prec):
+    print("case rank", R, prec, flush=True)
+    inds, vals, dims = ps.synth_coord([300, 200, 100], 30000, zipf=[1.2, 1.1, 0.0],
+                                      vals="lognormal", seed=R)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    s = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=ps.CSF_ONEMODE))
+    mats = [ps.synth_matrix(d, R, 7 + m) for m, d in enumerate(dims)]
+    _set_precision(prec)
+    for mode in range(3):
+        got = s.mttkrp(mode, mats)
+        if prec == ps.F64:
+            assert rel(got, orc.mttkrp_coo(coo, mats, mode)) <= TOL64
+        else:
+            m32 = [m.astype(np.float32) for m in mats]
+            assert rel(got, orc.mttkrp_coo(coo, m32, mode, f32=True)) <= TOL32
+
+
+@pytest.mark.parametrize("nmodes", [4, 5, 6, 8])
+def test_higher_order(gpu_available, nmodes):
+    print("case nmodes", nmodes, flush=True)
+    dims = [40, 30, 20, 12, 9, 7, 5, 4][:nmodes]
+    inds, vals, d2 = ps.synth_coord(dims, 40000, zipf=[1.1] * nmodes, seed=nmodes)
+    coo = orc.Coo.from_arrays(inds, vals, d2)
+    mats = [ps.synth_matrix(d, 8, 30 + m) for m, d in enumerate(d2)]
+    for kind in (ps.CSF_ONEMODE, ps.CSF_TWOMODE, ps.CSF_ALLMODE):
+        s = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=kind))
+        for mode in range(nmodes):
+            assert rel(s.mttkrp(mode, mats), orc.mttkrp_coo(coo, mats, mode)) <= TOL64
+
+
+@pytest.mark.parametrize("tilelevel", [1, 2])
+def test_tiled(gpu_available, tilelevel):
+    coo = orc.Coo.read(tensor_path("med4.tns"))
+    opts = ps.Opts(csf_alloc=ps.CSF_TWOMODE, tile=ps.DENSETILE, tilelevel=tilelevel,
+                   nthreads=5)
+    s = ps.CsfSet.convert(coo.inds(), coo.vals(), opts)
+    mats = [ps.synth_matrix(d, 16, 40 + m) for m, d in enumerate(coo.dims)]
+    for mode in range(4):
+        assert rel(s.mttkrp(mode, mats), orc.mttkrp_coo(coo, mats, mode)) <= TOL64
+
+
+def test_heavy_zipf_slices(gpu_available):
+    # a few slices hold most nonzeros, so they span many work items and are
+    # completed through the partial (atomic) flushes
+    inds, vals, dims = ps.synth_coord([2000, 1500, 1000], 1000000, zipf=[1.6, 1.6, 1.6],
+                                      vals="lognormal", seed=3)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    mats = [ps.synth_matrix(d, 32, 50 + m) for m, d in enumerate(dims)]
+    for kind in (ps.CSF_ONEMODE, ps.CSF_ALLMODE):
+        s = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=kind))
+        for mode in range(3):
+            assert rel(s.mttkrp(mode, mats), orc.mttkrp_coo(coo, mats, mode)) <= TOL64
+
+
+def test_device_entry_point(gpu_available):
+    inds, vals, dims = ps.synth_coord([500, 400, 300], 50000, zipf=[1.2] * 3, seed=9)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    opts = ps.Opts(csf_alloc=ps.CSF_ALLMODE)
+    s = ps.CsfSet.convert(inds, vals, opts)
+    R = 32
+    mats = [ps.synth_matrix(d, R, 60 + m) for m, d in enumerate(dims)]
+    L = ps.lib()
+    for prec, dt in ((ps.F64, np.float64), (ps.F32, np.float32)):
+        dm = [ps.DevBuf.from_host(m.astype(dt)) for m in mats]
+        arr = (ctypes.c_void_p * 3)(*[b.ptr for b in dm])
+        for mode in range(3):
+            out = ps.DevBuf(dims[mode] * R * np.dtype(dt).itemsize)
+            assert L.splatt_amd_mttkrp_dev(mode, R, s.ptr, arr, out.ptr, prec, opts.ptr,
+                                           None) == ps.SUCCESS
+            assert L.splatt_amd_sync() == ps.SUCCESS
+            got = out.to_host((dims[mode], R), dt)
+            if prec == ps.F64:
+                assert rel(got, orc.mttkrp_coo(coo, mats, mode)) <= TOL64
+            else:
+                m32 = [m.astype(np.float32) for m in mats]
+                assert rel(got, orc.mttkrp_coo(coo, m32, mode, f32=True)) <= TOL32
+
+
+def test_inner_product_is_mode_independent(gpu_available):
+    # <X, [[A_1,...,A_N]]> = sum_i sum_r M_m[i,r] A_m[i,r] for every mode m
+    inds, vals, dims = ps.synth_coord([20000, 15000, 5000], 5000000, zipf=[1.2] * 3,
+                                      vals="lognormal", seed=12)
+    s = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=ps.CSF_ONEMODE))
+    mats = [ps.synth_matrix(d, 16, 70 + m) for m, d in enumerate(dims)]
+    inner = [float(np.sum(s.mttkrp(m, mats) * mats[m])) for m in range(3)]
+    scale = float(np.sum(np.abs(vals))) * 1.0
+    assert max(inner) - min(inner) <= 1e-10 * scale
+    # linearity in the values
+    s2 = ps.CsfSet.convert(inds, 2.0 * vals, ps.Opts(csf_alloc=ps.CSF_ONEMODE))
+    assert rel(s2.mttkrp(0, mats), 2.0 * s.mttkrp(0, mats)) <= 1e-14

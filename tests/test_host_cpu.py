@@ -1,0 +1,139 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+everything include/ declares, option defaults, file input, mode ordering,
+compaction and the MTTKRP workspace map (all host C code)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import DATASETS, ROOT, tensor_path
+import oracle_py as orc
+import pysplatt as ps
+
+
+def _declared_functions():
+    names = set()
+    pat = re.compile(r"^[A-Za-z_][\w \*]*?\b(splatt_\w+)\s*\(", re.M)
+    for h in glob.glob(os.path.join(ROOT, "include", "**", "*.h"), recursive=True):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in pat.finditer(text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_exports_every_declared_symbol():
+    L = ps.lib()
+    names = _declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_version():
+    assert ps.lib().splatt_version_major() == 2
+
+
+def test_default_opts():
+    o = ps.Opts()
+    a = o.arr
+    assert a[ps.OPT_TOLERANCE] == 1e-5
+    assert a[ps.OPT_NITER] == 50
+    assert a[ps.OPT_CSF_ALLOC] == ps.CSF_TWOMODE
+    assert a[ps.OPT_TILE] == ps.NOTILE
+    assert a[ps.OPT_TILELEVEL] == 1
+    assert a[ps.OPT_PRIVTHRESH] == 0.02
+    assert a[ps.OPT_NTHREADS] >= 1
+
+
+@pytest.mark.parametrize("name", DATASETS + ["small4_zeroidx.tns"])
+def test_tt_read_matches_oracle(name):
+    L = ps.lib()
+    p = L.splatt_tt_read(tensor_path(name).encode())
+    assert p
+    tt = p[0]
+    o = orc.Coo.read(tensor_path(name))
+    N = int(tt.nmodes)
+    nnz = int(tt.nnz)
+    assert N == o.nmodes and nnz == o.nnz
+    assert [int(tt.dims[m]) for m in range(N)] == o.dims
+    for m in range(N):
+        assert np.array_equal(np.ctypeslib.as_array(tt.ind[m], shape=(nnz,)), o.inds()[m])
+    assert np.array_equal(np.ctypeslib.as_array(tt.vals, shape=(nnz,)), o.vals())
+    L.splatt_tt_free(p)
+
+
+def test_coord_load_and_accessors():
+    L = ps.lib()
+    c = L.splatt_alloc_coord(3, 1)
+    assert L.splatt_coord_load(tensor_path("small.tns").encode(), c) == ps.SUCCESS
+    L.splatt_coord_get_nnz.restype = C.c_uint64
+    L.splatt_coord_get_modes.restype = C.c_uint64
+    assert L.splatt_coord_get_nnz(c) == 6 and L.splatt_coord_get_modes(c) == 3
+    L.splatt_free_coord(c)
+
+
+def test_mode_order_matches_oracle():
+    L = ps.lib()
+    rng = np.random.default_rng(0)
+    for trial in range(200):
+        n = int(rng.integers(2, 9))
+        dims = rng.integers(1, 6, n).astype(np.uint64)
+        for typ in range(4):
+            mode = int(rng.integers(0, n))
+            perm = np.zeros(n, dtype=np.uint64)
+            L.splatt_csf_find_mode_order(dims.ctypes.data_as(ps.u64p), n, typ, mode,
+                                         perm.ctypes.data_as(ps.u64p))
+            assert list(perm) == orc.mode_order(list(dims), typ, mode)
+
+
+def test_remove_empty_matches_oracle():
+    L = ps.lib()
+    rng = np.random.default_rng(4)
+    dims = [30, 20, 25]
+    nnz = 120
+    inds = [rng.choice(np.arange(0, d, 2), nnz) for d in dims]
+    vals = rng.random(nnz)
+    p = L.splatt_alloc_coord(3, nnz)
+    for m in range(3):
+        p[0].dims[m] = dims[m]
+        np.ctypeslib.as_array(p[0].ind[m], shape=(nnz,))[:] = inds[m]
+    np.ctypeslib.as_array(p[0].vals, shape=(nnz,))[:] = vals
+    removed = L.splatt_tt_remove_empty(p)
+    o = orc.Coo.from_arrays(inds, vals, dims)
+    assert removed == o.remove_empty()
+    for m in range(3):
+        assert int(p[0].dims[m]) == o.dims[m]
+        assert np.array_equal(np.ctypeslib.as_array(p[0].ind[m], shape=(nnz,)), o.inds()[m])
+        # indmap keeps local -> global
+        im = np.ctypeslib.as_array(p[0].indmap[m], shape=(o.dims[m],))
+        assert np.array_equal(im, np.unique(inds[m]))
+    L.splatt_free_coord(p)
+
+
+def test_mttkrp_ws_map():
+    # host-only tensor header: the map and privatisation rule of
+    # src/mttkrp.c:1831-1929 need only dims, nnz and dim_iperm
+    arr = (ps.SplattCsf * 3)()
+    dims = [100, 7, 50]
+    for i in range(3):
+        arr[i].nnz = 10000
+        arr[i].nmodes = 3
+        for m in range(3):
+            arr[i].dims[m] = dims[m]
+    perm = orc.mode_order(dims, 0)       # [1, 2, 0]
+    for l in range(3):
+        arr[0].dim_perm[l] = perm[l]
+        arr[0].dim_iperm[perm[l]] = l
+    L = ps.lib()
+    for kind, want in ((ps.CSF_ONEMODE, [0, 0, 0]), (ps.CSF_TWOMODE, [1, 0, 0]),
+                       (ps.CSF_ALLMODE, [0, 1, 2])):
+        o = ps.Opts(csf_alloc=kind, nthreads=8)
+        ws = L.splatt_mttkrp_alloc_ws(arr, 16, o.ptr)
+        assert [int(ws[0].mode_csf_map[m]) for m in range(3)] == want
+        # privatise iff dims*T <= 0.02*nnz: only mode 1 (7*8 = 56 <= 200)
+        assert [bool(ws[0].is_privatized[m]) for m in range(3)] == [False, True, False]
+        L.splatt_mttkrp_free_ws(ws)

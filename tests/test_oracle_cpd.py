@@ -1,0 +1,164 @@
+"""Pins the oracle's dense kernels, the rand() initialisation and CPD-ALS.
+
+  or_mat_aTa           numpy A.T @ A.
+  or_cholesky/_solve   L L^T reconstructs G, L equals numpy's Cholesky, the
+                       solve's residual ||G x - b|| is at rounding level.
+  or_mat_normalize     unit column norms and A_out * lambda == A_in.
+  or_fill_rand         the glibc rand() stream called directly: each value
+                       consumes two draws, magnitude 3*r/RAND_MAX, negative
+                       when the second draw is even (src/util.c:15-24).
+  or_cpd_als           (a) one sweep against a textbook ALS written with
+                       numpy (Khatri-Rao + normal equations) from the same
+                       start; (b) the reported fit of the last iteration
+                       equals 1 - ||X - X_hat||^2/||X||^2 computed on the
+                       dense tensor; (c) fit non-decreasing across iterations
+                       (the reference asserts err <= olderr,
+                       src/cpd/cpd.c:258); (d) an exactly rank-2 tensor is
+                       fitted to within 1e-6.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_py as orc
+
+libc = ctypes.CDLL("libc.so.6")
+RAND_MAX = 2147483647
+
+
+def test_aTa_and_normalize():
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((37, 5))
+    out = np.zeros((5, 5))
+    orc.lib().or_mat_aTa(orc._p(A, ctypes.c_double), 37, 5, orc._p(out, ctypes.c_double))
+    assert np.allclose(out, A.T @ A, rtol=1e-13, atol=1e-12)
+    B = A.copy()
+    lam = np.zeros(5)
+    orc.lib().or_mat_normalize(orc._p(B, ctypes.c_double), 37, 5, orc._p(lam, ctypes.c_double))
+    assert np.allclose(np.linalg.norm(B, axis=0), 1.0)
+    assert np.allclose(B * lam, A)
+
+
+def test_cholesky_and_solve():
+    rng = np.random.default_rng(2)
+    M = rng.standard_normal((9, 9))
+    G = M @ M.T + 9 * np.eye(9)
+    L = G.copy()
+    assert orc.lib().or_cholesky(orc._p(L, ctypes.c_double), 9) == 0
+    low = np.tril(L)
+    assert np.allclose(low @ low.T, G, rtol=1e-12, atol=1e-12)
+    assert np.allclose(low, np.linalg.cholesky(G), rtol=1e-12, atol=1e-12)
+    Bm = rng.standard_normal((13, 9))
+    X = Bm.copy()
+    orc.lib().or_cholesky_solve(orc._p(L, ctypes.c_double), 9, orc._p(X, ctypes.c_double), 13)
+    assert np.max(np.abs(X @ G - Bm)) <= 1e-10
+    # a non-positive pivot is reported like LAPACK's info
+    bad = -np.eye(3)
+    assert orc.lib().or_cholesky(orc._p(bad, ctypes.c_double), 3) == 1
+
+
+def test_fill_rand_stream():
+    libc.srand(1234)
+    got = orc.fill_rand(20)
+    libc.srand(1234)
+    for v in got:
+        a = libc.rand()
+        b = libc.rand()
+        mag = 3.0 * a / RAND_MAX
+        assert abs(abs(v) - mag) <= 1e-15
+        assert (v < 0) == (b % 2 == 0) or mag == 0.0
+
+
+def _dense(inds, vals, dims):
+    X = np.zeros(dims)
+    np.add.at(X, tuple(inds), vals)
+    return X
+
+
+def _khatri_rao(mats):
+    R = mats[0].shape[1]
+    out = mats[0]
+    for M in mats[1:]:
+        out = (out[:, None, :] * M[None, :, :]).reshape(-1, R)
+    return out
+
+
+def _numpy_als_sweep(X, facs):
+    """One textbook ALS sweep with per-mode 2-norm column normalisation."""
+    N = X.ndim
+    facs = [f.copy() for f in facs]
+    lam = None
+    for m in range(N):
+        others = [facs[k] for k in range(N) if k != m]
+        Xm = np.moveaxis(X, m, 0).reshape(X.shape[m], -1)
+        KR = _khatri_rao(others)
+        G = np.ones((facs[0].shape[1],) * 2)
+        for o in others:
+            G *= o.T @ o
+        A = np.linalg.solve(G, (Xm @ KR).T).T
+        lam = np.linalg.norm(A, axis=0)
+        facs[m] = A / lam
+    return facs, lam
+
+
+def _tensor(dims, nnz, seed, rank_r=None):
+    rng = np.random.default_rng(seed)
+    if rank_r is None:
+        cells = rng.choice(int(np.prod(dims)), nnz, replace=False)
+        inds = list(np.unravel_index(cells, dims))
+        vals = rng.random(nnz) + 0.1
+    else:
+        fs = [rng.random((d, rank_r)) + 0.2 for d in dims]
+        X = np.einsum("ir,jr,kr->ijk", *fs)
+        inds = list(np.indices(dims).reshape(len(dims), -1))
+        vals = X.reshape(-1)
+    return inds, vals
+
+
+def _init(dims, R, seed):
+    libc.srand(seed)
+    return [orc.fill_rand(d * R).reshape(d, R) for d in dims]
+
+
+def test_cpd_one_sweep_matches_numpy():
+    dims = [6, 5, 7]
+    inds, vals = _tensor(dims, 90, 3)
+    c = orc.Coo.from_arrays(inds, vals, dims)
+    csfs = [orc.Csf(c, orc.mode_order(dims, 3, m)) for m in range(3)]
+    init = _init(dims, 3, 42)
+    facs, lam, fits = orc.cpd_als(csfs, [0, 1, 2], init, 1, 0.0)
+    # the oracle normalises the start first (src/cpd/cpd.c:179-188)
+    start = [f / np.linalg.norm(f, axis=0) for f in init]
+    want, wl = _numpy_als_sweep(_dense(inds, vals, dims), start)
+    # after the final post-processing the oracle's factors are unit columns
+    # and lambda carries the norms: compare the models
+    for m in range(3):
+        assert np.allclose(facs[m], want[m], rtol=1e-9, atol=1e-9)
+    assert np.allclose(lam, wl, rtol=1e-9)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_cpd_fit_matches_dense_residual_and_is_monotone(seed):
+    dims = [8, 6, 7]
+    inds, vals = _tensor(dims, 150, seed)
+    c = orc.Coo.from_arrays(inds, vals, dims)
+    csfs = [orc.Csf(c, orc.mode_order(dims, 0)), orc.Csf(c, orc.mode_order(dims, 3, 1))]
+    perm0 = orc.mode_order(dims, 0)
+    mode_csf = [1 if perm0[-1] == m else 0 for m in range(3)]
+    facs, lam, fits = orc.cpd_als(csfs, mode_csf, _init(dims, 4, seed), 15, 0.0)
+    assert len(fits) == 15
+    assert np.all(np.diff(fits) >= -1e-12)
+    X = _dense(inds, vals, dims)
+    Xh = np.einsum("r,ir,jr,kr->ijk", lam, *facs)
+    fit = 1 - np.sum((X - Xh) ** 2) / np.sum(X ** 2)
+    assert abs(fit - fits[-1]) <= 1e-10
+
+
+def test_cpd_exact_low_rank():
+    dims = [7, 6, 5]
+    inds, vals = _tensor(dims, None, 11, rank_r=2)
+    c = orc.Coo.from_arrays(inds, vals, dims)
+    csfs = [orc.Csf(c, orc.mode_order(dims, 3, m)) for m in range(3)]
+    facs, lam, fits = orc.cpd_als(csfs, [0, 1, 2], _init(dims, 2, 5), 200, 1e-12)
+    assert fits[-1] >= 1 - 1e-6
