@@ -51,10 +51,14 @@ splatt_error_type splatt_amd_mttkrp_cost(
     double * bytes,
     double * flops);
 
-/* Device time (ms) of the main kernel of the last MTTKRP, measured with HIP
- * events on the stream it ran on, when timing was enabled; <0 otherwise. */
+/* Kernel timing. While enabled (which also clears the record), a pair of HIP
+ * events is recorded around the main kernel of every MTTKRP, on the stream
+ * it runs on. last_ms returns the last such duration; kernel_times copies
+ * the durations of the last `max` timed launches (oldest first, at most
+ * 4096 kept) and returns how many. Both wait for the events. */
 void splatt_amd_mttkrp_timing(int enable);
 double splatt_amd_mttkrp_last_ms(void);
+int splatt_amd_mttkrp_kernel_times(double * ms, int max);
 
 /* Device memory helpers for callers that hold device-resident factors. */
 void * splatt_amd_dev_alloc(size_t bytes);

@@ -101,6 +101,7 @@ int spd_mttkrp_model(spd_csf const * csf, int mode, uint64_t R, int dtype,
  * last such time in ms, or <0 if none was recorded. */
 int spd_mttkrp_timing(int enable);
 double spd_mttkrp_last_kernel_ms(void);
+int spd_mttkrp_kernel_times(double * ms, int max);
 
 /* ---- dense / CPD ------------------------------------------------------- */
 

@@ -39,6 +39,7 @@ struct spd_csf {
   uint64_t * tile_nnz;             // host, ntiles+1 prefix of nnz per tile
   float * vals32;                  // FP32 copy of u.vals, made on demand
   MttkrpPlan * plans;              // launch plans, one per item size
+  struct HotPlan * hot;            // output-row rankings, one per level
 };
 
 void spd_csf_drop_plans(spd_csf * c);

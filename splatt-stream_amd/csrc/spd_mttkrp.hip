@@ -16,8 +16,16 @@
 // its accumulator is multiplied by the node's factor row and added to the
 // parent's accumulator (root output), or flushed to the output row
 // (internal/leaf output). Partial nodes at item edges are folded the same
-// way (the products are linear) and flushed with atomics; a root slice that
-// starts and ends inside one item is written with a plain store.
+// way (the products are linear).
+//
+// Root output needs no atomics in the walker: a slice that starts and ends
+// inside one item is stored; the partial sums of a slice cut by item edges
+// go to a carry buffer (two rows per item) that a fix-up kernel reduces, so
+// a Zipf slice spread over thousands of items costs a few dozen atomics.
+// Internal and leaf outputs scatter to rows shared by many fibers; the rows
+// with the most contributions (top-K, chosen once per tensor and level) are
+// accumulated per block in LDS with LDS atomics and written once per block,
+// the remaining (cold) rows take global atomics.
 #include "spd_internal.hpp"
 #include "spd_csf_dev.hpp"
 #include <stdlib.h>
@@ -81,6 +89,11 @@ struct WalkArgs {
   T * out;
   uint32_t const * plan;          // per item: node at levels 0..N-2
   uint8_t const * head;           // per item: 1 if its first slice began earlier
+  T * carry;                      // root: 2 partial rows per item (R wide)
+  uint32_t * carry_row;           // root: output row of each partial, or NONE
+  int32_t const * hot_slot;       // internal/leaf: LDS slot of an output row, -1 cold
+  uint32_t const * hot_rows;      // internal/leaf: output row of each LDS slot
+  uint32_t nhot;                  // internal/leaf: rows privatised in LDS
   uint64_t nnz;
   uint32_t R;
   uint32_t S;
@@ -89,18 +102,36 @@ struct WalkArgs {
   int rows_are_positions;         // root row = slice position (untiled host CSF)
 };
 
+#define CARRY_NONE 0xFFFFFFFFu
+
+// Output of a completed or partial node (internal/leaf kernels): rows with
+// an LDS slot accumulate there with LDS atomics, the rest go to HBM.
+template<typename T, int V, int L>
+__device__ __forceinline__ void
+flush_priv(WalkArgs<T> const & a, T * lds, uint32_t row, uint32_t col, uint32_t c,
+    bool cok, typename VecT<T, V>::type v)
+{
+  int32_t const slot = (a.nhot > 0) ? a.hot_slot[row] : -1;
+  if(slot >= 0 && (uint32_t) slot < a.nhot) {
+    if(!cok) return;
+    T * p = lds + (uint32_t) slot * (L * V) + c * V;
+    if constexpr (V == 1) {
+      atomicAdd(p, v);
+    } else {
+      #pragma unroll
+      for(int k=0; k < V; ++k) atomicAdd(p + k, v[k]);
+    }
+  } else {
+    vadd_atomic<T, V>(a.out, row, a.R, col, cok, v);
+  }
+}
+
 template<typename T, int N, int KIND, int V, int L>
-__global__ void __launch_bounds__(MK_THREADS)
-k_walk(WalkArgs<T> a)
+__device__ __forceinline__ void
+walk_item(WalkArgs<T> const & a, uint32_t item, uint32_t c, T * lds)
 {
   typedef typename VecT<T, V>::type VT;
-  constexpr int G = 64 / L;
   constexpr int NL = N - 1;        // internal levels 0..N-2
-  int const lane = threadIdx.x & 63;
-  int const c = lane % L;
-  int const g = lane / L;
-  uint32_t const item = ((uint32_t) blockIdx.x * (MK_THREADS / 64) + (threadIdx.x >> 6)) * G + g;
-  if(item >= a.nitems) return;
   uint32_t const col = blockIdx.y * (L * V) + c * V;
   bool const cok = col < a.R;
   uint32_t const R = a.R;
@@ -121,6 +152,8 @@ k_walk(WalkArgs<T> a)
   }
   bool head_partial = a.head[item] != 0;
   int const depth = (KIND == KIND_ROOT) ? 0 : ((KIND == KIND_LEAF) ? N - 1 : a.depth);
+  uint32_t carry0 = CARRY_NONE;   // root: row of the completed partial slice
+  uint32_t carry1 = CARRY_NONE;   // root: row of the unfinished last slice
 
   VT acc[NL];
   #pragma unroll
@@ -150,7 +183,7 @@ k_walk(WalkArgs<T> a)
       for(; n < e; ++n) {
         T const v = a.vals[n];
         uint32_t const j = a.fids[N - 1][n];
-        vadd_atomic<T, V>(a.out, j, R, col, cok, v * h);
+        flush_priv<T, V, L>(a, lds, j, col, c, cok, v * h);
       }
     } else {
       T const * __restrict__ ul = a.um[N - 1];
@@ -192,7 +225,9 @@ k_walk(WalkArgs<T> a)
           if(l == 0) {
             uint32_t const row = a.rows_are_positions ? node[0] : a.fids[0][node[0]];
             if(head_partial) {
-              vadd_atomic<T, V>(a.out, row, R, col, cok, acc[0]);
+              // began in an earlier item: hand the partial to the fix-up
+              vstore<T, V>(a.carry, 2 * item, R, col, cok, acc[0]);
+              carry0 = row;
             } else {
               vstore<T, V>(a.out, row, R, col, cok, acc[0]);
             }
@@ -213,7 +248,7 @@ k_walk(WalkArgs<T> a)
             acc[l - 1] += acc[l] * r;
             acc[l] = vzero<VT>();
           } else if(l == depth) {
-            vadd_atomic<T, V>(a.out, a.fids[l][node[l]], R, col, cok,
+            flush_priv<T, V, L>(a, lds, a.fids[l][node[l]], col, c, cok,
                 acc[l] * pre[l > 0 ? l - 1 : 0]);
             acc[l] = vzero<VT>();
           }
@@ -259,8 +294,12 @@ k_walk(WalkArgs<T> a)
           acc[l - 1] += acc[l] * r;
         }
       }
-      uint32_t const row = a.rows_are_positions ? node[0] : a.fids[0][node[0]];
-      vadd_atomic<T, V>(a.out, row, R, col, cok, acc[0]);
+      carry1 = a.rows_are_positions ? node[0] : a.fids[0][node[0]];
+      vstore<T, V>(a.carry, 2 * item + 1, R, col, cok, acc[0]);
+    }
+    if(c == 0 && blockIdx.y == 0) {
+      a.carry_row[2 * item] = carry0;
+      a.carry_row[2 * item + 1] = carry1;
     }
   } else if constexpr (KIND == KIND_INTL) {
     if(stop > depth) {
@@ -274,10 +313,406 @@ k_walk(WalkArgs<T> a)
       #pragma unroll
       for(int l=0; l < NL; ++l) {
         if(l == depth) {
-          vadd_atomic<T, V>(a.out, a.fids[l][node[l]], R, col, cok,
+          flush_priv<T, V, L>(a, lds, a.fids[l][node[l]], col, c, cok,
               acc[l] * pre[l > 0 ? l - 1 : 0]);
         }
       }
+    }
+  }
+}
+
+template<typename T, int N, int KIND, int V, int L>
+__global__ void __launch_bounds__(MK_THREADS)
+k_walk(WalkArgs<T> a)
+{
+  constexpr int G = 64 / L;
+  constexpr int CW = L * V;        // columns of this column tile
+  int const lane = threadIdx.x & 63;
+  uint32_t const c = lane % L;
+  uint32_t const g = lane / L;
+  uint32_t const item = ((uint32_t) blockIdx.x * (MK_THREADS / 64) + (threadIdx.x >> 6)) * G + g;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T * lds = (T *) lds_raw;
+  if constexpr (KIND != KIND_ROOT) {
+    for(uint32_t x=threadIdx.x; x < a.nhot * CW; x += MK_THREADS) lds[x] = (T) 0;
+    __syncthreads();
+  }
+  if(item < a.nitems) {
+    walk_item<T, N, KIND, V, L>(a, item, c, lds);
+  }
+  if constexpr (KIND != KIND_ROOT) {
+    __syncthreads();
+    // hot rows accumulated by this block go to HBM once per block
+    for(uint32_t x=threadIdx.x; x < a.nhot * CW; x += MK_THREADS) {
+      uint32_t const slot = x / CW;
+      uint32_t const cc = x % CW;
+      uint32_t const colx = blockIdx.y * CW + cc;
+      T const v = lds[x];
+      if(colx < a.R && v != (T) 0) {
+        vatomic(a.out + (uint64_t) a.hot_rows[slot] * a.R + colx, v);
+      }
+    }
+  }
+}
+
+// Root fix-up: sums the partial rows of slices that spanned several items.
+// Each group of L lanes owns a chunk of FIX_CHUNK consecutive carry entries;
+// entries of one slice are consecutive, so a chunk flushes each of its runs
+// with one atomic row-add (a slice spanning P items costs ~P/FIX_CHUNK
+// atomics on its row).
+#define FIX_CHUNK 64
+
+template<typename T, int V, int L>
+__global__ void __launch_bounds__(MK_THREADS)
+k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
+    uint32_t nentries, T * __restrict__ out, uint32_t R)
+{
+  typedef typename VecT<T, V>::type VT;
+  constexpr int G = 64 / L;
+  int const lane = threadIdx.x & 63;
+  uint32_t const c = lane % L;
+  uint32_t const g = lane / L;
+  uint32_t const chunk = ((uint32_t) blockIdx.x * (MK_THREADS / 64) + (threadIdx.x >> 6)) * G + g;
+  uint64_t const e0 = (uint64_t) chunk * FIX_CHUNK;
+  if(e0 >= nentries) return;
+  uint64_t e1 = e0 + FIX_CHUNK;
+  if(e1 > nentries) e1 = nentries;
+  uint32_t const col = blockIdx.y * (L * V) + c * V;
+  bool const cok = col < R;
+  uint32_t cur = CARRY_NONE;
+  VT acc = vzero<VT>();
+  for(uint64_t e=e0; e < e1; ++e) {
+    uint32_t const row = crow[e];
+    if(row == CARRY_NONE) continue;
+    if(row != cur) {
+      if(cur != CARRY_NONE) vadd_atomic<T, V>(out, cur, R, col, cok, acc);
+      cur = row;
+      acc = vzero<VT>();
+    }
+    acc += vload<T, V>(carry, (uint32_t) e, R, col, cok);
+  }
+  if(cur != CARRY_NONE) vadd_atomic<T, V>(out, cur, R, col, cok, acc);
+}
+
+// ---- 3-mode root kernel with LDS staging ---------------------------------
+//
+// The hot path of an all-mode (or two-mode) CSF: out[s] = sum over fibers f
+// of slice s of A[fid(f)] o sum_{nnz j in f} v_j B[leaf(j)].
+//
+// A wave owns 256 consecutive nonzeros, split into G = 64/L items of 256/G
+// nonzeros (one per group of L lanes). Before walking, the wave stages into
+// LDS, with coalesced loads: the values and leaf ids of its nonzeros, the
+// end and id of every fiber and the end and output row of every slice that
+// overlaps its range. A group then walks its item in batches of 8 nonzeros:
+// 8 independent B-row gathers are in flight per batch, every boundary test
+// reads LDS, and the A row of the next fiber is prefetched one fiber ahead.
+// Slices completed inside an item are stored; the partial sums at item
+// edges are combined per block in LDS, and only a block's first and last
+// slice (when they cross the block's edges) reach the carry buffer.
+
+#define R3_WAVES 4
+#define R3_WN 256                  // nonzeros per wave
+#define R3_BATCH 8
+
+template<typename T>
+struct Root3Args {
+  uint32_t const * fptr0;          // slice -> first fiber
+  uint32_t const * fids0;          // slice -> output row (tiled host CSF)
+  uint32_t const * fptr1;          // fiber -> first nonzero
+  uint32_t const * fids1;          // fiber -> A row
+  uint32_t const * leaf;           // nonzero -> B row
+  T const * vals;
+  T const * A;
+  T const * B;
+  T * out;
+  uint32_t const * plan;           // per item: slice, fiber containing its first nnz
+  uint8_t const * head;            // per item: first slice began earlier
+  T * carry;                       // 2 rows per block
+  uint32_t * carry_row;
+  uint64_t nnz;
+  uint32_t nslices;
+  uint32_t nfibers;
+  uint32_t nitems;
+  uint32_t R;
+  int rows_are_positions;
+};
+
+// Inclusive prefix sum over a wave of 64 lanes.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+  int const lane = threadIdx.x & 63;
+  #pragma unroll
+  for(int o=1; o < 64; o <<= 1) {
+    uint32_t const t = __shfl_up(v, o, 64);
+    if(lane >= o) v += t;
+  }
+  return v;
+}
+
+
+// Sum runs of equal rows over entries [e0, e1) (rows[e] == CARRY_NONE are
+// empty). The first run is "open" if open_first, the run containing entry
+// e_last is open at the end. Open runs go to slot 1 (open at the end) or
+// slot 0 (open at the start only) of (orow, oval) -- or, when oval is null,
+// to rows of carry (cv + slot*R, the caller's row range); closed runs are
+// stored to out. Empty slots report CARRY_NONE in orow.
+template<typename T, int V>
+__device__ __forceinline__ void
+combine_runs(uint32_t const * rows, T const * vals, int ld, uint32_t e0, uint32_t e1,
+    bool open_first, uint32_t e_last, uint32_t * orow, T * oval, int old_,
+    T * out, uint32_t R, uint32_t col, uint32_t c, bool cok, T * cv = nullptr)
+{
+  typedef typename VecT<T, V>::type VT;
+  uint32_t cur = CARRY_NONE;
+  bool cur_first = false, cur_last = false, seen = false;
+  uint32_t r0 = CARRY_NONE, r1 = CARRY_NONE;
+  VT acc = vzero<VT>();
+  for(uint32_t e=e0; e <= e1; ++e) {
+    uint32_t const r = (e < e1) ? rows[e] : CARRY_NONE;
+    if(e < e1 && r == CARRY_NONE) continue;
+    if(r != cur || e == e1) {
+      if(cur != CARRY_NONE) {
+        int slot = -1;
+        if(cur_last) slot = 1;
+        else if(cur_first && open_first) slot = 0;
+        if(slot < 0) {
+          vstore<T, V>(out, cur, R, col, cok, acc);
+        } else {
+          if(slot == 0) r0 = cur; else r1 = cur;
+          if(cok) {
+            if(oval != nullptr) *(VT *) (oval + slot * old_ + c * V) = acc;
+            else vstore<T, V>(cv, (uint32_t) slot, R, col, cok, acc);
+          }
+        }
+      }
+      if(e == e1) break;
+      cur = r;
+      cur_first = !seen;
+      cur_last = false;
+      seen = true;
+      acc = vzero<VT>();
+    }
+    if(cok) acc += *(VT const *) (vals + (uint64_t) e * ld + c * V);
+    if(e == e_last) cur_last = true;
+  }
+  orow[0] = r0;
+  orow[1] = r1;
+}
+
+template<typename T, int V, int L>
+__global__ void __launch_bounds__(R3_WAVES * 64)
+k_root3(Root3Args<T> a)
+{
+  typedef typename VecT<T, V>::type VT;
+  constexpr int G = 64 / L;
+  constexpr int S = R3_WN / G;
+  constexpr int CW = L * V;
+  constexpr int NE = R3_WAVES * G * 2;        // combine entries per block
+  constexpr int PER = R3_WN / 64;             // staged nonzeros per lane
+  __shared__ T s_val[R3_WAVES][R3_WN];
+  __shared__ uint32_t s_leaf[R3_WAVES][R3_WN];
+  __shared__ uint32_t s_arow[R3_WAVES][R3_WN + 1];   // A row of local fiber k
+  __shared__ uint16_t s_fi[R3_WAVES][R3_WN];     // local fiber of each nonzero
+  __shared__ uint16_t s_si[R3_WAVES][R3_WN];     // local slice of each nonzero
+  __shared__ uint32_t s_srow[R3_WAVES][R3_WN + 1];   // output row of slice S0 + k
+  __shared__ uint32_t s_crow[NE];
+  __shared__ __align__(16) T s_cval[NE][CW];
+  __shared__ uint32_t s_wrow[R3_WAVES][2];
+  __shared__ __align__(16) T s_wval[R3_WAVES][2][CW];
+
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  uint32_t const c = lane % L;
+  uint32_t const g = lane / L;
+  uint32_t const R = a.R;
+  uint32_t const col = blockIdx.y * CW + c * V;
+  bool const cok = col < R;
+
+  uint64_t const wid = (uint64_t) blockIdx.x * R3_WAVES + wave;
+  uint64_t const W0 = wid * R3_WN;
+  uint64_t const W1 = (W0 + R3_WN < a.nnz) ? W0 + R3_WN : a.nnz;
+  uint32_t const item0 = (uint32_t) (wid * G);
+  uint32_t const wn = (W0 < a.nnz) ? (uint32_t) (W1 - W0) : 0u;
+
+  // ---- stage this wave's range ----
+  // Each lane owns PER consecutive positions p = lane*PER + q. Fiber and
+  // slice starts inside the range are marked (s_fi/s_si temporarily hold
+  // the marks) and turned into local indices with a wave prefix sum.
+  uint32_t F0 = 0, S0 = 0;
+  if(wn > 0) {
+    S0 = a.plan[2ull * item0 + 0];
+    F0 = a.plan[2ull * item0 + 1];
+    uint32_t const nfw = (a.nfibers - F0 < (uint32_t) (R3_WN + 1)) ? a.nfibers - F0 : (uint32_t) (R3_WN + 1);
+    uint32_t const nsw = (a.nslices - S0 < (uint32_t) (R3_WN + 1)) ? a.nslices - S0 : (uint32_t) (R3_WN + 1);
+    for(uint32_t k=lane; k < R3_WN; k += 64) {
+      s_fi[wave][k] = 0;
+      s_si[wave][k] = 0;
+    }
+    for(uint32_t k=lane; k < wn; k += 64) {
+      s_val[wave][k] = a.vals[W0 + k];
+      s_leaf[wave][k] = a.leaf[W0 + k];
+    }
+    for(uint32_t k=lane; k < nsw; k += 64) {
+      s_srow[wave][k] = a.rows_are_positions ? S0 + k : a.fids0[S0 + k];
+    }
+    for(uint32_t k=lane; k < nfw; k += 64) {
+      s_arow[wave][k] = a.fids1[F0 + k];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // fiber k (local, >= 1) starts at fptr1[F0+k]; slice q (local, >= 1)
+    // starts at fiber fptr0[S0+q], i.e. at nonzero fptr1[fptr0[S0+q]]
+    for(uint32_t k=1 + lane; k < nfw; k += 64) {
+      uint64_t const st = a.fptr1[F0 + k];
+      if(st >= W1) break;
+      s_fi[wave][st - W0] = 1;
+    }
+    for(uint32_t q=1 + lane; q < nsw; q += 64) {
+      uint32_t const fs = a.fptr0[S0 + q];
+      if(fs >= a.nfibers) break;
+      uint64_t const st = a.fptr1[fs];
+      if(st >= W1) break;
+      s_si[wave][st - W0] = 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t fm[PER], sm[PER];
+    uint32_t fsum = 0, ssum = 0;
+    #pragma unroll
+    for(int q=0; q < PER; ++q) {
+      fm[q] = s_fi[wave][lane * PER + q];
+      sm[q] = s_si[wave][lane * PER + q];
+      fsum += fm[q];
+      ssum += sm[q];
+    }
+    uint32_t fpre = wave_incl_scan(fsum) - fsum;
+    uint32_t spre = wave_incl_scan(ssum) - ssum;
+    #pragma unroll
+    for(int q=0; q < PER; ++q) {
+      fpre += fm[q];
+      spre += sm[q];
+      uint32_t const p = lane * PER + q;
+      s_fi[wave][p] = (uint16_t) fpre;
+      s_si[wave][p] = (uint16_t) spre;
+    }
+  }
+  for(uint32_t e=threadIdx.x; e < NE; e += R3_WAVES * 64) s_crow[e] = CARRY_NONE;
+  __syncthreads();
+
+  // ---- walk this group's item ----
+  uint32_t const item = item0 + g;
+  uint64_t const n0 = (uint64_t) item * S;
+  uint32_t const e0 = (wave * G + g) * 2;
+  if(n0 < a.nnz) {
+    uint64_t const n1 = (n0 + S < a.nnz) ? n0 + S : a.nnz;
+    bool head_partial = a.head[item] != 0;
+    uint32_t const k0 = (uint32_t) (n0 - W0);
+    uint32_t const k1 = (uint32_t) (n1 - W0);
+    uint32_t cur_s = s_si[wave][k0];
+    uint32_t prev_f = 0xFFFFFFFFu;
+    VT aprev = vzero<VT>();
+    VT accs = vzero<VT>();
+    bool stouch = false;
+    for(uint32_t k=k0; k < k1; k += R3_BATCH) {
+      T v[R3_BATCH];
+      VT b[R3_BATCH];
+      VT ar[R3_BATCH];
+      uint32_t fi[R3_BATCH];
+      bool nf[R3_BATCH];
+      #pragma unroll
+      for(int j=0; j < R3_BATCH; ++j) {
+        bool const ok = k + j < k1;
+        uint32_t const kk = ok ? k + j : k0;
+        v[j] = ok ? s_val[wave][kk] : (T) 0;
+        fi[j] = s_fi[wave][kk];
+        b[j] = vload<T, V>(a.B, s_leaf[wave][kk], R, col, cok && ok);
+      }
+      #pragma unroll
+      for(int j=0; j < R3_BATCH; ++j) {
+        uint32_t const pf = (j == 0) ? prev_f : fi[j > 0 ? j - 1 : 0];
+        nf[j] = (k + j < k1) && (fi[j] != pf);
+        ar[j] = vload<T, V>(a.A, s_arow[wave][nf[j] ? fi[j] : 0], R, col, cok && nf[j]);
+      }
+      #pragma unroll
+      for(int j=0; j < R3_BATCH; ++j) {
+        if(k + j < k1) {
+          VT const aj = nf[j] ? ar[j] : aprev;
+          aprev = aj;
+          uint32_t const sj = s_si[wave][k + j];
+          if(sj != cur_s) {
+            // slice cur_s ended before this nonzero
+            if(stouch) {
+              uint32_t const orow = s_srow[wave][cur_s];
+              if(head_partial) {
+                s_crow[e0] = orow;
+                if(cok) *(VT *) &s_cval[e0][c * V] = accs;
+              } else {
+                vstore<T, V>(a.out, orow, R, col, cok, accs);
+              }
+            }
+            head_partial = false;
+            accs = vzero<VT>();
+            cur_s = sj;
+          }
+          accs += v[j] * (b[j] * aj);
+          stouch = true;
+        }
+      }
+      prev_f = fi[R3_BATCH - 1];
+    }
+    if(stouch) {
+      // is the last slice complete at n1? It is if n1 is the start of a new
+      // slice (or the tensor's end)
+      bool complete = (n1 == a.nnz);
+      if(!complete && n1 < W1) {
+        complete = s_si[wave][k1] != cur_s;
+      } else if(!complete) {
+        // the item ends at the wave's end: the slice is complete iff the
+        // next slice starts exactly there
+        uint32_t const fs = a.fptr0[S0 + cur_s + 1];
+        complete = (fs < a.nfibers) && ((uint64_t) a.fptr1[fs] == n1);
+      }
+      uint32_t const orow = s_srow[wave][cur_s];
+      if(complete && !head_partial) {
+        vstore<T, V>(a.out, orow, R, col, cok, accs);
+      } else if(complete) {
+        s_crow[e0] = orow;
+        if(cok) *(VT *) &s_cval[e0][c * V] = accs;
+      } else {
+        s_crow[e0 + 1] = orow;
+        if(cok) *(VT *) &s_cval[e0 + 1][c * V] = accs;
+      }
+    }
+  }
+  // ---- combine partial slices: per wave, then per block ----
+  // Entries are in nonzero order, so one slice's entries are adjacent. A run
+  // (entries of one row) that may continue past an edge is kept open; a run
+  // closed on both sides is complete and stored. Open runs of a wave go to
+  // its two wave slots (slot 1 if the run reaches the wave's end), and the
+  // block combines the wave slots the same way into its two carry rows.
+  if(g == 0) {
+    uint32_t const e_begin = wave * G * 2;
+    bool const wopen_first = (item0 < a.nitems) && (wn > 0) && a.head[item0] != 0;
+    uint32_t last_item = item0 + G - 1;
+    if(last_item >= a.nitems) last_item = a.nitems - 1;
+    uint32_t const e_lastp1 = e_begin + (last_item - item0) * 2 + 1;
+    combine_runs<T, V>(s_crow, &s_cval[0][0], CW, e_begin, e_begin + 2 * G,
+        wn > 0 ? wopen_first : false, wn > 0 ? e_lastp1 : 0xFFFFFFFFu,
+        &s_wrow[wave][0], &s_wval[wave][0][0], CW, a.out, R, col, c, cok);
+  }
+  __syncthreads();
+  if(wave == 0 && g == 0) {
+    uint32_t const bitem0 = (uint32_t) blockIdx.x * R3_WAVES * G;
+    bool const block_head = (bitem0 < a.nitems) && a.head[bitem0] != 0;
+    // the last wave of the block with nonzeros
+    uint64_t const bw0 = (uint64_t) blockIdx.x * R3_WAVES;
+    uint64_t const nw_total = (a.nnz + R3_WN - 1) / R3_WN;
+    uint32_t const lastw = (uint32_t) (((bw0 + R3_WAVES <= nw_total) ? R3_WAVES : nw_total - bw0) - 1);
+    uint32_t rows[2];
+    T * cv = a.carry + (uint64_t) (2 * blockIdx.x) * R;
+    combine_runs<T, V>(&s_wrow[0][0], &s_wval[0][0][0], CW, 0, 2 * R3_WAVES,
+        block_head, 2 * lastw + 1, rows, nullptr, 0, a.out, R, col, c, cok, cv);
+    if(c == 0 && blockIdx.y == 0) {
+      a.carry_row[2 * blockIdx.x] = rows[0];
+      a.carry_row[2 * blockIdx.x + 1] = rows[1];
     }
   }
 }
@@ -335,6 +770,17 @@ __global__ void k_plan(PlanArgs p)
   p.head[it] = (s < n0) ? 1 : 0;
 }
 
+// Rows of one output level ranked by how many nodes of that level carry
+// them (descending): rank[row] < nhot selects the rows a block privatises.
+struct HotPlan {
+  int level;
+  uint32_t nrows;
+  int32_t * rank;        // device, nrows
+  uint32_t * rows;       // device, nrows, most contributions first
+  std::vector<uint32_t> counts;   // host, counts in rank order (first 4096)
+  HotPlan * next;
+};
+
 void spd_csf_drop_plans(spd_csf * c)
 {
   MttkrpPlan * p = c->plans;
@@ -346,6 +792,100 @@ void spd_csf_drop_plans(spd_csf * c)
     p = nx;
   }
   c->plans = nullptr;
+  HotPlan * h = c->hot;
+  while(h) {
+    HotPlan * nx = h->next;
+    spd_free(h->rank);
+    spd_free(h->rows);
+    delete h;
+    h = nx;
+  }
+  c->hot = nullptr;
+}
+
+__global__ void k_count_rows(uint32_t const * __restrict__ ids, uint64_t n,
+    uint32_t * __restrict__ cnt)
+{
+  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  for(; i < n; i += stride) atomicAdd(&cnt[ids[i]], 1u);
+}
+
+__global__ void k_rank_keys(uint32_t const * __restrict__ cnt, uint32_t n,
+    uint64_t * __restrict__ key, uint32_t * __restrict__ idx)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) {
+    key[i] = (uint64_t) (0xFFFFFFFFu - cnt[i]);
+    idx[i] = i;
+  }
+}
+
+__global__ void k_rank_scatter(uint32_t const * __restrict__ rows, uint32_t n,
+    int32_t * __restrict__ rank)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n && rows[i] < n) rank[rows[i]] = (int32_t) i;
+}
+
+static int get_hot(spd_csf * c, int level, HotPlan ** out, hipStream_t st)
+{
+  for(HotPlan * h = c->hot; h; h = h->next) {
+    if(h->level == level) { *out = h; return SPD_OK; }
+  }
+  uint32_t const nrows = (uint32_t) c->dims[c->perm[level]];
+  uint64_t const nids = c->u.nfibs[level];
+  uint32_t * cnt = nullptr;
+  uint64_t * k0 = nullptr; uint64_t * k1 = nullptr;
+  uint32_t * v0 = nullptr; uint32_t * v1 = nullptr;
+  HotPlan * h = new HotPlan();
+  h->level = level;
+  h->nrows = nrows;
+  int rc = spd_malloc((void **) &cnt, nrows * 4ull);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &k0, nrows * 8ull);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &k1, nrows * 8ull);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &v0, nrows * 4ull);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &v1, nrows * 4ull);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &h->rank, nrows * 4ull);
+  if(rc == SPD_OK) rc = spd_zero_async(cnt, nrows * 4ull, st);
+  if(rc == SPD_OK) {
+    unsigned grid = (unsigned) (spd_cdiv(nids, 256) < 8192 ? spd_cdiv(nids, 256) : 8192);
+    hipLaunchKernelGGL(k_count_rows, dim3(grid), dim3(256), 0, st, c->u.fids[level], nids, cnt);
+    hipLaunchKernelGGL(k_rank_keys, dim3((unsigned) spd_cdiv(nrows, 256)), dim3(256), 0, st,
+        cnt, nrows, k0, v0);
+    if(hipGetLastError() != hipSuccess) rc = SPD_EHIP;
+  }
+  int which = 0;
+  if(rc == SPD_OK) rc = spd_radix_sort(k0, v0, k1, v1, nrows, 32, &which, st);
+  uint64_t * sk = which ? k1 : k0;
+  uint32_t * sv = which ? v1 : v0;
+  if(rc == SPD_OK) {
+    hipLaunchKernelGGL(k_rank_scatter, dim3((unsigned) spd_cdiv(nrows, 256)), dim3(256), 0, st,
+        sv, nrows, h->rank);
+    if(hipGetLastError() != hipSuccess) rc = SPD_EHIP;
+  }
+  if(rc == SPD_OK) {
+    uint32_t const keep = nrows < 4096 ? nrows : 4096;
+    std::vector<uint64_t> hk(keep);
+    rc = spd_d2h(hk.data(), sk, keep * 8ull);
+    h->counts.resize(keep);
+    for(uint32_t i=0; i < keep; ++i) h->counts[i] = 0xFFFFFFFFu - (uint32_t) hk[i];
+  }
+  spd_free(cnt);
+  spd_free(k0); spd_free(k1);
+  if(rc == SPD_OK) {
+    h->rows = sv;                  // keep the sorted row ids
+    spd_free(which ? v0 : v1);
+  } else {
+    spd_free(v0); spd_free(v1);
+    spd_free(h->rank);
+    delete h;
+    return rc == SPD_EHIP ? (spd_set_error(SPD_EHIP, "hot-row ranking failed"), rc) : rc;
+  }
+  h->next = c->hot;
+  c->hot = h;
+  *out = h;
+  return SPD_OK;
 }
 
 static int get_plan(spd_csf * c, uint32_t S, MttkrpPlan ** out, hipStream_t st)
@@ -388,15 +928,33 @@ static int get_plan(spd_csf * c, uint32_t S, MttkrpPlan ** out, hipStream_t st)
 
 // ---- dispatch -------------------------------------------------------------
 
+// LDS for privatised output rows, per block.
+#define HOT_LDS_BYTES (48 * 1024)
+
 template<typename T, int N, int KIND, int V, int L>
-static int launch_walk(WalkArgs<T> const & a, hipStream_t st)
+static int launch_walk(WalkArgs<T> a, hipStream_t st)
 {
   constexpr int G = 64 / L;
+  constexpr int CW = L * V;
   uint32_t const groups_per_block = (MK_THREADS / 64) * G;
   dim3 grid((unsigned) spd_cdiv(a.nitems, groups_per_block),
-            (unsigned) spd_cdiv(a.R, (uint64_t) (L * V)));
-  hipLaunchKernelGGL((k_walk<T, N, KIND, V, L>), grid, dim3(MK_THREADS), 0, st, a);
+            (unsigned) spd_cdiv(a.R, (uint64_t) CW));
+  size_t shmem = 0;
+  if constexpr (KIND != KIND_ROOT) {
+    uint32_t const cap = HOT_LDS_BYTES / (CW * sizeof(T));
+    if(a.nhot > cap) a.nhot = cap;
+    shmem = (size_t) a.nhot * CW * sizeof(T);
+  }
+  hipLaunchKernelGGL((k_walk<T, N, KIND, V, L>), grid, dim3(MK_THREADS), shmem, st, a);
   SPD_LAUNCH_CHECK();
+  if constexpr (KIND == KIND_ROOT) {
+    uint32_t const nent = 2 * a.nitems;
+    uint32_t const chunks = (uint32_t) spd_cdiv(nent, FIX_CHUNK);
+    dim3 g2((unsigned) spd_cdiv(chunks, groups_per_block), grid.y);
+    hipLaunchKernelGGL((k_root_fixup<T, V, L>), g2, dim3(MK_THREADS), 0, st,
+        (T const *) a.carry, (uint32_t const *) a.carry_row, nent, a.out, a.R);
+    SPD_LAUNCH_CHECK();
+  }
   return SPD_OK;
 }
 
@@ -423,6 +981,67 @@ static int dispatch_vl(WalkArgs<T> const & a, hipStream_t st)
   }
   if(R <= 16) return launch_walk<T, N, KIND, 1, 16>(a, st);
   return launch_walk<T, N, KIND, 1, 64>(a, st);
+}
+
+template<typename T, int V, int L>
+static int launch_root3(Root3Args<T> const & a, uint32_t nblocks, hipStream_t st)
+{
+  constexpr int CW = L * V;
+  dim3 grid(nblocks, (unsigned) spd_cdiv(a.R, (uint64_t) CW));
+  hipLaunchKernelGGL((k_root3<T, V, L>), grid, dim3(R3_WAVES * 64), 0, st, a);
+  SPD_LAUNCH_CHECK();
+  uint32_t const nent = 2 * nblocks;
+  uint32_t const chunks = (uint32_t) spd_cdiv(nent, FIX_CHUNK);
+  uint32_t const groups_per_block = (MK_THREADS / 64) * (64 / L);
+  dim3 g2((unsigned) spd_cdiv(chunks, groups_per_block), grid.y);
+  hipLaunchKernelGGL((k_root_fixup<T, V, L>), g2, dim3(MK_THREADS), 0, st,
+      (T const *) a.carry, (uint32_t const *) a.carry_row, nent, a.out, a.R);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+// Lane layout (V columns per lane, L lanes per row chunk) for a rank R.
+static void lane_layout(uint32_t R, int dtype, int * V, int * L)
+{
+  if(dtype == SPD_F32 && R % 4 == 0 && R >= 8) {
+    uint32_t const lanes = R / 4;
+    *V = 4;
+    *L = lanes <= 2 ? 2 : lanes <= 4 ? 4 : lanes <= 8 ? 8 : 16;
+  } else if(dtype == SPD_F64 && R % 2 == 0 && R >= 8) {
+    uint32_t const lanes = R / 2;
+    *V = 2;
+    *L = lanes <= 4 ? 4 : lanes <= 8 ? 8 : lanes <= 16 ? 16 : 32;
+  } else {
+    *V = 1;
+    *L = R <= 16 ? 16 : 64;
+  }
+}
+
+template<typename T>
+static int dispatch_root3(Root3Args<T> const & a, uint32_t nblocks, int V, int L,
+    hipStream_t st)
+{
+  if constexpr (sizeof(T) == 4) {
+    if(V == 4) {
+      switch(L) {
+      case 2: return launch_root3<T, 4, 2>(a, nblocks, st);
+      case 4: return launch_root3<T, 4, 4>(a, nblocks, st);
+      case 8: return launch_root3<T, 4, 8>(a, nblocks, st);
+      default: return launch_root3<T, 4, 16>(a, nblocks, st);
+      }
+    }
+  } else {
+    if(V == 2) {
+      switch(L) {
+      case 4: return launch_root3<T, 2, 4>(a, nblocks, st);
+      case 8: return launch_root3<T, 2, 8>(a, nblocks, st);
+      case 16: return launch_root3<T, 2, 16>(a, nblocks, st);
+      default: return launch_root3<T, 2, 32>(a, nblocks, st);
+      }
+    }
+  }
+  if(L == 16) return launch_root3<T, 1, 16>(a, nblocks, st);
+  return launch_root3<T, 1, 64>(a, nblocks, st);
 }
 
 template<typename T, int KIND>
@@ -459,27 +1078,52 @@ static uint32_t choose_S(uint64_t nnz, uint32_t R, int dtype)
   return (uint32_t) S;
 }
 
+// Kernel timing: when enabled, a pair of HIP events is recorded around every
+// walker launch on the stream it runs on, into a ring of SPD_NEV pairs.
+#define SPD_NEV 4096
 static int g_timing = 0;
-static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
-static int g_ev_valid = 0;
+static hipEvent_t g_ev[SPD_NEV][2];
+static int g_ev_made = 0;
+static uint64_t g_ev_count = 0;
 
 extern "C" int spd_mttkrp_timing(int enable)
 {
-  g_timing = enable;
-  if(enable && g_ev0 == nullptr) {
-    SPD_HIP(hipEventCreate(&g_ev0));
-    SPD_HIP(hipEventCreate(&g_ev1));
+  if(enable && !g_ev_made) {
+    for(int i=0; i < SPD_NEV; ++i) {
+      SPD_HIP(hipEventCreate(&g_ev[i][0]));
+      SPD_HIP(hipEventCreate(&g_ev[i][1]));
+    }
+    g_ev_made = 1;
   }
+  g_timing = enable;
+  g_ev_count = 0;
   return SPD_OK;
+}
+
+static double ev_ms(uint64_t k)
+{
+  float ms = -1.f;
+  int const s = (int) (k % SPD_NEV);
+  if(hipEventSynchronize(g_ev[s][1]) != hipSuccess) return -1.0;
+  if(hipEventElapsedTime(&ms, g_ev[s][0], g_ev[s][1]) != hipSuccess) return -1.0;
+  return (double) ms;
 }
 
 extern "C" double spd_mttkrp_last_kernel_ms(void)
 {
-  if(!g_ev_valid) return -1.0;
-  float ms = -1.f;
-  if(hipEventSynchronize(g_ev1) != hipSuccess) return -1.0;
-  if(hipEventElapsedTime(&ms, g_ev0, g_ev1) != hipSuccess) return -1.0;
-  return (double) ms;
+  if(g_ev_count == 0) return -1.0;
+  return ev_ms(g_ev_count - 1);
+}
+
+// Durations (ms) of the last min(max, recorded, SPD_NEV) timed launches, in
+// launch order. Returns how many were written.
+extern "C" int spd_mttkrp_kernel_times(double * ms, int max)
+{
+  uint64_t n = g_ev_count < (uint64_t) SPD_NEV ? g_ev_count : (uint64_t) SPD_NEV;
+  if((uint64_t) max < n) n = (uint64_t) max;
+  uint64_t const first = g_ev_count - n;
+  for(uint64_t k=0; k < n; ++k) ms[k] = ev_ms(first + k);
+  return (int) n;
 }
 
 static int ensure_vals32(spd_csf * c)
@@ -490,7 +1134,55 @@ static int ensure_vals32(spd_csf * c)
   return SPD_OK;
 }
 
-int spd_zero_async(void * ptr, size_t bytes, hipStream_t st);
+template<typename T>
+static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
+    int rows_pos, hipStream_t st, int dtype)
+{
+  int V = 1, L = 16;
+  lane_layout((uint32_t) R, dtype, &V, &L);
+  uint32_t const G = 64 / L;
+  uint32_t const S = R3_WN / G;
+  MttkrpPlan * plan = nullptr;
+  SPD_TRY(get_plan(c, S, &plan, spd_hs(nullptr)));
+  if(st != spd_hs(nullptr)) {
+    SPD_HIP(hipStreamSynchronize(spd_hs(nullptr)));
+  }
+  uint64_t const nwaves = spd_cdiv(c->nnz, R3_WN);
+  uint32_t const nblocks = (uint32_t) spd_cdiv(nwaves, R3_WAVES);
+  void * cv = nullptr;
+  void * cr = nullptr;
+  SPD_TRY(spd_scratch(2, 2ull * nblocks * R * sizeof(T), &cv));
+  SPD_TRY(spd_scratch(3, 2ull * nblocks * sizeof(uint32_t), &cr));
+  Root3Args<T> a = {};
+  a.fptr0 = c->u.fptr[0];
+  a.fids0 = c->u.fids[0];
+  a.fptr1 = c->u.fptr[1];
+  a.fids1 = c->u.fids[1];
+  a.leaf = c->u.fids[2];
+  a.vals = (sizeof(T) == 4) ? (T const *) c->vals32 : (T const *) c->u.vals;
+  a.A = (T const *) mats[c->perm[1]];
+  a.B = (T const *) mats[c->perm[2]];
+  a.out = (T *) out;
+  a.plan = plan->nodes;
+  a.head = plan->head;
+  a.carry = (T *) cv;
+  a.carry_row = (uint32_t *) cr;
+  a.nnz = c->nnz;
+  a.nslices = (uint32_t) c->u.nfibs[0];
+  a.nfibers = (uint32_t) c->u.nfibs[1];
+  a.nitems = plan->nitems;
+  a.R = (uint32_t) R;
+  a.rows_are_positions = rows_pos;
+  SPD_TRY(spd_zero_async(out, c->dims[c->perm[0]] * R * sizeof(T), st));
+  int const slot = (int) (g_ev_count % SPD_NEV);
+  if(g_timing) SPD_HIP(hipEventRecord(g_ev[slot][0], st));
+  int const rc = dispatch_root3<T>(a, nblocks, V, L, st);
+  if(g_timing) {
+    SPD_HIP(hipEventRecord(g_ev[slot][1], st));
+    ++g_ev_count;
+  }
+  return rc;
+}
 
 template<typename T>
 static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
@@ -502,6 +1194,9 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   if(depth < 0) {
     spd_set_error(SPD_EINPUT, "mode %d not in tensor", mode);
     return SPD_EINPUT;
+  }
+  if(N == 3 && depth == 0) {
+    return run_root3<T>(c, R, mats, out, rows_pos, st, dtype);
   }
   uint32_t const S = choose_S(c->nnz, (uint32_t) R, dtype);
   MttkrpPlan * plan = nullptr;
@@ -526,15 +1221,36 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   a.nitems = plan->nitems;
   a.depth = depth;
   a.rows_are_positions = rows_pos;
+  if(depth == 0) {
+    void * cv = nullptr;
+    void * cr = nullptr;
+    SPD_TRY(spd_scratch(2, 2ull * plan->nitems * R * sizeof(T), &cv));
+    SPD_TRY(spd_scratch(3, 2ull * plan->nitems * sizeof(uint32_t), &cr));
+    a.carry = (T *) cv;
+    a.carry_row = (uint32_t *) cr;
+  } else {
+    HotPlan * hp = nullptr;
+    SPD_TRY(get_hot(c, depth, &hp, spd_hs(nullptr)));
+    if(st != spd_hs(nullptr)) {
+      SPD_HIP(hipStreamSynchronize(spd_hs(nullptr)));
+    }
+    // privatise rows that collect at least two contributions
+    uint32_t nhot = 0;
+    while(nhot < hp->counts.size() && hp->counts[nhot] >= 2) ++nhot;
+    a.hot_slot = hp->rank;
+    a.hot_rows = hp->rows;
+    a.nhot = nhot;
+  }
   SPD_TRY(spd_zero_async(out, c->dims[mode] * R * sizeof(T), st));
-  if(g_timing) SPD_HIP(hipEventRecord(g_ev0, st));
+  int const slot = (int) (g_ev_count % SPD_NEV);
+  if(g_timing) SPD_HIP(hipEventRecord(g_ev[slot][0], st));
   int rc;
   if(depth == 0) rc = dispatch_n<T, KIND_ROOT>(N, a, st);
   else if(depth == N - 1) rc = dispatch_n<T, KIND_LEAF>(N, a, st);
   else rc = dispatch_n<T, KIND_INTL>(N, a, st);
   if(g_timing) {
-    SPD_HIP(hipEventRecord(g_ev1, st));
-    g_ev_valid = 1;
+    SPD_HIP(hipEventRecord(g_ev[slot][1], st));
+    ++g_ev_count;
   }
   return rc;
 }

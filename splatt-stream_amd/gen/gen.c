@@ -14,8 +14,10 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 #define GEN_MAX_MODES 8
+#define GEN_CHUNK (1u << 20)
 
 static inline uint64_t sm64(uint64_t * s)
 {
@@ -30,18 +32,39 @@ static inline double u01(uint64_t * s)
   return (double) (sm64(s) >> 11) * (1.0 / 9007199254740992.0);
 }
 
-/* LSD radix sort of 64-bit keys, 16-bit digits, in place via a buffer. */
+/* LSD radix sort of 64-bit keys, 16-bit digits, OpenMP-parallel with
+ * per-thread histograms (stable, so the result is thread-count independent). */
 static void sort_u64(uint64_t * a, uint64_t n, int bits)
 {
-  uint64_t * b = malloc((n ? n : 1) * sizeof(uint64_t));
-  uint64_t * cnt = malloc(65537 * sizeof(uint64_t));
+  if(n < 2) return;
+  uint64_t * b = malloc(n * sizeof(uint64_t));
+  int const T = omp_get_max_threads();
+  uint64_t * cnt = malloc((size_t) T * 65536 * sizeof(uint64_t));
   uint64_t * src = a;
   uint64_t * dst = b;
   for(int sh=0; sh < bits; sh += 16) {
-    memset(cnt, 0, 65537 * sizeof(uint64_t));
-    for(uint64_t i=0; i < n; ++i) cnt[1 + ((src[i] >> sh) & 0xFFFF)]++;
-    for(int d=0; d < 65536; ++d) cnt[d+1] += cnt[d];
-    for(uint64_t i=0; i < n; ++i) dst[cnt[(src[i] >> sh) & 0xFFFF]++] = src[i];
+    #pragma omp parallel num_threads(T)
+    {
+      int const t = omp_get_thread_num();
+      uint64_t const lo = n * (uint64_t) t / T;
+      uint64_t const hi = n * (uint64_t) (t + 1) / T;
+      uint64_t * c = cnt + (size_t) t * 65536;
+      memset(c, 0, 65536 * sizeof(uint64_t));
+      for(uint64_t i=lo; i < hi; ++i) c[(src[i] >> sh) & 0xFFFF]++;
+      #pragma omp barrier
+      #pragma omp single
+      {
+        uint64_t run = 0;
+        for(int d=0; d < 65536; ++d) {
+          for(int u=0; u < T; ++u) {
+            uint64_t const x = cnt[(size_t) u * 65536 + d];
+            cnt[(size_t) u * 65536 + d] = run;
+            run += x;
+          }
+        }
+      }
+      for(uint64_t i=lo; i < hi; ++i) dst[c[(src[i] >> sh) & 0xFFFF]++] = src[i];
+    }
     uint64_t * t = src; src = dst; dst = t;
   }
   if(src != a) memcpy(a, src, n * sizeof(uint64_t));
@@ -119,29 +142,56 @@ int splatt_gen_coord(int nmodes, uint64_t const * dims, double const * zipf_s,
     }
   }
 
-  uint64_t cap = nnz + nnz / 4 + 1024;
-  uint64_t * keys = malloc(cap * sizeof(uint64_t));
+  /* Draw in rounds; each round draws fixed-size chunks whose streams depend
+   * only on (seed, round, chunk), sorts the new keys, and merges them into
+   * the sorted set of distinct keys. The size of the next round follows the
+   * fraction of new distinct keys the last round produced. */
+  uint64_t * keys = malloc((nnz + 1) * sizeof(uint64_t));
   uint64_t have = 0;
-  while(have < nnz) {
-    uint64_t want = (nnz - have) + (nnz - have) / 8 + 64;
-    if(have + want > cap) {
-      cap = have + want;
-      keys = realloc(keys, cap * sizeof(uint64_t));
-    }
-    for(uint64_t i=0; i < want; ++i) {
-      uint64_t k = 0;
-      for(int m=0; m < nmodes; ++m) {
-        k = k * dims[m] + draw(tabs + m, dims[m], zipf_s[m], &s_coord);
+  double yield_est = 1.0;
+  for(uint64_t round=0; have < nnz; ++round) {
+    uint64_t const deficit = nnz - have;
+    uint64_t want = (uint64_t) ((double) deficit / yield_est * 1.05) + 1024;
+    uint64_t * fresh = malloc(want * sizeof(uint64_t));
+    uint64_t const nchunks = (want + GEN_CHUNK - 1) / GEN_CHUNK;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for(uint64_t ch=0; ch < nchunks; ++ch) {
+      uint64_t st = s_coord ^ (0x9E3779B97F4A7C15ull * (round * 65536 + ch + 1));
+      uint64_t const lo = ch * GEN_CHUNK;
+      uint64_t const hi = (lo + GEN_CHUNK < want) ? lo + GEN_CHUNK : want;
+      for(uint64_t i=lo; i < hi; ++i) {
+        uint64_t k = 0;
+        for(int m=0; m < nmodes; ++m) {
+          k = k * dims[m] + draw(tabs + m, dims[m], zipf_s[m], &st);
+        }
+        fresh[i] = k;
       }
-      keys[have + i] = k;
     }
-    uint64_t const tot = have + want;
-    sort_u64(keys, tot, bits);
+    sort_u64(fresh, want, bits);
     uint64_t u = 0;
-    for(uint64_t i=0; i < tot; ++i) {
-      if(u == 0 || keys[i] != keys[u-1]) keys[u++] = keys[i];
+    for(uint64_t i=0; i < want; ++i) {
+      if(u == 0 || fresh[i] != fresh[u-1]) fresh[u++] = fresh[i];
     }
-    have = u;
+    /* merge into the sorted set of distinct keys */
+    uint64_t * merged = malloc((have + u) * sizeof(uint64_t));
+    uint64_t i = 0, j = 0, w = 0;
+    while(i < have || j < u) {
+      uint64_t v;
+      if(j >= u || (i < have && keys[i] <= fresh[j])) {
+        v = keys[i++];
+        if(j < u && fresh[j] == v) ++j;
+      } else {
+        v = fresh[j++];
+      }
+      merged[w++] = v;
+    }
+    uint64_t const added = w - have;
+    yield_est = (double) added / (double) want;
+    if(yield_est < 0.01) yield_est = 0.01;
+    free(fresh);
+    free(keys);
+    keys = merged;
+    have = w;
   }
   /* keep a uniformly chosen subset of exactly nnz keys when overshooting */
   if(have > nnz) {
@@ -183,21 +233,27 @@ int splatt_gen_coord(int nmodes, uint64_t const * dims, double const * zipf_s,
       free(map);
     }
   }
-  for(uint64_t i=0; i < nnz; ++i) {
-    switch(val_kind) {
-    case 1: {
-      double u1 = u01(&s_val);
-      double const u2 = u01(&s_val);
-      if(u1 < 1e-300) u1 = 1e-300;
-      vals[i] = exp(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
-      break;
-    }
-    case 2:
-      vals[i] = 2.0 * u01(&s_val) - 1.0;
-      break;
-    default:
-      vals[i] = u01(&s_val);
-      break;
+  uint64_t const vchunks = (nnz + GEN_CHUNK - 1) / GEN_CHUNK;
+  #pragma omp parallel for schedule(dynamic, 1)
+  for(uint64_t ch=0; ch < vchunks; ++ch) {
+    uint64_t sv = s_val ^ (0xD1B54A32D192ED03ull * (ch + 1));
+    uint64_t const hi = ((ch + 1) * GEN_CHUNK < nnz) ? (ch + 1) * GEN_CHUNK : nnz;
+    for(uint64_t i=ch * GEN_CHUNK; i < hi; ++i) {
+      switch(val_kind) {
+      case 1: {
+        double u1 = u01(&sv);
+        double const u2 = u01(&sv);
+        if(u1 < 1e-300) u1 = 1e-300;
+        vals[i] = exp(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+        break;
+      }
+      case 2:
+        vals[i] = 2.0 * u01(&sv) - 1.0;
+        break;
+      default:
+        vals[i] = u01(&sv);
+        break;
+      }
     }
   }
   return 0;

@@ -201,6 +201,11 @@ double splatt_amd_mttkrp_last_ms(void)
   return spd_mttkrp_last_kernel_ms();
 }
 
+int splatt_amd_mttkrp_kernel_times(double * ms, int max)
+{
+  return spd_mttkrp_kernel_times(ms, max);
+}
+
 void * splatt_amd_dev_alloc(size_t bytes)
 {
   void * p = NULL;

@@ -145,6 +145,8 @@ def lib():
                                          f64p, f64p]
     L.splatt_amd_mttkrp_timing.argtypes = [C.c_int]
     L.splatt_amd_mttkrp_last_ms.restype = C.c_double
+    L.splatt_amd_mttkrp_kernel_times.restype = C.c_int
+    L.splatt_amd_mttkrp_kernel_times.argtypes = [f64p, C.c_int]
     L.splatt_amd_dev_alloc.restype = C.c_void_p
     L.splatt_amd_dev_alloc.argtypes = [C.c_size_t]
     L.splatt_amd_dev_free.argtypes = [C.c_void_p]
