@@ -135,6 +135,10 @@ def main():
     ap.add_argument("--primary-only", action="store_true",
                     help="only the FP32 all-mode measurement (for profiling)")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a streaming kernel of known traffic (FP64->FP32 "
+                         "conversion of 2^26 values: 512 MiB read, 256 MiB written) "
+                         "so hardware byte counters can be scaled")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
@@ -143,6 +147,16 @@ def main():
 
     L = ps.lib()
     R = args.rank
+    if args.calibrate:
+        n = 1 << 26
+        src = ps.DevBuf(n * 8)
+        dst = ps.DevBuf(n * 4)
+        L.spd_f64_to_f32.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.spd_f64_to_f32(src.ptr, dst.ptr, n, None)
+        L.splatt_amd_sync()
+        src.free()
+        dst.free()
+        log("calibration kernel done (k_f64_to_f32, %d values)" % n)
     t0 = time.perf_counter()
     inds, vals, dims = ps.synth_coord([200_000, 150_000, 50_000], args.nnz,
                                       zipf=[1.2, 1.2, 1.2], vals="lognormal",

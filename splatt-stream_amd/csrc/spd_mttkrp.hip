@@ -509,11 +509,13 @@ k_root3(Root3Args<T> a)
   constexpr int CW = L * V;
   constexpr int NE = R3_WAVES * G * 2;        // combine entries per block
   constexpr int PER = R3_WN / 64;             // staged nonzeros per lane
-  __shared__ T s_val[R3_WAVES][R3_WN];
-  __shared__ uint32_t s_leaf[R3_WAVES][R3_WN];
-  __shared__ uint32_t s_arow[R3_WAVES][R3_WN + 1];   // A row of local fiber k
-  __shared__ uint16_t s_fi[R3_WAVES][R3_WN];     // local fiber of each nonzero
-  __shared__ uint16_t s_si[R3_WAVES][R3_WN];     // local slice of each nonzero
+  // per nonzero: byte offset of its B row, byte offset of its fiber's A row,
+  // local slice index, and (FP32) the value's bits; FP64 values in s_v64
+  __shared__ uint4 s_rec[R3_WAVES][R3_WN];
+  __shared__ T s_v64[sizeof(T) == 8 ? R3_WAVES : 1][sizeof(T) == 8 ? R3_WN : 1];
+  __shared__ uint32_t s_arow[R3_WAVES][R3_WN + 1];   // A row offset of local fiber k
+  __shared__ uint16_t s_fi[R3_WAVES][R3_WN];     // fiber-start marks
+  __shared__ uint16_t s_si[R3_WAVES][R3_WN];     // slice-start marks
   __shared__ uint32_t s_srow[R3_WAVES][R3_WN + 1];   // output row of slice S0 + k
   __shared__ uint32_t s_crow[NE];
   __shared__ __align__(16) T s_cval[NE][CW];
@@ -525,8 +527,11 @@ k_root3(Root3Args<T> a)
   uint32_t const c = lane % L;
   uint32_t const g = lane / L;
   uint32_t const R = a.R;
+  uint32_t const rowb = R * (uint32_t) sizeof(T);
   uint32_t const col = blockIdx.y * CW + c * V;
   bool const cok = col < R;
+  // lanes past the last column read column 0 (in bounds, never stored)
+  uint32_t const colb = cok ? col * (uint32_t) sizeof(T) : 0u;
 
   uint64_t const wid = (uint64_t) blockIdx.x * R3_WAVES + wave;
   uint64_t const W0 = wid * R3_WN;
@@ -536,8 +541,8 @@ k_root3(Root3Args<T> a)
 
   // ---- stage this wave's range ----
   // Each lane owns PER consecutive positions p = lane*PER + q. Fiber and
-  // slice starts inside the range are marked (s_fi/s_si temporarily hold
-  // the marks) and turned into local indices with a wave prefix sum.
+  // slice starts inside the range are marked and turned into local indices
+  // with a wave prefix sum; each position then gets its record.
   uint32_t F0 = 0, S0 = 0;
   if(wn > 0) {
     S0 = a.plan[2ull * item0 + 0];
@@ -548,15 +553,11 @@ k_root3(Root3Args<T> a)
       s_fi[wave][k] = 0;
       s_si[wave][k] = 0;
     }
-    for(uint32_t k=lane; k < wn; k += 64) {
-      s_val[wave][k] = a.vals[W0 + k];
-      s_leaf[wave][k] = a.leaf[W0 + k];
-    }
     for(uint32_t k=lane; k < nsw; k += 64) {
       s_srow[wave][k] = a.rows_are_positions ? S0 + k : a.fids0[S0 + k];
     }
     for(uint32_t k=lane; k < nfw; k += 64) {
-      s_arow[wave][k] = a.fids1[F0 + k];
+      s_arow[wave][k] = a.fids1[F0 + k] * rowb;
     }
     __builtin_amdgcn_wave_barrier();
     // fiber k (local, >= 1) starts at fptr1[F0+k]; slice q (local, >= 1)
@@ -590,96 +591,109 @@ k_root3(Root3Args<T> a)
       fpre += fm[q];
       spre += sm[q];
       uint32_t const p = lane * PER + q;
-      s_fi[wave][p] = (uint16_t) fpre;
-      s_si[wave][p] = (uint16_t) spre;
+      uint32_t leaf = 0;
+      T v = (T) 0;
+      if(p < wn) {
+        leaf = a.leaf[W0 + p];
+        v = a.vals[W0 + p];
+      }
+      uint4 r;
+      r.x = leaf * rowb;
+      r.y = s_arow[wave][fpre];
+      r.z = spre;
+      if constexpr (sizeof(T) == 4) {
+        r.w = __float_as_uint(v);
+      } else {
+        r.w = 0;
+        s_v64[wave][p] = v;
+      }
+      s_rec[wave][p] = r;
     }
   }
   for(uint32_t e=threadIdx.x; e < NE; e += R3_WAVES * 64) s_crow[e] = CARRY_NONE;
   __syncthreads();
 
   // ---- walk this group's item ----
+  // Every nonzero contributes v * B[leaf] * A[fiber row] to its slice's
+  // accumulator; a change of local slice flushes the finished slice.
   uint32_t const item = item0 + g;
   uint64_t const n0 = (uint64_t) item * S;
   uint32_t const e0 = (wave * G + g) * 2;
+  char const * const Bb = (char const *) a.B;
+  char const * const Ab = (char const *) a.A;
   if(n0 < a.nnz) {
     uint64_t const n1 = (n0 + S < a.nnz) ? n0 + S : a.nnz;
     bool head_partial = a.head[item] != 0;
     uint32_t const k0 = (uint32_t) (n0 - W0);
     uint32_t const k1 = (uint32_t) (n1 - W0);
-    uint32_t cur_s = s_si[wave][k0];
-    uint32_t prev_f = 0xFFFFFFFFu;
-    VT aprev = vzero<VT>();
+    uint32_t cur_s = s_rec[wave][k0].z;
     VT accs = vzero<VT>();
-    bool stouch = false;
+    auto rec_val = [&](uint4 const & r, uint32_t k) -> T {
+      if constexpr (sizeof(T) == 4) {
+        return __uint_as_float(r.w);
+      } else {
+        return s_v64[wave][k];
+      }
+    };
+    auto step = [&](uint4 const & r, T v, VT b, VT av) {
+      if(r.z != cur_s) {
+        // slice cur_s ended before this nonzero
+        uint32_t const orow = s_srow[wave][cur_s];
+        if(head_partial) {
+          s_crow[e0] = orow;
+          if(cok) *(VT *) &s_cval[e0][c * V] = accs;
+        } else {
+          vstore<T, V>(a.out, orow, R, col, cok, accs);
+        }
+        head_partial = false;
+        accs = vzero<VT>();
+        cur_s = r.z;
+      }
+      accs += v * (b * av);
+    };
     for(uint32_t k=k0; k < k1; k += R3_BATCH) {
-      T v[R3_BATCH];
-      VT b[R3_BATCH];
-      VT ar[R3_BATCH];
-      uint32_t fi[R3_BATCH];
-      bool nf[R3_BATCH];
-      #pragma unroll
-      for(int j=0; j < R3_BATCH; ++j) {
-        bool const ok = k + j < k1;
-        uint32_t const kk = ok ? k + j : k0;
-        v[j] = ok ? s_val[wave][kk] : (T) 0;
-        fi[j] = s_fi[wave][kk];
-        b[j] = vload<T, V>(a.B, s_leaf[wave][kk], R, col, cok && ok);
-      }
-      #pragma unroll
-      for(int j=0; j < R3_BATCH; ++j) {
-        uint32_t const pf = (j == 0) ? prev_f : fi[j > 0 ? j - 1 : 0];
-        nf[j] = (k + j < k1) && (fi[j] != pf);
-        ar[j] = vload<T, V>(a.A, s_arow[wave][nf[j] ? fi[j] : 0], R, col, cok && nf[j]);
-      }
-      #pragma unroll
-      for(int j=0; j < R3_BATCH; ++j) {
-        if(k + j < k1) {
-          VT const aj = nf[j] ? ar[j] : aprev;
-          aprev = aj;
-          uint32_t const sj = s_si[wave][k + j];
-          if(sj != cur_s) {
-            // slice cur_s ended before this nonzero
-            if(stouch) {
-              uint32_t const orow = s_srow[wave][cur_s];
-              if(head_partial) {
-                s_crow[e0] = orow;
-                if(cok) *(VT *) &s_cval[e0][c * V] = accs;
-              } else {
-                vstore<T, V>(a.out, orow, R, col, cok, accs);
-              }
-            }
-            head_partial = false;
-            accs = vzero<VT>();
-            cur_s = sj;
-          }
-          accs += v[j] * (b[j] * aj);
-          stouch = true;
+      if(k + R3_BATCH <= k1) {
+        uint4 r[R3_BATCH];
+        VT b[R3_BATCH];
+        VT av[R3_BATCH];
+        #pragma unroll
+        for(int j=0; j < R3_BATCH; ++j) r[j] = s_rec[wave][k + j];
+        #pragma unroll
+        for(int j=0; j < R3_BATCH; ++j) {
+          b[j] = *(VT const *) (Bb + (r[j].x + colb));
+          av[j] = *(VT const *) (Ab + (r[j].y + colb));
+        }
+        #pragma unroll
+        for(int j=0; j < R3_BATCH; ++j) step(r[j], rec_val(r[j], k + j), b[j], av[j]);
+      } else {
+        for(uint32_t kk=k; kk < k1; ++kk) {
+          uint4 const r = s_rec[wave][kk];
+          VT const b = *(VT const *) (Bb + (r.x + colb));
+          VT const av = *(VT const *) (Ab + (r.y + colb));
+          step(r, rec_val(r, kk), b, av);
         }
       }
-      prev_f = fi[R3_BATCH - 1];
     }
-    if(stouch) {
-      // is the last slice complete at n1? It is if n1 is the start of a new
-      // slice (or the tensor's end)
-      bool complete = (n1 == a.nnz);
-      if(!complete && n1 < W1) {
-        complete = s_si[wave][k1] != cur_s;
-      } else if(!complete) {
-        // the item ends at the wave's end: the slice is complete iff the
-        // next slice starts exactly there
-        uint32_t const fs = a.fptr0[S0 + cur_s + 1];
-        complete = (fs < a.nfibers) && ((uint64_t) a.fptr1[fs] == n1);
-      }
-      uint32_t const orow = s_srow[wave][cur_s];
-      if(complete && !head_partial) {
-        vstore<T, V>(a.out, orow, R, col, cok, accs);
-      } else if(complete) {
-        s_crow[e0] = orow;
-        if(cok) *(VT *) &s_cval[e0][c * V] = accs;
-      } else {
-        s_crow[e0 + 1] = orow;
-        if(cok) *(VT *) &s_cval[e0 + 1][c * V] = accs;
-      }
+    // is the last slice complete at n1? It is if n1 is the start of a new
+    // slice (or the tensor's end)
+    bool complete = (n1 == a.nnz);
+    if(!complete && n1 < W1) {
+      complete = s_rec[wave][k1].z != cur_s;
+    } else if(!complete) {
+      // the item ends at the wave's end: the slice is complete iff the
+      // next slice starts exactly there
+      uint32_t const fs = a.fptr0[S0 + cur_s + 1];
+      complete = (fs < a.nfibers) && ((uint64_t) a.fptr1[fs] == n1);
+    }
+    uint32_t const orow = s_srow[wave][cur_s];
+    if(complete && !head_partial) {
+      vstore<T, V>(a.out, orow, R, col, cok, accs);
+    } else if(complete) {
+      s_crow[e0] = orow;
+      if(cok) *(VT *) &s_cval[e0][c * V] = accs;
+    } else {
+      s_crow[e0 + 1] = orow;
+      if(cok) *(VT *) &s_cval[e0 + 1][c * V] = accs;
     }
   }
   // ---- combine partial slices: per wave, then per block ----
@@ -1195,7 +1209,10 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
     spd_set_error(SPD_EINPUT, "mode %d not in tensor", mode);
     return SPD_EINPUT;
   }
-  if(N == 3 && depth == 0) {
+  // the 3-mode root kernel addresses factor rows with 32-bit byte offsets
+  uint64_t const lim = 0xFFFFFFFFull;
+  if(N == 3 && depth == 0 && c->dims[c->perm[1]] * R * sizeof(T) <= lim &&
+      c->dims[c->perm[2]] * R * sizeof(T) <= lim) {
     return run_root3<T>(c, R, mats, out, rows_pos, st, dtype);
   }
   uint32_t const S = choose_S(c->nnz, (uint32_t) R, dtype);
