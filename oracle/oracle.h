@@ -97,6 +97,34 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
     uint64_t nmodes, uint64_t rank, double ** factors, double * lambda,
     uint64_t max_its, double tol, double * fits, int nthreads);
 
+/* A constraint as the CPD restatement runs it: kind 0 = unconstrained
+ * closed form, 1 = closed form with diag_add on the Gram diagonal
+ * (src/cpd/constraints/frob.c:25-36), 2 = ADMM (src/cpd/admm.c:346-406,
+ * one chunk) with prox 1 = max-col-norm (constraints/maxcolnorm.c:32-61),
+ * 2 = non-negative max-col-norm (maxcolnorm-nonneg.c:32-66),
+ * 3 = non-negative (nonneg.c), 0 = none. */
+typedef struct { int kind; int prox; double diag_add; } or_con;
+
+/* or_cpd_als with per-mode constraints; `unconstrained` is the reference's
+ * ws->unconstrained (every constraint UNCONSTRAINED: normalise columns into
+ * lambda). max_inner/inner_tol drive the ADMM modes. */
+int or_cpd_als_con(or_csf * const * csfs, uint64_t const * mode_csf,
+    uint64_t nmodes, uint64_t rank, double ** factors, double * lambda,
+    uint64_t max_its, double tol, uint64_t max_inner, double inner_tol,
+    or_con const * cons, int unconstrained, double * fits, int nthreads);
+
+/* ADMM for one factor (src/cpd/admm.c:240-308, :346-406 with one chunk):
+ * G (R x R, full, including any regularisation) is overwritten by its
+ * Cholesky factor after rho = trace(G)/R is added to the diagonal. P (the
+ * factor, I x R) and D (its dual) are updated in place. Returns the number
+ * of inner iterations; *rho_out receives rho when not NULL. */
+uint64_t or_admm(int prox, double const * M, double * P, double * D,
+    double * G, uint64_t I, uint64_t R, uint64_t max_inner, double inner_tol,
+    double * rho_out);
+
+/* The proximal operators (row-major I x R, in place). */
+void or_prox(int prox, double * P, uint64_t I, uint64_t R);
+
 /* Dense helpers exposed for pinning tests (src/matrix.c). */
 void or_mat_aTa(double const * A, uint64_t I, uint64_t R, double * out);
 int or_cholesky(double * G, uint64_t R);            /* in place, lower */

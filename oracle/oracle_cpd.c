@@ -114,9 +114,111 @@ void or_mat_normalize(double * A, uint64_t I, uint64_t R, double * lambda)
   }
 }
 
+/* src/cpd/constraints/maxcolnorm.c:32-61, maxcolnorm-nonneg.c:32-66,
+ * nonneg.c: column norms capped at 1 (after clamping negatives for the
+ * non-negative variant); plain clamping for nonneg. */
+void or_prox(int prox, double * P, uint64_t I, uint64_t R)
+{
+  if(prox == 3) {
+    for(uint64_t x=0; x < I * R; ++x) {
+      P[x] = (P[x] > 0.) ? P[x] : 0.;
+    }
+    return;
+  }
+  if(prox != 1 && prox != 2) {
+    return;
+  }
+  for(uint64_t j=0; j < R; ++j) {
+    double norm = 0.;
+    for(uint64_t i=0; i < I; ++i) {
+      double * p = P + j + i * R;
+      if(prox == 2 && *p < 0.) {
+        *p = 0.;
+      } else {
+        norm += (*p) * (*p);
+      }
+    }
+    norm = sqrt(norm);
+    if(norm <= 1.) {
+      continue;
+    }
+    for(uint64_t i=0; i < I; ++i) {
+      P[j + i * R] /= norm;
+    }
+  }
+}
+
+uint64_t or_admm(int prox, double const * M, double * P, double * D,
+    double * G, uint64_t I, uint64_t R, uint64_t max_inner, double inner_tol,
+    double * rho_out)
+{
+  /* src/cpd/admm.c:346-354 */
+  double tr = 0.;
+  for(uint64_t i=0; i < R; ++i) {
+    tr += G[i * R + i];
+  }
+  double const rho = tr / (double) R;
+  for(uint64_t i=0; i < R; ++i) {
+    G[i * R + i] += rho;
+  }
+  if(rho_out) {
+    *rho_out = rho;
+  }
+  or_cholesky(G, R);
+  double * aux = malloc(I * R * sizeof(double));
+  double * init = malloc(I * R * sizeof(double));
+  uint64_t it;
+  for(it=0; it < max_inner; ++it) {
+    memcpy(init, P, I * R * sizeof(double));
+    for(uint64_t x=0; x < I * R; ++x) {
+      aux[x] = M[x] + rho * (P[x] + D[x]);
+    }
+    or_cholesky_solve(G, R, aux, I);
+    for(uint64_t x=0; x < I * R; ++x) {
+      P[x] = aux[x] - D[x];
+    }
+    or_prox(prox, P, I, R);
+    double dual_norm = 0.;
+    for(uint64_t x=0; x < I * R; ++x) {
+      D[x] += P[x] - aux[x];
+      dual_norm += D[x] * D[x];
+    }
+    double p_norm = 0., p_res = 0., d_res = 0.;
+    for(uint64_t x=0; x < I * R; ++x) {
+      double const pd = P[x] - aux[x];
+      double const dd = P[x] - init[x];
+      p_norm += P[x] * P[x];
+      p_res += pd * pd;
+      d_res += dd * dd;
+    }
+    if(p_res <= inner_tol * p_norm && d_res <= inner_tol * dual_norm) {
+      ++it;
+      break;
+    }
+  }
+  free(aux);
+  free(init);
+  return it;
+}
+
 int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
     uint64_t nmodes, uint64_t rank, double ** factors, double * lambda,
     uint64_t max_its, double tol, double * fits, int nthreads)
+{
+  or_con cons[OR_MAX_NMODES];
+  for(uint64_t m=0; m < OR_MAX_NMODES; ++m) {
+    cons[m].kind = 0;
+    cons[m].prox = 0;
+    cons[m].diag_add = 0.;
+  }
+  return or_cpd_als_con(csfs, mode_csf, nmodes, rank, factors, lambda, max_its,
+      tol, 50, 1e-2, cons, 1, fits, nthreads);
+}
+
+int or_cpd_als_con(or_csf * const * csfs, uint64_t const * mode_csf,
+    uint64_t nmodes, uint64_t rank, double ** factors, double * lambda,
+    uint64_t max_its, double tol, uint64_t max_inner, double inner_tol,
+    or_con const * cons, int unconstrained, double * fits, int nthreads)
 {
   uint64_t const R = rank;
   or_csf const * c0 = csfs[0];
@@ -125,8 +227,10 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
     if(c0->dims[m] > maxdim) maxdim = c0->dims[m];
   }
   double * aTa[OR_MAX_NMODES];
+  double * duals[OR_MAX_NMODES];
   for(uint64_t m=0; m < nmodes; ++m) {
     aTa[m] = calloc(R * R, sizeof(double));
+    duals[m] = (cons[m].kind == 2) ? calloc(c0->dims[m] * R, sizeof(double)) : NULL;
   }
   double * gram = malloc(R * R * sizeof(double));
   double * mttkrp = malloc(maxdim * R * sizeof(double));
@@ -149,7 +253,8 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
     for(uint64_t m=0; m < nmodes; ++m) {
       uint64_t const I = c0->dims[m];
       or_mttkrp_csf(csfs[mode_csf[m]], factors, R, m, mttkrp, nthreads);
-      /* src/matrix.c:311-345 then src/cpd/admm.c:217-235 */
+      /* src/matrix.c:311-345, the constraint's gram_func, then
+       * src/cpd/admm.c:217-235 (closed form) or :346-406 (ADMM) */
       for(uint64_t x=0; x < R * R; ++x) {
         gram[x] = 1.;
       }
@@ -159,10 +264,20 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
           gram[x] *= aTa[mm][x];
         }
       }
-      or_cholesky(gram, R);
-      memcpy(factors[m], mttkrp, I * R * sizeof(double));
-      or_cholesky_solve(gram, R, factors[m], I);
-      or_mat_normalize(factors[m], I, R, lambda);
+      for(uint64_t i=0; i < R; ++i) {
+        gram[i * R + i] += cons[m].diag_add;
+      }
+      if(cons[m].kind == 2) {
+        or_admm(cons[m].prox, mttkrp, factors[m], duals[m], gram, I, R,
+            max_inner, inner_tol, NULL);
+      } else {
+        or_cholesky(gram, R);
+        memcpy(factors[m], mttkrp, I * R * sizeof(double));
+        or_cholesky_solve(gram, R, factors[m], I);
+        if(unconstrained) {
+          or_mat_normalize(factors[m], I, R, lambda);
+        }
+      }
       or_mat_aTa(factors[m], I, R, aTa[m]);
     }
     /* src/cpd/cpd.c:453-487 */
@@ -196,15 +311,18 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
     }
     olderr = err;
   }
-  /* src/cpd/cpd.c:313-332 */
-  for(uint64_t m=0; m < nmodes; ++m) {
-    or_mat_normalize(factors[m], c0->dims[m], R, tmp);
-    for(uint64_t r=0; r < R; ++r) {
-      lambda[r] *= tmp[r];
+  /* src/cpd/cpd.c:296-301 and :313-332 */
+  if(unconstrained) {
+    for(uint64_t m=0; m < nmodes; ++m) {
+      or_mat_normalize(factors[m], c0->dims[m], R, tmp);
+      for(uint64_t r=0; r < R; ++r) {
+        lambda[r] *= tmp[r];
+      }
     }
   }
   for(uint64_t m=0; m < nmodes; ++m) {
     free(aTa[m]);
+    free(duals[m]);
   }
   free(gram);
   free(mttkrp);

@@ -225,3 +225,55 @@ def cpd_als(csfs, mode_csf, factors, max_its, tol, nthreads=1):
     its = lib().or_cpd_als(cptrs, _p(mc, C.c_uint64), N, R, fptrs, _p(lam, C.c_double),
                            max_its, tol, _p(fits, C.c_double), nthreads)
     return facs, lam, fits[:its]
+
+
+class _OrCon(C.Structure):
+    _fields_ = [("kind", C.c_int), ("prox", C.c_int), ("diag_add", C.c_double)]
+
+
+def cpd_als_con(csfs, mode_csf, factors, max_its, tol, cons, unconstrained,
+                max_inner=50, inner_tol=1e-2, nthreads=1):
+    """or_cpd_als_con on copies of `factors`. cons: list of (kind, prox,
+    diag_add) per mode. Returns (factors, lambda, fits)."""
+    N = len(factors)
+    R = factors[0].shape[1]
+    facs = [np.ascontiguousarray(f, dtype=np.float64).copy() for f in factors]
+    lam = np.zeros(R, dtype=np.float64)
+    fits = np.zeros(max_its, dtype=np.float64)
+    cptrs = (C.POINTER(OrCsf) * len(csfs))(*[c.ptr for c in csfs])
+    mc = np.asarray(mode_csf, dtype=np.uint64)
+    fptrs = (f64p * N)(*[_p(f, C.c_double) for f in facs])
+    carr = (_OrCon * 8)()
+    for m, (k, p, d) in enumerate(cons):
+        carr[m].kind, carr[m].prox, carr[m].diag_add = k, p, d
+    L = lib()
+    L.or_cpd_als_con.restype = C.c_int
+    its = L.or_cpd_als_con(cptrs, _p(mc, C.c_uint64), C.c_uint64(N), C.c_uint64(R),
+                           fptrs, _p(lam, C.c_double), C.c_uint64(max_its),
+                           C.c_double(tol), C.c_uint64(max_inner), C.c_double(inner_tol),
+                           carr, C.c_int(1 if unconstrained else 0),
+                           _p(fits, C.c_double), C.c_int(nthreads))
+    return facs, lam, fits[:its]
+
+
+def admm(prox, M, P, D, G, max_inner, inner_tol):
+    """or_admm on copies; returns (P, D, iterations, rho)."""
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    P = np.ascontiguousarray(P, dtype=np.float64).copy()
+    D = np.ascontiguousarray(D, dtype=np.float64).copy()
+    G = np.ascontiguousarray(G, dtype=np.float64).copy()
+    I, R = P.shape
+    rho = C.c_double(0.)
+    L = lib()
+    L.or_admm.restype = C.c_uint64
+    its = L.or_admm(C.c_int(prox), _p(M, C.c_double), _p(P, C.c_double),
+                    _p(D, C.c_double), _p(G, C.c_double), C.c_uint64(I), C.c_uint64(R),
+                    C.c_uint64(max_inner), C.c_double(inner_tol), C.byref(rho))
+    return P, D, int(its), rho.value
+
+
+def prox(kind, P):
+    P = np.ascontiguousarray(P, dtype=np.float64).copy()
+    I, R = P.shape
+    lib().or_prox(C.c_int(kind), _p(P, C.c_double), C.c_uint64(I), C.c_uint64(R))
+    return P

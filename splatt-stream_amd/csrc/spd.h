@@ -109,6 +109,68 @@ int spd_mttkrp_kernel_times(double * ms, int max);
 int spd_f64_to_f32(double const * src, float * dst, uint64_t n, void * stream);
 int spd_f32_to_f64(float const * src, double * dst, uint64_t n, void * stream);
 
+/* Dense steps on device arrays (library stream, asynchronous), exported
+ * for the tests:
+ *   gram:     out (R x R) = X^T X, after X[:,r] /= div[r] when div != NULL;
+ *             X32 (optional) receives an FP32 copy of X.
+ *   chol_inv: G = Hadamard of aTa[m], m != mode, + diag_add (+ rho =
+ *             trace/R when admm); ginv (RP x RP, RP = R rounded up to 16)
+ *             = G^{-1} from its Cholesky factor; *info = failing pivot.
+ *   apply:    X = M ginv (M FP64 or FP32 by mdtype); colnorm[r] = column
+ *             2-norms of X when colnorm != NULL.
+ *   admm_step: one ADMM inner iteration; sums needs 4 + R doubles:
+ *             ||P||^2, ||P-aux||^2, ||P-init||^2, ||D||^2, then the column
+ *             norms used by the prox. */
+int spd_dense_gram(double * X, uint64_t I, uint64_t R, double const * div,
+    float * X32, double * out);
+int spd_dense_chol_inv(double const * const * aTa, int nmodes, int mode,
+    uint64_t R, double diag_add, int admm, double * ginv, double * rho, int * info);
+int spd_dense_apply(void const * M, int mdtype, double const * ginv,
+    uint64_t I, uint64_t R, double * X, double * colnorm);
+int spd_dense_admm_step(void const * M, int mdtype, double const * ginv,
+    double const * rho, uint64_t I, uint64_t R, double * P, double * aux, double * D,
+    double * init, int prox, double * sums);
+
+/* ---- CPD-ALS (src/cpd/cpd.c:161-332, src/cpd/admm.c:316-406) ---------- */
+
+#define SPD_CON_NONE   0   /* unconstrained: closed form, columns normalised */
+#define SPD_CON_CLOSED 1   /* closed form, Gram diagonal += diag_add */
+#define SPD_CON_ADMM   2   /* ADMM with a proximal operator */
+
+#define SPD_PROX_NONE 0
+#define SPD_PROX_MAXCOLNORM 1
+#define SPD_PROX_MAXCOLNORM_NONNEG 2
+#define SPD_PROX_NONNEG 3
+
+typedef struct {
+  int kind;          /* SPD_CON_* */
+  int prox;          /* SPD_PROX_* (ADMM) */
+  double diag_add;   /* Frobenius regularisation added to the Gram diagonal */
+} spd_con;
+
+typedef struct {
+  int nmodes;
+  uint64_t rank;
+  uint64_t max_its;
+  double tol;
+  uint64_t max_inner;
+  double inner_tol;
+  int dtype;           /* MTTKRP precision, SPD_F64 / SPD_F32 */
+  int verbosity;       /* > 0: one line per iteration */
+  int unconstrained;   /* every mode unconstrained: normalise, absorb into lambda */
+  spd_con con[SPD_MAX_MODES];
+} spd_cpd_cfg;
+
+/* ALS sweeps on the GPU. csfs[k] are device CSFs, mode_csf[m] the one used
+ * for mode m, rows_pos[k] the root-row convention of CSF k. factors[m]
+ * (host, dims[m] x rank, FP64) are the initial factors on entry and the
+ * result on return; lambda (host, rank) is the result. fits (host, max_its
+ * entries, may be NULL) receives the fit of every iteration; *its the number
+ * of iterations and *fit the final fit. */
+int spd_cpd_als(spd_csf * const * csfs, int const * mode_csf,
+    int const * rows_pos, spd_cpd_cfg const * cfg, double * const * factors,
+    double * lambda, double * fits, uint64_t * its, double * fit);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,904 @@
+// spd_dense.hip -- dense FP64 factor algebra for CPD on gfx950.
+//
+// The reference does these steps with BLAS/LAPACK (src/matrix.c: syrk for
+// A^T A, potrf + potrs for the normal equations) and OpenMP loops (column
+// norms, ADMM updates, fit). Here:
+//   - Grams A^T A run on MFMA (v_mfma_f64_16x16x4f64): a wave holds the
+//     16x16 output tiles of the upper triangle in registers and streams
+//     4-row slices of A through them; blocks write partial Grams that a
+//     second kernel sums in a fixed order (results are deterministic).
+//   - The R x R normal-equation matrix is formed, Cholesky-factored and
+//     inverted in one single-wave kernel (R^3/3 flops, latency-bound).
+//   - The solve of every row against it (potrs in the reference) becomes a
+//     product with the inverse on MFMA, with the rows of the right-hand
+//     side staged through LDS; the column sums of squares needed for
+//     normalisation or the max-column-norm prox are fused into it.
+//   - Column reductions use fixed-order block partials, so repeated runs
+//     give bit-identical results.
+// Ranks up to 64 take the MFMA kernels; larger ranks take plain kernels.
+#include "spd_dense.hpp"
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// D(16x16) += A(16x4) B(4x16). Lane l holds A[l%16][l/16] and B[l/16][l%16];
+// D comes back as D[l/16 + 4*r][l%16] in component r.
+__device__ __forceinline__ d4 mfma16(double a, double b, d4 c)
+{
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+#define DN_THREADS 256
+#define DN_MAX_BLOCKS 1024
+#define GR_U 4                 // 4-row MFMA steps per wave batch
+
+static uint32_t dn_blocks(uint64_t work_items, uint64_t per_block)
+{
+  uint64_t nb = spd_cdiv(work_items, per_block);
+  if(nb < 1) nb = 1;
+  if(nb > DN_MAX_BLOCKS) nb = DN_MAX_BLOCKS;
+  return (uint32_t) nb;
+}
+
+template<typename T> __device__ __forceinline__ double ld_as_f64(T const * p, uint64_t i)
+{
+  return (double) p[i];
+}
+
+// ---- Gram on MFMA (R <= 64) -------------------------------------------------
+
+template<int TT, bool DIV>
+__global__ void __launch_bounds__(DN_THREADS)
+k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
+    double const * __restrict__ div, float * __restrict__ X32,
+    double * __restrict__ part)
+{
+  constexpr int RP = TT * 16;
+  __shared__ double s_acc[RP * RP];
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const kk = lane >> 4;
+  int const cc = lane & 15;
+  for(int e=threadIdx.x; e < RP * RP; e += DN_THREADS) s_acc[e] = 0.;
+
+  d4 acc[TT][TT];
+  #pragma unroll
+  for(int i=0; i < TT; ++i) {
+    #pragma unroll
+    for(int j=0; j < TT; ++j) acc[i][j] = (d4) 0.;
+  }
+  bool cok[TT];
+  double dv[TT];
+  #pragma unroll
+  for(int t=0; t < TT; ++t) {
+    uint32_t const col = t * 16 + cc;
+    cok[t] = col < R;
+    dv[t] = (DIV && cok[t]) ? div[col] : 1.;
+  }
+  uint64_t const nsteps = (I + 3) / 4;
+  uint64_t const stride = (uint64_t) gridDim.x * 4 * GR_U;
+  for(uint64_t s0 = ((uint64_t) blockIdx.x * 4 + wave) * GR_U; s0 < nsteps; s0 += stride) {
+    double a[GR_U][TT];
+    #pragma unroll
+    for(int u=0; u < GR_U; ++u) {
+      uint64_t const row = (s0 + u) * 4 + kk;
+      #pragma unroll
+      for(int t=0; t < TT; ++t) {
+        bool const ok = (row < I) && cok[t];
+        a[u][t] = ok ? X[row * R + t * 16 + cc] : 0.;
+      }
+    }
+    if(DIV || X32 != nullptr) {
+      #pragma unroll
+      for(int u=0; u < GR_U; ++u) {
+        uint64_t const row = (s0 + u) * 4 + kk;
+        #pragma unroll
+        for(int t=0; t < TT; ++t) {
+          if((row < I) && cok[t]) {
+            uint64_t const idx = row * R + t * 16 + cc;
+            double v = a[u][t];
+            if(DIV) {
+              v = v / dv[t];
+              X[idx] = v;
+              a[u][t] = v;
+            }
+            if(X32 != nullptr) X32[idx] = (float) v;
+          }
+        }
+      }
+    }
+    #pragma unroll
+    for(int u=0; u < GR_U; ++u) {
+      #pragma unroll
+      for(int i=0; i < TT; ++i) {
+        #pragma unroll
+        for(int j=i; j < TT; ++j) acc[i][j] = mfma16(a[u][i], a[u][j], acc[i][j]);
+      }
+    }
+  }
+  // waves add their tiles in a fixed order
+  __syncthreads();
+  for(int w=0; w < 4; ++w) {
+    if(wave == w) {
+      #pragma unroll
+      for(int i=0; i < TT; ++i) {
+        #pragma unroll
+        for(int j=i; j < TT; ++j) {
+          #pragma unroll
+          for(int r=0; r < 4; ++r) {
+            s_acc[(i * 16 + kk + 4 * r) * RP + j * 16 + cc] += acc[i][j][r];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double * const out = part + (uint64_t) blockIdx.x * RP * RP;
+  for(int e=threadIdx.x; e < RP * RP; e += DN_THREADS) out[e] = s_acc[e];
+}
+
+// Sum of the block partials (upper triangle), mirrored into the full R x R.
+__global__ void k_gram_reduce(double const * __restrict__ part, uint32_t nb,
+    uint32_t R, uint32_t ld, double * __restrict__ out)
+{
+  uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= R * R) return;
+  uint32_t const i = e / R;
+  uint32_t const j = e - i * R;
+  uint32_t const a = i < j ? i : j;
+  uint32_t const b = i < j ? j : i;
+  double s = 0.;
+  for(uint32_t k=0; k < nb; ++k) s += part[(uint64_t) k * ld * ld + a * ld + b];
+  out[e] = s;
+}
+
+// ---- plain kernels for any rank -----------------------------------------------
+
+// Gram block partials for ranks above 64: entry (i <= j) of the block's rows.
+__global__ void __launch_bounds__(DN_THREADS)
+k_gram_plain(double const * __restrict__ X, uint64_t I, uint32_t R,
+    uint64_t rows_per_block, double * __restrict__ part)
+{
+  uint64_t const r0 = (uint64_t) blockIdx.x * rows_per_block;
+  uint64_t const r1 = (r0 + rows_per_block < I) ? r0 + rows_per_block : I;
+  double * const out = part + (uint64_t) blockIdx.x * R * R;
+  for(uint32_t e=threadIdx.x; e < R * R; e += DN_THREADS) {
+    uint32_t const i = e / R;
+    uint32_t const j = e - i * R;
+    if(i > j) continue;
+    double s = 0.;
+    for(uint64_t r=r0; r < r1; ++r) s += X[r * R + i] * X[r * R + j];
+    out[e] = s;
+  }
+}
+
+// X[:,r] /= div[r], FP32 copy.
+__global__ void k_div_cols(double * __restrict__ X, uint64_t n, uint32_t R,
+    double const * __restrict__ div, float * __restrict__ X32)
+{
+  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  uint32_t c = (uint32_t) (e % R);
+  uint32_t const sc = (uint32_t) (stride % R);
+  for(; e < n; e += stride) {
+    double v = X[e];
+    if(div != nullptr) {
+      v = v / div[c];
+      X[e] = v;
+    }
+    if(X32 != nullptr) X32[e] = (float) v;
+    c += sc;
+    if(c >= R) c -= R;
+  }
+}
+
+// Per block and column: sum over the block's rows of f(X[i,c]) * Y[i,c]
+// (Y == nullptr: f(X)^2, f = max(x,0) when CLAMP). Block partials in
+// part[b * R + c].
+template<typename TY, bool CLAMP>
+__global__ void __launch_bounds__(DN_THREADS)
+k_coldot(double const * __restrict__ X, TY const * __restrict__ Y, uint64_t I,
+    uint32_t R, uint64_t rows_per_block, double * __restrict__ part)
+{
+  __shared__ double s[DN_THREADS];
+  uint64_t const r0 = (uint64_t) blockIdx.x * rows_per_block;
+  uint64_t const r1 = (r0 + rows_per_block < I) ? r0 + rows_per_block : I;
+  int const t = threadIdx.x;
+  if(R <= DN_THREADS) {
+    uint32_t const nq = DN_THREADS / R;
+    uint32_t const c = t % R;
+    uint32_t const q = t / R;
+    double acc = 0.;
+    if(q < nq) {
+      for(uint64_t i=r0 + q; i < r1; i += nq) {
+        double x = X[i * R + c];
+        if(CLAMP) x = (x < 0.) ? 0. : x;
+        double const y = (Y == nullptr) ? x : (double) Y[i * R + c];
+        acc += x * y;
+      }
+    }
+    s[t] = acc;
+    __syncthreads();
+    if(t < (int) R) {
+      double tot = 0.;
+      for(uint32_t k=0; k < nq; ++k) tot += s[t + k * R];
+      part[(uint64_t) blockIdx.x * R + t] = tot;
+    }
+  } else {
+    for(uint32_t c=t; c < R; c += DN_THREADS) {
+      double acc = 0.;
+      for(uint64_t i=r0; i < r1; ++i) {
+        double x = X[i * R + c];
+        if(CLAMP) x = (x < 0.) ? 0. : x;
+        double const y = (Y == nullptr) ? x : (double) Y[i * R + c];
+        acc += x * y;
+      }
+      part[(uint64_t) blockIdx.x * R + c] = acc;
+    }
+  }
+}
+
+// Column totals of block partials (row stride ld): mode 0 writes
+// out[c] = sqrt(total_c); mode 1 writes out[0] = sum_c total_c * lambda[c].
+__global__ void __launch_bounds__(DN_THREADS)
+k_colreduce(double const * __restrict__ part, uint32_t nb, uint32_t R, uint32_t ld,
+    int mode, double const * __restrict__ lambda, double * __restrict__ out)
+{
+  __shared__ double s[DN_THREADS];
+  double mine = 0.;
+  for(uint32_t c=threadIdx.x; c < R; c += DN_THREADS) {
+    double v = 0.;
+    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * ld + c];
+    if(mode == 0) {
+      out[c] = sqrt(v);
+    } else {
+      // fixed order: thread t sums its columns c = t, t+256, ...
+      mine += v * lambda[c];
+    }
+  }
+  if(mode == 1) {
+    s[threadIdx.x] = mine;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+      double tot = 0.;
+      for(int k=0; k < DN_THREADS; ++k) tot += s[k];
+      out[0] = tot;
+    }
+  }
+}
+
+// ---- normal equations: form, Cholesky, inverse -----------------------------------
+
+struct GramSrc { double const * aTa[SPD_MAX_MODES]; };
+
+// One wave. W is 2 R^2 doubles (LDS when it fits, else global scratch).
+__global__ void __launch_bounds__(64)
+k_chol_inv(GramSrc g, int nmodes, int mode, uint32_t R, uint32_t RP,
+    double diag_add, int admm, double * gW, double * __restrict__ ginv,
+    double * __restrict__ rho_out, int * __restrict__ info)
+{
+  extern __shared__ double dyn[];
+  double * const A = gW ? gW : dyn;       // R x R working matrix, then L
+  double * const Li = A + (uint64_t) R * R;   // R x R, L^{-1}
+  int const t = threadIdx.x;
+
+  for(uint32_t e=t; e < R * R; e += 64) {
+    uint32_t const i = e / R;
+    uint32_t const j = e - i * R;
+    uint32_t const a = i < j ? i : j;
+    uint32_t const b = i < j ? j : i;
+    double h = 1.;
+    for(int m=0; m < nmodes; ++m) {
+      if(m != mode) h *= g.aTa[m][a * R + b];
+    }
+    A[e] = h;
+    Li[e] = 0.;
+  }
+  __syncthreads();
+  if(t == 0) {
+    for(uint32_t i=0; i < R; ++i) A[i * R + i] += diag_add;
+    if(admm) {
+      double tr = 0.;
+      for(uint32_t i=0; i < R; ++i) tr += A[i * R + i];
+      double const rho = tr / (double) R;
+      for(uint32_t i=0; i < R; ++i) A[i * R + i] += rho;
+      *rho_out = rho;
+    }
+  }
+  __syncthreads();
+
+  // right-looking Cholesky on the lower triangle; thread t owns rows t+64k
+  for(uint32_t j=0; j < R; ++j) {
+    double const d = A[j * R + j];
+    if(t == 0 && !(d > 0.) && *info == 0) *info = (int) j + 1;
+    double const ljj = sqrt(d);
+    __syncthreads();
+    for(uint32_t i=j + 1 + t; i < R; i += 64) A[i * R + j] /= ljj;
+    if(t == 0) A[j * R + j] = ljj;
+    __syncthreads();
+    for(uint32_t i=j + 1 + t; i < R; i += 64) {
+      double const lij = A[i * R + j];
+      for(uint32_t k=j + 1; k <= i; ++k) A[i * R + k] -= lij * A[k * R + j];
+    }
+    __syncthreads();
+  }
+  // L^{-1}, column c by thread c (forward substitution against e_c)
+  for(uint32_t c=t; c < R; c += 64) {
+    Li[c * R + c] = 1. / A[c * R + c];
+    for(uint32_t i=c + 1; i < R; ++i) {
+      double s = 0.;
+      for(uint32_t k=c; k < i; ++k) s += A[i * R + k] * Li[k * R + c];
+      Li[i * R + c] = -s / A[i * R + i];
+    }
+  }
+  __syncthreads();
+  // G^{-1} = L^{-T} L^{-1}; padded entries are zero
+  for(uint32_t e=t; e < RP * RP; e += 64) {
+    uint32_t const i = e / RP;
+    uint32_t const j = e - i * RP;
+    if(i >= R || j >= R) ginv[e] = 0.;
+  }
+  for(uint32_t j=t; j < R; j += 64) {
+    for(uint32_t i=0; i <= j; ++i) {
+      double s = 0.;
+      for(uint32_t k=j; k < R; ++k) s += Li[k * R + i] * Li[k * R + j];
+      ginv[i * RP + j] = s;
+      ginv[j * RP + i] = s;
+    }
+  }
+}
+
+// ---- apply the inverse on MFMA (R <= 64) ---------------------------------------
+
+template<typename TM>
+struct ApplyArgs {
+  TM const * M;
+  double const * ginv;      // RP x RP
+  uint64_t I;
+  uint32_t R;
+  double * X;               // KIND 0: output; KIND 1: primal (in: old, out: aux - D)
+  double * aux;             // KIND 1
+  double const * dual;      // KIND 1
+  double * init;            // KIND 1
+  double const * rho;       // KIND 1
+  int clamp;                // KIND 1: column sums of max(P, 0)^2
+  double * colsq;           // per-block partial sums of squares (RP per block), or null
+};
+
+template<int TT, typename TM, int KIND>
+__global__ void __launch_bounds__(DN_THREADS)
+k_apply_mfma(ApplyArgs<TM> a)
+{
+  constexpr int RP = TT * 16;
+  constexpr int LD = RP + 1;
+  constexpr int RPI = (64 / RP) > 0 ? 64 / RP : 1;   // rows staged per pass
+  __shared__ double s_g[RP * RP];
+  __shared__ double s_m[4][16 * LD];
+  __shared__ double s_cs[RP];
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const kk = lane >> 4;
+  int const cc = lane & 15;
+  uint64_t const I = a.I;
+  uint32_t const R = a.R;
+  for(int e=threadIdx.x; e < RP * RP; e += DN_THREADS) s_g[e] = a.ginv[e];
+  for(int e=threadIdx.x; e < RP; e += DN_THREADS) s_cs[e] = 0.;
+  __syncthreads();
+  double const rho = (KIND == 1) ? *a.rho : 0.;
+  double cs[TT];
+  #pragma unroll
+  for(int t=0; t < TT; ++t) cs[t] = 0.;
+
+  int const lr = lane / RP;
+  int const lc = lane - lr * RP;
+  uint64_t const ngroups = (I + 15) / 16;
+  for(uint64_t gi = (uint64_t) blockIdx.x * 4 + wave; gi < ngroups; gi += (uint64_t) gridDim.x * 4) {
+    uint64_t const row0 = gi * 16;
+    // stage 16 rows of the right-hand side (coalesced: the rows are contiguous)
+    #pragma unroll
+    for(int rr=0; rr < 16; rr += RPI) {
+      int const r = rr + lr;
+      if(lr < RPI && r < 16) {
+        uint64_t const row = row0 + r;
+        double v = 0.;
+        if(row < I && (uint32_t) lc < R) {
+          uint64_t const idx = row * R + lc;
+          if(KIND == 0) {
+            v = (double) a.M[idx];
+          } else {
+            double const p = a.X[idx];
+            double const d = a.dual[idx];
+            a.init[idx] = p;
+            v = (double) a.M[idx] + rho * (p + d);
+          }
+        }
+        s_m[wave][r * LD + lc] = v;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    d4 acc[TT];
+    #pragma unroll
+    for(int t=0; t < TT; ++t) acc[t] = (d4) 0.;
+    #pragma unroll
+    for(int k0=0; k0 < RP; k0 += 4) {
+      double const av = s_m[wave][cc * LD + k0 + kk];
+      #pragma unroll
+      for(int t=0; t < TT; ++t) {
+        acc[t] = mfma16(av, s_g[(k0 + kk) * RP + t * 16 + cc], acc[t]);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    #pragma unroll
+    for(int t=0; t < TT; ++t) {
+      uint32_t const col = t * 16 + cc;
+      #pragma unroll
+      for(int r=0; r < 4; ++r) {
+        uint64_t const row = row0 + kk + 4 * r;
+        if(row < I && col < R) {
+          uint64_t const idx = row * R + col;
+          double const x = acc[t][r];
+          if(KIND == 0) {
+            a.X[idx] = x;
+            cs[t] += x * x;
+          } else {
+            a.aux[idx] = x;
+            double const p = x - a.dual[idx];
+            a.X[idx] = p;
+            double const q = (a.clamp && p < 0.) ? 0. : p;
+            cs[t] += q * q;
+          }
+        }
+      }
+    }
+  }
+  if(a.colsq != nullptr) {
+    #pragma unroll
+    for(int t=0; t < TT; ++t) {
+      cs[t] += __shfl_xor(cs[t], 16, 64);
+      cs[t] += __shfl_xor(cs[t], 32, 64);
+    }
+    for(int w=0; w < 4; ++w) {
+      if(wave == w && kk == 0) {
+        #pragma unroll
+        for(int t=0; t < TT; ++t) s_cs[t * 16 + cc] += cs[t];
+      }
+      __syncthreads();
+    }
+    for(int e=threadIdx.x; e < RP; e += DN_THREADS) {
+      a.colsq[(uint64_t) blockIdx.x * RP + e] = s_cs[e];
+    }
+  }
+}
+
+// Plain product for ranks above 64: out[i,j] = sum_k In[i,k] ginv[k,j].
+template<typename TM>
+__global__ void k_apply_plain(TM const * __restrict__ In, double const * __restrict__ ginv,
+    uint64_t I, uint32_t R, uint32_t ld, double * __restrict__ out)
+{
+  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const n = I * R;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  for(; e < n; e += stride) {
+    uint64_t const i = e / R;
+    uint32_t const j = (uint32_t) (e - i * R);
+    double s = 0.;
+    for(uint32_t k=0; k < R; ++k) s += (double) In[i * R + k] * ginv[(uint64_t) k * ld + j];
+    out[e] = s;
+  }
+}
+
+// ADMM, plain path: T = M + rho (P + D), init = P.
+template<typename TM>
+__global__ void k_admm_form(TM const * __restrict__ M, double const * __restrict__ rho,
+    uint64_t n, double const * __restrict__ P, double const * __restrict__ D,
+    double * __restrict__ T, double * __restrict__ init)
+{
+  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  double const r = *rho;
+  for(; e < n; e += stride) {
+    double const p = P[e];
+    init[e] = p;
+    T[e] = (double) M[e] + r * (p + D[e]);
+  }
+}
+
+// ADMM, plain path: P = aux - D.
+__global__ void k_admm_sub(double const * __restrict__ aux, double const * __restrict__ D,
+    uint64_t n, double * __restrict__ P)
+{
+  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  for(; e < n; e += stride) P[e] = aux[e] - D[e];
+}
+
+// ADMM: prox, dual update and the four residual sums (block partials).
+__global__ void __launch_bounds__(DN_THREADS)
+k_admm_post(double * __restrict__ P, double const * __restrict__ aux,
+    double * __restrict__ D, double const * __restrict__ init,
+    double const * __restrict__ norms, int prox, uint64_t n, uint32_t R,
+    double * __restrict__ part)
+{
+  __shared__ double s[4][DN_THREADS];
+  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  uint32_t c = (uint32_t) (e % R);
+  uint32_t const sc = (uint32_t) (stride % R);
+  double pn = 0., pr = 0., dr = 0., dn = 0.;
+  for(; e < n; e += stride) {
+    double p = P[e];
+    if(prox == DN_PROX_NONNEG) {
+      p = (p > 0.) ? p : 0.;
+    } else if(prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG) {
+      if(prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) p = 0.;
+      double const nc = norms[c];
+      if(!(nc <= 1.)) p /= nc;
+    }
+    P[e] = p;
+    double const x = aux[e];
+    double const d = D[e] + (p - x);
+    D[e] = d;
+    double const q = p - x;
+    double const w = p - init[e];
+    pn += p * p;
+    pr += q * q;
+    dr += w * w;
+    dn += d * d;
+    c += sc;
+    if(c >= R) c -= R;
+  }
+  int const t = threadIdx.x;
+  s[0][t] = pn; s[1][t] = pr; s[2][t] = dr; s[3][t] = dn;
+  __syncthreads();
+  for(int h=DN_THREADS / 2; h > 0; h >>= 1) {
+    if(t < h) {
+      #pragma unroll
+      for(int k=0; k < 4; ++k) s[k][t] += s[k][t + h];
+    }
+    __syncthreads();
+  }
+  if(t < 4) part[(uint64_t) blockIdx.x * 4 + t] = s[t][0];
+}
+
+__global__ void k_sum4(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
+{
+  int const k = threadIdx.x;
+  if(k < 4) {
+    double v = 0.;
+    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * 4 + k];
+    out[k] = v;
+  }
+}
+
+__global__ void k_cpd_norm(GramSrc g, int nmodes, uint32_t R,
+    double const * __restrict__ lambda, double * __restrict__ out)
+{
+  __shared__ double s[64];
+  int const t = threadIdx.x;
+  double mine = 0.;
+  for(uint32_t i=t; i < R; i += 64) {
+    double row = 0.;
+    for(uint32_t j=i; j < R; ++j) {
+      double h = 1.;
+      for(int m=0; m < nmodes; ++m) h *= g.aTa[m][i * R + j];
+      row += (i == j) ? h * lambda[i] * lambda[i] : h * lambda[i] * lambda[j] * 2.;
+    }
+    mine += row;
+  }
+  s[t] = mine;
+  __syncthreads();
+  if(t == 0) {
+    double tot = 0.;
+    for(int k=0; k < 64; ++k) tot += s[k];
+    out[0] = fabs(tot);
+  }
+}
+
+// ---- host wrappers -----------------------------------------------------------------
+
+#define SLOT_GRAM 4
+#define SLOT_COL 5
+#define SLOT_CHOL 6
+#define SLOT_TMP 7
+
+template<int TT>
+static int launch_gram_mfma(double * X, uint64_t I, uint32_t R, double const * div,
+    float * X32, double * part, uint32_t nb, hipStream_t st)
+{
+  if(div) {
+    hipLaunchKernelGGL((k_gram_mfma<TT, true>), dim3(nb), dim3(DN_THREADS), 0, st,
+        X, I, R, div, X32, part);
+  } else {
+    hipLaunchKernelGGL((k_gram_mfma<TT, false>), dim3(nb), dim3(DN_THREADS), 0, st,
+        X, I, R, div, X32, part);
+  }
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int dn_div_cols(double * X, uint64_t I, uint32_t R, double const * div,
+    float * X32, hipStream_t st)
+{
+  uint64_t const n = I * R;
+  if(n == 0) return SPD_OK;
+  hipLaunchKernelGGL(k_div_cols, dim3(dn_blocks(n, DN_THREADS * 8)), dim3(DN_THREADS),
+      0, st, X, n, R, div, X32);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+template<typename TY, bool CLAMP>
+static int coldot(double const * X, TY const * Y, uint64_t I, uint32_t R, int mode,
+    double const * lambda, double * out, hipStream_t st)
+{
+  uint32_t const nb = dn_blocks(I, 64);
+  uint64_t const rpb = spd_cdiv(I > 0 ? I : 1, nb);
+  void * part = nullptr;
+  SPD_TRY(spd_scratch(SLOT_COL, (size_t) nb * R * sizeof(double), &part));
+  hipLaunchKernelGGL((k_coldot<TY, CLAMP>), dim3(nb), dim3(DN_THREADS), 0, st,
+      X, Y, I, R, rpb, (double *) part);
+  SPD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_colreduce, dim3(1), dim3(DN_THREADS), 0, st,
+      (double const *) part, nb, R, R, mode, lambda, out);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int dn_colnorm(double const * X, uint64_t I, uint32_t R, double * lam, hipStream_t st)
+{
+  return coldot<double, false>(X, (double const *) nullptr, I, R, 0, nullptr, lam, st);
+}
+
+int dn_innerprod(double const * X, void const * M, int mdtype, uint64_t I,
+    uint32_t R, double const * lambda, double * out, hipStream_t st)
+{
+  if(mdtype == SPD_F32) {
+    return coldot<float, false>(X, (float const *) M, I, R, 1, lambda, out, st);
+  }
+  return coldot<double, false>(X, (double const *) M, I, R, 1, lambda, out, st);
+}
+
+int dn_gram(double * X, uint64_t I, uint32_t R, double const * div, float * X32,
+    double * out, hipStream_t st)
+{
+  if(R > 64) {
+    // plain path: divide / copy first, then a column-pair reduction per entry
+    if(div || X32) SPD_TRY(dn_div_cols(X, I, R, div, X32, st));
+    uint32_t const nb = dn_blocks(I, 64);
+    uint64_t const rpb = spd_cdiv(I > 0 ? I : 1, nb);
+    void * part = nullptr;
+    SPD_TRY(spd_scratch(SLOT_GRAM, (size_t) nb * R * R * sizeof(double), &part));
+    hipLaunchKernelGGL(k_gram_plain, dim3(nb), dim3(DN_THREADS), 0, st,
+        (double const *) X, I, R, rpb, (double *) part);
+    SPD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 256)),
+        dim3(256), 0, st, (double const *) part, nb, R, R, out);
+    SPD_LAUNCH_CHECK();
+    return SPD_OK;
+  }
+  uint32_t const TT = dn_rpad(R) / 16;
+  uint32_t const RP = TT * 16;
+  uint64_t const nsteps = (I + 3) / 4;
+  uint32_t const nb = dn_blocks(nsteps, 4 * GR_U * 4);
+  void * part = nullptr;
+  SPD_TRY(spd_scratch(SLOT_GRAM, (size_t) nb * RP * RP * sizeof(double), &part));
+  switch(TT) {
+  case 1: SPD_TRY(launch_gram_mfma<1>(X, I, R, div, X32, (double *) part, nb, st)); break;
+  case 2: SPD_TRY(launch_gram_mfma<2>(X, I, R, div, X32, (double *) part, nb, st)); break;
+  case 3: SPD_TRY(launch_gram_mfma<3>(X, I, R, div, X32, (double *) part, nb, st)); break;
+  default: SPD_TRY(launch_gram_mfma<4>(X, I, R, div, X32, (double *) part, nb, st)); break;
+  }
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 256)),
+      dim3(256), 0, st, (double const *) part, nb, R, RP, out);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int dn_chol_inv(double const * const * aTa, int nmodes, int mode, uint32_t R,
+    double diag_add, int admm, double * ginv, double * rho, int * info,
+    hipStream_t st)
+{
+  if(nmodes > SPD_MAX_MODES || R == 0) {
+    spd_set_error(SPD_EINPUT, "bad Gram shape");
+    return SPD_EINPUT;
+  }
+  GramSrc g = {};
+  for(int m=0; m < nmodes; ++m) g.aTa[m] = aTa[m];
+  size_t const wbytes = 2ull * R * R * sizeof(double);
+  double * gW = nullptr;
+  size_t shmem = 0;
+  if(wbytes <= 64 * 1024) {
+    shmem = wbytes;
+  } else {
+    void * p = nullptr;
+    SPD_TRY(spd_scratch(SLOT_CHOL, wbytes, &p));
+    gW = (double *) p;
+  }
+  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(64), shmem, st, g, nmodes, mode, R,
+      dn_rpad(R), diag_add, admm, gW, ginv, rho, info);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+template<int TT, typename TM, int KIND>
+static int launch_apply(ApplyArgs<TM> const & a, hipStream_t st, uint32_t nb)
+{
+  hipLaunchKernelGGL((k_apply_mfma<TT, TM, KIND>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+template<typename TM, int KIND>
+static int dispatch_apply(ApplyArgs<TM> const & a, uint32_t TT, uint32_t nb, hipStream_t st)
+{
+  switch(TT) {
+  case 1: return launch_apply<1, TM, KIND>(a, st, nb);
+  case 2: return launch_apply<2, TM, KIND>(a, st, nb);
+  case 3: return launch_apply<3, TM, KIND>(a, st, nb);
+  default: return launch_apply<4, TM, KIND>(a, st, nb);
+  }
+}
+
+static uint32_t apply_blocks(uint64_t I)
+{
+  return dn_blocks((I + 15) / 16, 4 * 4);
+}
+
+template<typename TM>
+static int apply_t(TM const * M, double const * ginv, uint64_t I, uint32_t R,
+    double * X, double * colnorm, hipStream_t st)
+{
+  if(I == 0) return SPD_OK;
+  uint32_t const RP = dn_rpad(R);
+  if(R > 64) {
+    hipLaunchKernelGGL((k_apply_plain<TM>), dim3(dn_blocks(I * R, DN_THREADS * 4)),
+        dim3(DN_THREADS), 0, st, M, ginv, I, R, RP, X);
+    SPD_LAUNCH_CHECK();
+    if(colnorm) SPD_TRY(dn_colnorm(X, I, R, colnorm, st));
+    return SPD_OK;
+  }
+  uint32_t const nb = apply_blocks(I);
+  void * part = nullptr;
+  if(colnorm) SPD_TRY(spd_scratch(SLOT_COL, (size_t) nb * RP * sizeof(double), &part));
+  ApplyArgs<TM> a = {};
+  a.M = M;
+  a.ginv = ginv;
+  a.I = I;
+  a.R = R;
+  a.X = X;
+  a.colsq = (double *) part;
+  SPD_TRY((dispatch_apply<TM, 0>(a, RP / 16, nb, st)));
+  if(colnorm) {
+    hipLaunchKernelGGL(k_colreduce, dim3(1), dim3(DN_THREADS), 0, st,
+        (double const *) part, nb, R, RP, 0, (double const *) nullptr, colnorm);
+    SPD_LAUNCH_CHECK();
+  }
+  return SPD_OK;
+}
+
+int dn_apply(void const * M, int mdtype, double const * ginv, uint64_t I,
+    uint32_t R, double * X, double * colnorm, hipStream_t st)
+{
+  if(mdtype == SPD_F32) return apply_t<float>((float const *) M, ginv, I, R, X, colnorm, st);
+  return apply_t<double>((double const *) M, ginv, I, R, X, colnorm, st);
+}
+
+template<typename TM>
+static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
+    uint32_t R, double * P, double * aux, double * D, double * init, int prox,
+    double * sums, hipStream_t st)
+{
+  if(I == 0) return SPD_OK;
+  uint32_t const RP = dn_rpad(R);
+  uint64_t const n = I * R;
+  bool const need_norms = (prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG);
+  int const clamp = (prox == DN_PROX_MAXCOLNORM_NONNEG);
+  // column norms go into the tail of the sums buffer: sums[4 .. 4+R)
+  double * const norms = sums + 4;
+  if(R > 64) {
+    void * tmp = nullptr;
+    SPD_TRY(spd_scratch(SLOT_TMP, n * sizeof(double), &tmp));
+    uint32_t const nbe = dn_blocks(n, DN_THREADS * 4);
+    hipLaunchKernelGGL((k_admm_form<TM>), dim3(nbe), dim3(DN_THREADS), 0, st,
+        M, rho, n, (double const *) P, (double const *) D, (double *) tmp, init);
+    SPD_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_apply_plain<double>), dim3(nbe), dim3(DN_THREADS), 0, st,
+        (double const *) tmp, ginv, I, R, RP, aux);
+    SPD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_admm_sub, dim3(nbe), dim3(DN_THREADS), 0, st,
+        (double const *) aux, (double const *) D, n, P);
+    SPD_LAUNCH_CHECK();
+    if(need_norms) {
+      if(clamp) {
+        SPD_TRY((coldot<double, true>(P, (double const *) nullptr, I, R, 0, nullptr, norms, st)));
+      } else {
+        SPD_TRY((coldot<double, false>(P, (double const *) nullptr, I, R, 0, nullptr, norms, st)));
+      }
+    }
+  } else {
+    uint32_t const nb = apply_blocks(I);
+    void * part = nullptr;
+    if(need_norms) SPD_TRY(spd_scratch(SLOT_COL, (size_t) nb * RP * sizeof(double), &part));
+    ApplyArgs<TM> a = {};
+    a.M = M;
+    a.ginv = ginv;
+    a.I = I;
+    a.R = R;
+    a.X = P;
+    a.aux = aux;
+    a.dual = D;
+    a.init = init;
+    a.rho = rho;
+    a.clamp = clamp;
+    a.colsq = (double *) part;
+    SPD_TRY((dispatch_apply<TM, 1>(a, RP / 16, nb, st)));
+    if(need_norms) {
+      hipLaunchKernelGGL(k_colreduce, dim3(1), dim3(DN_THREADS), 0, st,
+          (double const *) part, nb, R, RP, 0, (double const *) nullptr, norms);
+      SPD_LAUNCH_CHECK();
+    }
+  }
+  uint32_t const nb2 = dn_blocks(n, DN_THREADS * 8);
+  void * part2 = nullptr;
+  SPD_TRY(spd_scratch(SLOT_GRAM, (size_t) nb2 * 4 * sizeof(double), &part2));
+  hipLaunchKernelGGL(k_admm_post, dim3(nb2), dim3(DN_THREADS), 0, st, P,
+      (double const *) aux, D, (double const *) init, (double const *) norms, prox,
+      n, R, (double *) part2);
+  SPD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum4, dim3(1), dim3(64), 0, st, (double const *) part2, nb2, sums);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int dn_admm_step(void const * M, int mdtype, double const * ginv,
+    double const * rho, uint64_t I, uint32_t R, double * P, double * aux,
+    double * D, double * init, int prox, double * sums, hipStream_t st)
+{
+  if(mdtype == SPD_F32) {
+    return admm_t<float>((float const *) M, ginv, rho, I, R, P, aux, D, init, prox, sums, st);
+  }
+  return admm_t<double>((double const *) M, ginv, rho, I, R, P, aux, D, init, prox, sums, st);
+}
+
+int dn_cpd_norm(double const * const * aTa, int nmodes, uint32_t R,
+    double const * lambda, double * out, hipStream_t st)
+{
+  GramSrc g = {};
+  for(int m=0; m < nmodes; ++m) g.aTa[m] = aTa[m];
+  hipLaunchKernelGGL(k_cpd_norm, dim3(1), dim3(64), 0, st, g, nmodes, R, lambda, out);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+// ---- exported for tests ------------------------------------------------------------
+
+extern "C" int spd_dense_gram(double * X, uint64_t I, uint64_t R, double const * div,
+    float * X32, double * out)
+{
+  SPD_TRY(spd_init());
+  return dn_gram(X, I, (uint32_t) R, div, X32, out, spd_hs(nullptr));
+}
+
+extern "C" int spd_dense_chol_inv(double const * const * aTa, int nmodes, int mode,
+    uint64_t R, double diag_add, int admm, double * ginv, double * rho, int * info)
+{
+  SPD_TRY(spd_init());
+  SPD_TRY(spd_zero_async(info, sizeof(int), spd_hs(nullptr)));
+  return dn_chol_inv(aTa, nmodes, mode, (uint32_t) R, diag_add, admm, ginv, rho, info,
+      spd_hs(nullptr));
+}
+
+extern "C" int spd_dense_apply(void const * M, int mdtype, double const * ginv,
+    uint64_t I, uint64_t R, double * X, double * colnorm)
+{
+  SPD_TRY(spd_init());
+  return dn_apply(M, mdtype, ginv, I, (uint32_t) R, X, colnorm, spd_hs(nullptr));
+}
+
+extern "C" int spd_dense_admm_step(void const * M, int mdtype, double const * ginv,
+    double const * rho, uint64_t I, uint64_t R, double * P, double * aux, double * D,
+    double * init, int prox, double * sums)
+{
+  SPD_TRY(spd_init());
+  return dn_admm_step(M, mdtype, ginv, rho, I, (uint32_t) R, P, aux, D, init, prox,
+      sums, spd_hs(nullptr));
+}

@@ -1,0 +1,66 @@
+// spd_dense.hpp -- dense FP64 factor algebra used by CPD-ALS and streaming
+// CPD: Grams, the normal-equation solve, column norms, ADMM steps and the
+// fit reductions. All matrices are row-major I x R in device memory; all
+// results stay on the device (nothing here synchronises).
+#pragma once
+#include "spd_internal.hpp"
+
+// Proximal operators recognised on the device (the reference's
+// constraints/maxcolnorm.c, maxcolnorm-nonneg.c, nonneg.c).
+#define DN_PROX_NONE 0
+#define DN_PROX_MAXCOLNORM 1
+#define DN_PROX_MAXCOLNORM_NONNEG 2
+#define DN_PROX_NONNEG 3
+
+// out (R x R, full, symmetric) = X^T X. When div != nullptr the columns of X
+// are first divided in place by div[r] (X[:,r] /= div[r], the reference's
+// normalisation); when X32 != nullptr an FP32 copy of the (divided) X is
+// written as well.
+int dn_gram(double * X, uint64_t I, uint32_t R, double const * div, float * X32,
+    double * out, hipStream_t st);
+
+// Normal-equation matrix of `mode` and its inverse: G = Hadamard product of
+// aTa[m] over m != mode (upper triangle, mirrored), plus diag_add on the
+// diagonal; with admm != 0 also rho = trace(G)/R is stored in *rho and added
+// to the diagonal. G is Cholesky-factored (G = L L^T) and
+// ginv (RP x RP, RP = R rounded up to 16, zero padded) = L^{-T} L^{-1}.
+// *info gets the 1-based column of a non-positive pivot (0 if none); it must
+// be zero on entry.
+int dn_chol_inv(double const * const * aTa, int nmodes, int mode, uint32_t R,
+    double diag_add, int admm, double * ginv, double * rho, int * info,
+    hipStream_t st);
+
+// Row-padded size used for ginv.
+static inline uint32_t dn_rpad(uint32_t R) { return (R + 15) / 16 * 16; }
+
+// X = M ginv (rows of M times the inverse, i.e. the Cholesky solve of the
+// reference's potrs). M is FP64 (mdtype SPD_F64) or FP32. When colnorm is
+// given, colnorm[r] = ||X[:,r]||_2.
+int dn_apply(void const * M, int mdtype, double const * ginv, uint64_t I,
+    uint32_t R, double * X, double * colnorm, hipStream_t st);
+
+// One ADMM inner iteration (src/cpd/admm.c:240-308) on device:
+//   init = P; aux = (M + rho (P + D)) ginv; P = prox(aux - D); D += P - aux;
+// sums[0..3] = ||P||^2, ||P - aux||^2, ||P - init||^2, ||D||^2 (after the
+// update); sums[4 .. 4+R) receive the column norms the prox used, so sums
+// must hold 4 + R doubles.
+int dn_admm_step(void const * M, int mdtype, double const * ginv,
+    double const * rho, uint64_t I, uint32_t R, double * P, double * aux,
+    double * D, double * init, int prox, double * sums, hipStream_t st);
+
+// lam[r] = ||X[:,r]||_2.
+int dn_colnorm(double const * X, uint64_t I, uint32_t R, double * lam,
+    hipStream_t st);
+
+// X[:,r] /= div[r] (and the FP32 copy when X32 != nullptr).
+int dn_div_cols(double * X, uint64_t I, uint32_t R, double const * div,
+    float * X32, hipStream_t st);
+
+// *out = |lambda^T (Hadamard of all aTa) lambda| over the upper triangle
+// (src/cpd/cpd.c:453-487).
+int dn_cpd_norm(double const * const * aTa, int nmodes, uint32_t R,
+    double const * lambda, double * out, hipStream_t st);
+
+// *out = sum_r lambda[r] sum_i X[i,r] M[i,r] (src/cpd/cpd.c:490-531).
+int dn_innerprod(double const * X, void const * M, int mdtype, uint64_t I,
+    uint32_t R, double const * lambda, double * out, hipStream_t st);

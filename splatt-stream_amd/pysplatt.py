@@ -72,6 +72,20 @@ class SplattKruskal(C.Structure):
                 ("fit", C.c_double)]
 
 
+class CpdOpts(C.Structure):
+    _fields_ = [("tolerance", C.c_double),
+                ("max_iterations", C.c_uint64),
+                ("inner_tolerance", C.c_double),
+                ("max_inner_iterations", C.c_uint64),
+                ("constraints", C.c_void_p * MAX_NMODES),
+                ("chunk_sizes", C.c_uint64 * MAX_NMODES)]
+
+
+class GlobalOpts(C.Structure):
+    _fields_ = [("num_threads", C.c_int), ("verbosity", C.c_int),
+                ("random_seed", C.c_int)]
+
+
 class MatrixT(C.Structure):
     _fields_ = [("I", C.c_uint64), ("J", C.c_uint64), ("vals", f64p),
                 ("rowmajor", C.c_int)]
@@ -156,6 +170,36 @@ def lib():
     L.splatt_amd_d2h.restype = C.c_int
     L.splatt_amd_sync.restype = C.c_int
     L.splatt_version_major.restype = C.c_int
+    L.splatt_alloc_cpd.restype = C.POINTER(SplattKruskal)
+    L.splatt_alloc_cpd.argtypes = [C.POINTER(SplattCsf), C.c_uint64]
+    L.splatt_free_cpd.argtypes = [C.POINTER(SplattKruskal)]
+    L.splatt_cpd.restype = C.c_int
+    L.splatt_cpd.argtypes = [C.POINTER(SplattCsf), C.c_uint64, C.POINTER(CpdOpts),
+                             C.POINTER(GlobalOpts), C.POINTER(SplattKruskal)]
+    L.splatt_alloc_cpd_opts.restype = C.POINTER(CpdOpts)
+    L.splatt_free_cpd_opts.argtypes = [C.POINTER(CpdOpts)]
+    L.splatt_alloc_global_opts.restype = C.POINTER(GlobalOpts)
+    L.splatt_free_global_opts.argtypes = [C.POINTER(GlobalOpts)]
+    L.splatt_register_frob.restype = C.c_int
+    L.splatt_register_frob.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
+    for name in ("splatt_register_maxcolnorm", "splatt_register_maxcolnorm_nonneg",
+                 "splatt_register_nonneg"):
+        getattr(L, name).restype = C.c_int
+        getattr(L, name).argtypes = [C.POINTER(CpdOpts), u64p, C.c_uint64]
+    L.spd_dense_gram.restype = C.c_int
+    L.spd_dense_gram.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                 C.c_void_p, C.c_void_p]
+    L.spd_dense_chol_inv.restype = C.c_int
+    L.spd_dense_chol_inv.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_uint64,
+                                     C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]
+    L.spd_dense_apply.restype = C.c_int
+    L.spd_dense_apply.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64,
+                                  C.c_uint64, C.c_void_p, C.c_void_p]
+    L.spd_dense_admm_step.restype = C.c_int
+    L.spd_dense_admm_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
     _lib = L
     return L
 
@@ -350,3 +394,48 @@ class DevBuf:
             self.free()
         except Exception:
             pass
+
+
+_libc = None
+
+
+def srand(seed):
+    """Seed glibc rand(), which splatt_alloc_cpd draws from."""
+    global _libc
+    if _libc is None:
+        _libc = C.CDLL(None)
+    _libc.srand(C.c_uint(seed))
+
+
+def run_cpd(csfset, rank, max_its=20, tol=0.0, seed=1, configure=None,
+            verbosity=0, max_inner=None, inner_tol=None):
+    """splatt_alloc_cpd (after srand(seed)) + splatt_cpd through the C API.
+    configure(L, opts_ptr) may register constraints. Returns a dict with the
+    initial factors, the result factors, lambda, fit and the return code."""
+    L = lib()
+    srand(seed)
+    K = L.splatt_alloc_cpd(csfset.ptr, rank)
+    N = int(K[0].nmodes)
+    dims = [int(K[0].dims[m]) for m in range(N)]
+    init = [np.ctypeslib.as_array(K[0].factors[m], shape=(dims[m], rank)).copy()
+            for m in range(N)]
+    co = L.splatt_alloc_cpd_opts()
+    co[0].max_iterations = max_its
+    co[0].tolerance = tol
+    if max_inner is not None:
+        co[0].max_inner_iterations = max_inner
+    if inner_tol is not None:
+        co[0].inner_tolerance = inner_tol
+    if configure is not None:
+        configure(L, co)
+    go = L.splatt_alloc_global_opts()
+    go[0].verbosity = verbosity
+    rc = L.splatt_cpd(csfset.ptr, rank, co, go, K)
+    out = [np.ctypeslib.as_array(K[0].factors[m], shape=(dims[m], rank)).copy()
+           for m in range(N)]
+    lam = np.ctypeslib.as_array(K[0].lambda_, shape=(rank,)).copy()
+    fit = float(K[0].fit)
+    L.splatt_free_cpd_opts(co)
+    L.splatt_free_global_opts(go)
+    L.splatt_free_cpd(K)
+    return dict(init=init, factors=out, lam=lam, fit=fit, rc=rc, dims=dims)

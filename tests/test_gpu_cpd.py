@@ -1,0 +1,292 @@
+"""GPU parity of the dense CPD steps and of splatt_cpd against the oracle.
+
+Dense kernels (device arrays through the exported spd_dense_* entries):
+  Gram         X^T X on MFMA against numpy, with and without the in-place
+               column division and the FP32 copy; ranks on both sides of 64.
+  chol_inv     ginv G = I for the Hadamard of Grams (+ Frobenius term,
+               + rho = trace/R for ADMM); a non-positive pivot is reported.
+  apply        M ginv against numpy's solve, FP64 and FP32 right-hand sides,
+               and the fused column norms.
+  admm_step    one inner iteration against the oracle's or_admm with
+               max_inner = 1, for every proximal operator.
+
+splatt_cpd (C API; factors from splatt_alloc_cpd after srand):
+  FP64: fit per run, lambda and factors against or_cpd_als_con from the same
+  start (relative 1e-8 on the fit; the two differ only in rounding:
+  explicit inverse vs triangular solves, summation order).
+  FP32 MTTKRP: fit within 1e-4 of the FP64 oracle.
+  Frobenius on one mode, and ADMM with max-column-norm constraints.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_py as orc
+import pysplatt as ps
+from conftest import tensor_path
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return ps.DevBuf.from_host(np.ascontiguousarray(a))
+
+
+def _rp(R):
+    return (R + 15) // 16 * 16
+
+
+@pytest.mark.parametrize("R", [1, 5, 16, 32, 40, 64, 100])
+@pytest.mark.parametrize("I", [1, 37, 1000])
+def test_gram(gpu_available, I, R):
+    L = ps.lib()
+    rng = np.random.default_rng(I * 131 + R)
+    X = rng.standard_normal((I, R))
+    dX = _dev(X)
+    out = ps.DevBuf(R * R * 8)
+    assert L.spd_dense_gram(dX.ptr, I, R, None, None, out.ptr) == 0
+    G = out.to_host((R, R), np.float64)
+    want = X.T @ X
+    assert np.allclose(G, want, rtol=1e-12, atol=1e-12 * np.max(np.abs(want)))
+    assert np.array_equal(G, G.T)
+    # with column division and FP32 copy
+    div = np.abs(rng.standard_normal(R)) + 0.5
+    dd = _dev(div)
+    x32 = ps.DevBuf(I * R * 4)
+    assert L.spd_dense_gram(dX.ptr, I, R, dd.ptr, x32.ptr, out.ptr) == 0
+    Xd = dX.to_host((I, R), np.float64)
+    assert np.array_equal(Xd, X / div)
+    assert np.array_equal(x32.to_host((I, R), np.float32), (X / div).astype(np.float32))
+    G2 = out.to_host((R, R), np.float64)
+    want2 = (X / div).T @ (X / div)
+    assert np.allclose(G2, want2, rtol=1e-12, atol=1e-12 * np.max(np.abs(want2)))
+
+
+def _grams(R, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        A = rng.standard_normal((2 * R + 3, R))
+        out.append(A.T @ A)
+    return out
+
+
+@pytest.mark.parametrize("R", [3, 16, 33, 64, 80])
+@pytest.mark.parametrize("admm", [0, 1])
+def test_chol_inv(gpu_available, R, admm):
+    L = ps.lib()
+    N = 3
+    grams = _grams(R, N, R + 7 * admm)
+    bufs = [_dev(g) for g in grams]
+    arr = (C.c_void_p * N)(*[b.ptr for b in bufs])
+    RP = _rp(R)
+    ginv = ps.DevBuf(RP * RP * 8)
+    rho = ps.DevBuf(8)
+    info = ps.DevBuf(8)
+    mode, mu = 1, 0.25
+    assert L.spd_dense_chol_inv(arr, N, mode, R, mu, admm, ginv.ptr, rho.ptr, info.ptr) == 0
+    G = np.ones((R, R))
+    for m in range(N):
+        if m != mode:
+            G *= np.triu(grams[m]) + np.triu(grams[m], 1).T
+    G += mu * np.eye(R)
+    if admm:
+        r = rho.to_host((1,), np.float64)[0]
+        assert abs(r - np.trace(G) / R) <= 1e-13 * abs(r)
+        G += r * np.eye(R)
+    Gi = ginv.to_host((RP, RP), np.float64)
+    assert np.all(Gi[R:, :] == 0) and np.all(Gi[:, R:] == 0)
+    Gi = Gi[:R, :R]
+    assert np.array_equal(Gi, Gi.T)
+    err = np.max(np.abs(Gi @ G - np.eye(R)))
+    assert err <= 1e-9, err
+    assert info.to_host((1,), np.int32)[0] == 0
+
+
+def test_chol_reports_bad_pivot(gpu_available):
+    L = ps.lib()
+    R = 4
+    g = -np.eye(R)
+    bufs = [_dev(np.eye(R)), _dev(g)]
+    arr = (C.c_void_p * 2)(*[b.ptr for b in bufs])
+    ginv = ps.DevBuf(16 * 16 * 8)
+    rho = ps.DevBuf(8)
+    info = ps.DevBuf(8)
+    assert L.spd_dense_chol_inv(arr, 2, 0, R, 0.0, 0, ginv.ptr, rho.ptr, info.ptr) == 0
+    assert info.to_host((1,), np.int32)[0] == 1
+
+
+@pytest.mark.parametrize("R", [2, 16, 24, 48, 64, 70])
+@pytest.mark.parametrize("f32", [False, True])
+def test_apply(gpu_available, R, f32):
+    L = ps.lib()
+    I = 777
+    rng = np.random.default_rng(R)
+    G = _grams(R, 1, R)[0] + np.eye(R)
+    bufs = [_dev(G), _dev(np.ones((R, R)))]
+    arr = (C.c_void_p * 2)(*[b.ptr for b in bufs])
+    RP = _rp(R)
+    ginv = ps.DevBuf(RP * RP * 8)
+    rho = ps.DevBuf(8)
+    info = ps.DevBuf(8)
+    assert L.spd_dense_chol_inv(arr, 2, 1, R, 0.0, 0, ginv.ptr, rho.ptr, info.ptr) == 0
+    M = rng.standard_normal((I, R))
+    if f32:
+        M = M.astype(np.float32)
+    dM = _dev(M)
+    X = ps.DevBuf(I * R * 8)
+    cn = ps.DevBuf(R * 8)
+    assert L.spd_dense_apply(dM.ptr, 1 if f32 else 0, ginv.ptr, I, R, X.ptr, cn.ptr) == 0
+    got = X.to_host((I, R), np.float64)
+    want = np.linalg.solve(G, M.astype(np.float64).T).T
+    assert np.allclose(got, want, rtol=1e-9, atol=1e-9 * np.max(np.abs(want)))
+    norms = cn.to_host((R,), np.float64)
+    assert np.allclose(norms, np.linalg.norm(got, axis=0), rtol=1e-12)
+
+
+@pytest.mark.parametrize("R", [8, 32, 70])
+@pytest.mark.parametrize("prox", [0, 1, 2, 3])
+def test_admm_step_matches_oracle(gpu_available, R, prox):
+    L = ps.lib()
+    I = 300
+    rng = np.random.default_rng(R * 10 + prox)
+    grams = _grams(R, 3, R + prox)
+    mode = 2
+    Gfull = np.ones((R, R))
+    for m in range(3):
+        if m != mode:
+            Gfull *= grams[m]
+    M = rng.standard_normal((I, R)) * 5
+    P0 = rng.standard_normal((I, R))
+    D0 = rng.standard_normal((I, R)) * 0.1
+    wantP, wantD, its, wrho = orc.admm(prox, M, P0, D0, Gfull, 1, 0.0)
+    bufs = [_dev(g) for g in grams]
+    arr = (C.c_void_p * 3)(*[b.ptr for b in bufs])
+    RP = _rp(R)
+    ginv = ps.DevBuf(RP * RP * 8)
+    rho = ps.DevBuf(8)
+    info = ps.DevBuf(8)
+    assert L.spd_dense_chol_inv(arr, 3, mode, R, 0.0, 1, ginv.ptr, rho.ptr, info.ptr) == 0
+    dM, dP, dD = _dev(M), _dev(P0), _dev(D0)
+    aux = ps.DevBuf(I * R * 8)
+    init = ps.DevBuf(I * R * 8)
+    sums = ps.DevBuf((4 + R) * 8)
+    assert L.spd_dense_admm_step(dM.ptr, 0, ginv.ptr, rho.ptr, I, R, dP.ptr, aux.ptr,
+                                 dD.ptr, init.ptr, prox, sums.ptr) == 0
+    P = dP.to_host((I, R), np.float64)
+    D = dD.to_host((I, R), np.float64)
+    s = sums.to_host((4 + R,), np.float64)
+    assert abs(rho.to_host((1,), np.float64)[0] - wrho) <= 1e-13 * wrho
+    tol = 1e-8
+    assert np.allclose(P, wantP, rtol=tol, atol=tol * np.max(np.abs(wantP)))
+    # D is a difference of O(|P|) terms (zero up to rounding for prox 0)
+    assert np.allclose(D, wantD, rtol=tol, atol=tol * np.max(np.abs(wantP)))
+    assert np.array_equal(init.to_host((I, R), np.float64), P0)
+    A = aux.to_host((I, R), np.float64)
+    assert np.allclose(s[:4], [np.sum(P * P), np.sum((P - A) ** 2), np.sum((P - P0) ** 2),
+                               np.sum(D * D)], rtol=1e-12)
+
+
+# ---- splatt_cpd ---------------------------------------------------------------
+
+def _twomode(path=None, coo=None, alloc=ps.CSF_TWOMODE):
+    opts = ps.Opts(csf_alloc=alloc)
+    if path is not None:
+        nm = C.c_uint64()
+        out = C.POINTER(ps.SplattCsf)()
+        assert ps.lib().splatt_csf_load(path.encode(), C.byref(nm), C.byref(out),
+                                        opts.ptr) == ps.SUCCESS
+        return ps.CsfSet(out, opts)
+    inds, vals = coo
+    return ps.CsfSet.convert(inds, vals, opts)
+
+
+def _oracle_from_path(path):
+    coo = orc.Coo.read(path)
+    coo.remove_empty()
+    return coo
+
+
+def _run_both(path, R, its, cons=None, unconstrained=True, configure=None,
+              max_inner=50, inner_tol=1e-2, f32=False, seed=7):
+    cs = _twomode(path=path)
+    L = ps.lib()
+    L.splatt_amd_set_mttkrp_precision(1 if f32 else 0)
+    try:
+        res = ps.run_cpd(cs, R, max_its=its, tol=0.0, seed=seed, configure=configure,
+                         max_inner=max_inner, inner_tol=inner_tol)
+    finally:
+        L.splatt_amd_set_mttkrp_precision(0)
+    assert res["rc"] == ps.SUCCESS, ps.last_error()
+    coo = _oracle_from_path(path)
+    dims = res["dims"]
+    N = len(dims)
+    a0 = cs.arrays(0)
+    perm0 = a0["perm"]
+    ocsfs = [orc.Csf(coo, perm0)]
+    if cs.count >= 2:
+        ocsfs.append(orc.Csf(coo, cs.arrays(1)["perm"]))
+    mode_csf = [1 if (cs.count >= 2 and perm0[-1] == m) else 0 for m in range(N)]
+    if cons is None:
+        cons = [(0, 0, 0.0)] * N
+    facs, lam, fits = orc.cpd_als_con(ocsfs, mode_csf, res["init"], its, 0.0, cons,
+                                      unconstrained, max_inner, inner_tol)
+    return res, facs, lam, fits
+
+
+def _model_close(res, facs, lam, rtol):
+    """Compare the two Kruskal models column by column (scaled factors)."""
+    for m in range(len(facs)):
+        a = res["factors"][m] * (res["lam"] if m == 0 else 1.0)
+        b = facs[m] * (lam if m == 0 else 1.0)
+        scale = np.max(np.abs(b))
+        assert np.allclose(a, b, rtol=rtol, atol=rtol * scale), m
+
+
+@pytest.mark.parametrize("name,R", [("small.tns", 4), ("med.tns", 8), ("med4.tns", 6),
+                                    ("med5.tns", 5)])
+def test_cpd_fp64_matches_oracle(gpu_available, name, R):
+    res, facs, lam, fits = _run_both(tensor_path(name), R, 5)
+    assert abs(res["fit"] - fits[-1]) <= 1e-8 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-6)
+
+
+def test_cpd_fp32_fit_within_1e4(gpu_available):
+    res, facs, lam, fits = _run_both(tensor_path("med.tns"), 8, 10, f32=True)
+    assert abs(res["fit"] - fits[-1]) <= 1e-4
+
+
+def test_cpd_frob_one_mode(gpu_available):
+    def configure(L, co):
+        modes = np.array([1], dtype=np.uint64)
+        L.splatt_register_frob(co, 0.1, modes.ctypes.data_as(ps.u64p), 1)
+    res, facs, lam, fits = _run_both(tensor_path("med.tns"), 6, 4,
+                                     cons=[(1, 0, 0.0), (1, 0, 0.1), (1, 0, 0.0)],
+                                     unconstrained=False, configure=configure)
+    assert np.allclose(res["lam"], 1.0)
+    assert abs(res["fit"] - fits[-1]) <= 1e-8 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-6)
+
+
+def test_cpd_admm_maxcolnorm(gpu_available):
+    def configure(L, co):
+        modes = np.array([0, 2], dtype=np.uint64)
+        L.splatt_register_maxcolnorm(co, modes.ctypes.data_as(ps.u64p), 2)
+    res, facs, lam, fits = _run_both(tensor_path("med.tns"), 5, 3,
+                                     cons=[(2, 1, 0.0), (1, 0, 0.0), (2, 1, 0.0)],
+                                     unconstrained=False, configure=configure,
+                                     max_inner=25, inner_tol=7e-2)
+    for m in (0, 2):
+        assert np.all(np.linalg.norm(res["factors"][m], axis=0) <= 1 + 1e-9)
+    assert abs(res["fit"] - fits[-1]) <= 1e-6 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-5)
+
+
+def test_cpd_rejects_chunked_nonneg(gpu_available):
+    def configure(L, co):
+        modes = np.array([0], dtype=np.uint64)
+        L.splatt_register_nonneg(co, modes.ctypes.data_as(ps.u64p), 1)
+    cs = _twomode(path=tensor_path("small.tns"))
+    res = ps.run_cpd(cs, 3, max_its=2, configure=configure)
+    assert res["rc"] != ps.SUCCESS

@@ -162,3 +162,128 @@ def test_cpd_exact_low_rank():
     csfs = [orc.Csf(c, orc.mode_order(dims, 3, m)) for m in range(3)]
     facs, lam, fits = orc.cpd_als(csfs, [0, 1, 2], _init(dims, 2, 5), 200, 1e-12)
     assert fits[-1] >= 1 - 1e-6
+
+
+# ---- constraints and ADMM (src/cpd/admm.c, src/cpd/constraints/*.c) ------
+#
+#   or_admm, prox none   with the identity prox the dual vanishes after one
+#                        step and P <- (G + rho I)^{-1}(M + rho P) is a
+#                        fixed-point iteration whose limit is G^{-1} M.
+#   or_admm, nonneg      the limit satisfies the KKT conditions of the
+#                        non-negative least squares problem per row:
+#                        p >= 0, g = G p - m >= 0 where p = 0, g = 0 where
+#                        p > 0 (a wrong dual sign or prox placement fails).
+#   or_admm, max-col-norm  when the unconstrained solution has column norms
+#                        below 1 the constraint is inactive: the limit is
+#                        G^{-1} M; otherwise every column norm is <= 1.
+#   or_prox              closed forms of the three operators.
+#   or_cpd_als_con       frob with multiplier 0 equals the unconstrained
+#                        solve without normalisation; with multiplier mu the
+#                        solved factor satisfies (G + mu I) U^T = M^T.
+
+
+def _spd(R, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((3 * R, R))
+    return scale * (A.T @ A) + 0.1 * np.eye(R)
+
+
+def test_prox_closed_forms():
+    rng = np.random.default_rng(4)
+    P = rng.standard_normal((50, 6)) * 0.5
+    P[:, 0] *= 10.0
+    nn = orc.prox(3, P)
+    assert np.array_equal(nn, np.where(P > 0, P, 0.0))
+    mc = orc.prox(1, P)
+    norms = np.linalg.norm(P, axis=0)
+    want = np.where(norms > 1, P / np.where(norms > 1, norms, 1.0), P)
+    assert np.allclose(mc, want, rtol=1e-15, atol=0)
+    mcn = orc.prox(2, P)
+    Q = np.where(P < 0, 0.0, P)
+    qn = np.linalg.norm(Q, axis=0)
+    assert np.allclose(mcn, np.where(qn > 1, Q / np.where(qn > 1, qn, 1.0), Q))
+
+
+def test_admm_identity_prox_converges_to_solve():
+    R, I = 5, 40
+    G = _spd(R, 1)
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((I, R))
+    P0 = rng.standard_normal((I, R))
+    P, D, its, rho = orc.admm(0, M, P0, np.zeros((I, R)), G, 400, 0.0)
+    # with tolerance 0 it stops only at an exact floating-point fixed point
+    assert 1 < its <= 400
+    assert abs(rho - np.trace(G) / R) <= 1e-14 * rho
+    assert np.allclose(P, np.linalg.solve(G, M.T).T, rtol=1e-8, atol=1e-10)
+    assert np.max(np.abs(D)) <= 1e-12
+
+
+def test_admm_nonneg_kkt():
+    R, I = 6, 30
+    G = _spd(R, 2)
+    rng = np.random.default_rng(6)
+    M = rng.standard_normal((I, R)) * 3.0
+    P, D, its, rho = orc.admm(3, M, np.zeros((I, R)), np.zeros((I, R)), G, 3000, 0.0)
+    assert np.all(P >= 0)
+    g = P @ G - M           # gradient of 1/2 p^T G p - m^T p per row
+    scale = np.max(np.abs(M))
+    assert np.all(g[P == 0] >= -1e-6 * scale)
+    assert np.max(np.abs(g[P > 1e-9])) <= 1e-6 * scale
+
+
+def test_admm_maxcolnorm_inactive_and_active():
+    R, I = 4, 25
+    G = _spd(R, 3, scale=5.0)
+    rng = np.random.default_rng(7)
+    M = rng.standard_normal((I, R)) * 0.01
+    ls = np.linalg.solve(G, M.T).T
+    assert np.all(np.linalg.norm(ls, axis=0) < 1)
+    P, D, its, rho = orc.admm(1, M, np.zeros((I, R)), np.zeros((I, R)), G, 2000, 0.0)
+    assert np.allclose(P, ls, rtol=1e-7, atol=1e-10)
+    M2 = M * 1e4
+    P2, D2, its2, rho2 = orc.admm(1, M2, np.zeros((I, R)), np.zeros((I, R)), G, 2000, 0.0)
+    assert np.all(np.linalg.norm(P2, axis=0) <= 1 + 1e-12)
+
+
+def test_admm_stops_on_tolerance():
+    R, I = 4, 20
+    G = _spd(R, 8)
+    rng = np.random.default_rng(9)
+    M = rng.standard_normal((I, R))
+    P, D, its, rho = orc.admm(3, M, np.zeros((I, R)), np.zeros((I, R)), G, 50, 1e-2)
+    assert 1 <= its < 50
+
+
+def test_cpd_frob_closed_form():
+    dims = [8, 6, 7]
+    inds, vals = _tensor(dims, 150, 3)
+    c = orc.Coo.from_arrays(inds, vals, dims)
+    csfs = [orc.Csf(c, orc.mode_order(dims, 3, m)) for m in range(3)]
+    init = _init(dims, 3, 17)
+    cons0 = [(1, 0, 0.0)] * 3
+    f0, l0, fits0 = orc.cpd_als_con(csfs, [0, 1, 2], init, 1, 0.0, cons0, False)
+    # mu = 0: a plain normal-equation solve of every mode, lambda stays 1
+    assert np.allclose(l0, 1.0)
+    start = [f / np.linalg.norm(f, axis=0) for f in init]
+    X = _dense(inds, vals, dims)
+    cur = [s.copy() for s in start]
+    for m in range(3):
+        others = [cur[k] for k in range(3) if k != m]
+        G = np.ones((3, 3))
+        for o in others:
+            G *= o.T @ o
+        sub = "ijk," + ",".join("%sr" % "ijk"[k] for k in range(3) if k != m) + \
+              "->%sr" % "ijk"[m]
+        Mm = np.einsum(sub, X, *others)
+        cur[m] = np.linalg.solve(G, Mm.T).T
+    for m in range(3):
+        assert np.allclose(f0[m], cur[m], rtol=1e-9, atol=1e-9)
+    # mu > 0 on mode 1: (G + mu I) U1^T = M1^T for the final sweep's inputs
+    mu = 0.5
+    cons = [(1, 0, 0.0), (1, 0, mu), (1, 0, 0.0)]
+    f1, l1, fits1 = orc.cpd_als_con(csfs, [0, 1, 2], init, 1, 0.0, cons, False)
+    # mode 1 is solved after mode 0, before mode 2: its Gram uses the new U0
+    # and the starting U2
+    G = (f1[0].T @ f1[0]) * (start[2].T @ start[2]) + mu * np.eye(3)
+    M1 = np.einsum("ijk,ir,kr->jr", X, f1[0], start[2])
+    assert np.allclose(f1[1] @ G, M1, rtol=1e-9, atol=1e-9)
