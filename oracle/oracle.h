@@ -125,6 +125,49 @@ uint64_t or_admm(int prox, double const * M, double * P, double * D,
 /* The proximal operators (row-major I x R, in place). */
 void or_prox(int prox, double * P, uint64_t I, uint64_t R);
 
+/* ---- streaming (the files under src/cpd/stream) --------------------- */
+
+/* Sort in place into lexicographic order of perm (src/sort.c:912-956). */
+void or_coo_sort(or_coo * tt, uint64_t const * perm);
+
+/* COO MTTKRP in FP64, src/mttkrp.c:1699-1769. */
+void or_mttkrp_stream(or_coo const * tt, double * const * mats, uint64_t rank,
+    uint64_t mode, double * out);
+
+/* src/cpd/cpd.c:533-577 and :580-656 (unsquared relative error). */
+double or_kruskal_norm(double * const * factors, uint64_t const * dims,
+    uint64_t nmodes, uint64_t rank, double const * lambda);
+double or_cpd_error(or_coo const * tt, double * const * factors,
+    uint64_t const * fdims, uint64_t rank, double const * lambda);
+
+/* ParserSimple.cc:13-80: sort by (t, t+1, ...) cyclically and relabel the
+ * other modes by first appearance; iperm[m] (optional) gets new -> old. */
+void or_stream_prepare(or_coo * tt, uint64_t stream_mode, uint64_t ** iperm);
+
+typedef struct {
+  uint64_t stream_mode;
+  uint64_t rank;
+  double forget;
+  uint64_t max_its;                 /* outer ALS iterations per slice */
+  or_con cons[OR_MAX_NMODES];       /* per mode, after the stream's rules */
+} or_stream_cfg;
+
+typedef struct {
+  uint64_t max_batches;             /* capacity of time / local_err / outer_its */
+  double * time;                    /* max_batches x rank */
+  double * factors[OR_MAX_NMODES];  /* non-stream modes, caller sized */
+  double * local_err;               /* per slice, squared (may be NULL) */
+  uint64_t * outer_its;             /* per slice (may be NULL) */
+  uint64_t nbatches;
+  uint64_t dims[OR_MAX_NMODES];
+  double final_err;                 /* squared, as printed by the reference */
+} or_stream_out;
+
+/* StreamCPD::compute (StreamCPD.cc:311-571) on a tensor already prepared
+ * by or_stream_prepare; the caller seeds rand(). */
+int or_stream_cpd(or_coo const * tt, or_stream_cfg const * cfg,
+    or_stream_out * out);
+
 /* Dense helpers exposed for pinning tests (src/matrix.c). */
 void or_mat_aTa(double const * A, uint64_t I, uint64_t R, double * out);
 int or_cholesky(double * G, uint64_t R);            /* in place, lower */

@@ -155,6 +155,25 @@ static uint64_t * p_lex_order(or_coo const * tt, uint64_t const * perm)
   return idx;
 }
 
+/* Sort the COO in place into the lexicographic order of perm (the order
+ * tt_sort produces, src/sort.c:912-956). */
+void or_coo_sort(or_coo * tt, uint64_t const * perm)
+{
+  uint64_t const n = tt->nnz;
+  uint64_t * idx = p_lex_order(tt, perm);
+  uint64_t * tmp = malloc((n ? n : 1) * sizeof(uint64_t));
+  for(uint64_t m=0; m < tt->nmodes; ++m) {
+    for(uint64_t i=0; i < n; ++i) tmp[i] = tt->ind[m][idx[i]];
+    memcpy(tt->ind[m], tmp, n * sizeof(uint64_t));
+  }
+  double * tv = malloc((n ? n : 1) * sizeof(double));
+  for(uint64_t i=0; i < n; ++i) tv[i] = tt->vals[idx[i]];
+  memcpy(tt->vals, tv, n * sizeof(double));
+  free(tv);
+  free(tmp);
+  free(idx);
+}
+
 /* ---- compression, src/csf.c:230-372 ------------------------------------ */
 
 /* Build the sparsity of nonzeros [0, n) of the already-ordered arrays

@@ -277,3 +277,77 @@ def prox(kind, P):
     I, R = P.shape
     lib().or_prox(C.c_int(kind), _p(P, C.c_double), C.c_uint64(I), C.c_uint64(R))
     return P
+
+
+# ---- streaming ---------------------------------------------------------------
+
+class _OrStreamCfg(C.Structure):
+    _fields_ = [("stream_mode", C.c_uint64), ("rank", C.c_uint64),
+                ("forget", C.c_double), ("max_its", C.c_uint64),
+                ("cons", _OrCon * MAXM)]
+
+
+class _OrStreamOut(C.Structure):
+    _fields_ = [("max_batches", C.c_uint64), ("time", f64p),
+                ("factors", f64p * MAXM), ("local_err", f64p),
+                ("outer_its", u64p), ("nbatches", C.c_uint64),
+                ("dims", C.c_uint64 * MAXM), ("final_err", C.c_double)]
+
+
+def mttkrp_stream(coo, mats, mode):
+    R = mats[0].shape[1]
+    keep = [np.ascontiguousarray(m, dtype=np.float64) for m in mats]
+    ptrs = (f64p * len(keep))(*[_p(m, C.c_double) for m in keep])
+    out = np.zeros((coo.dims[mode], R), dtype=np.float64)
+    lib().or_mttkrp_stream(coo.ptr, ptrs, C.c_uint64(R), C.c_uint64(mode), _p(out, C.c_double))
+    return out
+
+
+def cpd_error(coo, factors, lam):
+    R = factors[0].shape[1]
+    keep = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    ptrs = (f64p * len(keep))(*[_p(f, C.c_double) for f in keep])
+    fd = np.array([f.shape[0] for f in keep], dtype=np.uint64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    L = lib()
+    L.or_cpd_error.restype = C.c_double
+    return L.or_cpd_error(coo.ptr, ptrs, _p(fd, C.c_uint64), C.c_uint64(R), _p(lam, C.c_double))
+
+
+def stream_prepare(coo, stream_mode):
+    """or_stream_prepare in place; returns iperm per mode (new -> original)."""
+    N = coo.nmodes
+    ip = [np.zeros(max(d, 1), dtype=np.uint64) for d in coo.dims]
+    arr = (u64p * MAXM)(*([_p(a, C.c_uint64) for a in ip] + [None] * (MAXM - N)))
+    lib().or_stream_prepare(coo.ptr, C.c_uint64(stream_mode), arr)
+    return [a[:d] for a, d in zip(ip, coo.dims)]
+
+
+def stream_cpd(coo, stream_mode, rank, forget, max_its, cons):
+    """or_stream_cpd on a prepared tensor (caller seeds rand()). cons: list of
+    (kind, prox, diag_add) per mode. Returns a dict of numpy results."""
+    N = coo.nmodes
+    dims = coo.dims
+    T = dims[stream_mode]
+    cfg = _OrStreamCfg()
+    cfg.stream_mode, cfg.rank, cfg.forget, cfg.max_its = stream_mode, rank, forget, max_its
+    for m, (k, p, d) in enumerate(cons):
+        cfg.cons[m].kind, cfg.cons[m].prox, cfg.cons[m].diag_add = k, p, d
+    time = np.zeros((max(T, 1), rank), dtype=np.float64)
+    facs = [np.zeros((max(d, 1), rank), dtype=np.float64) for d in dims]
+    lerr = np.zeros(max(T, 1), dtype=np.float64)
+    oits = np.zeros(max(T, 1), dtype=np.uint64)
+    out = _OrStreamOut()
+    out.max_batches = T
+    out.time = _p(time, C.c_double)
+    for m in range(N):
+        if m != stream_mode:
+            out.factors[m] = _p(facs[m], C.c_double)
+    out.local_err = _p(lerr, C.c_double)
+    out.outer_its = _p(oits, C.c_uint64)
+    lib().or_stream_cpd(coo.ptr, C.byref(cfg), C.byref(out))
+    nb = int(out.nbatches)
+    fd = [int(out.dims[m]) for m in range(N)]
+    return dict(time=time[:nb], factors=[f[:fd[m]] for m, f in enumerate(facs)],
+                local_err=lerr[:nb], outer_its=oits[:nb], final_err=float(out.final_err),
+                nbatches=nb, dims=fd)

@@ -53,6 +53,11 @@ int spd_coo_upload(int nmodes, uint64_t nnz, uint64_t const * dims,
     uint64_t const * const * inds, double const * vals, spd_coo ** out);
 void spd_coo_free(spd_coo * coo);
 
+/* Lexicographic order of the coordinates with perm[0] most significant:
+ * order_host[k] (n entries, host) = position of the k-th smallest. Equal
+ * coordinates keep their input order. */
+int spd_coo_lexsort(spd_coo const * coo, int const * perm, uint32_t * order_host);
+
 /* ---- CSF --------------------------------------------------------------- */
 typedef struct spd_csf spd_csf;
 

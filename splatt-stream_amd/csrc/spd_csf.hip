@@ -369,6 +369,76 @@ static int compress(CsfArrays * a, DevPtrs32 lind, double * svals, uint64_t n,
   return rc;
 }
 
+// Lexicographic order of n coordinates under the level order given by
+// lind64 (level l -> index array) with per-level sizes dims_l: the stable
+// LSD radix sort over one or more packed keys. On return *order points at
+// the permutation (one of v0/v1); k0/k1/v0/v1 are n-element work buffers.
+static int lex_order(DevPtrs const & lind64, uint64_t const * dims_l, int N, uint64_t n,
+    uint64_t * k0, uint64_t * k1, uint32_t * v0, uint32_t * v1, uint32_t ** order_out,
+    hipStream_t st)
+{
+  unsigned grid = (unsigned) (spd_cdiv(n, 256) < 16384 ? spd_cdiv(n, 256) : 16384);
+  int rc = SPD_OK;
+  std::vector<KeySpec> groups;
+  {
+    int packed_bits = 0;
+    unsigned __int128 prod = 1;
+    int bits_l[SPD_MAX_MODES];
+    for(int l=0; l < N; ++l) {
+      uint64_t const d = dims_l[l] ? dims_l[l] : 1;
+      bits_l[l] = bits_for(d);
+      packed_bits += bits_l[l];
+      if(prod <= ((unsigned __int128) 1 << 64)) prod *= d;
+    }
+    if(packed_bits <= 64) {
+      KeySpec ks = {};
+      ks.nlev = N;
+      for(int l=0; l < N; ++l) { ks.lev[l] = l; ks.mul[l] = 1ull << bits_l[l]; }
+      ks.bits = packed_bits;
+      groups.push_back(ks);
+    } else if(prod < ((unsigned __int128) 1 << 64)) {
+      KeySpec ks = {};
+      ks.nlev = N;
+      for(int l=0; l < N; ++l) { ks.lev[l] = l; ks.mul[l] = dims_l[l]; }
+      ks.bits = bits_for(prod);
+      groups.push_back(ks);
+    } else {
+      int l = N - 1;
+      while(l >= 0) {
+        int acc = 0;
+        int first = l;
+        while(first >= 0 && acc + bits_l[first] <= 64) { acc += bits_l[first]; --first; }
+        if(first == l) { acc = bits_l[l]; first = l - 1; } // a single level wider than 64 bits cannot happen
+        KeySpec ks = {};
+        ks.nlev = l - first;
+        for(int q=0; q < ks.nlev; ++q) {
+          ks.lev[q] = first + 1 + q;
+          ks.mul[q] = 1ull << bits_l[first + 1 + q];
+        }
+        ks.bits = acc;
+        groups.push_back(ks);
+        l = first;
+      }
+    }
+  }
+  uint32_t * order = nullptr;   // current permutation (device)
+  for(size_t g=0; g < groups.size() && rc == SPD_OK; ++g) {
+    KeySpec ks = groups[g];
+    hipLaunchKernelGGL(k_make_key, dim3(grid), dim3(256), 0, st, ks, lind64, n,
+        order, k0, v0);
+    if(hipGetLastError() != hipSuccess) { rc = SPD_EHIP; break; }
+    int which = 0;
+    rc = spd_radix_sort(k0, v0, k1, v1, n, ks.bits, &which, st);
+    if(rc != SPD_OK) break;
+    if(which) { uint64_t * t = k0; k0 = k1; k1 = t; uint32_t * tv = v0; v0 = v1; v1 = tv; }
+    // v0 holds the permutation; the next group writes its payload into v1
+    order = v0;
+    uint32_t * t = v0; v0 = v1; v1 = t;
+  }
+  *order_out = order;
+  return rc;
+}
+
 extern "C" int spd_csf_build(spd_coo const * coo, int const * perm,
     uint64_t const * tile_dims, spd_csf ** out)
 {
@@ -406,63 +476,11 @@ extern "C" int spd_csf_build(spd_coo const * coo, int const * perm,
   if(rc == SPD_OK) rc = spd_malloc((void **) &v1, n * 4);
   unsigned grid = (unsigned) (spd_cdiv(n, 256) < 16384 ? spd_cdiv(n, 256) : 16384);
 
-  // key groups, least significant first
-  std::vector<KeySpec> groups;
+  uint32_t * order = nullptr;
   {
-    int packed_bits = 0;
-    unsigned __int128 prod = 1;
-    int bits_l[SPD_MAX_MODES];
-    for(int l=0; l < N; ++l) {
-      uint64_t const d = c->dims[perm[l]] ? c->dims[perm[l]] : 1;
-      bits_l[l] = bits_for(d);
-      packed_bits += bits_l[l];
-      if(prod <= ((unsigned __int128) 1 << 64)) prod *= d;
-    }
-    if(packed_bits <= 64) {
-      KeySpec ks = {};
-      ks.nlev = N;
-      for(int l=0; l < N; ++l) { ks.lev[l] = l; ks.mul[l] = 1ull << bits_l[l]; }
-      ks.bits = packed_bits;
-      groups.push_back(ks);
-    } else if(prod < ((unsigned __int128) 1 << 64)) {
-      KeySpec ks = {};
-      ks.nlev = N;
-      for(int l=0; l < N; ++l) { ks.lev[l] = l; ks.mul[l] = c->dims[perm[l]]; }
-      ks.bits = bits_for(prod);
-      groups.push_back(ks);
-    } else {
-      int l = N - 1;
-      while(l >= 0) {
-        int acc = 0;
-        int first = l;
-        while(first >= 0 && acc + bits_l[first] <= 64) { acc += bits_l[first]; --first; }
-        if(first == l) { acc = bits_l[l]; first = l - 1; } // a single level wider than 64 bits cannot happen
-        KeySpec ks = {};
-        ks.nlev = l - first;
-        for(int q=0; q < ks.nlev; ++q) {
-          ks.lev[q] = first + 1 + q;
-          ks.mul[q] = 1ull << bits_l[first + 1 + q];
-        }
-        ks.bits = acc;
-        groups.push_back(ks);
-        l = first;
-      }
-    }
-  }
-  uint32_t * order = nullptr;   // current permutation (device)
-  for(size_t g=0; g < groups.size() && rc == SPD_OK; ++g) {
-    KeySpec ks = groups[g];
-    // express levels through the level-ordered pointers
-    hipLaunchKernelGGL(k_make_key, dim3(grid), dim3(256), 0, st, ks, lind64, n,
-        order, k0, v0);
-    if(hipGetLastError() != hipSuccess) { rc = SPD_EHIP; break; }
-    int which = 0;
-    rc = spd_radix_sort(k0, v0, k1, v1, n, ks.bits, &which, st);
-    if(rc != SPD_OK) break;
-    if(which) { uint64_t * t = k0; k0 = k1; k1 = t; uint32_t * tv = v0; v0 = v1; v1 = tv; }
-    // v0 holds the permutation; keep it in v1's partner slot for next group
-    order = v0;
-    uint32_t * t = v0; v0 = v1; v1 = t;   // next group writes payload into the other
+    uint64_t dl[SPD_MAX_MODES];
+    for(int l=0; l < N; ++l) dl[l] = c->dims[perm[l]];
+    if(rc == SPD_OK) rc = lex_order(lind64, dl, N, n, k0, k1, v0, v1, &order, st);
   }
   // canonical order is in `order`
   DevPtrs32 lind = {};
@@ -659,4 +677,30 @@ extern "C" int spd_csf_frobsq(spd_csf const * c, double * out)
   for(unsigned b=0; b < nb; ++b) s += h[b];
   *out = s;
   return SPD_OK;
+}
+
+extern "C" int spd_coo_lexsort(spd_coo const * coo, int const * perm, uint32_t * order_host)
+{
+  SPD_TRY(spd_init());
+  hipStream_t st = spd_hs(nullptr);
+  int const N = coo->nmodes;
+  uint64_t const n = coo->nnz;
+  if(n == 0) return SPD_OK;
+  DevPtrs lind64 = {};
+  uint64_t dl[SPD_MAX_MODES];
+  for(int l=0; l < N; ++l) {
+    lind64.p[l] = coo->ind[perm[l]];
+    dl[l] = coo->dims[perm[l]];
+  }
+  uint64_t * k0 = nullptr; uint64_t * k1 = nullptr;
+  uint32_t * v0 = nullptr; uint32_t * v1 = nullptr;
+  int rc = spd_malloc((void **) &k0, n * 8);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &k1, n * 8);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &v0, n * 4);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &v1, n * 4);
+  uint32_t * order = nullptr;
+  if(rc == SPD_OK) rc = lex_order(lind64, dl, N, n, k0, k1, v0, v1, &order, st);
+  if(rc == SPD_OK) rc = spd_d2h(order_host, order, n * sizeof(uint32_t));
+  spd_free(k0); spd_free(k1); spd_free(v0); spd_free(v1);
+  return rc;
 }
