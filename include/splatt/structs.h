@@ -135,6 +135,15 @@ void splatt_tt_free(sptensor_t * tt);
 splatt_idx_t splatt_tt_remove_empty(sptensor_t * const tt);
 splatt_val_t splatt_tt_normsq(sptensor_t const * const tt);
 
+/* src/cpd/stream.h:24-30 (src/cpd/stream/stream.cc:48): the entry behind
+ * `splatt cpd --stream=M`. Reads the tensor, runs the streaming CPD on the
+ * GPU, prints the reference's summary lines and writes mode<N>.mat (rows at
+ * their original indices) into the working directory. */
+splatt_error_type splatt_cpd_stream(char const * const filename,
+    splatt_idx_t rank, splatt_idx_t const stream_mode, double const forget,
+    splatt_cpd_opts * const cpd_options,
+    splatt_global_opts const * const global_options);
+
 #ifdef __cplusplus
 }
 #endif

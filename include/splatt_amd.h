@@ -60,6 +60,38 @@ void splatt_amd_mttkrp_timing(int enable);
 double splatt_amd_mttkrp_last_ms(void);
 int splatt_amd_mttkrp_kernel_times(double * ms, int max);
 
+/* Streaming CPD on a tensor in memory: splatt_cpd_stream without the file
+ * and the output files. The stream is the tensor sorted by stream_mode (its
+ * index is the time slice); the other modes are relabelled by first
+ * appearance, so row i of factors[m] belongs to original index iperm[m][i].
+ * cpd_options is modified as the reference modifies it (max-column-norm
+ * constraints on the other modes, stream tolerances). Returns NULL on
+ * error. */
+typedef struct
+{
+  splatt_idx_t nmodes;
+  splatt_idx_t rank;
+  splatt_idx_t stream_mode;
+  splatt_idx_t nbatches;                      /* time slices processed */
+  splatt_idx_t dims[SPLATT_MAX_NMODES];       /* rows of factors[m] (time: nbatches) */
+  splatt_val_t * factors[SPLATT_MAX_NMODES];  /* time mode: one row per slice */
+  splatt_idx_t * iperm[SPLATT_MAX_NMODES];    /* relabelled row -> original index */
+  double * local_err;                         /* per slice: squared error of the slice */
+  double * batch_seconds;                     /* per slice: wall time, device synchronised */
+  splatt_idx_t * outer_its;                   /* per slice: ALS iterations run */
+  double final_err;                           /* squared error of the whole stream */
+} splatt_amd_stream_result;
+
+splatt_amd_stream_result * splatt_amd_cpd_stream_coo(
+    struct _splatt_coord const * const tt,
+    splatt_idx_t rank,
+    splatt_idx_t stream_mode,
+    double forget,
+    splatt_cpd_opts * const cpd_options,
+    splatt_global_opts const * const global_options);
+
+void splatt_amd_free_stream_result(splatt_amd_stream_result * result);
+
 /* Device memory helpers for callers that hold device-resident factors. */
 void * splatt_amd_dev_alloc(size_t bytes);
 void splatt_amd_dev_free(void * ptr);

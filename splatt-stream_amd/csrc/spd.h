@@ -176,6 +176,38 @@ int spd_cpd_als(spd_csf * const * csfs, int const * mode_csf,
     int const * rows_pos, spd_cpd_cfg const * cfg, double * const * factors,
     double * lambda, double * fits, uint64_t * its, double * fit);
 
+/* ---- streaming CPD (src/cpd/stream/StreamCPD.cc:311-571) ------------- */
+
+typedef struct {
+  int stream_mode;
+  uint64_t rank;
+  double forget;
+  uint64_t max_its;        /* outer ALS iterations per slice */
+  double tol;              /* outer tolerance on the change measure */
+  uint64_t max_inner;      /* ADMM */
+  double inner_tol;
+  int verbosity;
+  spd_con con[SPD_MAX_MODES];   /* streamed mode: closed form only */
+  void (*fill_rand)(double * vals, uint64_t n);   /* new factor rows */
+} spd_stream_cfg;
+
+typedef struct {
+  uint64_t nbatches;
+  uint64_t dims[SPD_MAX_MODES];      /* rows of each result (time: nbatches) */
+  double * factors[SPD_MAX_MODES];   /* host buffers sized by the caller, or NULL */
+  double * local_err;                /* per slice, squared; NULL to skip */
+  double * batch_seconds;            /* per slice wall time; NULL to skip */
+  uint64_t * outer_its;              /* per slice; NULL to skip */
+  double * final_err;                /* squared; NULL to skip */
+} spd_stream_result;
+
+/* The stream: N-mode coordinates (host, uint32) already sorted by time
+ * (inds[stream_mode], the slice index) and relabelled, values (host), the
+ * tensor's dims. Runs every slice on the GPU. */
+int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
+    uint32_t const * const * inds, double const * vals, spd_stream_cfg const * cfg,
+    spd_stream_result * res);
+
 #ifdef __cplusplus
 }
 #endif

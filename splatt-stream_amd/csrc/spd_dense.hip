@@ -46,11 +46,11 @@ template<typename T> __device__ __forceinline__ double ld_as_f64(T const * p, ui
 
 // ---- Gram on MFMA (R <= 64) -------------------------------------------------
 
-template<int TT, bool DIV>
+template<int TT, bool DIV, bool CROSS>
 __global__ void __launch_bounds__(DN_THREADS)
 k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
     double const * __restrict__ div, float * __restrict__ X32,
-    double * __restrict__ part)
+    double * __restrict__ part, double const * __restrict__ Y)
 {
   constexpr int RP = TT * 16;
   __shared__ double s_acc[RP * RP];
@@ -78,6 +78,7 @@ k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
   uint64_t const stride = (uint64_t) gridDim.x * 4 * GR_U;
   for(uint64_t s0 = ((uint64_t) blockIdx.x * 4 + wave) * GR_U; s0 < nsteps; s0 += stride) {
     double a[GR_U][TT];
+    double b[CROSS ? GR_U : 1][TT];
     #pragma unroll
     for(int u=0; u < GR_U; ++u) {
       uint64_t const row = (s0 + u) * 4 + kk;
@@ -85,6 +86,7 @@ k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
       for(int t=0; t < TT; ++t) {
         bool const ok = (row < I) && cok[t];
         a[u][t] = ok ? X[row * R + t * 16 + cc] : 0.;
+        if(CROSS) b[CROSS ? u : 0][t] = ok ? Y[row * R + t * 16 + cc] : 0.;
       }
     }
     if(DIV || X32 != nullptr) {
@@ -111,7 +113,9 @@ k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
       #pragma unroll
       for(int i=0; i < TT; ++i) {
         #pragma unroll
-        for(int j=i; j < TT; ++j) acc[i][j] = mfma16(a[u][i], a[u][j], acc[i][j]);
+        for(int j=(CROSS ? 0 : i); j < TT; ++j) {
+          acc[i][j] = mfma16(a[u][i], CROSS ? b[CROSS ? u : 0][j] : a[u][j], acc[i][j]);
+        }
       }
     }
   }
@@ -122,7 +126,7 @@ k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
       #pragma unroll
       for(int i=0; i < TT; ++i) {
         #pragma unroll
-        for(int j=i; j < TT; ++j) {
+        for(int j=(CROSS ? 0 : i); j < TT; ++j) {
           #pragma unroll
           for(int r=0; r < 4; ++r) {
             s_acc[(i * 16 + kk + 4 * r) * RP + j * 16 + cc] += acc[i][j][r];
@@ -136,16 +140,17 @@ k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
   for(int e=threadIdx.x; e < RP * RP; e += DN_THREADS) out[e] = s_acc[e];
 }
 
-// Sum of the block partials (upper triangle), mirrored into the full R x R.
+// Sum of the block partials into the full R x R: the upper triangle
+// mirrored (sym) or every entry as computed (cross products).
 __global__ void k_gram_reduce(double const * __restrict__ part, uint32_t nb,
-    uint32_t R, uint32_t ld, double * __restrict__ out)
+    uint32_t R, uint32_t ld, double * __restrict__ out, int sym)
 {
   uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
   if(e >= R * R) return;
   uint32_t const i = e / R;
   uint32_t const j = e - i * R;
-  uint32_t const a = i < j ? i : j;
-  uint32_t const b = i < j ? j : i;
+  uint32_t const a = (sym && j < i) ? j : i;
+  uint32_t const b = (sym && j < i) ? i : j;
   double s = 0.;
   for(uint32_t k=0; k < nb; ++k) s += part[(uint64_t) k * ld * ld + a * ld + b];
   out[e] = s;
@@ -153,20 +158,22 @@ __global__ void k_gram_reduce(double const * __restrict__ part, uint32_t nb,
 
 // ---- plain kernels for any rank -----------------------------------------------
 
-// Gram block partials for ranks above 64: entry (i <= j) of the block's rows.
+// Gram block partials for ranks above 64: entry (i, j) of X^T Y over the
+// block's rows (Y == X: upper triangle only).
 __global__ void __launch_bounds__(DN_THREADS)
-k_gram_plain(double const * __restrict__ X, uint64_t I, uint32_t R,
-    uint64_t rows_per_block, double * __restrict__ part)
+k_gram_plain(double const * __restrict__ X, double const * __restrict__ Y,
+    uint64_t I, uint32_t R, uint64_t rows_per_block, double * __restrict__ part)
 {
   uint64_t const r0 = (uint64_t) blockIdx.x * rows_per_block;
   uint64_t const r1 = (r0 + rows_per_block < I) ? r0 + rows_per_block : I;
   double * const out = part + (uint64_t) blockIdx.x * R * R;
+  bool const sym = (X == Y);
   for(uint32_t e=threadIdx.x; e < R * R; e += DN_THREADS) {
     uint32_t const i = e / R;
     uint32_t const j = e - i * R;
-    if(i > j) continue;
+    if(sym && i > j) continue;
     double s = 0.;
-    for(uint64_t r=r0; r < r1; ++r) s += X[r * R + i] * X[r * R + j];
+    for(uint64_t r=r0; r < r1; ++r) s += X[r * R + i] * Y[r * R + j];
     out[e] = s;
   }
 }
@@ -402,7 +409,7 @@ k_apply_mfma(ApplyArgs<TM> a)
         double v = 0.;
         if(row < I && (uint32_t) lc < R) {
           uint64_t const idx = row * R + lc;
-          if(KIND == 0) {
+          if(KIND != 1) {
             v = (double) a.M[idx];
           } else {
             double const p = a.X[idx];
@@ -439,6 +446,8 @@ k_apply_mfma(ApplyArgs<TM> a)
           if(KIND == 0) {
             a.X[idx] = x;
             cs[t] += x * x;
+          } else if(KIND == 2) {
+            a.X[idx] += x;
           } else {
             a.aux[idx] = x;
             double const p = x - a.dual[idx];
@@ -483,6 +492,23 @@ __global__ void k_apply_plain(TM const * __restrict__ In, double const * __restr
     double s = 0.;
     for(uint32_t k=0; k < R; ++k) s += (double) In[i * R + k] * ginv[(uint64_t) k * ld + j];
     out[e] = s;
+  }
+}
+
+// Plain accumulate for ranks above 64: out[i,j] += sum_k In[i,k] G[k,j].
+__global__ void k_apply_plain_acc(double const * __restrict__ In,
+    double const * __restrict__ G, uint64_t I, uint32_t R, uint32_t ld,
+    double * __restrict__ out)
+{
+  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const n = I * R;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  for(; e < n; e += stride) {
+    uint64_t const i = e / R;
+    uint32_t const j = (uint32_t) (e - i * R);
+    double s = 0.;
+    for(uint32_t k=0; k < R; ++k) s += In[i * R + k] * G[(uint64_t) k * ld + j];
+    out[e] += s;
   }
 }
 
@@ -602,14 +628,17 @@ __global__ void k_cpd_norm(GramSrc g, int nmodes, uint32_t R,
 
 template<int TT>
 static int launch_gram_mfma(double * X, uint64_t I, uint32_t R, double const * div,
-    float * X32, double * part, uint32_t nb, hipStream_t st)
+    float * X32, double * part, uint32_t nb, hipStream_t st, double const * Y)
 {
-  if(div) {
-    hipLaunchKernelGGL((k_gram_mfma<TT, true>), dim3(nb), dim3(DN_THREADS), 0, st,
-        X, I, R, div, X32, part);
+  if(Y) {
+    hipLaunchKernelGGL((k_gram_mfma<TT, false, true>), dim3(nb), dim3(DN_THREADS), 0, st,
+        X, I, R, div, X32, part, Y);
+  } else if(div) {
+    hipLaunchKernelGGL((k_gram_mfma<TT, true, false>), dim3(nb), dim3(DN_THREADS), 0, st,
+        X, I, R, div, X32, part, Y);
   } else {
-    hipLaunchKernelGGL((k_gram_mfma<TT, false>), dim3(nb), dim3(DN_THREADS), 0, st,
-        X, I, R, div, X32, part);
+    hipLaunchKernelGGL((k_gram_mfma<TT, false, false>), dim3(nb), dim3(DN_THREADS), 0, st,
+        X, I, R, div, X32, part, Y);
   }
   SPD_LAUNCH_CHECK();
   return SPD_OK;
@@ -657,21 +686,22 @@ int dn_innerprod(double const * X, void const * M, int mdtype, uint64_t I,
   return coldot<double, false>(X, (double const *) M, I, R, 1, lambda, out, st);
 }
 
-int dn_gram(double * X, uint64_t I, uint32_t R, double const * div, float * X32,
-    double * out, hipStream_t st)
+static int gram_impl(double * X, double const * Y, uint64_t I, uint32_t R,
+    double const * div, float * X32, double * out, hipStream_t st)
 {
+  int const sym = (Y == nullptr);
   if(R > 64) {
-    // plain path: divide / copy first, then a column-pair reduction per entry
+    // plain path: divide / copy first, then one entry per thread
     if(div || X32) SPD_TRY(dn_div_cols(X, I, R, div, X32, st));
     uint32_t const nb = dn_blocks(I, 64);
     uint64_t const rpb = spd_cdiv(I > 0 ? I : 1, nb);
     void * part = nullptr;
     SPD_TRY(spd_scratch(SLOT_GRAM, (size_t) nb * R * R * sizeof(double), &part));
     hipLaunchKernelGGL(k_gram_plain, dim3(nb), dim3(DN_THREADS), 0, st,
-        (double const *) X, I, R, rpb, (double *) part);
+        (double const *) X, sym ? (double const *) X : Y, I, R, rpb, (double *) part);
     SPD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 256)),
-        dim3(256), 0, st, (double const *) part, nb, R, R, out);
+        dim3(256), 0, st, (double const *) part, nb, R, R, out, sym);
     SPD_LAUNCH_CHECK();
     return SPD_OK;
   }
@@ -682,15 +712,27 @@ int dn_gram(double * X, uint64_t I, uint32_t R, double const * div, float * X32,
   void * part = nullptr;
   SPD_TRY(spd_scratch(SLOT_GRAM, (size_t) nb * RP * RP * sizeof(double), &part));
   switch(TT) {
-  case 1: SPD_TRY(launch_gram_mfma<1>(X, I, R, div, X32, (double *) part, nb, st)); break;
-  case 2: SPD_TRY(launch_gram_mfma<2>(X, I, R, div, X32, (double *) part, nb, st)); break;
-  case 3: SPD_TRY(launch_gram_mfma<3>(X, I, R, div, X32, (double *) part, nb, st)); break;
-  default: SPD_TRY(launch_gram_mfma<4>(X, I, R, div, X32, (double *) part, nb, st)); break;
+  case 1: SPD_TRY(launch_gram_mfma<1>(X, I, R, div, X32, (double *) part, nb, st, Y)); break;
+  case 2: SPD_TRY(launch_gram_mfma<2>(X, I, R, div, X32, (double *) part, nb, st, Y)); break;
+  case 3: SPD_TRY(launch_gram_mfma<3>(X, I, R, div, X32, (double *) part, nb, st, Y)); break;
+  default: SPD_TRY(launch_gram_mfma<4>(X, I, R, div, X32, (double *) part, nb, st, Y)); break;
   }
   hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 256)),
-      dim3(256), 0, st, (double const *) part, nb, R, RP, out);
+      dim3(256), 0, st, (double const *) part, nb, R, RP, out, sym);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
+}
+
+int dn_gram(double * X, uint64_t I, uint32_t R, double const * div, float * X32,
+    double * out, hipStream_t st)
+{
+  return gram_impl(X, nullptr, I, R, div, X32, out, st);
+}
+
+int dn_xgram(double const * X, double const * Y, uint64_t I, uint32_t R, double * out,
+    hipStream_t st)
+{
+  return gram_impl((double *) X, Y, I, R, nullptr, nullptr, out, st);
 }
 
 int dn_chol_inv(double const * const * aTa, int nmodes, int mode, uint32_t R,
@@ -773,6 +815,27 @@ static int apply_t(TM const * M, double const * ginv, uint64_t I, uint32_t R,
     SPD_LAUNCH_CHECK();
   }
   return SPD_OK;
+}
+
+int dn_apply_acc(double const * In, double const * G, uint64_t I, uint32_t R,
+    double * out, hipStream_t st)
+{
+  if(I == 0) return SPD_OK;
+  uint32_t const RP = dn_rpad(R);
+  if(R > 64) {
+    hipLaunchKernelGGL(k_apply_plain_acc, dim3(dn_blocks(I * R, DN_THREADS * 4)),
+        dim3(DN_THREADS), 0, st, In, G, I, R, RP, out);
+    SPD_LAUNCH_CHECK();
+    return SPD_OK;
+  }
+  ApplyArgs<double> a = {};
+  a.M = In;
+  a.ginv = G;
+  a.I = I;
+  a.R = R;
+  a.X = out;
+  a.colsq = nullptr;
+  return dispatch_apply<double, 2>(a, RP / 16, apply_blocks(I), st);
 }
 
 int dn_apply(void const * M, int mdtype, double const * ginv, uint64_t I,

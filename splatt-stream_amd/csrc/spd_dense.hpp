@@ -48,6 +48,15 @@ int dn_admm_step(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint32_t R, double * P, double * aux,
     double * D, double * init, int prox, double * sums, hipStream_t st);
 
+// out (R x R, full) = X^T Y for two I x R matrices.
+int dn_xgram(double const * X, double const * Y, uint64_t I, uint32_t R,
+    double * out, hipStream_t st);
+
+// out += In G for In (I x R) and a general G (RP x RP, zero padded, RP =
+// R rounded up to 16).
+int dn_apply_acc(double const * In, double const * G, uint64_t I, uint32_t R,
+    double * out, hipStream_t st);
+
 // lam[r] = ||X[:,r]||_2.
 int dn_colnorm(double const * X, uint64_t I, uint32_t R, double * lam,
     hipStream_t st);

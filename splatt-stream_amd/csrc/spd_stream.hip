@@ -1,0 +1,627 @@
+// spd_stream.hip -- streaming CPD on the GPU: the reference's
+// StreamCPD::compute (src/cpd/stream/StreamCPD.cc:311-571) over a tensor
+// that is resident in HBM for the whole stream.
+//
+// The whole stream (sorted by time, other modes relabelled by first
+// appearance) is uploaded once; a time slice is a contiguous range of it, so
+// a batch costs no copy. Factor matrices are allocated at their final row
+// counts up front; "growing" a factor only activates rows (filled with the
+// reference's rand() sequence on the host) and zero rows of the history and
+// dual matrices stay zero. Per slice everything runs on the device: the COO
+// MTTKRP (src/mttkrp.c:1699-1769: atomics for scattered rows, fixed-order
+// block partials for the single time row), the closed-form time solve, the
+// time Gram update, the history term (cross Grams on MFMA and a product
+// with the old factor), ADMM with the max-column-norm prox, the Grams and
+// the change measure. The host reads the ADMM and outer convergence scalars.
+#include "spd_internal.hpp"
+#include "spd_dense.hpp"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <vector>
+
+#define CO_THREADS 256
+
+struct CooArgs {
+  uint32_t const * ind[SPD_MAX_MODES];
+  double const * vals;
+  double const * mats[SPD_MAX_MODES];
+  double * out;        // rows > 1: atomic target
+  double * part;       // single row: per-block partial rows
+  uint64_t nnz;
+  uint32_t R;
+  int N;
+  int mode;
+  int reduce;
+};
+
+// COO MTTKRP: per nonzero v * prod_{m != mode} U_m[ind_m] (modes ascending,
+// as the reference multiplies), added to the output row. L lanes per
+// nonzero, NC column chunks of L.
+template<int L, int NC>
+__global__ void __launch_bounds__(CO_THREADS)
+k_coo_mttkrp(CooArgs a)
+{
+  constexpr int G = 64 / L;
+  constexpr int W = CO_THREADS / 64;
+  __shared__ double s_red[W * G][L * NC];
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const c = lane % L;
+  int const g = lane / L;
+  uint32_t const R = a.R;
+  uint64_t const group = ((uint64_t) blockIdx.x * W + wave) * G + g;
+  uint64_t const ngroups = (uint64_t) gridDim.x * W * G;
+  double acc[NC];
+  #pragma unroll
+  for(int k=0; k < NC; ++k) acc[k] = 0.;
+  for(uint64_t n=group; n < a.nnz; n += ngroups) {
+    double const v = a.vals[n];
+    double p[NC];
+    #pragma unroll
+    for(int k=0; k < NC; ++k) p[k] = v;
+    #pragma unroll
+    for(int m=0; m < SPD_MAX_MODES; ++m) {
+      if(m >= a.N) break;
+      if(m == a.mode) continue;
+      double const * row = a.mats[m] + (uint64_t) a.ind[m][n] * R;
+      #pragma unroll
+      for(int k=0; k < NC; ++k) {
+        uint32_t const col = c + k * L;
+        if(col < R) p[k] *= row[col];
+      }
+    }
+    if(a.reduce) {
+      #pragma unroll
+      for(int k=0; k < NC; ++k) acc[k] += p[k];
+    } else {
+      double * orow = a.out + (uint64_t) a.ind[a.mode][n] * R;
+      #pragma unroll
+      for(int k=0; k < NC; ++k) {
+        uint32_t const col = c + k * L;
+        if(col < R) unsafeAtomicAdd(orow + col, p[k]);
+      }
+    }
+  }
+  if(a.reduce) {
+    #pragma unroll
+    for(int k=0; k < NC; ++k) s_red[wave * G + g][c + k * L] = acc[k];
+    __syncthreads();
+    for(uint32_t col=threadIdx.x; col < R; col += CO_THREADS) {
+      double s = 0.;
+      for(int q=0; q < W * G; ++q) s += s_red[q][col];
+      a.part[(uint64_t) blockIdx.x * R + col] = s;
+    }
+  }
+}
+
+// out[c] (+)= sum_b part[b * R + c], fixed order.
+__global__ void k_rowsum(double const * __restrict__ part, uint32_t nb, uint32_t R,
+    double * __restrict__ out)
+{
+  uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
+  if(c >= R) return;
+  double s = 0.;
+  for(uint32_t b=0; b < nb; ++b) s += part[(uint64_t) b * R + c];
+  out[c] = s;
+}
+
+// Time Gram: symmetric copy of the upper triangle into old, then
+// A = old + t t^T (StreamCPD.cc:420-435). One block.
+__global__ void k_time_gram(double * __restrict__ A, double const * __restrict__ t,
+    double * __restrict__ old, uint32_t R)
+{
+  for(uint32_t e=threadIdx.x; e < R * R; e += blockDim.x) {
+    uint32_t const i = e / R;
+    uint32_t const j = e - i * R;
+    old[e] = (i > j) ? A[j * R + i] : A[e];
+  }
+  __syncthreads();
+  for(uint32_t e=threadIdx.x; e < R * R; e += blockDim.x) {
+    uint32_t const i = e / R;
+    uint32_t const j = e - i * R;
+    A[e] = old[e] + t[i] * t[j];
+  }
+}
+
+struct HistSrc { double const * m[SPD_MAX_MODES]; };
+
+// H (RP x RP, zero padded) = old_gram o cross_0 o cross_1 ...
+__global__ void k_hist(double const * __restrict__ old, HistSrc cr, int ncross,
+    uint32_t R, uint32_t RP, double * __restrict__ H)
+{
+  for(uint32_t e=threadIdx.x; e < RP * RP; e += blockDim.x) {
+    uint32_t const i = e / RP;
+    uint32_t const j = e - i * RP;
+    double v = 0.;
+    if(i < R && j < R) {
+      v = old[i * R + j];
+      for(int k=0; k < ncross; ++k) v *= cr.m[k][i * R + j];
+    }
+    H[e] = v;
+  }
+}
+
+__global__ void k_scale(double * __restrict__ x, uint64_t n, double f)
+{
+  uint64_t const e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if(e < n) x[e] *= f;
+}
+
+// Block partials of sum (a-b)^2 and sum b^2.
+__global__ void __launch_bounds__(256)
+k_diff2(double const * __restrict__ a, double const * __restrict__ b, uint64_t n,
+    double * __restrict__ part)
+{
+  __shared__ double s[2][256];
+  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  double d2 = 0., b2 = 0.;
+  for(; e < n; e += stride) {
+    double const x = a[e] - b[e];
+    d2 += x * x;
+    b2 += b[e] * b[e];
+  }
+  int const t = threadIdx.x;
+  s[0][t] = d2;
+  s[1][t] = b2;
+  __syncthreads();
+  for(int h=128; h > 0; h >>= 1) {
+    if(t < h) {
+      s[0][t] += s[0][t + h];
+      s[1][t] += s[1][t + h];
+    }
+    __syncthreads();
+  }
+  if(t < 2) part[(uint64_t) blockIdx.x * 2 + t] = s[t][0];
+}
+
+__global__ void k_sum2(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
+{
+  int const k = threadIdx.x;
+  if(k < 2) {
+    double v = 0.;
+    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * 2 + k];
+    out[k] = v;
+  }
+}
+
+__global__ void k_fill_f64(double * __restrict__ x, uint64_t n, double v)
+{
+  uint64_t const e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if(e < n) x[e] = v;
+}
+
+namespace {
+
+struct Mem {
+  std::vector<void *> ptrs;
+  ~Mem() { for(void * p : ptrs) spd_free(p); }
+  int get(void ** out, size_t bytes) {
+    *out = nullptr;
+    if(bytes == 0) bytes = 8;
+    SPD_TRY(spd_malloc(out, bytes));
+    ptrs.push_back(*out);
+    return SPD_OK;
+  }
+};
+
+double wall_s()
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double) ts.tv_sec + 1e-9 * (double) ts.tv_nsec;
+}
+
+template<int L, int NC>
+int launch_coo(CooArgs const & a, uint32_t nb, hipStream_t st)
+{
+  hipLaunchKernelGGL((k_coo_mttkrp<L, NC>), dim3(nb), dim3(CO_THREADS), 0, st, a);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+// COO MTTKRP of nnz nonzeros (ind[m][x], vals[x]) into out (rows x R,
+// overwritten). With rows == 1 the output-mode indices are not read.
+int coo_mttkrp(uint32_t const * const * ind, double const * vals,
+    uint64_t nnz, int N, int mode, double const * const * mats, uint32_t R,
+    uint64_t rows, double * out, hipStream_t st)
+{
+  CooArgs a = {};
+  for(int m=0; m < N; ++m) {
+    a.ind[m] = ind[m];
+    a.mats[m] = mats[m];
+  }
+  a.vals = vals;
+  a.nnz = nnz;
+  a.R = R;
+  a.N = N;
+  a.mode = mode;
+  a.reduce = (rows == 1);
+  int L = 64, NC = 1;
+  if(R <= 16) { L = 16; NC = 1; }
+  else if(R <= 32) { L = 32; NC = 1; }
+  else if(R <= 64) { L = 64; NC = 1; }
+  else if(R <= 128) { L = 64; NC = 2; }
+  else if(R <= 256) { L = 64; NC = 4; }
+  else {
+    spd_set_error(SPD_EINPUT, "streaming MTTKRP supports rank <= 256 (got %u)", R);
+    return SPD_EINPUT;
+  }
+  uint64_t const per_block = (uint64_t) (CO_THREADS / L);
+  uint64_t nb64 = spd_cdiv(nnz > 0 ? nnz : 1, per_block * 4);
+  uint32_t const cap = a.reduce ? 256u : 8192u;
+  uint32_t const nb = (uint32_t) (nb64 < cap ? nb64 : cap);
+  void * part = nullptr;
+  if(a.reduce) {
+    SPD_TRY(spd_scratch(13, (size_t) nb * R * sizeof(double), &part));
+    a.part = (double *) part;
+  } else {
+    SPD_TRY(spd_zero_async(out, rows * R * sizeof(double), st));
+    a.out = out;
+  }
+  if(nnz > 0 || a.reduce) {
+    if(L == 16) SPD_TRY((launch_coo<16, 1>(a, nb, st)));
+    else if(L == 32) SPD_TRY((launch_coo<32, 1>(a, nb, st)));
+    else if(NC == 1) SPD_TRY((launch_coo<64, 1>(a, nb, st)));
+    else if(NC == 2) SPD_TRY((launch_coo<64, 2>(a, nb, st)));
+    else SPD_TRY((launch_coo<64, 4>(a, nb, st)));
+  }
+  if(a.reduce) {
+    hipLaunchKernelGGL(k_rowsum, dim3((unsigned) spd_cdiv(R, 256)), dim3(256), 0, st,
+        (double const *) part, nb, R, out);
+    SPD_LAUNCH_CHECK();
+  }
+  return SPD_OK;
+}
+
+// sum (a-b)^2 and sum b^2 over n entries -> out[0..1] (device)
+int diff2(double const * a, double const * b, uint64_t n, double * out, hipStream_t st)
+{
+  uint64_t nb64 = spd_cdiv(n > 0 ? n : 1, 256 * 8);
+  uint32_t const nb = (uint32_t) (nb64 < 1024 ? nb64 : 1024);
+  void * part = nullptr;
+  SPD_TRY(spd_scratch(14, (size_t) nb * 2 * sizeof(double), &part));
+  hipLaunchKernelGGL(k_diff2, dim3(nb), dim3(256), 0, st, a, b, n, (double *) part);
+  SPD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum2, dim3(1), dim3(64), 0, st, (double const *) part, nb, out);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+}  // namespace
+
+extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
+    uint32_t const * const * inds, double const * vals, spd_stream_cfg const * cfg,
+    spd_stream_result * res)
+{
+  SPD_TRY(spd_init());
+  hipStream_t const st = spd_hs(nullptr);
+  int const t = cfg->stream_mode;
+  uint64_t const R64 = cfg->rank;
+  if(N < 2 || N > SPD_MAX_MODES || t < 0 || t >= N || R64 == 0 || R64 > 256 ||
+      nnz == 0 || nnz >= (1ull << 32)) {
+    spd_set_error(SPD_EINPUT, "stream: bad shape (nmodes %d, stream mode %d, rank %llu,"
+        " nnz %llu)", N, t, (unsigned long long) R64, (unsigned long long) nnz);
+    return SPD_EINPUT;
+  }
+  for(int m=0; m < N; ++m) {
+    if(m != t && cfg->con[m].kind == SPD_CON_NONE) {
+      spd_set_error(SPD_EINPUT, "stream: mode %d needs a constraint", m);
+      return SPD_EINPUT;
+    }
+  }
+  if(cfg->con[t].kind == SPD_CON_ADMM) {
+    spd_set_error(SPD_EINPUT, "stream: the streamed mode takes closed-form "
+        "constraints only");
+    return SPD_EINPUT;
+  }
+  uint32_t const R = (uint32_t) R64;
+  uint32_t const RP = dn_rpad(R);
+
+  // ---- batches (ParserSimple::next_batch, ParserSimple.cc:89-149) ----
+  std::vector<uint64_t> bstart, bend;
+  std::vector<std::vector<uint64_t>> bdims;
+  {
+    uint64_t ptr = 0;
+    uint64_t prev[SPD_MAX_MODES] = {0};
+    uint64_t it = 0;
+    while(ptr < nnz) {
+      uint64_t s = ptr;
+      while(s < nnz && inds[t][s] < it) ++s;
+      uint64_t e = s;
+      while(e < nnz && inds[t][e] < it + 1) ++e;
+      if(e == s) {
+        fprintf(stderr, "SPLATT: time slice %llu is empty; the stream ends there.\n",
+            (unsigned long long) it);
+        break;
+      }
+      std::vector<uint64_t> d(N, 1);
+      for(int m=0; m < N; ++m) {
+        if(m == t) continue;
+        uint64_t mx = 0;
+        for(uint64_t x=s; x < e; ++x) if(inds[m][x] > mx) mx = inds[m][x];
+        d[m] = (mx + 1 > prev[m]) ? mx + 1 : prev[m];
+        prev[m] = d[m];
+      }
+      bstart.push_back(s);
+      bend.push_back(e);
+      bdims.push_back(d);
+      ptr = e;
+      ++it;
+    }
+  }
+  uint64_t const T = bstart.size();
+  uint64_t maxrows[SPD_MAX_MODES] = {0};
+  for(int m=0; m < N; ++m) {
+    maxrows[m] = (m == t) ? 1 : (T ? bdims[T - 1][m] : 1);
+  }
+  uint64_t maxdim = 1;
+  for(int m=0; m < N; ++m) if(m != t && maxrows[m] > maxdim) maxdim = maxrows[m];
+
+  // ---- device state ----
+  Mem mem;
+  uint32_t * d_ind[SPD_MAX_MODES] = {};
+  double * d_vals = nullptr;
+  for(int m=0; m < N; ++m) {
+    SPD_TRY(mem.get((void **) &d_ind[m], nnz * sizeof(uint32_t)));
+    SPD_TRY(spd_h2d(d_ind[m], inds[m], nnz * sizeof(uint32_t)));
+  }
+  SPD_TRY(mem.get((void **) &d_vals, nnz * sizeof(double)));
+  SPD_TRY(spd_h2d(d_vals, vals, nnz * sizeof(double)));
+  // stream mode of a batch is all zeros
+  uint32_t * d_zero_ind = nullptr;
+  uint64_t maxb = 0;
+  for(uint64_t b=0; b < T; ++b) if(bend[b] - bstart[b] > maxb) maxb = bend[b] - bstart[b];
+  SPD_TRY(mem.get((void **) &d_zero_ind, (maxb ? maxb : 1) * sizeof(uint32_t)));
+  SPD_TRY(spd_zero_async(d_zero_ind, (maxb ? maxb : 1) * sizeof(uint32_t), st));
+
+  double * Unew[SPD_MAX_MODES] = {};
+  double * Uold[SPD_MAX_MODES] = {};
+  double * Dual[SPD_MAX_MODES] = {};
+  double * aTa[SPD_MAX_MODES] = {};
+  for(int m=0; m < N; ++m) {
+    SPD_TRY(mem.get((void **) &aTa[m], (size_t) R * R * sizeof(double)));
+    SPD_TRY(spd_zero_async(aTa[m], (size_t) R * R * sizeof(double), st));
+    if(m == t) continue;
+    size_t const bytes = maxrows[m] * R * sizeof(double);
+    SPD_TRY(mem.get((void **) &Unew[m], bytes));
+    SPD_TRY(mem.get((void **) &Uold[m], bytes));
+    SPD_TRY(mem.get((void **) &Dual[m], bytes));
+    SPD_TRY(spd_zero_async(Unew[m], bytes, st));
+    SPD_TRY(spd_zero_async(Uold[m], bytes, st));
+    SPD_TRY(spd_zero_async(Dual[m], bytes, st));
+  }
+  double * time_row = nullptr;
+  double * gtime = nullptr;
+  SPD_TRY(mem.get((void **) &time_row, R * sizeof(double)));
+  SPD_TRY(spd_zero_async(time_row, R * sizeof(double), st));
+  SPD_TRY(mem.get((void **) &gtime, (T ? T : 1) * R * sizeof(double)));
+  SPD_TRY(spd_zero_async(gtime, (T ? T : 1) * R * sizeof(double), st));
+  double * Mbuf = nullptr;
+  double * aux = nullptr;
+  double * init = nullptr;
+  SPD_TRY(mem.get((void **) &Mbuf, maxdim * R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &aux, maxdim * R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &init, maxdim * R * sizeof(double)));
+  double * old_gram = nullptr;
+  double * ginv = nullptr;
+  double * H = nullptr;
+  double * cross = nullptr;
+  double * tgram = nullptr;
+  SPD_TRY(mem.get((void **) &old_gram, (size_t) R * R * sizeof(double)));
+  SPD_TRY(spd_zero_async(old_gram, (size_t) R * R * sizeof(double), st));
+  SPD_TRY(mem.get((void **) &ginv, (size_t) RP * RP * sizeof(double)));
+  SPD_TRY(mem.get((void **) &H, (size_t) RP * RP * sizeof(double)));
+  SPD_TRY(mem.get((void **) &cross, (size_t) SPD_MAX_MODES * R * R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &tgram, (size_t) R * R * sizeof(double)));
+  double * sc = nullptr;   // [0] rho, [2..3] diff sums, [4..5] inner, [8..] admm sums
+  SPD_TRY(mem.get((void **) &sc, (16 + 4 + R) * sizeof(double)));
+  int * info = nullptr;
+  SPD_TRY(mem.get((void **) &info, sizeof(int)));
+  SPD_TRY(spd_zero_async(info, sizeof(int), st));
+  double * lam1 = nullptr;
+  double * tmpv = nullptr;
+  SPD_TRY(mem.get((void **) &lam1, R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &tmpv, R * sizeof(double)));
+  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, lam1, (uint64_t) R, 1.0);
+  SPD_LAUNCH_CHECK();
+  double * const d_rho = sc + 0;
+  double * const d_diff = sc + 2;
+  double * const d_inner = sc + 4;
+  double * const d_sums = sc + 8;
+
+  double const * aTa_c[SPD_MAX_MODES];
+  for(int m=0; m < N; ++m) aTa_c[m] = aTa[m];
+  uint32_t const * ind_b[SPD_MAX_MODES];
+  double const * mats[SPD_MAX_MODES];
+  uint64_t rows[SPD_MAX_MODES] = {0};
+  std::vector<double> hrand;
+
+  for(uint64_t b=0; b < T; ++b) {
+    SPD_TRY(spd_sync());
+    double const t0 = wall_s();
+    uint64_t const s0 = bstart[b];
+    uint64_t const bn = bend[b] - bstart[b];
+    std::vector<uint64_t> const & bd = bdims[b];
+    for(int m=0; m < N; ++m) ind_b[m] = (m == t) ? d_zero_ind : d_ind[m] + s0;
+    double const * const vals_b = d_vals + s0;
+
+    // grow_mats (StreamCPD.cc:192-220): new rows from rand(), Grams
+    for(int m=0; m < N; ++m) {
+      if(m == t) continue;
+      if(bd[m] > rows[m]) {
+        uint64_t const nnew = (bd[m] - rows[m]) * R;
+        hrand.resize(nnew);
+        cfg->fill_rand(hrand.data(), nnew);
+        SPD_TRY(spd_h2d(Unew[m] + rows[m] * R, hrand.data(), nnew * sizeof(double)));
+        rows[m] = bd[m];
+      }
+      SPD_TRY(dn_gram(Unew[m], rows[m], R, nullptr, nullptr, aTa[m], st));
+    }
+    if(b == 0) {
+      for(int m=0; m < N; ++m) {
+        if(m == t) continue;
+        SPD_TRY(dn_colnorm(Unew[m], rows[m], R, tmpv, st));
+        SPD_TRY(dn_div_cols(Unew[m], rows[m], R, tmpv, nullptr, st));
+      }
+    }
+    for(int m=0; m < N; ++m) mats[m] = (m == t) ? time_row : Unew[m];
+
+    double prev_delta = 0.;
+    uint64_t outer = 0;
+    for(outer=0; outer < cfg->max_its; ++outer) {
+      double delta = 0.;
+      // time row: MTTKRP to one row and the closed-form solve
+      SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, t, mats, R, 1, Mbuf, st));
+      SPD_TRY(dn_chol_inv(aTa_c, N, t, R, cfg->con[t].diag_add, 0, ginv, d_rho, info, st));
+      SPD_TRY(dn_apply(Mbuf, SPD_F64, ginv, 1, R, time_row, nullptr, st));
+      hipLaunchKernelGGL(k_time_gram, dim3(1), dim3(256), 0, st, aTa[t],
+          (double const *) time_row, old_gram, R);
+      SPD_LAUNCH_CHECK();
+      for(int m=0; m < N; ++m) {
+        if(m == t) continue;
+        uint64_t const I = bd[m];
+        SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, m, mats, R, I, Mbuf, st));
+        // add_historical (StreamCPD.cc:223-294)
+        HistSrc hs = {};
+        int nc = 0;
+        for(int mm=0; mm < N; ++mm) {
+          if(mm == m || mm == t) continue;
+          double * cm = cross + (size_t) nc * R * R;
+          SPD_TRY(dn_xgram(Uold[mm], Unew[mm], rows[mm], R, cm, st));
+          hs.m[nc++] = cm;
+        }
+        hipLaunchKernelGGL(k_hist, dim3(1), dim3(256), 0, st, (double const *) old_gram,
+            hs, nc, R, RP, H);
+        SPD_LAUNCH_CHECK();
+        SPD_TRY(dn_apply_acc(Uold[m], H, I, R, Mbuf, st));
+        SPD_TRY(spd_zero_async(Dual[m], I * R * sizeof(double), st));
+        spd_con const & con = cfg->con[m];
+        if(con.kind == SPD_CON_ADMM) {
+          SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
+          for(uint64_t k=0; k < cfg->max_inner; ++k) {
+            SPD_TRY(dn_admm_step(Mbuf, SPD_F64, ginv, d_rho, I, R, Unew[m], aux, Dual[m],
+                init, con.prox, d_sums, st));
+            double s4[4];
+            SPD_TRY(spd_d2h(s4, d_sums, sizeof(s4)));
+            if(s4[1] <= cfg->inner_tol * s4[0] && s4[2] <= cfg->inner_tol * s4[3]) break;
+          }
+        } else {
+          SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 0, ginv, d_rho, info, st));
+          SPD_TRY(dn_apply(Mbuf, SPD_F64, ginv, I, R, Unew[m], nullptr, st));
+        }
+        SPD_TRY(dn_gram(Unew[m], I, R, nullptr, nullptr, aTa[m], st));
+        SPD_TRY(diff2(Uold[m], Unew[m], I * R, d_diff, st));
+        double dd[2];
+        SPD_TRY(spd_d2h(dd, d_diff, sizeof(dd)));
+        delta += sqrt(dd[0]) / sqrt(dd[1]);
+      }
+      if(cfg->verbosity > 1) {
+        printf("  delta: %e prev_delta: %e (%e diff)\n", delta, prev_delta,
+            fabs(delta - prev_delta));
+      }
+      if(outer > 0 && fabs(delta - prev_delta) < cfg->tol) {
+        ++outer;
+        break;
+      }
+      prev_delta = delta;
+    }
+    // forget, old <- new, keep the time row (StreamCPD.cc:488-511)
+    hipLaunchKernelGGL(k_scale, dim3((unsigned) spd_cdiv((uint64_t) R * R, 256)), dim3(256),
+        0, st, aTa[t], (uint64_t) R * R, cfg->forget);
+    SPD_LAUNCH_CHECK();
+    for(int m=0; m < N; ++m) {
+      if(m == t) continue;
+      SPD_TRY(spd_d2d(Uold[m], Unew[m], rows[m] * R * sizeof(double)));
+    }
+    SPD_TRY(spd_d2d(gtime + b * R, time_row, R * sizeof(double)));
+    SPD_TRY(spd_sync());
+    double const secs = wall_s() - t0;
+    if(res->batch_seconds) res->batch_seconds[b] = secs;
+    if(res->outer_its) res->outer_its[b] = outer;
+
+    // local error of this slice (StreamCPD.cc:77-89): the model of the last
+    // time row against the slice (time index 0)
+    if(res->local_err) {
+      int small = 0;
+      for(int m=1; m < N; ++m) if(bd[m] < bd[small]) small = m;
+      uint64_t const nr = bd[small];
+      SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, small, mats, R, nr, Mbuf, st));
+      double const * fsm = (small == t) ? time_row : Unew[small];
+      SPD_TRY(dn_innerprod(fsm, Mbuf, SPD_F64, nr, R, lam1, d_inner, st));
+      SPD_TRY(dn_gram(time_row, 1, R, nullptr, nullptr, tgram, st));
+      double const * grams[SPD_MAX_MODES];
+      for(int m=0; m < N; ++m) grams[m] = (m == t) ? tgram : aTa[m];
+      SPD_TRY(dn_cpd_norm(grams, N, R, lam1, d_inner + 1, st));
+      double h[2];
+      SPD_TRY(spd_d2h(h, d_inner, sizeof(h)));
+      double xn = 0.;
+      for(uint64_t x=s0; x < bend[b]; ++x) xn += vals[x] * vals[x];
+      double const e2 = (xn + h[1] - 2. * h[0]) / xn;
+      res->local_err[b] = e2;
+      if(cfg->verbosity > 0) {
+        printf("batch %5llu: %7llu nnz (%0.5fs) (%0.3e NNZ/s) cpd: %+0.5f global: "
+            "%+0.5f local-1: %+0.5f local-10: %+0.5f\n", (unsigned long long) (b + 1),
+            (unsigned long long) bn, secs, (double) bn / secs, -1., -1., e2, -1.);
+        fflush(stdout);
+      }
+    }
+  }
+  int hinfo = 0;
+  SPD_TRY(spd_d2h(&hinfo, info, sizeof(int)));
+  if(hinfo != 0) {
+    fprintf(stderr, "SPLATT: Cholesky factorisation failed in the stream (pivot %d).\n",
+        hinfo);
+  }
+
+  // final error of the whole stream (StreamCPD.cc:549-556)
+  res->nbatches = T;
+  for(int m=0; m < N; ++m) res->dims[m] = (m == t) ? T : rows[m];
+  if(res->final_err && T > 0) {
+    int small = 0;
+    uint64_t fd[SPD_MAX_MODES];
+    for(int m=0; m < N; ++m) fd[m] = dims[m];
+    for(int m=1; m < N; ++m) if(fd[m] < fd[small]) small = m;
+    double const * fm[SPD_MAX_MODES];
+    for(int m=0; m < N; ++m) fm[m] = (m == t) ? gtime : Unew[m];
+    uint64_t const frows = (small == t) ? T : rows[small];
+    double * M2 = nullptr;
+    SPD_TRY(mem.get((void **) &M2, fd[small] * R * sizeof(double)));
+    if(small == t) {
+      // the stream is sorted by time: one single-row reduction per slice
+      SPD_TRY(spd_zero_async(M2, fd[small] * R * sizeof(double), st));
+      for(uint64_t b=0; b < T; ++b) {
+        uint32_t const * ib[SPD_MAX_MODES];
+        for(int m=0; m < N; ++m) ib[m] = d_ind[m] + bstart[b];
+        SPD_TRY(coo_mttkrp(ib, d_vals + bstart[b], bend[b] - bstart[b], N, t, fm, R, 1,
+            M2 + b * R, st));
+      }
+    } else {
+      uint32_t const * ib[SPD_MAX_MODES];
+      for(int m=0; m < N; ++m) ib[m] = d_ind[m];
+      SPD_TRY(coo_mttkrp(ib, d_vals, bend[T - 1], N, small, fm, R, fd[small], M2, st));
+    }
+    SPD_TRY(dn_innerprod(fm[small], M2, SPD_F64, frows, R, lam1, d_inner, st));
+    double const * grams[SPD_MAX_MODES];
+    SPD_TRY(dn_gram(gtime, T, R, nullptr, nullptr, tgram, st));
+    for(int m=0; m < N; ++m) grams[m] = (m == t) ? tgram : aTa[m];
+    SPD_TRY(dn_cpd_norm(grams, N, R, lam1, d_inner + 1, st));
+    double h[2];
+    SPD_TRY(spd_d2h(h, d_inner, sizeof(h)));
+    double xn = 0.;
+    for(uint64_t x=0; x < bend[T - 1]; ++x) xn += vals[x] * vals[x];
+    double const e = sqrt(xn + h[1] - 2. * h[0]) / sqrt(xn);
+    *res->final_err = e * e;
+  }
+  for(int m=0; m < N; ++m) {
+    if(res->factors[m] == nullptr) continue;
+    if(m == t) {
+      SPD_TRY(spd_d2h(res->factors[m], gtime, T * R * sizeof(double)));
+    } else {
+      SPD_TRY(spd_d2h(res->factors[m], Unew[m], rows[m] * R * sizeof(double)));
+    }
+  }
+  return SPD_OK;
+}

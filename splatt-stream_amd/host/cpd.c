@@ -43,7 +43,7 @@ void splatt_free_cpd(splatt_kruskal * factored)
 
 /* Translate a registered constraint into what the device driver runs.
  * Returns 0, or -1 for a constraint the device path does not provide. */
-static int p_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
+int spl_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
     idx_t chunk, spd_con * out)
 {
   out->kind = SPD_CON_NONE;
@@ -120,7 +120,7 @@ splatt_error_type splatt_cpd(splatt_csf const * const tensor,
     if(strcmp(con->description, "UNCONSTRAINED") != 0) {
       cfg.unconstrained = 0;
     }
-    if(p_map_constraint(con, m, copts->chunk_sizes[m], &cfg.con[m]) != 0) {
+    if(spl_map_constraint(con, m, copts->chunk_sizes[m], &cfg.con[m]) != 0) {
       ret = SPLATT_ERROR_BADINPUT;
     }
   }

@@ -39,6 +39,11 @@ void splatt_maxcolnorm_nonneg_prox(val_t * primal, idx_t const nrows,
 void splatt_nonneg_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
     idx_t const offset, void * data, val_t const rho, bool const should_parallelize);
 
+/* A registered constraint as the device drivers run it (0, or -1 when the
+ * GPU path does not provide it; the reason is printed). */
+int spl_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
+    idx_t chunk, spd_con * out);
+
 /* glibc rand() initialisation (src/util.c:15-40). */
 val_t spl_rand_val(void);
 void spl_fill_rand(val_t * vals, idx_t nelems);

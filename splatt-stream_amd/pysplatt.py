@@ -86,6 +86,16 @@ class GlobalOpts(C.Structure):
                 ("random_seed", C.c_int)]
 
 
+class StreamResult(C.Structure):
+    _fields_ = [("nmodes", C.c_uint64), ("rank", C.c_uint64),
+                ("stream_mode", C.c_uint64), ("nbatches", C.c_uint64),
+                ("dims", C.c_uint64 * MAX_NMODES),
+                ("factors", f64p * MAX_NMODES),
+                ("iperm", u64p * MAX_NMODES),
+                ("local_err", f64p), ("batch_seconds", f64p),
+                ("outer_its", u64p), ("final_err", C.c_double)]
+
+
 class MatrixT(C.Structure):
     _fields_ = [("I", C.c_uint64), ("J", C.c_uint64), ("vals", f64p),
                 ("rowmajor", C.c_int)]
@@ -200,6 +210,14 @@ def lib():
     L.spd_dense_admm_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    L.splatt_amd_cpd_stream_coo.restype = C.POINTER(StreamResult)
+    L.splatt_amd_cpd_stream_coo.argtypes = [C.POINTER(SplattCoord), C.c_uint64, C.c_uint64,
+                                            C.c_double, C.POINTER(CpdOpts),
+                                            C.POINTER(GlobalOpts)]
+    L.splatt_amd_free_stream_result.argtypes = [C.POINTER(StreamResult)]
+    L.splatt_cpd_stream.restype = C.c_int
+    L.splatt_cpd_stream.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_double,
+                                    C.POINTER(CpdOpts), C.POINTER(GlobalOpts)]
     _lib = L
     return L
 
@@ -439,3 +457,55 @@ def run_cpd(csfset, rank, max_its=20, tol=0.0, seed=1, configure=None,
     L.splatt_free_global_opts(go)
     L.splatt_free_cpd(K)
     return dict(init=init, factors=out, lam=lam, fit=fit, rc=rc, dims=dims)
+
+
+def coord_from_arrays(inds, vals, dims):
+    """A splatt_coord owned by libsplatt holding copies of the arrays."""
+    L = lib()
+    n = len(inds)
+    nnz = len(vals)
+    tt = L.splatt_alloc_coord(n, nnz)
+    for m in range(n):
+        tt[0].dims[m] = int(dims[m])
+        dst = np.ctypeslib.as_array(tt[0].ind[m], shape=(max(nnz, 1),))
+        dst[:nnz] = np.asarray(inds[m], dtype=np.uint64)
+    dv = np.ctypeslib.as_array(tt[0].vals, shape=(max(nnz, 1),))
+    dv[:nnz] = np.asarray(vals, dtype=np.float64)
+    return tt
+
+
+def run_stream(inds, vals, dims, rank, stream_mode, forget=0.95, max_its=5, seed=1,
+               frob=None, verbosity=0):
+    """splatt_amd_cpd_stream_coo after srand(seed); frob registers Frobenius
+    regularisation on the streamed mode. Returns a dict of numpy results."""
+    L = lib()
+    tt = coord_from_arrays(inds, vals, dims)
+    co = L.splatt_alloc_cpd_opts()
+    co[0].max_iterations = max_its
+    if frob is not None:
+        modes = np.array([stream_mode], dtype=np.uint64)
+        L.splatt_register_frob(co, frob, modes.ctypes.data_as(u64p), 1)
+    go = L.splatt_alloc_global_opts()
+    go[0].verbosity = verbosity
+    srand(seed)
+    r = L.splatt_amd_cpd_stream_coo(tt, rank, stream_mode, forget, co, go)
+    out = None
+    if r:
+        R = r[0]
+        N = int(R.nmodes)
+        nb = int(R.nbatches)
+        d = [int(R.dims[m]) for m in range(N)]
+        facs = [np.ctypeslib.as_array(R.factors[m], shape=(max(d[m], 1), rank))[:d[m]].copy()
+                for m in range(N)]
+        iperm = [np.ctypeslib.as_array(R.iperm[m], shape=(max(int(dims[m]), 1),))[:int(dims[m])].copy()
+                 for m in range(N)]
+        out = dict(nbatches=nb, dims=d, factors=facs, iperm=iperm,
+                   local_err=np.ctypeslib.as_array(R.local_err, shape=(max(nb, 1),))[:nb].copy(),
+                   batch_seconds=np.ctypeslib.as_array(R.batch_seconds, shape=(max(nb, 1),))[:nb].copy(),
+                   outer_its=np.ctypeslib.as_array(R.outer_its, shape=(max(nb, 1),))[:nb].copy(),
+                   final_err=float(R.final_err))
+        L.splatt_amd_free_stream_result(r)
+    L.splatt_free_cpd_opts(co)
+    L.splatt_free_global_opts(go)
+    L.splatt_free_coord(tt)
+    return out

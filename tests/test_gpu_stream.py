@@ -1,0 +1,155 @@
+"""GPU streaming CPD against the oracle's restatement of StreamCPD::compute.
+
+Same input, same rand() seed: the device result (time rows, non-stream
+factors in relabelled order, per-slice local error, outer iterations, final
+error) must match or_stream_cpd in FP64. Discrete decisions (ADMM and outer
+convergence tests) are taken on values that agree to ~1e-13, so the
+iteration counts agree too; values are compared at 1e-8 relative.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as orc
+import pysplatt as ps
+
+pytestmark = pytest.mark.gpu
+
+libc = ctypes.CDLL("libc.so.6")
+
+
+def _stream(T, dims, per_slice, seed, zipf=None):
+    """Non-empty slices of distinct coordinates; the last mode is time."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for t in range(T):
+        n = int(per_slice[0] + rng.integers(0, per_slice[1] - per_slice[0] + 1))
+        cols = []
+        for d in dims:
+            if zipf:
+                r = rng.zipf(zipf, size=4 * n) - 1
+                cols.append(np.minimum(r, d - 1))
+            else:
+                cols.append(rng.integers(0, d, size=4 * n))
+        key = np.zeros(4 * n, dtype=np.int64)
+        for c, d in zip(cols, dims):
+            key = key * d + c
+        _, first = np.unique(key, return_index=True)
+        first = np.sort(first)[:n]
+        parts.append([c[first] for c in cols] + [np.full(len(first), t)])
+    inds = [np.concatenate([p[m] for p in parts]).astype(np.uint64)
+            for m in range(len(dims) + 1)]
+    vals = rng.random(len(inds[0])) + 0.05
+    perm = rng.permutation(len(vals))          # input order must not matter
+    inds = [i[perm] for i in inds]
+    vals = vals[perm]
+    full = [int(max(i) + 1) for i in inds]
+    return inds, vals, full
+
+
+def _oracle(inds, vals, dims, stream_mode, R, forget, max_its, seed, frob):
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    iperm = orc.stream_prepare(coo, stream_mode)
+    N = len(dims)
+    cons = [(2, 1, 0.0)] * N
+    cons[stream_mode] = (1, 0, frob)
+    libc.srand(seed)
+    res = orc.stream_cpd(coo, stream_mode, R, forget, max_its, cons)
+    res["iperm"] = iperm
+    return res
+
+
+def _compare(dev, ref, stream_mode, rtol=1e-8):
+    assert dev is not None, ps.last_error()
+    assert dev["nbatches"] == ref["nbatches"]
+    assert np.array_equal(dev["outer_its"], ref["outer_its"])
+    N = len(dev["factors"])
+    for m in range(N):
+        if m == stream_mode:
+            got, want = dev["factors"][m], ref["time"]
+        else:
+            n = ref["dims"][m]
+            assert dev["dims"][m] == n
+            assert np.array_equal(dev["iperm"][m][:n], ref["iperm"][m][:n])
+            got, want = dev["factors"][m], ref["factors"][m]
+        scale = max(np.max(np.abs(want)), 1e-300)
+        assert np.allclose(got, want, rtol=rtol, atol=rtol * scale), m
+    assert np.allclose(dev["local_err"], ref["local_err"], rtol=rtol, atol=1e-12)
+    assert abs(dev["final_err"] - ref["final_err"]) <= rtol * max(1.0, ref["final_err"])
+
+
+@pytest.mark.parametrize("R", [4, 8, 16])
+def test_stream_small_matches_oracle(gpu_available, R):
+    inds, vals, dims = _stream(12, [40, 30], (20, 60), seed=R)
+    dev = ps.run_stream(inds, vals, dims, R, 2, forget=0.95, max_its=5, seed=3, frob=1e-2)
+    ref = _oracle(inds, vals, dims, 2, R, 0.95, 5, 3, 1e-2)
+    _compare(dev, ref, 2)
+
+
+def test_stream_zipf_matches_oracle(gpu_available):
+    inds, vals, dims = _stream(8, [3000, 2000], (1500, 3000), seed=5, zipf=1.3)
+    dev = ps.run_stream(inds, vals, dims, 16, 2, forget=0.95, max_its=5, seed=9, frob=1e-2)
+    ref = _oracle(inds, vals, dims, 2, 16, 0.95, 5, 9, 1e-2)
+    _compare(dev, ref, 2)
+
+
+def test_stream_first_mode_four_modes(gpu_available):
+    # stream mode 0 of a 4-mode tensor: the history term multiplies two
+    # cross Grams
+    inds, vals, dims = _stream(6, [25, 20, 15], (30, 80), seed=7)
+    inds = [inds[3]] + inds[:3]        # time first
+    dims = [dims[3]] + dims[:3]
+    dev = ps.run_stream(inds, vals, dims, 6, 0, forget=0.9, max_its=4, seed=2, frob=0.1)
+    ref = _oracle(inds, vals, dims, 0, 6, 0.9, 4, 2, 0.1)
+    _compare(dev, ref, 0)
+
+
+def test_stream_rank_above_64(gpu_available):
+    inds, vals, dims = _stream(5, [90, 80], (100, 200), seed=11)
+    dev = ps.run_stream(inds, vals, dims, 72, 2, forget=0.95, max_its=3, seed=4, frob=1e-2)
+    ref = _oracle(inds, vals, dims, 2, 72, 0.95, 3, 4, 1e-2)
+    _compare(dev, ref, 2)
+
+
+def test_stream_unconstrained_time_mode(gpu_available):
+    inds, vals, dims = _stream(6, [30, 25], (40, 90), seed=13)
+    dev = ps.run_stream(inds, vals, dims, 5, 2, forget=0.95, max_its=5, seed=6, frob=None)
+    ref = _oracle(inds, vals, dims, 2, 5, 0.95, 5, 6, 0.0)
+    _compare(dev, ref, 2)
+
+
+def test_cpd_stream_file_entry(gpu_available, tmp_path, capfd):
+    inds, vals, dims = _stream(5, [20, 15], (20, 40), seed=17)
+    path = tmp_path / "s.tns"
+    with open(path, "w") as f:
+        for n in range(len(vals)):
+            f.write(" ".join(str(int(i[n]) + 1) for i in inds) + " %.17g\n" % vals[n])
+    L = ps.lib()
+    co = L.splatt_alloc_cpd_opts()
+    co[0].max_iterations = 3
+    go = L.splatt_alloc_global_opts()
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        ps.srand(1)
+        rc = L.splatt_cpd_stream(str(path).encode(), 4, 2, 0.95, co, go)
+    finally:
+        os.chdir(cwd)
+    L.splatt_free_cpd_opts(co)
+    L.splatt_free_global_opts(go)
+    assert rc == ps.SUCCESS
+    out = capfd.readouterr().out
+    assert "final-err:" in out
+    ref = _oracle(inds, vals, dims, 2, 4, 0.95, 3, 1, 0.0)
+    line = [l for l in out.splitlines() if l.startswith("final-err:")][0]
+    assert abs(float(line.split()[1]) - ref["final_err"]) <= 1e-5
+    m1 = np.loadtxt(tmp_path / "mode1.mat")
+    m3 = np.loadtxt(tmp_path / "mode3.mat")
+    assert m1.shape == (dims[0], 4) and m3.shape == (dims[2], 4)
+    # rows sit at their original indices
+    want1 = np.zeros((dims[0], 4))
+    want1[ref["iperm"][0][:ref["dims"][0]].astype(int)] = ref["factors"][0]
+    assert np.allclose(m1, want1, rtol=1e-7, atol=1e-8)
+    assert np.allclose(m3, ref["time"], rtol=1e-7, atol=1e-8)
