@@ -132,6 +132,98 @@ def cpu_baseline(inds, vals, dims, mats, R, sample, threads):
                        % (sample, sample, N, len(times), best, total))
 
 
+def bench_cpd_c1(threads, seed=20261203):
+    """CPD-ALS, rank 16, 20 iterations, FP64, 1000x800x600 with 1M uniform
+    nonzeros (values U(0,1)), two-CSF allocation as splatt_cpd uses it. The
+    GPU run goes through splatt_cpd; the oracle runs the same iterations
+    from the same start on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+    R, its = 16, 20
+    inds, vals, dims = ps.synth_coord([1000, 800, 600], 1_000_000, vals="uniform", seed=seed)
+    opts = ps.Opts(csf_alloc=ps.CSF_TWOMODE)
+    cs = ps.CsfSet.convert(inds, vals, opts)
+    ps.run_cpd(cs, R, max_its=1, tol=0.0, seed=1)      # warm-up (plans, scratch)
+    t0 = time.perf_counter()
+    res = ps.run_cpd(cs, R, max_its=its, tol=0.0, seed=1)
+    gpu_s = time.perf_counter() - t0
+    if res["rc"] != ps.SUCCESS:
+        raise RuntimeError("splatt_cpd failed: " + ps.last_error())
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    perm0 = cs.arrays(0)["perm"]
+    ocsfs = [orc.Csf(coo, perm0), orc.Csf(coo, cs.arrays(1)["perm"])]
+    mode_csf = [1 if perm0[-1] == m else 0 for m in range(3)]
+    t0 = time.perf_counter()
+    facs, lam, fits = orc.cpd_als_con(ocsfs, mode_csf, res["init"], its, 0.0,
+                                      [(0, 0, 0.0)] * 3, True, nthreads=threads)
+    cpu_s = time.perf_counter() - t0
+    cs.free()
+    return dict(workload="CPD-ALS rank 16, 20 iterations, FP64, 1000x800x600, %d nnz "
+                         "uniform, two-CSF allocation" % len(vals),
+                gpu_s=gpu_s, gpu_ms_per_iteration=1e3 * gpu_s / its,
+                cpu_s=cpu_s, cpu_ms_per_iteration=1e3 * cpu_s / its, cpu_threads=threads,
+                cpu_kind="port", fit_gpu=res["fit"], fit_cpu=float(fits[-1]),
+                fit_abs_diff=abs(res["fit"] - float(fits[-1])))
+
+
+def stream_c4_slices(nslices, seed=20261206):
+    """The first nslices time slices of the streaming workload: 500000 x
+    200000 non-stream dims, per-slice nnz lognormal (median 2e5, sigma 1)
+    clipped to [1e4, 2e6], Zipf(1.2) coordinates distinct within a slice,
+    values U(0,1)."""
+    rng = np.random.default_rng(seed)
+    nn = np.clip(np.round(np.exp(np.log(2e5) + rng.standard_normal(365))), 1e4, 2e6)
+    nn = nn.astype(np.int64)[:nslices]
+    I, J, V, K = [], [], [], []
+    for t in range(nslices):
+        ij, v, _ = ps.synth_coord([500_000, 200_000], int(nn[t]), zipf=[1.2, 1.2],
+                                  vals="uniform", compact=False, seed=1000 + t, shuffle=False)
+        I.append(ij[0]); J.append(ij[1]); V.append(v)
+        K.append(np.full(len(v), t, dtype=np.uint64))
+    inds = [np.concatenate(I), np.concatenate(J), np.concatenate(K)]
+    vals = np.concatenate(V)
+    return inds, vals, [500_000, 200_000, nslices]
+
+
+def bench_stream_c4(nslices, cpu_slices):
+    """Streaming CPD, rank 32, frob 1e-2 on the time mode, 5 outer
+    iterations per slice, FP64, over the first nslices slices. The oracle's
+    restatement of the reference runs the first cpu_slices slices (one host
+    thread, as the reference's per-slice ALS is dominated by its serial
+    dense passes here) and must report the same per-slice errors."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+    R = 32
+    inds, vals, dims = stream_c4_slices(nslices)
+    t0 = time.perf_counter()
+    res = ps.run_stream(inds, vals, dims, R, 2, forget=0.95, max_its=5, seed=4, frob=1e-2)
+    wall = time.perf_counter() - t0
+    if res is None:
+        raise RuntimeError("stream failed: " + ps.last_error())
+    bs = res["batch_seconds"]
+    out = dict(workload="streaming CPD rank 32, frob 1e-2 on time, 5 outer its, FP64, "
+                        "first %d slices of 500000x200000x365 (%d nnz)" % (nslices, len(vals)),
+               slices=int(res["nbatches"]), slices_per_s=float(len(bs) / bs.sum()),
+               p50_slice_s=float(np.percentile(bs, 50)), p99_slice_s=float(np.percentile(bs, 99)),
+               wall_s=wall, final_err=res["final_err"],
+               mean_local_err=float(np.mean(res["local_err"])))
+    if cpu_slices > 0:
+        sel = inds[2] < cpu_slices
+        cinds = [i[sel] for i in inds]
+        coo = orc.Coo.from_arrays(cinds, vals[sel], [dims[0], dims[1], cpu_slices])
+        orc.stream_prepare(coo, 2)
+        ps.srand(4)
+        t0 = time.perf_counter()
+        ref = orc.stream_cpd(coo, 2, R, 0.95, 5, [(2, 1, 0.0), (2, 1, 0.0), (1, 0, 1e-2)])
+        cpu_s = time.perf_counter() - t0
+        out.update(cpu_slices=cpu_slices, cpu_s=cpu_s, cpu_slices_per_s=cpu_slices / cpu_s,
+                   cpu_threads=1, cpu_kind="port",
+                   gpu_slices_per_s_same_slices=float(cpu_slices / bs[:cpu_slices].sum()),
+                   local_err_max_abs_diff=float(np.max(np.abs(
+                       res["local_err"][:cpu_slices] - ref["local_err"]))))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +236,10 @@ def main():
     ap.add_argument("--primary-only", action="store_true",
                     help="only the FP32 all-mode measurement (for profiling)")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
+    ap.add_argument("--skip-extras", action="store_true",
+                    help="skip the CPD (1000x800x600) and streaming measurements")
+    ap.add_argument("--stream-slices", type=int, default=60)
+    ap.add_argument("--stream-cpu-slices", type=int, default=2)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a streaming kernel of known traffic (FP64->FP32 "
                          "conversion of 2^26 values: 512 MiB read, 256 MiB written) "
@@ -216,6 +312,18 @@ def main():
         log("cpu baseline %.2f GB/s on %d threads (%.1fs)" % (cpu["value"], threads,
                                                               time.perf_counter() - t0))
 
+    if not args.primary_only and not args.skip_extras:
+        threads = min(16, os.cpu_count() or 1)
+        c1 = bench_cpd_c1(threads)
+        log("CPD c1: GPU %.2f ms/it, CPU %.2f ms/it (%d threads), fit diff %.2e"
+            % (c1["gpu_ms_per_iteration"], c1["cpu_ms_per_iteration"], threads,
+               c1["fit_abs_diff"]))
+        extra["cpd_c1"] = c1
+        c4 = bench_stream_c4(args.stream_slices, 0 if args.skip_cpu else args.stream_cpu_slices)
+        log("stream c4: %.2f slices/s, p50 %.4fs p99 %.4fs" % (
+            c4["slices_per_s"], c4["p50_slice_s"], c4["p99_slice_s"]))
+        extra["stream_c4"] = c4
+
     line = {
         "metric": "MTTKRP HBM GB/s",
         "value": round(primary["gbps"], 2),
@@ -247,9 +355,7 @@ def main():
         "cpu_baseline": cpu,
         "modes": primary["modes"],
         "csf_build_s": round(build_all, 3),
-        "extra": {k: (v if not isinstance(v, dict) else
-                      {kk: vv for kk, vv in v.items() if kk != "modes"} | {"modes": v["modes"]})
-                  for k, v in extra.items()},
+        "extra": extra,
         "parity": parity,
     }
     print(json.dumps(line))

@@ -8,6 +8,7 @@
 #define SPLATT_AMD_H
 
 #include <stddef.h>
+#include <stdint.h>
 #include "splatt.h"
 
 #ifdef __cplusplus
@@ -91,6 +92,54 @@ splatt_amd_stream_result * splatt_amd_cpd_stream_coo(
     splatt_global_opts const * const global_options);
 
 void splatt_amd_free_stream_result(splatt_amd_stream_result * result);
+
+/* ---- serving many streams on one GPU ----------------------------------
+ * No counterpart in the reference (one stream per process there). Each
+ * stream follows the reference's streaming rules and is the same
+ * computation as splatt_amd_cpd_stream_coo on its slices; its factor rows
+ * are initialised from its own rand() sequence seeded with `seed`. Streams
+ * are admitted while their device state fits the HBM budget; the others wait
+ * until closed streams release memory. Up to max_inflight slices (of
+ * different streams) run at once; the slices of one stream run in
+ * submission order. */
+typedef struct spd_server splatt_amd_server;
+
+/* hbm_budget_bytes 0: 80% of the free device memory. */
+splatt_amd_server * splatt_amd_server_create(size_t hbm_budget_bytes, int max_inflight);
+
+/* dims_cap[m]: original indices of mode m stay below it (the streamed
+ * mode's entry is ignored). cpd_options may be NULL. Returns the stream id,
+ * or -1. */
+int64_t splatt_amd_server_open(splatt_amd_server * srv, splatt_idx_t nmodes,
+    splatt_idx_t const * dims_cap, splatt_idx_t rank, splatt_idx_t stream_mode,
+    double forget, splatt_cpd_opts const * cpd_options, unsigned seed);
+
+/* One time slice (original indices; inds[stream_mode] is not read). Returns
+ * a ticket, or -1. */
+int64_t splatt_amd_server_submit(splatt_amd_server * srv, int64_t stream,
+    splatt_idx_t nnz, splatt_idx_t const * const * inds, splatt_val_t const * vals);
+
+/* Wait for every submitted slice. */
+splatt_error_type splatt_amd_server_drain(splatt_amd_server * srv);
+
+/* 0 when done (outputs filled), 1 while pending, < 0 on error: latency from
+ * submission to completion, device-side time, the slice's squared error,
+ * ALS iterations. */
+int splatt_amd_server_slice(splatt_amd_server * srv, int64_t ticket, double * latency_s,
+    double * gpu_s, double * local_err, splatt_idx_t * outer_its);
+
+/* Factor of `mode` after the stream's submitted slices (rows relabelled by
+ * first appearance; the streamed mode: one row per slice). out may be NULL
+ * to query *rows. */
+splatt_error_type splatt_amd_server_factors(splatt_amd_server * srv, int64_t stream,
+    splatt_idx_t mode, splatt_val_t * out, splatt_idx_t * rows);
+
+/* No more slices for the stream; its memory returns once its queue drains. */
+splatt_error_type splatt_amd_server_close(splatt_amd_server * srv, int64_t stream);
+
+void splatt_amd_server_stats(splatt_amd_server * srv, size_t * used, size_t * budget,
+    int * admitted, int * waiting);
+void splatt_amd_server_free(splatt_amd_server * srv);
 
 /* Device memory helpers for callers that hold device-resident factors. */
 void * splatt_amd_dev_alloc(size_t bytes);

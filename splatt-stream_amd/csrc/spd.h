@@ -188,7 +188,10 @@ typedef struct {
   double inner_tol;
   int verbosity;
   spd_con con[SPD_MAX_MODES];   /* streamed mode: closed form only */
-  void (*fill_rand)(double * vals, uint64_t n);   /* new factor rows */
+  /* values of new factor rows, in the order rows appear (the reference
+   * draws them from rand(), StreamMatrix.cc:70-83) */
+  void (*fill_rand)(void * ctx, double * vals, uint64_t n);
+  void * rand_ctx;
 } spd_stream_cfg;
 
 typedef struct {
@@ -207,6 +210,38 @@ typedef struct {
 int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
     uint32_t const * const * inds, double const * vals, spd_stream_cfg const * cfg,
     spd_stream_result * res);
+
+/* ---- serving: many streams on one GPU (no reference counterpart) ------ */
+
+typedef struct spd_server spd_server;
+
+/* budget: device bytes the streams' state may take (0: 80% of free memory);
+ * max_inflight: slices processed at once (worker threads). */
+int spd_server_create(size_t budget, int max_inflight, spd_server ** out);
+/* A stream with row capacities cap[m] (original indices < cap[m]) and the
+ * per-stream configuration (fill_rand is replaced by the stream's own rand()
+ * sequence from `seed`). Admitted when its footprint fits the budget. */
+int spd_server_open(spd_server * sv, int N, uint64_t const * cap,
+    spd_stream_cfg const * cfg, unsigned seed, int64_t * id_out);
+/* Queue one time slice (original labels; the streamed mode's array is not
+ * read). Returns a ticket. */
+int spd_server_submit(spd_server * sv, int64_t id, uint64_t nnz,
+    uint64_t const * const * inds, double const * vals, int64_t * ticket_out);
+/* Wait until every submitted slice is done. */
+int spd_server_drain(spd_server * sv);
+/* A slice's submit-to-done latency, device time, local error and outer
+ * iterations; returns 1 while it is pending. */
+int spd_server_slice(spd_server * sv, int64_t ticket, double * latency_s,
+    double * gpu_s, double * local_err, uint64_t * outer);
+/* Factor of `mode` (host, rows x rank; time mode: one row per slice) after
+ * the stream's queued slices; *rows gets the row count (out may be NULL). */
+int spd_server_factors(spd_server * sv, int64_t id, int mode, double * out,
+    uint64_t * rows);
+/* No more slices: memory returns to the budget once the queue drains. */
+int spd_server_close(spd_server * sv, int64_t id);
+void spd_server_stats(spd_server * sv, size_t * used, size_t * budget,
+    int * admitted, int * waiting);
+void spd_server_free(spd_server * sv);
 
 #ifdef __cplusplus
 }

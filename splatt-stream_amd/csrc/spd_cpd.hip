@@ -4,8 +4,8 @@
 // Everything between the initial upload of the factors and the final
 // download stays on the device: MTTKRP (spd_mttkrp), the Hadamard of Grams
 // and its Cholesky inverse, the solve, normalisation, Grams (MFMA) and the
-// fit terms. The host reads two scalars per iteration (the fit test) and,
-// for ADMM modes, four scalars per inner iteration (its convergence test).
+// fit terms. The host reads two scalars per iteration (the fit test); the
+// ADMM inner loop tests its convergence on the device (ranks up to 64).
 #include "spd_internal.hpp"
 #include "spd_csf_dev.hpp"
 #include "spd_dense.hpp"
@@ -102,6 +102,8 @@ extern "C" int spd_cpd_als(spd_csf * const * csfs, int const * mode_csf,
   int * info = nullptr;
   SPD_TRY(mem.get((void **) &info, sizeof(int)));
   SPD_TRY(spd_zero_async(info, sizeof(int), st));
+  int * ctl = nullptr;
+  SPD_TRY(mem.get((void **) &ctl, 2 * sizeof(int)));
   double * const d_rho = sc + 0;
   double * const d_norm = sc + 1;
   double * const d_inner = sc + 2;
@@ -139,18 +141,13 @@ extern "C" int spd_cpd_als(spd_csf * const * csfs, int const * mode_csf,
       inner_its[m] = 0.;
       if(con.kind == SPD_CON_ADMM) {
         SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
-        uint64_t k = 0;
-        for(k=0; k < cfg->max_inner; ++k) {
-          SPD_TRY(dn_admm_step(Mbuf, dt, ginv, d_rho, dims[m], R, U[m], aux, dual[m],
-              init, con.prox, d_sums, st));
-          double s4[4];
-          SPD_TRY(spd_d2h(s4, d_sums, sizeof(s4)));
-          if(s4[1] <= cfg->inner_tol * s4[0] && s4[2] <= cfg->inner_tol * s4[3]) {
-            ++k;
-            break;
-          }
+        SPD_TRY(dn_admm_solve(Mbuf, dt, ginv, d_rho, dims[m], R, U[m], dual[m], aux, init,
+            con.prox, cfg->max_inner, cfg->inner_tol, d_sums, ctl, st));
+        if(cfg->verbosity > 1) {
+          int hc[2];
+          SPD_TRY(spd_d2h(hc, ctl, sizeof(hc)));
+          inner_its[m] = (double) hc[1];
         }
-        inner_its[m] = (double) k;
         SPD_TRY(dn_gram(U[m], dims[m], R, nullptr, U32[m], aTa[m], st));
       } else if(con.kind == SPD_CON_CLOSED || !cfg->unconstrained) {
         SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 0, ginv, d_rho, info, st));

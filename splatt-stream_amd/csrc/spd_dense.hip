@@ -478,6 +478,194 @@ k_apply_mfma(ApplyArgs<TM> a)
   }
 }
 
+// ---- ADMM with the convergence test on the device (R <= 64) -------------
+//
+// One inner iteration (src/cpd/admm.c:240-308) in at most four launches and
+// no host round trip: pass 0 forms aux = (M + rho (P + D)) ginv and the
+// column sums of squares the prox needs; a one-block kernel turns them into
+// norms; pass 1 recomputes aux (the product is cheap on MFMA and saves
+// storing it), applies the prox, updates P and D and sums the residual
+// terms; a one-block kernel tests convergence and raises ctl[0]. Every
+// kernel returns at once when ctl[0] is set, so the host can queue all
+// max_inner iterations.
+
+template<typename TM>
+struct AdmmArgs {
+  TM const * M;
+  double const * ginv;      // RP x RP
+  uint64_t I;
+  uint32_t R;
+  double * P;
+  double * D;
+  double const * rho;
+  double const * norms;     // pass 1, max-column-norm prox
+  int prox;
+  double * part;            // pass 0: RP per block; pass 1: 4 per block
+  int const * ctl;          // ctl[0] != 0: converged
+};
+
+template<int TT, typename TM, int PASS>
+__global__ void __launch_bounds__(DN_THREADS)
+k_admm_mfma(AdmmArgs<TM> a)
+{
+  if(a.ctl[0] != 0) return;
+  constexpr int RP = TT * 16;
+  constexpr int LD = RP + 1;
+  constexpr int RPI = (64 / RP) > 0 ? 64 / RP : 1;
+  __shared__ double s_g[RP * RP];
+  __shared__ double s_m[4][16 * LD];
+  __shared__ double s_red[4][RP > 4 * 64 ? RP : 4 * 64];
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const kk = lane >> 4;
+  int const cc = lane & 15;
+  uint64_t const I = a.I;
+  uint32_t const R = a.R;
+  int const prox = a.prox;
+  for(int e=threadIdx.x; e < RP * RP; e += DN_THREADS) s_g[e] = a.ginv[e];
+  __syncthreads();
+  double const rho = *a.rho;
+  double cs[TT];
+  #pragma unroll
+  for(int t=0; t < TT; ++t) cs[t] = 0.;
+  double pn = 0., pr = 0., dr = 0., dn = 0.;
+  int const lr = lane / RP;
+  int const lc = lane - lr * RP;
+  uint64_t const ngroups = (I + 15) / 16;
+  for(uint64_t gi = (uint64_t) blockIdx.x * 4 + wave; gi < ngroups; gi += (uint64_t) gridDim.x * 4) {
+    uint64_t const row0 = gi * 16;
+    #pragma unroll
+    for(int rr=0; rr < 16; rr += RPI) {
+      int const r = rr + lr;
+      if(lr < RPI && r < 16) {
+        uint64_t const row = row0 + r;
+        double v = 0.;
+        if(row < I && (uint32_t) lc < R) {
+          uint64_t const idx = row * R + lc;
+          v = (double) a.M[idx] + rho * (a.P[idx] + a.D[idx]);
+        }
+        s_m[wave][r * LD + lc] = v;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    d4 acc[TT];
+    #pragma unroll
+    for(int t=0; t < TT; ++t) acc[t] = (d4) 0.;
+    #pragma unroll
+    for(int k0=0; k0 < RP; k0 += 4) {
+      double const av = s_m[wave][cc * LD + k0 + kk];
+      #pragma unroll
+      for(int t=0; t < TT; ++t) {
+        acc[t] = mfma16(av, s_g[(k0 + kk) * RP + t * 16 + cc], acc[t]);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    #pragma unroll
+    for(int t=0; t < TT; ++t) {
+      uint32_t const col = t * 16 + cc;
+      #pragma unroll
+      for(int r=0; r < 4; ++r) {
+        uint64_t const row = row0 + kk + 4 * r;
+        if(row < I && col < R) {
+          uint64_t const idx = row * R + col;
+          double const x = acc[t][r];
+          double const d_old = a.D[idx];
+          double p = x - d_old;
+          if(PASS == 0) {
+            double const q = (prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) ? 0. : p;
+            cs[t] += q * q;
+          } else {
+            if(prox == DN_PROX_NONNEG) {
+              p = (p > 0.) ? p : 0.;
+            } else if(prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG) {
+              if(prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) p = 0.;
+              double const nc = a.norms[col];
+              if(!(nc <= 1.)) p /= nc;
+            }
+            double const p_old = a.P[idx];
+            double const d_new = d_old + (p - x);
+            a.P[idx] = p;
+            a.D[idx] = d_new;
+            double const q = p - x;
+            double const w = p - p_old;
+            pn += p * p;
+            pr += q * q;
+            dr += w * w;
+            dn += d_new * d_new;
+          }
+        }
+      }
+    }
+  }
+  // block partials in a fixed order
+  if(PASS == 0) {
+    #pragma unroll
+    for(int t=0; t < TT; ++t) {
+      cs[t] += __shfl_xor(cs[t], 16, 64);
+      cs[t] += __shfl_xor(cs[t], 32, 64);
+    }
+    if(kk == 0) {
+      #pragma unroll
+      for(int t=0; t < TT; ++t) s_red[wave][t * 16 + cc] = cs[t];
+    }
+    __syncthreads();
+    for(int e=threadIdx.x; e < RP; e += DN_THREADS) {
+      a.part[(uint64_t) blockIdx.x * RP + e] = ((s_red[0][e] + s_red[1][e]) + s_red[2][e]) + s_red[3][e];
+    }
+  } else {
+    s_red[0][threadIdx.x] = pn;
+    s_red[1][threadIdx.x] = pr;
+    s_red[2][threadIdx.x] = dr;
+    s_red[3][threadIdx.x] = dn;
+    __syncthreads();
+    for(int h=DN_THREADS / 2; h > 0; h >>= 1) {
+      if((int) threadIdx.x < h) {
+        #pragma unroll
+        for(int k=0; k < 4; ++k) s_red[k][threadIdx.x] += s_red[k][threadIdx.x + h];
+      }
+      __syncthreads();
+    }
+    if(threadIdx.x < 4) a.part[(uint64_t) blockIdx.x * 4 + threadIdx.x] = s_red[threadIdx.x][0];
+  }
+}
+
+// norms[c] = sqrt(sum_b part[b * ld + c]) unless converged.
+__global__ void k_admm_norms(double const * __restrict__ part, uint32_t nb, uint32_t R,
+    uint32_t ld, double * __restrict__ norms, int const * __restrict__ ctl)
+{
+  if(ctl[0] != 0) return;
+  for(uint32_t c=threadIdx.x; c < R; c += blockDim.x) {
+    double v = 0.;
+    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * ld + c];
+    norms[c] = sqrt(v);
+  }
+}
+
+// sums = block totals; ctl[1] counts iterations; ctl[0] set on convergence.
+__global__ void k_admm_check(double const * __restrict__ part, uint32_t nb,
+    double * __restrict__ sums, int * __restrict__ ctl, double tol)
+{
+  if(ctl[0] != 0) return;
+  __shared__ double s[4];
+  int const k = threadIdx.x;
+  if(k < 4) {
+    double v = 0.;
+    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * 4 + k];
+    sums[k] = v;
+    s[k] = v;
+  }
+  __syncthreads();
+  if(k == 0) {
+    ctl[1] += 1;
+    if(s[1] <= tol * s[0] && s[2] <= tol * s[3]) ctl[0] = 1;
+  }
+}
+
+__global__ void k_ctl_reset(int * ctl)
+{
+  if(threadIdx.x < 2) ctl[threadIdx.x] = 0;
+}
+
 // Plain product for ranks above 64: out[i,j] = sum_k In[i,k] ginv[k,j].
 template<typename TM>
 __global__ void k_apply_plain(TM const * __restrict__ In, double const * __restrict__ ginv,
@@ -909,6 +1097,98 @@ static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_
   SPD_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum4, dim3(1), dim3(64), 0, st, (double const *) part2, nb2, sums);
   SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+template<int TT, typename TM, int PASS>
+static int launch_admm(AdmmArgs<TM> const & a, uint32_t nb, hipStream_t st)
+{
+  hipLaunchKernelGGL((k_admm_mfma<TT, TM, PASS>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+template<typename TM, int PASS>
+static int dispatch_admm(AdmmArgs<TM> const & a, uint32_t TT, uint32_t nb, hipStream_t st)
+{
+  switch(TT) {
+  case 1: return launch_admm<1, TM, PASS>(a, nb, st);
+  case 2: return launch_admm<2, TM, PASS>(a, nb, st);
+  case 3: return launch_admm<3, TM, PASS>(a, nb, st);
+  default: return launch_admm<4, TM, PASS>(a, nb, st);
+  }
+}
+
+template<typename TM>
+static int admm_solve_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
+    uint32_t R, double * P, double * D, int prox, uint64_t max_inner, double tol,
+    double * sums, int * ctl, hipStream_t st)
+{
+  uint32_t const RP = dn_rpad(R);
+  uint32_t const nb = apply_blocks(I);
+  bool const need_norms = (prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG);
+  void * pc = nullptr;
+  void * ps = nullptr;
+  SPD_TRY(spd_scratch(SLOT_COL, (size_t) nb * RP * sizeof(double), &pc));
+  SPD_TRY(spd_scratch(SLOT_GRAM, (size_t) nb * 4 * sizeof(double), &ps));
+  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, st, ctl);
+  SPD_LAUNCH_CHECK();
+  AdmmArgs<TM> a = {};
+  a.M = M;
+  a.ginv = ginv;
+  a.I = I;
+  a.R = R;
+  a.P = P;
+  a.D = D;
+  a.rho = rho;
+  a.norms = sums + 4;
+  a.prox = prox;
+  a.ctl = ctl;
+  for(uint64_t k=0; k < max_inner; ++k) {
+    if(need_norms) {
+      a.part = (double *) pc;
+      SPD_TRY((dispatch_admm<TM, 0>(a, RP / 16, nb, st)));
+      hipLaunchKernelGGL(k_admm_norms, dim3(1), dim3(64), 0, st, (double const *) pc, nb,
+          R, RP, sums + 4, (int const *) ctl);
+      SPD_LAUNCH_CHECK();
+    }
+    a.part = (double *) ps;
+    SPD_TRY((dispatch_admm<TM, 1>(a, RP / 16, nb, st)));
+    hipLaunchKernelGGL(k_admm_check, dim3(1), dim3(64), 0, st, (double const *) ps, nb,
+        sums, ctl, tol);
+    SPD_LAUNCH_CHECK();
+  }
+  return SPD_OK;
+}
+
+int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const * rho,
+    uint64_t I, uint32_t R, double * P, double * D, double * aux, double * init, int prox,
+    uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
+{
+  if(I == 0 || max_inner == 0) return SPD_OK;
+  if(R <= 64) {
+    if(mdtype == SPD_F32) {
+      return admm_solve_t<float>((float const *) M, ginv, rho, I, R, P, D, prox, max_inner,
+          tol, sums, ctl, st);
+    }
+    return admm_solve_t<double>((double const *) M, ginv, rho, I, R, P, D, prox, max_inner,
+        tol, sums, ctl, st);
+  }
+  // ranks above 64: one step at a time, the test on the host
+  int hctl[2] = {0, 0};
+  for(uint64_t k=0; k < max_inner; ++k) {
+    SPD_TRY(dn_admm_step(M, mdtype, ginv, rho, I, R, P, aux, D, init, prox, sums, st));
+    double s4[4];
+    SPD_HIP(hipMemcpyAsync(s4, sums, sizeof(s4), hipMemcpyDeviceToHost, st));
+    SPD_HIP(hipStreamSynchronize(st));
+    hctl[1] = (int) (k + 1);
+    if(s4[1] <= tol * s4[0] && s4[2] <= tol * s4[3]) {
+      hctl[0] = 1;
+      break;
+    }
+  }
+  SPD_HIP(hipMemcpyAsync(ctl, hctl, sizeof(hctl), hipMemcpyHostToDevice, st));
+  SPD_HIP(hipStreamSynchronize(st));
   return SPD_OK;
 }
 

@@ -48,6 +48,16 @@ int dn_admm_step(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint32_t R, double * P, double * aux,
     double * D, double * init, int prox, double * sums, hipStream_t st);
 
+// The whole ADMM inner loop (up to max_inner iterations, src/cpd/admm.c:
+// 240-308): P and D are updated in place; sums (4 + R doubles) ends with the
+// last iteration's residual terms and column norms; ctl (2 ints, device)
+// gets ctl[0] = converged, ctl[1] = iterations run. For R <= 64 the test
+// runs on the device and nothing synchronises; aux/init are scratch (I x R)
+// used only above rank 64.
+int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const * rho,
+    uint64_t I, uint32_t R, double * P, double * D, double * aux, double * init, int prox,
+    uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st);
+
 // out (R x R, full) = X^T Y for two I x R matrices.
 int dn_xgram(double const * X, double const * Y, uint64_t I, uint32_t R,
     double * out, hipStream_t st);

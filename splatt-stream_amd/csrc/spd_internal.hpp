@@ -50,8 +50,20 @@ int spd_scan_u32(uint32_t const * in, uint32_t * out, uint64_t n,
     uint32_t * total, hipStream_t st);
 
 // Scratch device memory owned by the library, grown on demand. Slot k keeps
-// its buffer until the next request for a larger size.
+// its buffer until the next request for a larger size. Slots: 0 radix
+// histograms, 1 norms, 2-3 root carries, 4-7 dense partials, 8-12 scan
+// levels, 13-14 streaming partials.
+#define SPD_NSCRATCH 16
+struct SpdScratchCtx {
+  void * p[SPD_NSCRATCH];
+  size_t bytes[SPD_NSCRATCH];
+  hipStream_t st;            // the stream that uses these buffers
+};
 int spd_scratch(int slot, size_t bytes, void ** ptr);
+// Route this thread's scratch requests to ctx (nullptr: the library's own).
+void spd_scratch_bind(SpdScratchCtx * ctx);
+// Free a context's buffers (after its stream has drained).
+void spd_scratch_release(SpdScratchCtx * ctx);
 
 // Zero fill with a kernel on stream st.
 int spd_zero_async(void * ptr, size_t bytes, hipStream_t st);

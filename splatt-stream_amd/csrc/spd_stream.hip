@@ -12,9 +12,11 @@
 // block partials for the single time row), the closed-form time solve, the
 // time Gram update, the history term (cross Grams on MFMA and a product
 // with the old factor), ADMM with the max-column-norm prox, the Grams and
-// the change measure. The host reads the ADMM and outer convergence scalars.
+// the change measure. ADMM tests its convergence on the device; the host
+// reads the two scalars of the outer change measure per mode.
 #include "spd_internal.hpp"
 #include "spd_dense.hpp"
+#include "spd_stream_state.hpp"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -195,25 +197,6 @@ __global__ void k_fill_f64(double * __restrict__ x, uint64_t n, double v)
 
 namespace {
 
-struct Mem {
-  std::vector<void *> ptrs;
-  ~Mem() { for(void * p : ptrs) spd_free(p); }
-  int get(void ** out, size_t bytes) {
-    *out = nullptr;
-    if(bytes == 0) bytes = 8;
-    SPD_TRY(spd_malloc(out, bytes));
-    ptrs.push_back(*out);
-    return SPD_OK;
-  }
-};
-
-double wall_s()
-{
-  struct timespec ts;
-  clock_gettime(CLOCK_MONOTONIC, &ts);
-  return (double) ts.tv_sec + 1e-9 * (double) ts.tv_nsec;
-}
-
 template<int L, int NC>
 int launch_coo(CooArgs const & a, uint32_t nb, hipStream_t st)
 {
@@ -292,40 +275,289 @@ int diff2(double const * a, double const * b, uint64_t n, double * out, hipStrea
 
 }  // namespace
 
+int sst_coo_mttkrp(uint32_t const * const * ind, double const * vals, uint64_t nnz,
+    int N, int mode, double const * const * mats, uint32_t R, uint64_t rows, double * out,
+    hipStream_t st)
+{
+  return coo_mttkrp(ind, vals, nnz, N, mode, mats, R, rows, out, st);
+}
+
+
+double sst_wall_s()
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double) ts.tv_sec + 1e-9 * (double) ts.tv_nsec;
+}
+
+static int st_h2d(void * dst, void const * src, size_t bytes, hipStream_t st)
+{
+  if(bytes == 0) return SPD_OK;
+  SPD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  SPD_HIP(hipStreamSynchronize(st));
+  return SPD_OK;
+}
+
+static int st_d2h(void * dst, void const * src, size_t bytes, hipStream_t st)
+{
+  if(bytes == 0) return SPD_OK;
+  SPD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+  SPD_HIP(hipStreamSynchronize(st));
+  return SPD_OK;
+}
+
+static int st_d2d(void * dst, void const * src, size_t bytes, hipStream_t st)
+{
+  if(bytes == 0) return SPD_OK;
+  SPD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+  return SPD_OK;
+}
+
+size_t StreamState::footprint(int N, int t, uint64_t R, uint64_t const * cap)
+{
+  uint64_t maxdim = 1;
+  size_t b = 0;
+  for(int m=0; m < N; ++m) {
+    b += R * R * sizeof(double);
+    if(m == t) continue;
+    b += 3 * cap[m] * R * sizeof(double);
+    if(cap[m] > maxdim) maxdim = cap[m];
+  }
+  uint64_t const RP = dn_rpad((uint32_t) R);
+  b += 3 * maxdim * R * sizeof(double);
+  b += (4 + SPD_MAX_MODES) * R * R * sizeof(double) + 2 * RP * RP * sizeof(double);
+  return b;
+}
+
+int StreamState::init(int nmodes, spd_stream_cfg const & c, uint64_t const * caps,
+    hipStream_t stream)
+{
+  N = nmodes;
+  cfg = c;
+  t = c.stream_mode;
+  R = (uint32_t) c.rank;
+  RP = dn_rpad(R);
+  st = stream;
+  nslices = 0;
+  zero_cap = 0;
+  d_zero_ind = nullptr;
+  for(int m=0; m < N; ++m) {
+    cap[m] = (m == t) ? 1 : caps[m];
+    rows[m] = 0;
+    prev_dim[m] = 0;
+  }
+  uint64_t maxdim = 1;
+  for(int m=0; m < N; ++m) if(m != t && cap[m] > maxdim) maxdim = cap[m];
+  for(int m=0; m < N; ++m) {
+    Unew[m] = Uold[m] = Dual[m] = nullptr;
+    SPD_TRY(mem.get((void **) &aTa[m], (size_t) R * R * sizeof(double)));
+    SPD_TRY(spd_zero_async(aTa[m], (size_t) R * R * sizeof(double), st));
+    if(m == t) continue;
+    size_t const bytes = cap[m] * R * sizeof(double);
+    SPD_TRY(mem.get((void **) &Unew[m], bytes));
+    SPD_TRY(mem.get((void **) &Uold[m], bytes));
+    SPD_TRY(mem.get((void **) &Dual[m], bytes));
+    SPD_TRY(spd_zero_async(Unew[m], bytes, st));
+    SPD_TRY(spd_zero_async(Uold[m], bytes, st));
+    SPD_TRY(spd_zero_async(Dual[m], bytes, st));
+  }
+  SPD_TRY(mem.get((void **) &time_row, R * sizeof(double)));
+  SPD_TRY(spd_zero_async(time_row, R * sizeof(double), st));
+  SPD_TRY(mem.get((void **) &Mbuf, maxdim * R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &aux, maxdim * R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &init_buf, maxdim * R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &old_gram, (size_t) R * R * sizeof(double)));
+  SPD_TRY(spd_zero_async(old_gram, (size_t) R * R * sizeof(double), st));
+  SPD_TRY(mem.get((void **) &ginv, (size_t) RP * RP * sizeof(double)));
+  SPD_TRY(mem.get((void **) &H, (size_t) RP * RP * sizeof(double)));
+  SPD_TRY(mem.get((void **) &cross, (size_t) SPD_MAX_MODES * R * R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &tgram, (size_t) R * R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &sc, (16 + 4 + R) * sizeof(double)));
+  SPD_TRY(mem.get((void **) &info, sizeof(int)));
+  SPD_TRY(spd_zero_async(info, sizeof(int), st));
+  SPD_TRY(mem.get((void **) &ctl, 2 * sizeof(int)));
+  SPD_TRY(mem.get((void **) &lam1, R * sizeof(double)));
+  SPD_TRY(mem.get((void **) &tmpv, R * sizeof(double)));
+  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, lam1, (uint64_t) R, 1.0);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int StreamState::slice(uint32_t const * const * ind_in, double const * vals_b, uint64_t bn,
+    double xnorm, double * local_err, uint64_t * outer_out, double * secs_out)
+{
+  SPD_HIP(hipStreamSynchronize(st));
+  double const t0 = sst_wall_s();
+  double * const d_rho = sc + 0;
+  double * const d_diff = sc + 2;
+  double * const d_inner = sc + 4;
+  double * const d_sums = sc + 8;
+  // the streamed mode of a slice is all zeros
+  if(bn > zero_cap) {
+    uint64_t nc = zero_cap ? zero_cap : 1024;
+    while(nc < bn) nc *= 2;
+    SPD_TRY(mem.get((void **) &d_zero_ind, nc * sizeof(uint32_t)));
+    SPD_TRY(spd_zero_async(d_zero_ind, nc * sizeof(uint32_t), st));
+    zero_cap = nc;
+  }
+  uint32_t const * ind_b[SPD_MAX_MODES];
+  for(int m=0; m < N; ++m) ind_b[m] = (m == t) ? d_zero_ind : ind_in[m];
+  // slice dims (ParserSimple.cc:131-140): running maxima of the labels
+  uint64_t bd[SPD_MAX_MODES];
+  for(int m=0; m < N; ++m) {
+    if(m == t) { bd[m] = 1; continue; }
+    bd[m] = (batch_max[m] + 1 > prev_dim[m]) ? batch_max[m] + 1 : prev_dim[m];
+    if(bd[m] > cap[m]) {
+      spd_set_error(SPD_EINPUT, "stream: mode %d needs %llu rows, capacity %llu", m,
+          (unsigned long long) bd[m], (unsigned long long) cap[m]);
+      return SPD_EINPUT;
+    }
+    prev_dim[m] = bd[m];
+  }
+  double const * aTa_c[SPD_MAX_MODES];
+  for(int m=0; m < N; ++m) aTa_c[m] = aTa[m];
+
+  // grow_mats (StreamCPD.cc:192-220): new rows from rand(), Grams
+  for(int m=0; m < N; ++m) {
+    if(m == t) continue;
+    if(bd[m] > rows[m]) {
+      uint64_t const nnew = (bd[m] - rows[m]) * R;
+      hrand.resize(nnew);
+      cfg.fill_rand(cfg.rand_ctx, hrand.data(), nnew);
+      SPD_TRY(st_h2d(Unew[m] + rows[m] * R, hrand.data(), nnew * sizeof(double), st));
+      rows[m] = bd[m];
+    }
+    SPD_TRY(dn_gram(Unew[m], rows[m], R, nullptr, nullptr, aTa[m], st));
+  }
+  if(nslices == 0) {
+    for(int m=0; m < N; ++m) {
+      if(m == t) continue;
+      SPD_TRY(dn_colnorm(Unew[m], rows[m], R, tmpv, st));
+      SPD_TRY(dn_div_cols(Unew[m], rows[m], R, tmpv, nullptr, st));
+    }
+  }
+  double const * mats[SPD_MAX_MODES];
+  for(int m=0; m < N; ++m) mats[m] = (m == t) ? time_row : Unew[m];
+
+  double prev_delta = 0.;
+  uint64_t outer = 0;
+  for(outer=0; outer < cfg.max_its; ++outer) {
+    double delta = 0.;
+    // time row: MTTKRP to one row and the closed-form solve
+    SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, t, mats, R, 1, Mbuf, st));
+    SPD_TRY(dn_chol_inv(aTa_c, N, t, R, cfg.con[t].diag_add, 0, ginv, d_rho, info, st));
+    SPD_TRY(dn_apply(Mbuf, SPD_F64, ginv, 1, R, time_row, nullptr, st));
+    hipLaunchKernelGGL(k_time_gram, dim3(1), dim3(256), 0, st, aTa[t],
+        (double const *) time_row, old_gram, R);
+    SPD_LAUNCH_CHECK();
+    for(int m=0; m < N; ++m) {
+      if(m == t) continue;
+      uint64_t const I = bd[m];
+      SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, m, mats, R, I, Mbuf, st));
+      // add_historical (StreamCPD.cc:223-294)
+      HistSrc hs = {};
+      int nc = 0;
+      for(int mm=0; mm < N; ++mm) {
+        if(mm == m || mm == t) continue;
+        double * cm = cross + (size_t) nc * R * R;
+        SPD_TRY(dn_xgram(Uold[mm], Unew[mm], rows[mm], R, cm, st));
+        hs.m[nc++] = cm;
+      }
+      hipLaunchKernelGGL(k_hist, dim3(1), dim3(256), 0, st, (double const *) old_gram,
+          hs, nc, R, RP, H);
+      SPD_LAUNCH_CHECK();
+      SPD_TRY(dn_apply_acc(Uold[m], H, I, R, Mbuf, st));
+      SPD_TRY(spd_zero_async(Dual[m], I * R * sizeof(double), st));
+      spd_con const & con = cfg.con[m];
+      if(con.kind == SPD_CON_ADMM) {
+        SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
+        SPD_TRY(dn_admm_solve(Mbuf, SPD_F64, ginv, d_rho, I, R, Unew[m], Dual[m], aux,
+            init_buf, con.prox, cfg.max_inner, cfg.inner_tol, d_sums, ctl, st));
+      } else {
+        SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 0, ginv, d_rho, info, st));
+        SPD_TRY(dn_apply(Mbuf, SPD_F64, ginv, I, R, Unew[m], nullptr, st));
+      }
+      SPD_TRY(dn_gram(Unew[m], I, R, nullptr, nullptr, aTa[m], st));
+      SPD_TRY(diff2(Uold[m], Unew[m], I * R, d_diff, st));
+      double dd[2];
+      SPD_TRY(st_d2h(dd, d_diff, sizeof(dd), st));
+      delta += sqrt(dd[0]) / sqrt(dd[1]);
+    }
+    if(cfg.verbosity > 1) {
+      printf("  delta: %e prev_delta: %e (%e diff)\n", delta, prev_delta,
+          fabs(delta - prev_delta));
+    }
+    if(outer > 0 && fabs(delta - prev_delta) < cfg.tol) {
+      ++outer;
+      break;
+    }
+    prev_delta = delta;
+  }
+  // forget, old <- new (StreamCPD.cc:488-506)
+  hipLaunchKernelGGL(k_scale, dim3((unsigned) spd_cdiv((uint64_t) R * R, 256)), dim3(256),
+      0, st, aTa[t], (uint64_t) R * R, cfg.forget);
+  SPD_LAUNCH_CHECK();
+  for(int m=0; m < N; ++m) {
+    if(m == t) continue;
+    SPD_TRY(st_d2d(Uold[m], Unew[m], rows[m] * R * sizeof(double), st));
+  }
+  SPD_HIP(hipStreamSynchronize(st));
+  if(secs_out) *secs_out = sst_wall_s() - t0;
+  if(outer_out) *outer_out = outer;
+  ++nslices;
+
+  // local error of this slice (StreamCPD.cc:77-89): the model of the last
+  // time row against the slice (time index 0)
+  if(local_err) {
+    int small = 0;
+    for(int m=1; m < N; ++m) if(bd[m] < bd[small]) small = m;
+    uint64_t const nr = bd[small];
+    SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, small, mats, R, nr, Mbuf, st));
+    double const * fsm = (small == t) ? time_row : Unew[small];
+    SPD_TRY(dn_innerprod(fsm, Mbuf, SPD_F64, nr, R, lam1, d_inner, st));
+    SPD_TRY(dn_gram(time_row, 1, R, nullptr, nullptr, tgram, st));
+    double const * grams[SPD_MAX_MODES];
+    for(int m=0; m < N; ++m) grams[m] = (m == t) ? tgram : aTa[m];
+    SPD_TRY(dn_cpd_norm(grams, N, R, lam1, d_inner + 1, st));
+    double h[2];
+    SPD_TRY(st_d2h(h, d_inner, sizeof(h), st));
+    *local_err = (xnorm + h[1] - 2. * h[0]) / xnorm;
+  }
+  return SPD_OK;
+}
+
+int StreamState::cholesky_info(int * out)
+{
+  return st_d2h(out, info, sizeof(int), st);
+}
+
 extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
-    uint32_t const * const * inds, double const * vals, spd_stream_cfg const * cfg,
+    uint32_t const * const * inds, double const * vals, spd_stream_cfg const * cfg_in,
     spd_stream_result * res)
 {
   SPD_TRY(spd_init());
   hipStream_t const st = spd_hs(nullptr);
-  int const t = cfg->stream_mode;
-  uint64_t const R64 = cfg->rank;
+  spd_stream_cfg const cfg = *cfg_in;
+  if(cfg.fill_rand == nullptr) {
+    spd_set_error(SPD_EINPUT, "stream: no random fill for new factor rows");
+    return SPD_EINPUT;
+  }
+  int const t = cfg.stream_mode;
+  uint64_t const R64 = cfg.rank;
   if(N < 2 || N > SPD_MAX_MODES || t < 0 || t >= N || R64 == 0 || R64 > 256 ||
       nnz == 0 || nnz >= (1ull << 32)) {
     spd_set_error(SPD_EINPUT, "stream: bad shape (nmodes %d, stream mode %d, rank %llu,"
         " nnz %llu)", N, t, (unsigned long long) R64, (unsigned long long) nnz);
     return SPD_EINPUT;
   }
-  for(int m=0; m < N; ++m) {
-    if(m != t && cfg->con[m].kind == SPD_CON_NONE) {
-      spd_set_error(SPD_EINPUT, "stream: mode %d needs a constraint", m);
-      return SPD_EINPUT;
-    }
-  }
-  if(cfg->con[t].kind == SPD_CON_ADMM) {
-    spd_set_error(SPD_EINPUT, "stream: the streamed mode takes closed-form "
-        "constraints only");
-    return SPD_EINPUT;
-  }
+  SPD_TRY(sst_check_cfg(N, cfg));
   uint32_t const R = (uint32_t) R64;
-  uint32_t const RP = dn_rpad(R);
 
   // ---- batches (ParserSimple::next_batch, ParserSimple.cc:89-149) ----
   std::vector<uint64_t> bstart, bend;
-  std::vector<std::vector<uint64_t>> bdims;
+  std::vector<std::vector<uint64_t>> bmax;
   {
     uint64_t ptr = 0;
-    uint64_t prev[SPD_MAX_MODES] = {0};
     uint64_t it = 0;
     while(ptr < nnz) {
       uint64_t s = ptr;
@@ -337,31 +569,26 @@ extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
             (unsigned long long) it);
         break;
       }
-      std::vector<uint64_t> d(N, 1);
+      std::vector<uint64_t> mx(N, 0);
       for(int m=0; m < N; ++m) {
         if(m == t) continue;
-        uint64_t mx = 0;
-        for(uint64_t x=s; x < e; ++x) if(inds[m][x] > mx) mx = inds[m][x];
-        d[m] = (mx + 1 > prev[m]) ? mx + 1 : prev[m];
-        prev[m] = d[m];
+        for(uint64_t x=s; x < e; ++x) if(inds[m][x] > mx[m]) mx[m] = inds[m][x];
       }
       bstart.push_back(s);
       bend.push_back(e);
-      bdims.push_back(d);
+      bmax.push_back(mx);
       ptr = e;
       ++it;
     }
   }
   uint64_t const T = bstart.size();
-  uint64_t maxrows[SPD_MAX_MODES] = {0};
+  uint64_t caps[SPD_MAX_MODES];
   for(int m=0; m < N; ++m) {
-    maxrows[m] = (m == t) ? 1 : (T ? bdims[T - 1][m] : 1);
+    caps[m] = 1;
+    for(uint64_t b=0; b < T; ++b) if(bmax[b][m] + 1 > caps[m]) caps[m] = bmax[b][m] + 1;
   }
-  uint64_t maxdim = 1;
-  for(int m=0; m < N; ++m) if(m != t && maxrows[m] > maxdim) maxdim = maxrows[m];
 
-  // ---- device state ----
-  Mem mem;
+  SpdMem mem;
   uint32_t * d_ind[SPD_MAX_MODES] = {};
   double * d_vals = nullptr;
   for(int m=0; m < N; ++m) {
@@ -370,207 +597,38 @@ extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
   }
   SPD_TRY(mem.get((void **) &d_vals, nnz * sizeof(double)));
   SPD_TRY(spd_h2d(d_vals, vals, nnz * sizeof(double)));
-  // stream mode of a batch is all zeros
-  uint32_t * d_zero_ind = nullptr;
-  uint64_t maxb = 0;
-  for(uint64_t b=0; b < T; ++b) if(bend[b] - bstart[b] > maxb) maxb = bend[b] - bstart[b];
-  SPD_TRY(mem.get((void **) &d_zero_ind, (maxb ? maxb : 1) * sizeof(uint32_t)));
-  SPD_TRY(spd_zero_async(d_zero_ind, (maxb ? maxb : 1) * sizeof(uint32_t), st));
-
-  double * Unew[SPD_MAX_MODES] = {};
-  double * Uold[SPD_MAX_MODES] = {};
-  double * Dual[SPD_MAX_MODES] = {};
-  double * aTa[SPD_MAX_MODES] = {};
-  for(int m=0; m < N; ++m) {
-    SPD_TRY(mem.get((void **) &aTa[m], (size_t) R * R * sizeof(double)));
-    SPD_TRY(spd_zero_async(aTa[m], (size_t) R * R * sizeof(double), st));
-    if(m == t) continue;
-    size_t const bytes = maxrows[m] * R * sizeof(double);
-    SPD_TRY(mem.get((void **) &Unew[m], bytes));
-    SPD_TRY(mem.get((void **) &Uold[m], bytes));
-    SPD_TRY(mem.get((void **) &Dual[m], bytes));
-    SPD_TRY(spd_zero_async(Unew[m], bytes, st));
-    SPD_TRY(spd_zero_async(Uold[m], bytes, st));
-    SPD_TRY(spd_zero_async(Dual[m], bytes, st));
-  }
-  double * time_row = nullptr;
   double * gtime = nullptr;
-  SPD_TRY(mem.get((void **) &time_row, R * sizeof(double)));
-  SPD_TRY(spd_zero_async(time_row, R * sizeof(double), st));
   SPD_TRY(mem.get((void **) &gtime, (T ? T : 1) * R * sizeof(double)));
   SPD_TRY(spd_zero_async(gtime, (T ? T : 1) * R * sizeof(double), st));
-  double * Mbuf = nullptr;
-  double * aux = nullptr;
-  double * init = nullptr;
-  SPD_TRY(mem.get((void **) &Mbuf, maxdim * R * sizeof(double)));
-  SPD_TRY(mem.get((void **) &aux, maxdim * R * sizeof(double)));
-  SPD_TRY(mem.get((void **) &init, maxdim * R * sizeof(double)));
-  double * old_gram = nullptr;
-  double * ginv = nullptr;
-  double * H = nullptr;
-  double * cross = nullptr;
-  double * tgram = nullptr;
-  SPD_TRY(mem.get((void **) &old_gram, (size_t) R * R * sizeof(double)));
-  SPD_TRY(spd_zero_async(old_gram, (size_t) R * R * sizeof(double), st));
-  SPD_TRY(mem.get((void **) &ginv, (size_t) RP * RP * sizeof(double)));
-  SPD_TRY(mem.get((void **) &H, (size_t) RP * RP * sizeof(double)));
-  SPD_TRY(mem.get((void **) &cross, (size_t) SPD_MAX_MODES * R * R * sizeof(double)));
-  SPD_TRY(mem.get((void **) &tgram, (size_t) R * R * sizeof(double)));
-  double * sc = nullptr;   // [0] rho, [2..3] diff sums, [4..5] inner, [8..] admm sums
-  SPD_TRY(mem.get((void **) &sc, (16 + 4 + R) * sizeof(double)));
-  int * info = nullptr;
-  SPD_TRY(mem.get((void **) &info, sizeof(int)));
-  SPD_TRY(spd_zero_async(info, sizeof(int), st));
-  double * lam1 = nullptr;
-  double * tmpv = nullptr;
-  SPD_TRY(mem.get((void **) &lam1, R * sizeof(double)));
-  SPD_TRY(mem.get((void **) &tmpv, R * sizeof(double)));
-  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, lam1, (uint64_t) R, 1.0);
-  SPD_LAUNCH_CHECK();
-  double * const d_rho = sc + 0;
-  double * const d_diff = sc + 2;
-  double * const d_inner = sc + 4;
-  double * const d_sums = sc + 8;
 
-  double const * aTa_c[SPD_MAX_MODES];
-  for(int m=0; m < N; ++m) aTa_c[m] = aTa[m];
-  uint32_t const * ind_b[SPD_MAX_MODES];
-  double const * mats[SPD_MAX_MODES];
-  uint64_t rows[SPD_MAX_MODES] = {0};
-  std::vector<double> hrand;
-
+  StreamState s;
+  SPD_TRY(s.init(N, cfg, caps, st));
   for(uint64_t b=0; b < T; ++b) {
-    SPD_TRY(spd_sync());
-    double const t0 = wall_s();
-    uint64_t const s0 = bstart[b];
-    uint64_t const bn = bend[b] - bstart[b];
-    std::vector<uint64_t> const & bd = bdims[b];
-    for(int m=0; m < N; ++m) ind_b[m] = (m == t) ? d_zero_ind : d_ind[m] + s0;
-    double const * const vals_b = d_vals + s0;
-
-    // grow_mats (StreamCPD.cc:192-220): new rows from rand(), Grams
+    uint32_t const * ib[SPD_MAX_MODES];
     for(int m=0; m < N; ++m) {
-      if(m == t) continue;
-      if(bd[m] > rows[m]) {
-        uint64_t const nnew = (bd[m] - rows[m]) * R;
-        hrand.resize(nnew);
-        cfg->fill_rand(hrand.data(), nnew);
-        SPD_TRY(spd_h2d(Unew[m] + rows[m] * R, hrand.data(), nnew * sizeof(double)));
-        rows[m] = bd[m];
-      }
-      SPD_TRY(dn_gram(Unew[m], rows[m], R, nullptr, nullptr, aTa[m], st));
+      ib[m] = d_ind[m] + bstart[b];
+      s.batch_max[m] = bmax[b][m];
     }
-    if(b == 0) {
-      for(int m=0; m < N; ++m) {
-        if(m == t) continue;
-        SPD_TRY(dn_colnorm(Unew[m], rows[m], R, tmpv, st));
-        SPD_TRY(dn_div_cols(Unew[m], rows[m], R, tmpv, nullptr, st));
-      }
-    }
-    for(int m=0; m < N; ++m) mats[m] = (m == t) ? time_row : Unew[m];
-
-    double prev_delta = 0.;
+    double xn = 0.;
+    for(uint64_t x=bstart[b]; x < bend[b]; ++x) xn += vals[x] * vals[x];
+    double le = 0., secs = 0.;
     uint64_t outer = 0;
-    for(outer=0; outer < cfg->max_its; ++outer) {
-      double delta = 0.;
-      // time row: MTTKRP to one row and the closed-form solve
-      SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, t, mats, R, 1, Mbuf, st));
-      SPD_TRY(dn_chol_inv(aTa_c, N, t, R, cfg->con[t].diag_add, 0, ginv, d_rho, info, st));
-      SPD_TRY(dn_apply(Mbuf, SPD_F64, ginv, 1, R, time_row, nullptr, st));
-      hipLaunchKernelGGL(k_time_gram, dim3(1), dim3(256), 0, st, aTa[t],
-          (double const *) time_row, old_gram, R);
-      SPD_LAUNCH_CHECK();
-      for(int m=0; m < N; ++m) {
-        if(m == t) continue;
-        uint64_t const I = bd[m];
-        SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, m, mats, R, I, Mbuf, st));
-        // add_historical (StreamCPD.cc:223-294)
-        HistSrc hs = {};
-        int nc = 0;
-        for(int mm=0; mm < N; ++mm) {
-          if(mm == m || mm == t) continue;
-          double * cm = cross + (size_t) nc * R * R;
-          SPD_TRY(dn_xgram(Uold[mm], Unew[mm], rows[mm], R, cm, st));
-          hs.m[nc++] = cm;
-        }
-        hipLaunchKernelGGL(k_hist, dim3(1), dim3(256), 0, st, (double const *) old_gram,
-            hs, nc, R, RP, H);
-        SPD_LAUNCH_CHECK();
-        SPD_TRY(dn_apply_acc(Uold[m], H, I, R, Mbuf, st));
-        SPD_TRY(spd_zero_async(Dual[m], I * R * sizeof(double), st));
-        spd_con const & con = cfg->con[m];
-        if(con.kind == SPD_CON_ADMM) {
-          SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
-          for(uint64_t k=0; k < cfg->max_inner; ++k) {
-            SPD_TRY(dn_admm_step(Mbuf, SPD_F64, ginv, d_rho, I, R, Unew[m], aux, Dual[m],
-                init, con.prox, d_sums, st));
-            double s4[4];
-            SPD_TRY(spd_d2h(s4, d_sums, sizeof(s4)));
-            if(s4[1] <= cfg->inner_tol * s4[0] && s4[2] <= cfg->inner_tol * s4[3]) break;
-          }
-        } else {
-          SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 0, ginv, d_rho, info, st));
-          SPD_TRY(dn_apply(Mbuf, SPD_F64, ginv, I, R, Unew[m], nullptr, st));
-        }
-        SPD_TRY(dn_gram(Unew[m], I, R, nullptr, nullptr, aTa[m], st));
-        SPD_TRY(diff2(Uold[m], Unew[m], I * R, d_diff, st));
-        double dd[2];
-        SPD_TRY(spd_d2h(dd, d_diff, sizeof(dd)));
-        delta += sqrt(dd[0]) / sqrt(dd[1]);
-      }
-      if(cfg->verbosity > 1) {
-        printf("  delta: %e prev_delta: %e (%e diff)\n", delta, prev_delta,
-            fabs(delta - prev_delta));
-      }
-      if(outer > 0 && fabs(delta - prev_delta) < cfg->tol) {
-        ++outer;
-        break;
-      }
-      prev_delta = delta;
-    }
-    // forget, old <- new, keep the time row (StreamCPD.cc:488-511)
-    hipLaunchKernelGGL(k_scale, dim3((unsigned) spd_cdiv((uint64_t) R * R, 256)), dim3(256),
-        0, st, aTa[t], (uint64_t) R * R, cfg->forget);
-    SPD_LAUNCH_CHECK();
-    for(int m=0; m < N; ++m) {
-      if(m == t) continue;
-      SPD_TRY(spd_d2d(Uold[m], Unew[m], rows[m] * R * sizeof(double)));
-    }
-    SPD_TRY(spd_d2d(gtime + b * R, time_row, R * sizeof(double)));
-    SPD_TRY(spd_sync());
-    double const secs = wall_s() - t0;
+    SPD_TRY(s.slice(ib, d_vals + bstart[b], bend[b] - bstart[b], xn,
+        res->local_err ? &le : nullptr, &outer, &secs));
+    SPD_TRY(spd_d2d(gtime + b * R, s.time_row, R * sizeof(double)));
+    if(res->local_err) res->local_err[b] = le;
     if(res->batch_seconds) res->batch_seconds[b] = secs;
     if(res->outer_its) res->outer_its[b] = outer;
-
-    // local error of this slice (StreamCPD.cc:77-89): the model of the last
-    // time row against the slice (time index 0)
-    if(res->local_err) {
-      int small = 0;
-      for(int m=1; m < N; ++m) if(bd[m] < bd[small]) small = m;
-      uint64_t const nr = bd[small];
-      SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, small, mats, R, nr, Mbuf, st));
-      double const * fsm = (small == t) ? time_row : Unew[small];
-      SPD_TRY(dn_innerprod(fsm, Mbuf, SPD_F64, nr, R, lam1, d_inner, st));
-      SPD_TRY(dn_gram(time_row, 1, R, nullptr, nullptr, tgram, st));
-      double const * grams[SPD_MAX_MODES];
-      for(int m=0; m < N; ++m) grams[m] = (m == t) ? tgram : aTa[m];
-      SPD_TRY(dn_cpd_norm(grams, N, R, lam1, d_inner + 1, st));
-      double h[2];
-      SPD_TRY(spd_d2h(h, d_inner, sizeof(h)));
-      double xn = 0.;
-      for(uint64_t x=s0; x < bend[b]; ++x) xn += vals[x] * vals[x];
-      double const e2 = (xn + h[1] - 2. * h[0]) / xn;
-      res->local_err[b] = e2;
-      if(cfg->verbosity > 0) {
-        printf("batch %5llu: %7llu nnz (%0.5fs) (%0.3e NNZ/s) cpd: %+0.5f global: "
-            "%+0.5f local-1: %+0.5f local-10: %+0.5f\n", (unsigned long long) (b + 1),
-            (unsigned long long) bn, secs, (double) bn / secs, -1., -1., e2, -1.);
-        fflush(stdout);
-      }
+    if(cfg.verbosity > 0) {
+      printf("batch %5llu: %7llu nnz (%0.5fs) (%0.3e NNZ/s) cpd: %+0.5f global: "
+          "%+0.5f local-1: %+0.5f local-10: %+0.5f\n", (unsigned long long) (b + 1),
+          (unsigned long long) (bend[b] - bstart[b]), secs,
+          (double) (bend[b] - bstart[b]) / secs, -1., -1., le, -1.);
+      fflush(stdout);
     }
   }
   int hinfo = 0;
-  SPD_TRY(spd_d2h(&hinfo, info, sizeof(int)));
+  SPD_TRY(s.cholesky_info(&hinfo));
   if(hinfo != 0) {
     fprintf(stderr, "SPLATT: Cholesky factorisation failed in the stream (pivot %d).\n",
         hinfo);
@@ -578,15 +636,16 @@ extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
 
   // final error of the whole stream (StreamCPD.cc:549-556)
   res->nbatches = T;
-  for(int m=0; m < N; ++m) res->dims[m] = (m == t) ? T : rows[m];
+  for(int m=0; m < N; ++m) res->dims[m] = (m == t) ? T : s.rows[m];
+  double * const d_inner = s.sc + 4;
   if(res->final_err && T > 0) {
     int small = 0;
     uint64_t fd[SPD_MAX_MODES];
     for(int m=0; m < N; ++m) fd[m] = dims[m];
     for(int m=1; m < N; ++m) if(fd[m] < fd[small]) small = m;
     double const * fm[SPD_MAX_MODES];
-    for(int m=0; m < N; ++m) fm[m] = (m == t) ? gtime : Unew[m];
-    uint64_t const frows = (small == t) ? T : rows[small];
+    for(int m=0; m < N; ++m) fm[m] = (m == t) ? gtime : s.Unew[m];
+    uint64_t const frows = (small == t) ? T : s.rows[small];
     double * M2 = nullptr;
     SPD_TRY(mem.get((void **) &M2, fd[small] * R * sizeof(double)));
     if(small == t) {
@@ -603,11 +662,11 @@ extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
       for(int m=0; m < N; ++m) ib[m] = d_ind[m];
       SPD_TRY(coo_mttkrp(ib, d_vals, bend[T - 1], N, small, fm, R, fd[small], M2, st));
     }
-    SPD_TRY(dn_innerprod(fm[small], M2, SPD_F64, frows, R, lam1, d_inner, st));
+    SPD_TRY(dn_innerprod(fm[small], M2, SPD_F64, frows, R, s.lam1, d_inner, st));
     double const * grams[SPD_MAX_MODES];
-    SPD_TRY(dn_gram(gtime, T, R, nullptr, nullptr, tgram, st));
-    for(int m=0; m < N; ++m) grams[m] = (m == t) ? tgram : aTa[m];
-    SPD_TRY(dn_cpd_norm(grams, N, R, lam1, d_inner + 1, st));
+    SPD_TRY(dn_gram(gtime, T, R, nullptr, nullptr, s.tgram, st));
+    for(int m=0; m < N; ++m) grams[m] = (m == t) ? s.tgram : s.aTa[m];
+    SPD_TRY(dn_cpd_norm(grams, N, R, s.lam1, d_inner + 1, st));
     double h[2];
     SPD_TRY(spd_d2h(h, d_inner, sizeof(h)));
     double xn = 0.;
@@ -620,8 +679,25 @@ extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
     if(m == t) {
       SPD_TRY(spd_d2h(res->factors[m], gtime, T * R * sizeof(double)));
     } else {
-      SPD_TRY(spd_d2h(res->factors[m], Unew[m], rows[m] * R * sizeof(double)));
+      SPD_TRY(spd_d2h(res->factors[m], s.Unew[m], s.rows[m] * R * sizeof(double)));
     }
+  }
+  return SPD_OK;
+}
+
+int sst_check_cfg(int N, spd_stream_cfg const & cfg)
+{
+  int const t = cfg.stream_mode;
+  for(int m=0; m < N; ++m) {
+    if(m != t && cfg.con[m].kind == SPD_CON_NONE) {
+      spd_set_error(SPD_EINPUT, "stream: mode %d needs a constraint", m);
+      return SPD_EINPUT;
+    }
+  }
+  if(cfg.con[t].kind == SPD_CON_ADMM) {
+    spd_set_error(SPD_EINPUT, "stream: the streamed mode takes closed-form "
+        "constraints only");
+    return SPD_EINPUT;
   }
   return SPD_OK;
 }

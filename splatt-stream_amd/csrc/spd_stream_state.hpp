@@ -1,0 +1,73 @@
+// spd_stream_state.hpp -- the per-stream device state of streaming CPD and
+// its per-slice step, shared by the whole-stream driver (spd_stream_cpd)
+// and the multi-stream server.
+#pragma once
+#include "spd_internal.hpp"
+#include <vector>
+
+// Device allocations freed together.
+struct SpdMem {
+  std::vector<void *> ptrs;
+  ~SpdMem() { for(void * p : ptrs) spd_free(p); }
+  int get(void ** out, size_t bytes) {
+    *out = nullptr;
+    if(bytes == 0) bytes = 8;
+    SPD_TRY(spd_malloc(out, bytes));
+    ptrs.push_back(*out);
+    return SPD_OK;
+  }
+};
+
+// One stream's factors, Grams and work buffers, allocated for fixed row
+// capacities; everything runs on `st`.
+struct StreamState {
+  int N = 0;
+  int t = 0;
+  uint32_t R = 0;
+  uint32_t RP = 0;
+  spd_stream_cfg cfg = {};
+  hipStream_t st = nullptr;
+  uint64_t cap[SPD_MAX_MODES] = {};
+  uint64_t rows[SPD_MAX_MODES] = {};       // active factor rows
+  uint64_t prev_dim[SPD_MAX_MODES] = {};   // running dims of the slices
+  uint64_t batch_max[SPD_MAX_MODES] = {};  // largest label of the next slice (set by the caller)
+  uint64_t nslices = 0;
+  SpdMem mem;
+  double * Unew[SPD_MAX_MODES] = {};
+  double * Uold[SPD_MAX_MODES] = {};
+  double * Dual[SPD_MAX_MODES] = {};
+  double * aTa[SPD_MAX_MODES] = {};
+  double * time_row = nullptr;
+  double * Mbuf = nullptr;
+  double * aux = nullptr;
+  double * init_buf = nullptr;
+  double * old_gram = nullptr;
+  double * ginv = nullptr;
+  double * H = nullptr;
+  double * cross = nullptr;
+  double * tgram = nullptr;
+  double * sc = nullptr;       // [0] rho, [2..3] change sums, [4..5] error terms, [8..] ADMM
+  double * lam1 = nullptr;
+  double * tmpv = nullptr;
+  int * info = nullptr;
+  int * ctl = nullptr;
+  uint32_t * d_zero_ind = nullptr;
+  uint64_t zero_cap = 0;
+  std::vector<double> hrand;
+
+  // device bytes init() allocates for these capacities
+  static size_t footprint(int N, int t, uint64_t R, uint64_t const * cap);
+  int init(int nmodes, spd_stream_cfg const & c, uint64_t const * caps, hipStream_t stream);
+  // One time slice: ind[m] (device, the slice's nonzeros, labels of every
+  // mode but the streamed one) and vals (device); xnorm = sum of squared
+  // values. batch_max[] must hold the slice's largest label per mode.
+  int slice(uint32_t const * const * ind, double const * vals, uint64_t nnz, double xnorm,
+      double * local_err, uint64_t * outer_its, double * seconds);
+  int cholesky_info(int * out);
+};
+
+int sst_check_cfg(int N, spd_stream_cfg const & cfg);
+int sst_coo_mttkrp(uint32_t const * const * ind, double const * vals, uint64_t nnz,
+    int N, int mode, double const * const * mats, uint32_t R, uint64_t rows, double * out,
+    hipStream_t st);
+double sst_wall_s();

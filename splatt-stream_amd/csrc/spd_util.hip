@@ -176,9 +176,27 @@ extern "C" int spd_f32_to_f64(float const * src, double * dst, uint64_t n, void 
 
 // ---- scratch cache -------------------------------------------------------
 
-#define SPD_NSCRATCH 16
-static void * g_scr[SPD_NSCRATCH];
-static size_t g_scr_bytes[SPD_NSCRATCH];
+// Each host thread draws scratch from its own context: the library's
+// context (on the library stream) unless the thread bound another one with
+// spd_scratch_bind (worker threads that run on their own stream).
+static SpdScratchCtx g_ctx_lib;
+static thread_local SpdScratchCtx * t_ctx = nullptr;
+
+void spd_scratch_bind(SpdScratchCtx * ctx)
+{
+  t_ctx = ctx;
+}
+
+void spd_scratch_release(SpdScratchCtx * ctx)
+{
+  if(ctx == nullptr) return;
+  if(ctx->st) (void) hipStreamSynchronize(ctx->st);
+  for(int k=0; k < SPD_NSCRATCH; ++k) {
+    if(ctx->p[k]) (void) hipFree(ctx->p[k]);
+    ctx->p[k] = nullptr;
+    ctx->bytes[k] = 0;
+  }
+}
 
 int spd_scratch(int slot, size_t bytes, void ** ptr)
 {
@@ -186,20 +204,22 @@ int spd_scratch(int slot, size_t bytes, void ** ptr)
     spd_set_error(SPD_EINPUT, "bad scratch slot %d", slot);
     return SPD_EINPUT;
   }
-  if(g_scr_bytes[slot] < bytes || g_scr[slot] == nullptr) {
-    if(g_scr[slot] != nullptr) {
+  SpdScratchCtx * c = t_ctx ? t_ctx : &g_ctx_lib;
+  hipStream_t const st = (c == &g_ctx_lib) ? g_stream : c->st;
+  if(c->bytes[slot] < bytes || c->p[slot] == nullptr) {
+    if(c->p[slot] != nullptr) {
       // the stream may still use the old buffer
-      SPD_HIP(hipStreamSynchronize(g_stream));
-      SPD_HIP(hipFree(g_scr[slot]));
-      g_scr[slot] = nullptr;
-      g_scr_bytes[slot] = 0;
+      SPD_HIP(hipStreamSynchronize(st));
+      SPD_HIP(hipFree(c->p[slot]));
+      c->p[slot] = nullptr;
+      c->bytes[slot] = 0;
     }
     size_t want = bytes < 4096 ? 4096 : bytes;
     want += want / 8;
-    SPD_TRY(spd_malloc(&g_scr[slot], want));
-    g_scr_bytes[slot] = want;
+    SPD_TRY(spd_malloc(&c->p[slot], want));
+    c->bytes[slot] = want;
   }
-  *ptr = g_scr[slot];
+  *ptr = c->p[slot];
   return SPD_OK;
 }
 

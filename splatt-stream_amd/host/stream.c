@@ -41,8 +41,9 @@ void splatt_amd_free_stream_result(splatt_amd_stream_result * r)
   free(r);
 }
 
-static void p_fill_rand(double * vals, uint64_t n)
+static void p_fill_rand(void * ctx, double * vals, uint64_t n)
 {
+  (void) ctx;
   spl_fill_rand(vals, n);
 }
 

@@ -218,6 +218,26 @@ def lib():
     L.splatt_cpd_stream.restype = C.c_int
     L.splatt_cpd_stream.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_double,
                                     C.POINTER(CpdOpts), C.POINTER(GlobalOpts)]
+    L.splatt_amd_server_create.restype = C.c_void_p
+    L.splatt_amd_server_create.argtypes = [C.c_size_t, C.c_int]
+    L.splatt_amd_server_open.restype = C.c_int64
+    L.splatt_amd_server_open.argtypes = [C.c_void_p, C.c_uint64, u64p, C.c_uint64, C.c_uint64,
+                                         C.c_double, C.POINTER(CpdOpts), C.c_uint]
+    L.splatt_amd_server_submit.restype = C.c_int64
+    L.splatt_amd_server_submit.argtypes = [C.c_void_p, C.c_int64, C.c_uint64,
+                                           C.POINTER(u64p), f64p]
+    L.splatt_amd_server_drain.restype = C.c_int
+    L.splatt_amd_server_drain.argtypes = [C.c_void_p]
+    L.splatt_amd_server_slice.restype = C.c_int
+    L.splatt_amd_server_slice.argtypes = [C.c_void_p, C.c_int64, f64p, f64p, f64p, u64p]
+    L.splatt_amd_server_factors.restype = C.c_int
+    L.splatt_amd_server_factors.argtypes = [C.c_void_p, C.c_int64, C.c_uint64, f64p, u64p]
+    L.splatt_amd_server_close.restype = C.c_int
+    L.splatt_amd_server_close.argtypes = [C.c_void_p, C.c_int64]
+    L.splatt_amd_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_size_t),
+                                          C.POINTER(C.c_size_t), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]
+    L.splatt_amd_server_free.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -509,3 +529,85 @@ def run_stream(inds, vals, dims, rank, stream_mode, forget=0.95, max_its=5, seed
     L.splatt_free_global_opts(go)
     L.splatt_free_coord(tt)
     return out
+
+
+class Server:
+    """The multi-stream server (splatt_amd_server_*)."""
+
+    def __init__(self, budget=0, max_inflight=4):
+        self.L = lib()
+        self.ptr = self.L.splatt_amd_server_create(budget, max_inflight)
+        if not self.ptr:
+            raise RuntimeError("server create failed: " + last_error())
+        self.ranks = {}
+        self.nmodes = {}
+        self.tmode = {}
+
+    def open(self, dims_cap, rank, stream_mode, forget=0.95, max_its=5, frob=None, seed=1):
+        co = self.L.splatt_alloc_cpd_opts()
+        co[0].max_iterations = max_its
+        if frob is not None:
+            modes = np.array([stream_mode], dtype=np.uint64)
+            self.L.splatt_register_frob(co, frob, modes.ctypes.data_as(u64p), 1)
+        cap = np.ascontiguousarray(dims_cap, dtype=np.uint64)
+        sid = self.L.splatt_amd_server_open(self.ptr, len(cap), cap.ctypes.data_as(u64p), rank,
+                                            stream_mode, forget, co, seed)
+        self.L.splatt_free_cpd_opts(co)
+        if sid < 0:
+            raise RuntimeError("server open failed: " + last_error())
+        self.ranks[sid] = rank
+        self.nmodes[sid] = len(cap)
+        self.tmode[sid] = stream_mode
+        return sid
+
+    def submit(self, sid, inds, vals):
+        keep = [np.ascontiguousarray(i, dtype=np.uint64) for i in inds]
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        ptrs = (u64p * len(keep))(*[k.ctypes.data_as(u64p) for k in keep])
+        t = self.L.splatt_amd_server_submit(self.ptr, sid, len(v), ptrs, v.ctypes.data_as(f64p))
+        if t < 0:
+            raise RuntimeError("submit failed: " + last_error())
+        return t
+
+    def drain(self):
+        if self.L.splatt_amd_server_drain(self.ptr) != SUCCESS:
+            raise RuntimeError("drain failed: " + last_error())
+
+    def slice(self, ticket):
+        lat, gs, le = C.c_double(), C.c_double(), C.c_double()
+        oi = C.c_uint64()
+        rc = self.L.splatt_amd_server_slice(self.ptr, ticket, C.byref(lat), C.byref(gs),
+                                            C.byref(le), C.byref(oi))
+        if rc < 0:
+            raise RuntimeError("slice failed: " + last_error())
+        if rc == 1:
+            return None
+        return dict(latency=lat.value, gpu=gs.value, local_err=le.value, outer=oi.value)
+
+    def factors(self, sid, mode):
+        rows = C.c_uint64()
+        self.L.splatt_amd_server_factors(self.ptr, sid, mode, None, C.byref(rows))
+        out = np.zeros((max(rows.value, 1), self.ranks[sid]), dtype=np.float64)
+        self.L.splatt_amd_server_factors(self.ptr, sid, mode, out.ctypes.data_as(f64p),
+                                         C.byref(rows))
+        return out[:rows.value]
+
+    def close(self, sid):
+        self.L.splatt_amd_server_close(self.ptr, sid)
+
+    def stats(self):
+        u, b = C.c_size_t(), C.c_size_t()
+        a, w = C.c_int(), C.c_int()
+        self.L.splatt_amd_server_stats(self.ptr, C.byref(u), C.byref(b), C.byref(a), C.byref(w))
+        return dict(used=u.value, budget=b.value, admitted=a.value, waiting=w.value)
+
+    def free(self):
+        if self.ptr:
+            self.L.splatt_amd_server_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
