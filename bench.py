@@ -224,6 +224,80 @@ def bench_stream_c4(nslices, cpu_slices):
     return out
 
 
+def serve_c5_streams(nstreams, per_stream, seed=20261207):
+    """The serving workload: independent 3-mode streams (time last) with
+    non-stream dims from U[10k, 1M] and rank from {8, 16, 32, 64}; per-slice
+    nnz lognormal (median 1e5, sigma 1) clipped to [5e3, 2e6], Zipf(1.2)
+    coordinates distinct within a slice, values U(0,1)."""
+    rng = np.random.default_rng(seed)
+    streams = []
+    for k in range(nstreams):
+        dims = [int(rng.integers(10_000, 1_000_001)), int(rng.integers(10_000, 1_000_001))]
+        R = int(rng.choice([8, 16, 32, 64]))
+        nn = np.clip(np.round(np.exp(np.log(1e5) + rng.standard_normal(per_stream))), 5e3, 2e6)
+        sl = []
+        for t in range(per_stream):
+            ij, v, _ = ps.synth_coord(dims, int(nn[t]), zipf=[1.2, 1.2], vals="uniform",
+                                      compact=False, seed=seed + 1000 * k + t, shuffle=True)
+            sl.append(([ij[0], ij[1], np.full(len(v), t, dtype=np.uint64)], v))
+        streams.append(dict(dims=dims, R=R, slices=sl))
+    return streams
+
+
+def run_server(streams, arrivals, max_inflight):
+    """Submit every slice at its arrival time (seconds from start, stream
+    order kept), wait, return per-slice latencies and the span."""
+    sv = ps.Server(budget=0, max_inflight=max_inflight)
+    ids = [sv.open(s["dims"] + [1], s["R"], 2, forget=0.95, max_its=5, frob=1e-2, seed=1 + k)
+           for k, s in enumerate(streams)]
+    order = sorted(arrivals)          # (time, stream, slice)
+    tickets = []
+    t0 = time.perf_counter()
+    for at, k, j in order:
+        now = time.perf_counter() - t0
+        if at > now:
+            time.sleep(at - now)
+        tickets.append(sv.submit(ids[k], *streams[k]["slices"][j]))
+    for sid in ids:
+        sv.close(sid)
+    sv.drain()
+    span = time.perf_counter() - t0
+    info = [sv.slice(t) for t in tickets]
+    sv.free()
+    lat = np.array([i["latency"] for i in info])
+    return dict(slices=len(lat), span_s=span, slices_per_s=len(lat) / span,
+                p50_latency_s=float(np.percentile(lat, 50)),
+                p99_latency_s=float(np.percentile(lat, 99)),
+                mean_gpu_s=float(np.mean([i["gpu"] for i in info])))
+
+
+def bench_serve_c5(nstreams, per_stream, rate, inflight):
+    """Serving: nstreams streams, Poisson arrivals at `rate` slices/s in
+    aggregate, served with `inflight` slices in flight against the
+    one-slice-at-a-time loop (the same server with one slice in flight)."""
+    streams = serve_c5_streams(nstreams, per_stream)
+    rng = np.random.default_rng(7)
+    arrivals = []
+    t = 0.0
+    nxt = [0] * nstreams
+    total = nstreams * per_stream
+    while len(arrivals) < total:
+        t += rng.exponential(1.0 / rate)
+        k = int(rng.integers(0, nstreams))
+        while nxt[k] >= per_stream:
+            k = (k + 1) % nstreams
+        arrivals.append((t, k, nxt[k]))
+        nxt[k] += 1
+    batched = run_server(streams, arrivals, inflight)
+    serial = run_server(streams, arrivals, 1)
+    nnz = sum(len(v) for s in streams for _, v in s["slices"])
+    return dict(workload="serving: %d streams x %d slices (%d nnz), Poisson arrivals %.0f "
+                         "slices/s, dims U[10k,1M], rank in {8,16,32,64}, frob 1e-2, 5 its, "
+                         "FP64" % (nstreams, per_stream, nnz, rate),
+                inflight=inflight, batched=batched, one_at_a_time=serial,
+                throughput_ratio=batched["slices_per_s"] / serial["slices_per_s"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +314,10 @@ def main():
                     help="skip the CPD (1000x800x600) and streaming measurements")
     ap.add_argument("--stream-slices", type=int, default=60)
     ap.add_argument("--stream-cpu-slices", type=int, default=2)
+    ap.add_argument("--serve-streams", type=int, default=64)
+    ap.add_argument("--serve-slices", type=int, default=3)
+    ap.add_argument("--serve-rate", type=float, default=500.0)
+    ap.add_argument("--serve-inflight", type=int, default=8)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a streaming kernel of known traffic (FP64->FP32 "
                          "conversion of 2^26 values: 512 MiB read, 256 MiB written) "
@@ -323,6 +401,12 @@ def main():
         log("stream c4: %.2f slices/s, p50 %.4fs p99 %.4fs" % (
             c4["slices_per_s"], c4["p50_slice_s"], c4["p99_slice_s"]))
         extra["stream_c4"] = c4
+        c5 = bench_serve_c5(args.serve_streams, args.serve_slices, args.serve_rate,
+                            args.serve_inflight)
+        log("serve c5: %.2f slices/s (p50 %.3fs p99 %.3fs) vs one-at-a-time %.2f slices/s"
+            % (c5["batched"]["slices_per_s"], c5["batched"]["p50_latency_s"],
+               c5["batched"]["p99_latency_s"], c5["one_at_a_time"]["slices_per_s"]))
+        extra["serve_c5"] = c5
 
     line = {
         "metric": "MTTKRP HBM GB/s",
