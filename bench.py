@@ -31,6 +31,21 @@ sys.path.insert(0, os.path.join(ROOT, "splatt-stream_amd"))
 import pysplatt as ps  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per root-kernel launch from the FETCH_SIZE / WRITE_SIZE counter
+# runs of this workload (scaled by a copy of known size), kept with the
+# rocprofv3 summary they come from
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "mttkrp_root3_f32_zipf50m_pmc.md")
+
+
+def measured_traffic():
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -318,6 +333,8 @@ def main():
     ap.add_argument("--serve-slices", type=int, default=3)
     ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--serve-inflight", type=int, default=8)
+    ap.add_argument("--only", choices=["cpd", "stream", "serve"], default=None,
+                    help="run one secondary workload alone and print its JSON (profiling)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a streaming kernel of known traffic (FP64->FP32 "
                          "conversion of 2^26 values: 512 MiB read, 256 MiB written) "
@@ -327,6 +344,19 @@ def main():
         print("bench.py runs on one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.only is not None:
+        threads = min(16, os.cpu_count() or 1)
+        if args.only == "cpd":
+            out = bench_cpd_c1(threads)
+        elif args.only == "stream":
+            out = bench_stream_c4(args.stream_slices,
+                                  0 if args.skip_cpu else args.stream_cpu_slices)
+        else:
+            out = bench_serve_c5(args.serve_streams, args.serve_slices, args.serve_rate,
+                                 args.serve_inflight)
+        print(json.dumps(out))
+        return
 
     L = ps.lib()
     R = args.rank
@@ -434,7 +464,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(primary["kernel_gbps"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
         },
         "cpu_baseline": cpu,
         "modes": primary["modes"],

@@ -39,9 +39,31 @@ static uint32_t dn_blocks(uint64_t work_items, uint64_t per_block)
   return (uint32_t) nb;
 }
 
-template<typename T> __device__ __forceinline__ double ld_as_f64(T const * p, uint64_t i)
+// Sums of block partials, part[b * ld + src], b < nb, for up to CW entries
+// at once by one block: thread t takes entry t % CW and every G-th partial
+// starting at t / CW (G = DN_THREADS / CW); the G partial sums are then added
+// in a fixed order. Returns the total in threads t < CW (src of the entry is
+// given by those threads). Deterministic for fixed nb and CW.
+template<int CW>
+__device__ __forceinline__ double block_partsum(double const * __restrict__ part,
+    uint32_t nb, uint32_t ld, bool valid, uint32_t src, double (*s)[CW])
 {
-  return (double) p[i];
+  constexpr int G = DN_THREADS / CW;
+  int const t = threadIdx.x;
+  int const e = t % CW;
+  int const q = t / CW;
+  double v = 0.;
+  if(q < G && valid) {
+    for(uint32_t b=q; b < nb; b += G) v += part[(uint64_t) b * ld + src];
+  }
+  if(q < G) s[q][e] = v;
+  __syncthreads();
+  double tot = 0.;
+  if(t < CW) {
+    for(int k=0; k < G; ++k) tot += s[k][t];
+  }
+  __syncthreads();
+  return tot;
 }
 
 // ---- Gram on MFMA (R <= 64) -------------------------------------------------
@@ -141,19 +163,21 @@ k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
 }
 
 // Sum of the block partials into the full R x R: the upper triangle
-// mirrored (sym) or every entry as computed (cross products).
-__global__ void k_gram_reduce(double const * __restrict__ part, uint32_t nb,
+// mirrored (sym) or every entry as computed (cross products). 32 entries
+// per block.
+__global__ void __launch_bounds__(DN_THREADS)
+k_gram_reduce(double const * __restrict__ part, uint32_t nb,
     uint32_t R, uint32_t ld, double * __restrict__ out, int sym)
 {
-  uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
-  if(e >= R * R) return;
-  uint32_t const i = e / R;
-  uint32_t const j = e - i * R;
+  __shared__ double s[DN_THREADS / 32][32];
+  uint32_t const e = blockIdx.x * 32 + threadIdx.x % 32;
+  bool const valid = e < R * R;
+  uint32_t const i = valid ? e / R : 0;
+  uint32_t const j = valid ? e - i * R : 0;
   uint32_t const a = (sym && j < i) ? j : i;
   uint32_t const b = (sym && j < i) ? i : j;
-  double s = 0.;
-  for(uint32_t k=0; k < nb; ++k) s += part[(uint64_t) k * ld * ld + a * ld + b];
-  out[e] = s;
+  double const tot = block_partsum<32>(part, nb, ld * ld, valid, a * ld + b, s);
+  if(threadIdx.x < 32 && valid) out[e] = tot;
 }
 
 // ---- plain kernels for any rank -----------------------------------------------
@@ -245,29 +269,30 @@ k_coldot(double const * __restrict__ X, TY const * __restrict__ Y, uint64_t I,
 }
 
 // Column totals of block partials (row stride ld): mode 0 writes
-// out[c] = sqrt(total_c); mode 1 writes out[0] = sum_c total_c * lambda[c].
+// out[c] = sqrt(total_c); mode 1 writes out[0] = sum_c total_c * lambda[c]
+// (columns taken in order). One block.
 __global__ void __launch_bounds__(DN_THREADS)
 k_colreduce(double const * __restrict__ part, uint32_t nb, uint32_t R, uint32_t ld,
     int mode, double const * __restrict__ lambda, double * __restrict__ out)
 {
-  __shared__ double s[DN_THREADS];
+  __shared__ double s[DN_THREADS / 32][32];
   double mine = 0.;
-  for(uint32_t c=threadIdx.x; c < R; c += DN_THREADS) {
-    double v = 0.;
-    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * ld + c];
-    if(mode == 0) {
-      out[c] = sqrt(v);
-    } else {
-      // fixed order: thread t sums its columns c = t, t+256, ...
-      mine += v * lambda[c];
+  for(uint32_t c0=0; c0 < R; c0 += 32) {
+    uint32_t const c = c0 + threadIdx.x % 32;
+    double const v = block_partsum<32>(part, nb, ld, c < R, c, s);
+    if(threadIdx.x < 32 && c < R) {
+      if(mode == 0) out[c] = sqrt(v);
+      else mine += v * lambda[c];
     }
   }
   if(mode == 1) {
-    s[threadIdx.x] = mine;
+    s[0][threadIdx.x % 32] = 0.;
+    __syncthreads();
+    if(threadIdx.x < 32) s[0][threadIdx.x] = mine;
     __syncthreads();
     if(threadIdx.x == 0) {
       double tot = 0.;
-      for(int k=0; k < DN_THREADS; ++k) tot += s[k];
+      for(int k=0; k < 32; ++k) tot += s[0][k];
       out[0] = tot;
     }
   }
@@ -350,6 +375,103 @@ k_chol_inv(GramSrc g, int nmodes, int mode, uint32_t R, uint32_t RP,
       for(uint32_t k=j; k < R; ++k) s += Li[k * R + i] * Li[k * R + j];
       ginv[i * RP + j] = s;
       ginv[j * RP + i] = s;
+    }
+  }
+}
+
+// The same for R <= RP <= 64 with the rows in registers: thread i holds row
+// i of the matrix (identity beyond R), column broadcasts go through lane
+// shuffles and LDS, and every loop is unrolled, so nothing waits on
+// dependent LDS round trips. The arithmetic (operations and their order)
+// is that of k_chol_inv, so the two give the same bits.
+template<int RP>
+__global__ void __launch_bounds__(64)
+k_chol_inv_reg(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
+    double * __restrict__ ginv, double * __restrict__ rho_out, int * __restrict__ info)
+{
+  __shared__ double sL[RP][RP + 1];
+  __shared__ double s_diag[64];
+  __shared__ double s_rho;
+  int const i = threadIdx.x;
+  double a[RP];
+  #pragma unroll
+  for(int k=0; k < RP; ++k) {
+    double h;
+    if(i < (int) R && k < (int) R) {
+      int const lo = i < k ? i : k;
+      int const hi = i < k ? k : i;
+      h = 1.;
+      for(int m=0; m < nmodes; ++m) {
+        if(m != mode) h *= g.aTa[m][lo * R + hi];
+      }
+      if(k == i) h += diag_add;
+    } else {
+      h = (k == i) ? 1. : 0.;
+    }
+    a[k] = h;
+  }
+  if(admm) {
+    double d = 0.;
+    #pragma unroll
+    for(int k=0; k < RP; ++k) if(k == i) d = a[k];
+    s_diag[i] = d;
+    __syncthreads();
+    if(i == 0) {
+      double tr = 0.;
+      for(uint32_t q=0; q < R; ++q) tr += s_diag[q];
+      double const rho = tr / (double) R;
+      *rho_out = rho;
+      s_rho = rho;
+    }
+    __syncthreads();
+    double const rho = s_rho;
+    if(i < (int) R) {
+      #pragma unroll
+      for(int k=0; k < RP; ++k) if(k == i) a[k] += rho;
+    }
+  }
+  // right-looking Cholesky: column j of L, then the trailing update of row i
+  #pragma unroll
+  for(int j=0; j < RP; ++j) {
+    double const ajj = __shfl(a[j], j, 64);
+    if(i == 0 && j < (int) R && !(ajj > 0.) && *info == 0) *info = j + 1;
+    double const ljj = sqrt(ajj);
+    double const lij = a[j] / ljj;
+    if(i == j) a[j] = ljj;
+    else if(i > j) a[j] = lij;
+    #pragma unroll
+    for(int k=j + 1; k < RP; ++k) {
+      double const lkj = __shfl(lij, k, 64);
+      if(i >= k) a[k] -= lij * lkj;
+    }
+  }
+  #pragma unroll
+  for(int k=0; k < RP; ++k) sL[i][k] = (k <= i) ? a[k] : 0.;
+  __syncthreads();
+  // column i of L^{-1} (forward substitution against e_i)
+  double x[RP];
+  #pragma unroll
+  for(int r=0; r < RP; ++r) {
+    double sum = 0.;
+    #pragma unroll
+    for(int k=0; k < r; ++k) sum += sL[r][k] * x[k];
+    double const lrr = sL[r][r];
+    x[r] = (r < i) ? 0. : ((r == i) ? 1. / lrr : -sum / lrr);
+  }
+  __syncthreads();
+  #pragma unroll
+  for(int k=0; k < RP; ++k) sL[k][i] = x[k];
+  __syncthreads();
+  // G^{-1} = L^{-T} L^{-1}: entry (q, i) for q <= i from columns q and i
+  #pragma unroll
+  for(int q=0; q < RP; ++q) {
+    double sum = 0.;
+    #pragma unroll
+    for(int k=0; k < RP; ++k) sum += sL[k][q] * x[k];
+    if(q <= i) {
+      bool const in = (i < (int) R) && (q < (int) R);
+      ginv[q * RP + i] = in ? sum : 0.;
+      ginv[i * RP + q] = in ? sum : 0.;
     }
   }
 }
@@ -629,35 +751,37 @@ k_admm_mfma(AdmmArgs<TM> a)
   }
 }
 
-// norms[c] = sqrt(sum_b part[b * ld + c]) unless converged.
-__global__ void k_admm_norms(double const * __restrict__ part, uint32_t nb, uint32_t R,
+// norms[c] = sqrt(sum_b part[b * ld + c]) unless converged. One block.
+__global__ void __launch_bounds__(DN_THREADS)
+k_admm_norms(double const * __restrict__ part, uint32_t nb, uint32_t R,
     uint32_t ld, double * __restrict__ norms, int const * __restrict__ ctl)
 {
   if(ctl[0] != 0) return;
-  for(uint32_t c=threadIdx.x; c < R; c += blockDim.x) {
-    double v = 0.;
-    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * ld + c];
-    norms[c] = sqrt(v);
+  __shared__ double s[DN_THREADS / 32][32];
+  for(uint32_t c0=0; c0 < R; c0 += 32) {
+    uint32_t const c = c0 + threadIdx.x % 32;
+    double const v = block_partsum<32>(part, nb, ld, c < R, c, s);
+    if(threadIdx.x < 32 && c < R) norms[c] = sqrt(v);
   }
 }
 
 // sums = block totals; ctl[1] counts iterations; ctl[0] set on convergence.
-__global__ void k_admm_check(double const * __restrict__ part, uint32_t nb,
+__global__ void __launch_bounds__(DN_THREADS)
+k_admm_check(double const * __restrict__ part, uint32_t nb,
     double * __restrict__ sums, int * __restrict__ ctl, double tol)
 {
   if(ctl[0] != 0) return;
-  __shared__ double s[4];
-  int const k = threadIdx.x;
-  if(k < 4) {
-    double v = 0.;
-    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * 4 + k];
-    sums[k] = v;
-    s[k] = v;
+  __shared__ double s[DN_THREADS / 4][4];
+  __shared__ double tot[4];
+  double const v = block_partsum<4>(part, nb, 4, true, threadIdx.x % 4, s);
+  if(threadIdx.x < 4) {
+    sums[threadIdx.x] = v;
+    tot[threadIdx.x] = v;
   }
   __syncthreads();
-  if(k == 0) {
+  if(threadIdx.x == 0) {
     ctl[1] += 1;
-    if(s[1] <= tol * s[0] && s[2] <= tol * s[3]) ctl[0] = 1;
+    if(tot[1] <= tol * tot[0] && tot[2] <= tol * tot[3]) ctl[0] = 1;
   }
 }
 
@@ -773,14 +897,12 @@ k_admm_post(double * __restrict__ P, double const * __restrict__ aux,
   if(t < 4) part[(uint64_t) blockIdx.x * 4 + t] = s[t][0];
 }
 
-__global__ void k_sum4(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
+__global__ void __launch_bounds__(DN_THREADS)
+k_sum4(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
 {
-  int const k = threadIdx.x;
-  if(k < 4) {
-    double v = 0.;
-    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * 4 + k];
-    out[k] = v;
-  }
+  __shared__ double s[DN_THREADS / 4][4];
+  double const v = block_partsum<4>(part, nb, 4, true, threadIdx.x % 4, s);
+  if(threadIdx.x < 4) out[threadIdx.x] = v;
 }
 
 __global__ void k_cpd_norm(GramSrc g, int nmodes, uint32_t R,
@@ -888,8 +1010,8 @@ static int gram_impl(double * X, double const * Y, uint64_t I, uint32_t R,
     hipLaunchKernelGGL(k_gram_plain, dim3(nb), dim3(DN_THREADS), 0, st,
         (double const *) X, sym ? (double const *) X : Y, I, R, rpb, (double *) part);
     SPD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 256)),
-        dim3(256), 0, st, (double const *) part, nb, R, R, out, sym);
+    hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 32)),
+        dim3(DN_THREADS), 0, st, (double const *) part, nb, R, R, out, sym);
     SPD_LAUNCH_CHECK();
     return SPD_OK;
   }
@@ -905,8 +1027,8 @@ static int gram_impl(double * X, double const * Y, uint64_t I, uint32_t R,
   case 3: SPD_TRY(launch_gram_mfma<3>(X, I, R, div, X32, (double *) part, nb, st, Y)); break;
   default: SPD_TRY(launch_gram_mfma<4>(X, I, R, div, X32, (double *) part, nb, st, Y)); break;
   }
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 256)),
-      dim3(256), 0, st, (double const *) part, nb, R, RP, out, sym);
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 32)),
+      dim3(DN_THREADS), 0, st, (double const *) part, nb, R, RP, out, sym);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
@@ -933,6 +1055,21 @@ int dn_chol_inv(double const * const * aTa, int nmodes, int mode, uint32_t R,
   }
   GramSrc g = {};
   for(int m=0; m < nmodes; ++m) g.aTa[m] = aTa[m];
+  uint32_t const RP = dn_rpad(R);
+  if(RP <= 64) {
+    switch(RP) {
+    case 16: hipLaunchKernelGGL((k_chol_inv_reg<16>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+                 R, diag_add, admm, ginv, rho, info); break;
+    case 32: hipLaunchKernelGGL((k_chol_inv_reg<32>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+                 R, diag_add, admm, ginv, rho, info); break;
+    case 48: hipLaunchKernelGGL((k_chol_inv_reg<48>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+                 R, diag_add, admm, ginv, rho, info); break;
+    default: hipLaunchKernelGGL((k_chol_inv_reg<64>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+                 R, diag_add, admm, ginv, rho, info); break;
+    }
+    SPD_LAUNCH_CHECK();
+    return SPD_OK;
+  }
   size_t const wbytes = 2ull * R * R * sizeof(double);
   double * gW = nullptr;
   size_t shmem = 0;
@@ -1095,7 +1232,7 @@ static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_
       (double const *) aux, D, (double const *) init, (double const *) norms, prox,
       n, R, (double *) part2);
   SPD_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum4, dim3(1), dim3(64), 0, st, (double const *) part2, nb2, sums);
+  hipLaunchKernelGGL(k_sum4, dim3(1), dim3(DN_THREADS), 0, st, (double const *) part2, nb2, sums);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
@@ -1148,13 +1285,13 @@ static int admm_solve_t(TM const * M, double const * ginv, double const * rho, u
     if(need_norms) {
       a.part = (double *) pc;
       SPD_TRY((dispatch_admm<TM, 0>(a, RP / 16, nb, st)));
-      hipLaunchKernelGGL(k_admm_norms, dim3(1), dim3(64), 0, st, (double const *) pc, nb,
+      hipLaunchKernelGGL(k_admm_norms, dim3(1), dim3(DN_THREADS), 0, st, (double const *) pc, nb,
           R, RP, sums + 4, (int const *) ctl);
       SPD_LAUNCH_CHECK();
     }
     a.part = (double *) ps;
     SPD_TRY((dispatch_admm<TM, 1>(a, RP / 16, nb, st)));
-    hipLaunchKernelGGL(k_admm_check, dim3(1), dim3(64), 0, st, (double const *) ps, nb,
+    hipLaunchKernelGGL(k_admm_check, dim3(1), dim3(DN_THREADS), 0, st, (double const *) ps, nb,
         sums, ctl, tol);
     SPD_LAUNCH_CHECK();
   }

@@ -40,7 +40,11 @@ struct CooArgs {
 
 // COO MTTKRP: per nonzero v * prod_{m != mode} U_m[ind_m] (modes ascending,
 // as the reference multiplies), added to the output row. L lanes per
-// nonzero, NC column chunks of L.
+// nonzero, NC column chunks of L. A group takes CO_CHUNK consecutive
+// nonzeros and adds runs of equal output rows before one atomic update per
+// run (a slice is sorted by the mode after the streamed one, so that mode's
+// rows come in runs).
+#define CO_CHUNK 16
 template<int L, int NC>
 __global__ void __launch_bounds__(CO_THREADS)
 k_coo_mttkrp(CooArgs a)
@@ -58,32 +62,53 @@ k_coo_mttkrp(CooArgs a)
   double acc[NC];
   #pragma unroll
   for(int k=0; k < NC; ++k) acc[k] = 0.;
-  for(uint64_t n=group; n < a.nnz; n += ngroups) {
-    double const v = a.vals[n];
-    double p[NC];
-    #pragma unroll
-    for(int k=0; k < NC; ++k) p[k] = v;
-    #pragma unroll
-    for(int m=0; m < SPD_MAX_MODES; ++m) {
-      if(m >= a.N) break;
-      if(m == a.mode) continue;
-      double const * row = a.mats[m] + (uint64_t) a.ind[m][n] * R;
+  uint32_t cur = 0xFFFFFFFFu;
+  for(uint64_t n0 = group * CO_CHUNK; n0 < a.nnz; n0 += ngroups * CO_CHUNK) {
+    uint64_t const n1 = (n0 + CO_CHUNK < a.nnz) ? n0 + CO_CHUNK : a.nnz;
+    for(uint64_t n=n0; n < n1; ++n) {
+      double const v = a.vals[n];
+      double p[NC];
       #pragma unroll
-      for(int k=0; k < NC; ++k) {
-        uint32_t const col = c + k * L;
-        if(col < R) p[k] *= row[col];
+      for(int k=0; k < NC; ++k) p[k] = v;
+      #pragma unroll
+      for(int m=0; m < SPD_MAX_MODES; ++m) {
+        if(m >= a.N) break;
+        if(m == a.mode) continue;
+        double const * row = a.mats[m] + (uint64_t) a.ind[m][n] * R;
+        #pragma unroll
+        for(int k=0; k < NC; ++k) {
+          uint32_t const col = c + k * L;
+          if(col < R) p[k] *= row[col];
+        }
       }
-    }
-    if(a.reduce) {
+      if(!a.reduce) {
+        uint32_t const orow = a.ind[a.mode][n];
+        if(orow != cur) {
+          if(cur != 0xFFFFFFFFu) {
+            double * o = a.out + (uint64_t) cur * R;
+            #pragma unroll
+            for(int k=0; k < NC; ++k) {
+              uint32_t const col = c + k * L;
+              if(col < R) unsafeAtomicAdd(o + col, acc[k]);
+            }
+          }
+          #pragma unroll
+          for(int k=0; k < NC; ++k) acc[k] = 0.;
+          cur = orow;
+        }
+      }
       #pragma unroll
       for(int k=0; k < NC; ++k) acc[k] += p[k];
-    } else {
-      double * orow = a.out + (uint64_t) a.ind[a.mode][n] * R;
+    }
+    if(!a.reduce && cur != 0xFFFFFFFFu) {
+      double * o = a.out + (uint64_t) cur * R;
       #pragma unroll
       for(int k=0; k < NC; ++k) {
         uint32_t const col = c + k * L;
-        if(col < R) unsafeAtomicAdd(orow + col, p[k]);
+        if(col < R) unsafeAtomicAdd(o + col, acc[k]);
+        acc[k] = 0.;
       }
+      cur = 0xFFFFFFFFu;
     }
   }
   if(a.reduce) {
@@ -98,15 +123,27 @@ k_coo_mttkrp(CooArgs a)
   }
 }
 
-// out[c] (+)= sum_b part[b * R + c], fixed order.
-__global__ void k_rowsum(double const * __restrict__ part, uint32_t nb, uint32_t R,
+// out[c] = sum_b part[b * R + c]: one block per 32 columns, the partials
+// split over 8 thread groups and added in a fixed order.
+__global__ void __launch_bounds__(256)
+k_rowsum(double const * __restrict__ part, uint32_t nb, uint32_t R,
     double * __restrict__ out)
 {
-  uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
-  if(c >= R) return;
-  double s = 0.;
-  for(uint32_t b=0; b < nb; ++b) s += part[(uint64_t) b * R + c];
-  out[c] = s;
+  __shared__ double s[8][32];
+  int const t = threadIdx.x;
+  uint32_t const c = blockIdx.x * 32 + t % 32;
+  int const q = t / 32;
+  double v = 0.;
+  if(c < R) {
+    for(uint32_t b=q; b < nb; b += 8) v += part[(uint64_t) b * R + c];
+  }
+  s[q][t % 32] = v;
+  __syncthreads();
+  if(t < 32 && c < R) {
+    double tot = 0.;
+    for(int k=0; k < 8; ++k) tot += s[k][t];
+    out[c] = tot;
+  }
 }
 
 // Time Gram: symmetric copy of the upper triangle into old, then
@@ -179,13 +216,21 @@ k_diff2(double const * __restrict__ a, double const * __restrict__ b, uint64_t n
   if(t < 2) part[(uint64_t) blockIdx.x * 2 + t] = s[t][0];
 }
 
-__global__ void k_sum2(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
+__global__ void __launch_bounds__(256)
+k_sum2(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
 {
-  int const k = threadIdx.x;
-  if(k < 2) {
-    double v = 0.;
-    for(uint32_t b=0; b < nb; ++b) v += part[(uint64_t) b * 2 + k];
-    out[k] = v;
+  __shared__ double s[128][2];
+  int const t = threadIdx.x;
+  int const k = t % 2;
+  int const q = t / 2;
+  double v = 0.;
+  for(uint32_t b=q; b < nb; b += 128) v += part[(uint64_t) b * 2 + k];
+  s[q][k] = v;
+  __syncthreads();
+  if(t < 2) {
+    double tot = 0.;
+    for(int j=0; j < 128; ++j) tot += s[j][t];
+    out[t] = tot;
   }
 }
 
@@ -232,8 +277,8 @@ int coo_mttkrp(uint32_t const * const * ind, double const * vals,
     spd_set_error(SPD_EINPUT, "streaming MTTKRP supports rank <= 256 (got %u)", R);
     return SPD_EINPUT;
   }
-  uint64_t const per_block = (uint64_t) (CO_THREADS / L);
-  uint64_t nb64 = spd_cdiv(nnz > 0 ? nnz : 1, per_block * 4);
+  uint64_t const per_block = (uint64_t) (CO_THREADS / L) * CO_CHUNK;
+  uint64_t nb64 = spd_cdiv(nnz > 0 ? nnz : 1, per_block);
   uint32_t const cap = a.reduce ? 256u : 8192u;
   uint32_t const nb = (uint32_t) (nb64 < cap ? nb64 : cap);
   void * part = nullptr;
@@ -252,7 +297,7 @@ int coo_mttkrp(uint32_t const * const * ind, double const * vals,
     else SPD_TRY((launch_coo<64, 4>(a, nb, st)));
   }
   if(a.reduce) {
-    hipLaunchKernelGGL(k_rowsum, dim3((unsigned) spd_cdiv(R, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_rowsum, dim3((unsigned) spd_cdiv(R, 32)), dim3(256), 0, st,
         (double const *) part, nb, R, out);
     SPD_LAUNCH_CHECK();
   }
@@ -268,7 +313,7 @@ int diff2(double const * a, double const * b, uint64_t n, double * out, hipStrea
   SPD_TRY(spd_scratch(14, (size_t) nb * 2 * sizeof(double), &part));
   hipLaunchKernelGGL(k_diff2, dim3(nb), dim3(256), 0, st, a, b, n, (double *) part);
   SPD_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum2, dim3(1), dim3(64), 0, st, (double const *) part, nb, out);
+  hipLaunchKernelGGL(k_sum2, dim3(1), dim3(256), 0, st, (double const *) part, nb, out);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
