@@ -66,7 +66,12 @@ int splatt_amd_mttkrp_kernel_times(double * ms, int max);
  * index is the time slice); the other modes are relabelled by first
  * appearance, so row i of factors[m] belongs to original index iperm[m][i].
  * cpd_options is modified as the reference modifies it (max-column-norm
- * constraints on the other modes, stream tolerances). Returns NULL on
+ * constraints on the other modes, stream tolerances). Every check_interval
+ * slices and after the slice that consumes the last nonzero, the errors of
+ * all slices so far and of the last 10 are computed, and with refit != 0 a
+ * full CPD of all slices so far (its factors drawn from rand(), as in the
+ * reference); splatt_cpd_stream uses 100 and refit, as the reference does
+ * (StreamCPD.cc:517-528). check_interval 0 skips them. Returns NULL on
  * error. */
 typedef struct
 {
@@ -81,6 +86,9 @@ typedef struct
   double * batch_seconds;                     /* per slice: wall time, device synchronised */
   splatt_idx_t * outer_its;                   /* per slice: ALS iterations run */
   double final_err;                           /* squared error of the whole stream */
+  double * global_err;                        /* per slice, -1 off checkpoints: all slices so far */
+  double * local10_err;                       /* per slice, -1 off checkpoints: the last 10 */
+  double * cpd_err;                           /* per slice, -1 off checkpoints: refit 1 - fit */
 } splatt_amd_stream_result;
 
 splatt_amd_stream_result * splatt_amd_cpd_stream_coo(
@@ -88,6 +96,8 @@ splatt_amd_stream_result * splatt_amd_cpd_stream_coo(
     splatt_idx_t rank,
     splatt_idx_t stream_mode,
     double forget,
+    splatt_idx_t check_interval,
+    int refit,
     splatt_cpd_opts * const cpd_options,
     splatt_global_opts const * const global_options);
 

@@ -102,8 +102,10 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
  * (src/cpd/constraints/frob.c:25-36), 2 = ADMM (src/cpd/admm.c:346-406,
  * one chunk) with prox 1 = max-col-norm (constraints/maxcolnorm.c:32-61),
  * 2 = non-negative max-col-norm (maxcolnorm-nonneg.c:32-66),
- * 3 = non-negative (nonneg.c), 0 = none. */
-typedef struct { int kind; int prox; double diag_add; } or_con;
+ * 3 = non-negative (nonneg.c), 0 = none. chunk > 0 runs ADMM in
+ * independent chunks of that many rows (admm.c:355-400; the caller passes
+ * 0 unless the prox is row-separable). */
+typedef struct { int kind; int prox; double diag_add; uint64_t chunk; } or_con;
 
 /* or_cpd_als with per-mode constraints; `unconstrained` is the reference's
  * ws->unconstrained (every constraint UNCONSTRAINED: normalise columns into
@@ -121,6 +123,13 @@ int or_cpd_als_con(or_csf * const * csfs, uint64_t const * mode_csf,
 uint64_t or_admm(int prox, double const * M, double * P, double * D,
     double * G, uint64_t I, uint64_t R, uint64_t max_inner, double inner_tol,
     double * rho_out);
+
+/* The same with the rows split into chunks of `chunk` rows (the last one
+ * ragged), each iterated to its own convergence (src/cpd/admm.c:355-400).
+ * Returns the sum over chunks of iterations x rows. */
+uint64_t or_admm_chunked(int prox, double const * M, double * P, double * D,
+    double * G, uint64_t I, uint64_t R, uint64_t chunk, uint64_t max_inner,
+    double inner_tol);
 
 /* The proximal operators (row-major I x R, in place). */
 void or_prox(int prox, double * P, uint64_t I, uint64_t R);
@@ -150,6 +159,8 @@ typedef struct {
   double forget;
   uint64_t max_its;                 /* outer ALS iterations per slice */
   or_con cons[OR_MAX_NMODES];       /* per mode, after the stream's rules */
+  uint64_t check_interval;          /* StreamCPD.cc:23-25 (100); 0: none */
+  int refit;                        /* full CPD at checkpoints (draws rand()) */
 } or_stream_cfg;
 
 typedef struct {
@@ -161,6 +172,11 @@ typedef struct {
   uint64_t nbatches;
   uint64_t dims[OR_MAX_NMODES];
   double final_err;                 /* squared, as printed by the reference */
+  /* per slice, -1 off checkpoints (may be NULL): squared errors of all
+   * slices so far and of the last 10, the refit's 1 - fit */
+  double * global_err;
+  double * local10_err;
+  double * cpd_err;
 } or_stream_out;
 
 /* StreamCPD::compute (StreamCPD.cc:311-571) on a tensor already prepared

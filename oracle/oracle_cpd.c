@@ -148,32 +148,21 @@ void or_prox(int prox, double * P, uint64_t I, uint64_t R)
   }
 }
 
-uint64_t or_admm(int prox, double const * M, double * P, double * D,
-    double * G, uint64_t I, uint64_t R, uint64_t max_inner, double inner_tol,
-    double * rho_out)
+/* p_admm_iterate_chunk (src/cpd/admm.c:240-308) on rows of one chunk: L is
+ * the Cholesky factor of G + rho I. */
+static uint64_t p_admm_iterate(int prox, double const * M, double * P, double * D,
+    double const * L, double rho, uint64_t I, uint64_t R, uint64_t max_inner,
+    double inner_tol)
 {
-  /* src/cpd/admm.c:346-354 */
-  double tr = 0.;
-  for(uint64_t i=0; i < R; ++i) {
-    tr += G[i * R + i];
-  }
-  double const rho = tr / (double) R;
-  for(uint64_t i=0; i < R; ++i) {
-    G[i * R + i] += rho;
-  }
-  if(rho_out) {
-    *rho_out = rho;
-  }
-  or_cholesky(G, R);
-  double * aux = malloc(I * R * sizeof(double));
-  double * init = malloc(I * R * sizeof(double));
+  double * aux = malloc((I ? I : 1) * R * sizeof(double));
+  double * init = malloc((I ? I : 1) * R * sizeof(double));
   uint64_t it;
   for(it=0; it < max_inner; ++it) {
     memcpy(init, P, I * R * sizeof(double));
     for(uint64_t x=0; x < I * R; ++x) {
       aux[x] = M[x] + rho * (P[x] + D[x]);
     }
-    or_cholesky_solve(G, R, aux, I);
+    or_cholesky_solve(L, R, aux, I);
     for(uint64_t x=0; x < I * R; ++x) {
       P[x] = aux[x] - D[x];
     }
@@ -201,6 +190,56 @@ uint64_t or_admm(int prox, double const * M, double * P, double * D,
   return it;
 }
 
+/* src/cpd/admm.c:346-354: rho = trace(G)/R added to the diagonal, then the
+ * Cholesky factorisation, once per mode. */
+static double p_admm_setup(double * G, uint64_t R)
+{
+  double tr = 0.;
+  for(uint64_t i=0; i < R; ++i) {
+    tr += G[i * R + i];
+  }
+  double const rho = tr / (double) R;
+  for(uint64_t i=0; i < R; ++i) {
+    G[i * R + i] += rho;
+  }
+  or_cholesky(G, R);
+  return rho;
+}
+
+uint64_t or_admm(int prox, double const * M, double * P, double * D,
+    double * G, uint64_t I, uint64_t R, uint64_t max_inner, double inner_tol,
+    double * rho_out)
+{
+  double const rho = p_admm_setup(G, R);
+  if(rho_out) {
+    *rho_out = rho;
+  }
+  return p_admm_iterate(prox, M, P, D, G, rho, I, R, max_inner, inner_tol);
+}
+
+uint64_t or_admm_chunked(int prox, double const * M, double * P, double * D,
+    double * G, uint64_t I, uint64_t R, uint64_t chunk, uint64_t max_inner,
+    double inner_tol)
+{
+  double const rho = p_admm_setup(G, R);
+  /* src/cpd/admm.c:355-400: chunks of `chunk` rows, the last one ragged,
+   * each iterated to its own convergence; the count is per row */
+  uint64_t nchunks = I / chunk;
+  if(I % chunk > 0) {
+    ++nchunks;
+  }
+  uint64_t total = 0;
+  for(uint64_t c=0; c < nchunks; ++c) {
+    uint64_t const start = c * chunk;
+    uint64_t const stop = (c == nchunks - 1) ? I : (c + 1) * chunk;
+    uint64_t const off = start * R;
+    uint64_t const its = p_admm_iterate(prox, M + off, P + off, D + off, G, rho,
+        stop - start, R, max_inner, inner_tol);
+    total += its * (stop - start);
+  }
+  return total;
+}
+
 int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
     uint64_t nmodes, uint64_t rank, double ** factors, double * lambda,
     uint64_t max_its, double tol, double * fits, int nthreads)
@@ -210,6 +249,7 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
     cons[m].kind = 0;
     cons[m].prox = 0;
     cons[m].diag_add = 0.;
+    cons[m].chunk = 0;
   }
   return or_cpd_als_con(csfs, mode_csf, nmodes, rank, factors, lambda, max_its,
       tol, 50, 1e-2, cons, 1, fits, nthreads);
@@ -267,7 +307,10 @@ int or_cpd_als_con(or_csf * const * csfs, uint64_t const * mode_csf,
       for(uint64_t i=0; i < R; ++i) {
         gram[i * R + i] += cons[m].diag_add;
       }
-      if(cons[m].kind == 2) {
+      if(cons[m].kind == 2 && cons[m].chunk > 0) {
+        or_admm_chunked(cons[m].prox, mttkrp, factors[m], duals[m], gram, I, R,
+            cons[m].chunk, max_inner, inner_tol);
+      } else if(cons[m].kind == 2) {
         or_admm(cons[m].prox, mttkrp, factors[m], duals[m], gram, I, R,
             max_inner, inner_tol, NULL);
       } else {

@@ -228,13 +228,14 @@ def cpd_als(csfs, mode_csf, factors, max_its, tol, nthreads=1):
 
 
 class _OrCon(C.Structure):
-    _fields_ = [("kind", C.c_int), ("prox", C.c_int), ("diag_add", C.c_double)]
+    _fields_ = [("kind", C.c_int), ("prox", C.c_int), ("diag_add", C.c_double),
+                ("chunk", C.c_uint64)]
 
 
 def cpd_als_con(csfs, mode_csf, factors, max_its, tol, cons, unconstrained,
                 max_inner=50, inner_tol=1e-2, nthreads=1):
     """or_cpd_als_con on copies of `factors`. cons: list of (kind, prox,
-    diag_add) per mode. Returns (factors, lambda, fits)."""
+    diag_add[, chunk]) per mode. Returns (factors, lambda, fits)."""
     N = len(factors)
     R = factors[0].shape[1]
     facs = [np.ascontiguousarray(f, dtype=np.float64).copy() for f in factors]
@@ -244,8 +245,9 @@ def cpd_als_con(csfs, mode_csf, factors, max_its, tol, cons, unconstrained,
     mc = np.asarray(mode_csf, dtype=np.uint64)
     fptrs = (f64p * N)(*[_p(f, C.c_double) for f in facs])
     carr = (_OrCon * 8)()
-    for m, (k, p, d) in enumerate(cons):
-        carr[m].kind, carr[m].prox, carr[m].diag_add = k, p, d
+    for m, c in enumerate(cons):
+        carr[m].kind, carr[m].prox, carr[m].diag_add = c[0], c[1], c[2]
+        carr[m].chunk = c[3] if len(c) > 3 else 0
     L = lib()
     L.or_cpd_als_con.restype = C.c_int
     its = L.or_cpd_als_con(cptrs, _p(mc, C.c_uint64), C.c_uint64(N), C.c_uint64(R),
@@ -272,6 +274,22 @@ def admm(prox, M, P, D, G, max_inner, inner_tol):
     return P, D, int(its), rho.value
 
 
+def admm_chunked(prox, M, P, D, G, chunk, max_inner, inner_tol):
+    """or_admm_chunked on copies; returns (P, D, iterations x rows summed)."""
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    P = np.ascontiguousarray(P, dtype=np.float64).copy()
+    D = np.ascontiguousarray(D, dtype=np.float64).copy()
+    G = np.ascontiguousarray(G, dtype=np.float64).copy()
+    I, R = P.shape
+    L = lib()
+    L.or_admm_chunked.restype = C.c_uint64
+    tot = L.or_admm_chunked(C.c_int(prox), _p(M, C.c_double), _p(P, C.c_double),
+                            _p(D, C.c_double), _p(G, C.c_double), C.c_uint64(I),
+                            C.c_uint64(R), C.c_uint64(chunk), C.c_uint64(max_inner),
+                            C.c_double(inner_tol))
+    return P, D, int(tot)
+
+
 def prox(kind, P):
     P = np.ascontiguousarray(P, dtype=np.float64).copy()
     I, R = P.shape
@@ -284,14 +302,16 @@ def prox(kind, P):
 class _OrStreamCfg(C.Structure):
     _fields_ = [("stream_mode", C.c_uint64), ("rank", C.c_uint64),
                 ("forget", C.c_double), ("max_its", C.c_uint64),
-                ("cons", _OrCon * MAXM)]
+                ("cons", _OrCon * MAXM), ("check_interval", C.c_uint64),
+                ("refit", C.c_int)]
 
 
 class _OrStreamOut(C.Structure):
     _fields_ = [("max_batches", C.c_uint64), ("time", f64p),
                 ("factors", f64p * MAXM), ("local_err", f64p),
                 ("outer_its", u64p), ("nbatches", C.c_uint64),
-                ("dims", C.c_uint64 * MAXM), ("final_err", C.c_double)]
+                ("dims", C.c_uint64 * MAXM), ("final_err", C.c_double),
+                ("global_err", f64p), ("local10_err", f64p), ("cpd_err", f64p)]
 
 
 def mttkrp_stream(coo, mats, mode):
@@ -323,9 +343,11 @@ def stream_prepare(coo, stream_mode):
     return [a[:d] for a, d in zip(ip, coo.dims)]
 
 
-def stream_cpd(coo, stream_mode, rank, forget, max_its, cons):
+def stream_cpd(coo, stream_mode, rank, forget, max_its, cons, check_interval=0,
+               refit=False):
     """or_stream_cpd on a prepared tensor (caller seeds rand()). cons: list of
-    (kind, prox, diag_add) per mode. Returns a dict of numpy results."""
+    (kind, prox, diag_add) per mode; check_interval/refit drive the
+    checkpoint errors. Returns a dict of numpy results."""
     N = coo.nmodes
     dims = coo.dims
     T = dims[stream_mode]
@@ -333,6 +355,8 @@ def stream_cpd(coo, stream_mode, rank, forget, max_its, cons):
     cfg.stream_mode, cfg.rank, cfg.forget, cfg.max_its = stream_mode, rank, forget, max_its
     for m, (k, p, d) in enumerate(cons):
         cfg.cons[m].kind, cfg.cons[m].prox, cfg.cons[m].diag_add = k, p, d
+    cfg.check_interval = check_interval
+    cfg.refit = 1 if refit else 0
     time = np.zeros((max(T, 1), rank), dtype=np.float64)
     facs = [np.zeros((max(d, 1), rank), dtype=np.float64) for d in dims]
     lerr = np.zeros(max(T, 1), dtype=np.float64)
@@ -345,9 +369,12 @@ def stream_cpd(coo, stream_mode, rank, forget, max_its, cons):
             out.factors[m] = _p(facs[m], C.c_double)
     out.local_err = _p(lerr, C.c_double)
     out.outer_its = _p(oits, C.c_uint64)
+    chk = [np.full(max(T, 1), -1.0) for _ in range(3)]
+    out.global_err, out.local10_err, out.cpd_err = [_p(a, C.c_double) for a in chk]
     lib().or_stream_cpd(coo.ptr, C.byref(cfg), C.byref(out))
     nb = int(out.nbatches)
     fd = [int(out.dims[m]) for m in range(N)]
     return dict(time=time[:nb], factors=[f[:fd[m]] for m, f in enumerate(facs)],
                 local_err=lerr[:nb], outer_its=oits[:nb], final_err=float(out.final_err),
-                nbatches=nb, dims=fd)
+                nbatches=nb, dims=fd, global_err=chk[0][:nb], local10_err=chk[1][:nb],
+                cpd_err=chk[2][:nb])

@@ -4,8 +4,9 @@
  * relabelling (src/cpd/stream/ParserSimple.cc:13-80, :89-149), the growth of
  * the factor matrices (StreamMatrix.cc:27-83, StreamCPD.cc:192-220), the
  * per-slice ALS with ADMM (StreamCPD.cc:311-571, add_historical :223-294),
- * the COO MTTKRP (src/mttkrp.c:1699-1769) and the error measures
- * (src/cpd/cpd.c:533-656, StreamCPD.cc:77-89).
+ * the COO MTTKRP (src/mttkrp.c:1699-1769), the error measures
+ * (src/cpd/cpd.c:533-656, StreamCPD.cc:77-89) and the checkpoints
+ * (StreamCPD.cc:89-106, :517-528).
  */
 #include "oracle.h"
 #include <math.h>
@@ -199,6 +200,73 @@ static void p_form_gram(double * const * aTa, uint64_t N, uint64_t mode,
   for(uint64_t i=0; i < R; ++i) gram[i * R + i] += diag;
 }
 
+/* stream_prev(nprev) (ParserSimple.cc:185-226) after `done` slices: the
+ * nonzeros of slices done-nprev .. done-1 (or from 0), time labels shifted
+ * to start at 0, the other dims at their running maxima. The caller frees
+ * ind[t]. */
+static or_coo p_window(or_coo const * tt, uint64_t t, uint64_t const * bstart,
+    uint64_t done, uint64_t nprev, uint64_t const * prev_dim)
+{
+  uint64_t const b0 = (nprev < done) ? done - nprev : 0;
+  uint64_t const x0 = bstart[b0];
+  uint64_t const x1 = bstart[done];
+  or_coo w;
+  memset(&w, 0, sizeof(w));
+  w.nmodes = tt->nmodes;
+  w.nnz = x1 - x0;
+  w.vals = tt->vals + x0;
+  for(uint64_t m=0; m < tt->nmodes; ++m) {
+    if(m == t) {
+      w.ind[m] = malloc((w.nnz ? w.nnz : 1) * sizeof(uint64_t));
+      for(uint64_t x=0; x < w.nnz; ++x) w.ind[m][x] = tt->ind[m][x0 + x] - b0;
+      w.dims[m] = done - b0;
+    } else {
+      w.ind[m] = tt->ind[m] + x0;
+      w.dims[m] = prev_dim[m];
+    }
+  }
+  return w;
+}
+
+/* compute_cpd_errorsq (StreamCPD.cc:89-106): factors drawn from rand() for
+ * every mode in order (splatt_alloc_cpd, src/cpd/cpd.c:112-135), then
+ * CPD-ALS with the default options (tolerance 1e-5, 200 iterations,
+ * src/cpd/cpd.c:82-99); returns 1 - fit. One CSF serves every mode. */
+static double p_refit(or_coo const * w, uint64_t R)
+{
+  uint64_t const N = w->nmodes;
+  or_coo c = *w;
+  for(uint64_t m=0; m < N; ++m) {
+    c.ind[m] = malloc((w->nnz ? w->nnz : 1) * sizeof(uint64_t));
+    memcpy(c.ind[m], w->ind[m], w->nnz * sizeof(uint64_t));
+  }
+  c.vals = malloc((w->nnz ? w->nnz : 1) * sizeof(double));
+  memcpy(c.vals, w->vals, w->nnz * sizeof(double));
+  uint64_t perm[OR_MAX_NMODES];
+  or_mode_order(c.dims, N, 0, 0, perm);
+  or_csf * csf = or_csf_build(&c, perm, 0, 0, 1, 0);
+  double * f[OR_MAX_NMODES];
+  uint64_t mode_csf[OR_MAX_NMODES];
+  for(uint64_t m=0; m < N; ++m) {
+    f[m] = malloc(c.dims[m] * R * sizeof(double));
+    or_fill_rand(f[m], c.dims[m] * R);
+    mode_csf[m] = 0;
+  }
+  double * lam = malloc(R * sizeof(double));
+  double * fits = malloc(200 * sizeof(double));
+  int const its = or_cpd_als(&csf, mode_csf, N, R, f, lam, 200, 1e-5, fits, 1);
+  double const err = (its > 0) ? 1. - fits[its - 1] : NAN;
+  for(uint64_t m=0; m < N; ++m) {
+    free(f[m]);
+    free(c.ind[m]);
+  }
+  free(c.vals);
+  free(lam);
+  free(fits);
+  or_csf_free(csf);
+  return err;
+}
+
 int or_stream_cpd(or_coo const * tt, or_stream_cfg const * cfg,
     or_stream_out * out)
 {
@@ -228,6 +296,8 @@ int or_stream_cpd(or_coo const * tt, or_stream_cfg const * cfg,
   uint64_t prev_dim[OR_MAX_NMODES] = {0};
   uint64_t nnz_ptr = 0;
   uint64_t it = 0;
+  /* bstart[b]: first nonzero of slice b; bstart[b + 1] its end */
+  uint64_t * bstart = calloc(tt->dims[t] + 2, sizeof(uint64_t));
   while(nnz_ptr < tt->nnz) {
     /* ParserSimple::next_batch (ParserSimple.cc:89-149) */
     uint64_t start = nnz_ptr;
@@ -238,6 +308,8 @@ int or_stream_cpd(or_coo const * tt, or_stream_cfg const * cfg,
     if(bnnz == 0) {
       break;   /* an empty time slice ends the stream (the reference asserts) */
     }
+    bstart[it] = start;
+    bstart[it + 1] = end;
     or_coo batch;
     memset(&batch, 0, sizeof(batch));
     batch.nmodes = N;
@@ -374,9 +446,46 @@ int or_stream_cpd(or_coo const * tt, or_stream_cfg const * cfg,
       out->local_err[it - 1] = e * e;
       free(ones);
     }
+    /* checkpoints (StreamCPD.cc:517-528): every check_interval slices and
+     * after the slice that consumes the last nonzero (last_batch,
+     * ParserSimple.cc:229-232) */
+    double ge = -1., l10 = -1., ce = -1.;
+    if(cfg->check_interval > 0 && it <= out->max_batches &&
+        (it % cfg->check_interval == 0 || end == tt->nnz)) {
+      double * ones = malloc(R * sizeof(double));
+      for(uint64_t r=0; r < R; ++r) ones[r] = 1.;
+      uint64_t const nps[2] = {it, 10};
+      double errs[2];
+      for(int k=0; k < 2; ++k) {
+        or_coo w = p_window(tt, t, bstart, it, nps[k], prev_dim);
+        uint64_t const b0 = (nps[k] < it) ? it - nps[k] : 0;
+        double * f[OR_MAX_NMODES];
+        uint64_t fd[OR_MAX_NMODES];
+        for(uint64_t m=0; m < N; ++m) {
+          f[m] = (m == t) ? out->time + b0 * R : snew[m].v;
+          fd[m] = (m == t) ? it - b0 : snew[m].rows;
+        }
+        double const e = or_cpd_error(&w, f, fd, R, ones);
+        errs[k] = e * e;
+        if(k == 0 && cfg->refit) {
+          ce = p_refit(&w, R);
+          if(isnan(ce)) ce = -1.;
+        }
+        free(w.ind[t]);
+      }
+      ge = errs[0];
+      l10 = errs[1];
+      free(ones);
+    }
+    if(it - 1 < out->max_batches) {
+      if(out->global_err) out->global_err[it - 1] = ge;
+      if(out->local10_err) out->local10_err[it - 1] = l10;
+      if(out->cpd_err) out->cpd_err[it - 1] = ce;
+    }
     free(zeros);
     nnz_ptr = end;
   }
+  free(bstart);
   out->nbatches = it;
   for(uint64_t m=0; m < N; ++m) {
     if(m == t) {

@@ -125,7 +125,9 @@ int spd_f32_to_f64(float const * src, double * dst, uint64_t n, void * stream);
  *             2-norms of X when colnorm != NULL.
  *   admm_step: one ADMM inner iteration; sums needs 4 + R doubles:
  *             ||P||^2, ||P-aux||^2, ||P-init||^2, ||D||^2, then the column
- *             norms used by the prox. */
+ *             norms used by the prox.
+ *   admm_chunked: ADMM in independent chunks of `chunk` rows (nonneg or no
+ *             prox) to convergence; *its_rows = sum of iterations x rows. */
 int spd_dense_gram(double * X, uint64_t I, uint64_t R, double const * div,
     float * X32, double * out);
 int spd_dense_chol_inv(double const * const * aTa, int nmodes, int mode,
@@ -135,6 +137,9 @@ int spd_dense_apply(void const * M, int mdtype, double const * ginv,
 int spd_dense_admm_step(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint64_t R, double * P, double * aux, double * D,
     double * init, int prox, double * sums);
+int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,
+    double const * rho, uint64_t I, uint64_t R, uint64_t chunk, double * P, double * D,
+    int prox, uint64_t max_inner, double tol, uint64_t * its_rows);
 
 /* ---- CPD-ALS (src/cpd/cpd.c:161-332, src/cpd/admm.c:316-406) ---------- */
 
@@ -151,6 +156,8 @@ typedef struct {
   int kind;          /* SPD_CON_* */
   int prox;          /* SPD_PROX_* (ADMM) */
   double diag_add;   /* Frobenius regularisation added to the Gram diagonal */
+  uint64_t chunk;    /* ADMM rows per independent chunk (row-separable prox
+                        only, src/cpd/admm.c:355-400); 0: one block */
 } spd_con;
 
 typedef struct {
@@ -192,6 +199,18 @@ typedef struct {
    * draws them from rand(), StreamMatrix.cc:70-83) */
   void (*fill_rand)(void * ctx, double * vals, uint64_t n);
   void * rand_ctx;
+  /* checkpoints (StreamCPD.cc:517-528): every check_interval slices and
+   * after the slice that consumes the last nonzero, the squared errors of
+   * all slices so far and of the last 10 against the current factors;
+   * 0 disables them */
+  uint64_t check_interval;
+  /* at a checkpoint: a full CPD of the first nslices slices (host arrays,
+   * time labels 0 .. nslices-1, dims[stream_mode] = nslices) returning its
+   * relative squared error, 1 - fit (StreamCPD.cc:89-106). Called from the
+   * thread running the stream; NULL: not computed (reported as -1). */
+  double (*refit)(void * ctx, int nmodes, uint64_t nnz, uint64_t const * dims,
+      uint32_t const * const * inds, double const * vals, uint64_t rank);
+  void * refit_ctx;
 } spd_stream_cfg;
 
 typedef struct {
@@ -202,6 +221,10 @@ typedef struct {
   double * batch_seconds;            /* per slice wall time; NULL to skip */
   uint64_t * outer_its;              /* per slice; NULL to skip */
   double * final_err;                /* squared; NULL to skip */
+  /* per slice, -1 except at checkpoints; NULL to skip each */
+  double * global_err;               /* squared, all slices so far */
+  double * local10_err;              /* squared, the last 10 slices */
+  double * cpd_err;                  /* the refit's 1 - fit */
 } spd_stream_result;
 
 /* The stream: N-mode coordinates (host, uint32) already sorted by time

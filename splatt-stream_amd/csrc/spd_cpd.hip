@@ -104,6 +104,8 @@ extern "C" int spd_cpd_als(spd_csf * const * csfs, int const * mode_csf,
   SPD_TRY(spd_zero_async(info, sizeof(int), st));
   int * ctl = nullptr;
   SPD_TRY(mem.get((void **) &ctl, 2 * sizeof(int)));
+  unsigned long long * its_rows = nullptr;
+  SPD_TRY(mem.get((void **) &its_rows, sizeof(unsigned long long)));
   double * const d_rho = sc + 0;
   double * const d_norm = sc + 1;
   double * const d_inner = sc + 2;
@@ -139,7 +141,19 @@ extern "C" int spd_cpd_als(spd_csf * const * csfs, int const * mode_csf,
       SPD_TRY(spd_mttkrp(csfs[mode_csf[m]], m, R, mats, Mbuf, dt,
           rows_pos[mode_csf[m]], st));
       inner_its[m] = 0.;
-      if(con.kind == SPD_CON_ADMM) {
+      if(con.kind == SPD_CON_ADMM && con.chunk > 0) {
+        // row-separable prox: independent chunks of rows (src/cpd/admm.c:355-400)
+        SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
+        SPD_TRY(spd_zero_async(its_rows, sizeof(unsigned long long), st));
+        SPD_TRY(dn_admm_chunked(Mbuf, dt, ginv, d_rho, dims[m], R, con.chunk, U[m], dual[m],
+            con.prox, cfg->max_inner, cfg->inner_tol, its_rows, st));
+        if(cfg->verbosity > 1) {
+          unsigned long long hr = 0;
+          SPD_TRY(spd_d2h(&hr, its_rows, sizeof(hr)));
+          inner_its[m] = (double) hr / (double) dims[m];
+        }
+        SPD_TRY(dn_gram(U[m], dims[m], R, nullptr, U32[m], aTa[m], st));
+      } else if(con.kind == SPD_CON_ADMM) {
         SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
         SPD_TRY(dn_admm_solve(Mbuf, dt, ginv, d_rho, dims[m], R, U[m], dual[m], aux, init,
             con.prox, cfg->max_inner, cfg->inner_tol, d_sums, ctl, st));

@@ -383,7 +383,9 @@ k_chol_inv(GramSrc g, int nmodes, int mode, uint32_t R, uint32_t RP,
 // i of the matrix (identity beyond R), column broadcasts go through lane
 // shuffles and LDS, and every loop is unrolled, so nothing waits on
 // dependent LDS round trips. The arithmetic (operations and their order)
-// is that of k_chol_inv, so the two give the same bits.
+// is that of k_chol_inv, so the two give the same bits. Launched with
+// exactly RP threads: thread i owns row i, and rows i >= RP do not exist
+// in sL or ginv.
 template<int RP>
 __global__ void __launch_bounds__(64)
 k_chol_inv_reg(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
@@ -790,6 +792,103 @@ __global__ void k_ctl_reset(int * ctl)
   if(threadIdx.x < 2) ctl[threadIdx.x] = 0;
 }
 
+// Chunked ADMM for a row-separable proximal operator (src/cpd/admm.c:355-400
+// around p_admm_iterate_chunk, :240-308): block b owns rows [b * chunk,
+// min((b + 1) * chunk, I)) and iterates them to their own convergence. One
+// pass per iteration and row tile: aux = M + rho (P + D) into LDS, then per
+// entry the product with the inverse, P = prox(aux - D), D += P - aux and
+// the residual sums. Sums over the chunk are added in a fixed order
+// (thread, wave butterfly, waves in order). GL: the inverse (R <= 64) sits
+// in LDS. its_rows += iterations x rows of the chunk.
+#define CH_TILE 3072
+template<typename TM, bool GL>
+__global__ void __launch_bounds__(DN_THREADS)
+k_admm_chunk(TM const * __restrict__ M, double const * __restrict__ ginv, uint32_t RP,
+    double const * __restrict__ rho_p, uint64_t I, uint32_t R, uint64_t chunk,
+    double * __restrict__ P, double * __restrict__ D, int prox, uint32_t max_inner,
+    double tol, unsigned long long * __restrict__ its_rows)
+{
+  __shared__ double s_aux[CH_TILE];
+  __shared__ double s_g[GL ? 64 * 64 : 1];
+  __shared__ double s_w[DN_THREADS / 64][4];
+  __shared__ int s_done;
+  int const t = threadIdx.x;
+  uint64_t const r0 = (uint64_t) blockIdx.x * chunk;
+  uint64_t const r1 = (r0 + chunk < I) ? r0 + chunk : I;
+  if(GL) {
+    for(uint32_t e=t; e < R * R; e += DN_THREADS) {
+      uint32_t const i = e / R;
+      s_g[e] = ginv[(uint64_t) i * RP + (e - i * R)];
+    }
+  }
+  uint32_t const TR = CH_TILE / R;
+  double const rho = *rho_p;
+  uint32_t it = 0;
+  while(it < max_inner) {
+    double pn = 0., pr = 0., dr = 0., dn = 0.;
+    for(uint64_t t0=r0; t0 < r1; t0 += TR) {
+      uint64_t const nr = (t0 + TR < r1) ? TR : r1 - t0;
+      uint32_t const ne = (uint32_t) nr * R;
+      uint64_t const base = t0 * R;
+      __syncthreads();
+      for(uint32_t e=t; e < ne; e += DN_THREADS) {
+        s_aux[e] = (double) M[base + e] + rho * (P[base + e] + D[base + e]);
+      }
+      __syncthreads();
+      for(uint32_t e=t; e < ne; e += DN_THREADS) {
+        uint32_t const i = e / R;
+        uint32_t const j = e - i * R;
+        double const * const arow = s_aux + (uint64_t) i * R;
+        double x = 0.;
+        for(uint32_t k=0; k < R; ++k) {
+          double const g = GL ? s_g[k * R + j] : ginv[(uint64_t) k * RP + j];
+          x += arow[k] * g;
+        }
+        double const p0 = P[base + e];
+        double const d0 = D[base + e];
+        double p = x - d0;
+        if(prox == DN_PROX_NONNEG) p = (p > 0.) ? p : 0.;
+        double const d = d0 + (p - x);
+        P[base + e] = p;
+        D[base + e] = d;
+        double const pd = p - x;
+        double const dd = p - p0;
+        pn += p * p;
+        pr += pd * pd;
+        dr += dd * dd;
+        dn += d * d;
+      }
+    }
+    for(int off=32; off > 0; off >>= 1) {
+      pn += __shfl_xor(pn, off, 64);
+      pr += __shfl_xor(pr, off, 64);
+      dr += __shfl_xor(dr, off, 64);
+      dn += __shfl_xor(dn, off, 64);
+    }
+    if((t & 63) == 0) {
+      s_w[t >> 6][0] = pn;
+      s_w[t >> 6][1] = pr;
+      s_w[t >> 6][2] = dr;
+      s_w[t >> 6][3] = dn;
+    }
+    __syncthreads();
+    if(t == 0) {
+      double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+      for(int w=0; w < DN_THREADS / 64; ++w) {
+        a0 += s_w[w][0];
+        a1 += s_w[w][1];
+        a2 += s_w[w][2];
+        a3 += s_w[w][3];
+      }
+      s_done = (a1 <= tol * a0 && a2 <= tol * a3) ? 1 : 0;
+    }
+    __syncthreads();
+    ++it;
+    if(s_done) break;
+  }
+  if(t == 0) atomicAdd(its_rows, (unsigned long long) it * (unsigned long long) (r1 - r0));
+}
+
 // Plain product for ranks above 64: out[i,j] = sum_k In[i,k] ginv[k,j].
 template<typename TM>
 __global__ void k_apply_plain(TM const * __restrict__ In, double const * __restrict__ ginv,
@@ -1058,11 +1157,11 @@ int dn_chol_inv(double const * const * aTa, int nmodes, int mode, uint32_t R,
   uint32_t const RP = dn_rpad(R);
   if(RP <= 64) {
     switch(RP) {
-    case 16: hipLaunchKernelGGL((k_chol_inv_reg<16>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+    case 16: hipLaunchKernelGGL((k_chol_inv_reg<16>), dim3(1), dim3(16), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
-    case 32: hipLaunchKernelGGL((k_chol_inv_reg<32>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+    case 32: hipLaunchKernelGGL((k_chol_inv_reg<32>), dim3(1), dim3(32), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
-    case 48: hipLaunchKernelGGL((k_chol_inv_reg<48>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+    case 48: hipLaunchKernelGGL((k_chol_inv_reg<48>), dim3(1), dim3(48), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
     default: hipLaunchKernelGGL((k_chol_inv_reg<64>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
@@ -1339,6 +1438,44 @@ int dn_admm_step(void const * M, int mdtype, double const * ginv,
   return admm_t<double>((double const *) M, ginv, rho, I, R, P, aux, D, init, prox, sums, st);
 }
 
+template<typename TM>
+static int admm_chunked_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
+    uint32_t R, uint64_t chunk, double * P, double * D, int prox, uint32_t max_inner,
+    double tol, unsigned long long * its_rows, hipStream_t st)
+{
+  uint64_t const nchunks = spd_cdiv(I, chunk);
+  uint32_t const RP = dn_rpad(R);
+  if(R <= 64) {
+    hipLaunchKernelGGL((k_admm_chunk<TM, true>), dim3((unsigned) nchunks), dim3(DN_THREADS),
+        0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, max_inner, tol, its_rows);
+  } else {
+    hipLaunchKernelGGL((k_admm_chunk<TM, false>), dim3((unsigned) nchunks), dim3(DN_THREADS),
+        0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, max_inner, tol, its_rows);
+  }
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int dn_admm_chunked(void const * M, int mdtype, double const * ginv, double const * rho,
+    uint64_t I, uint32_t R, uint64_t chunk, double * P, double * D, int prox,
+    uint64_t max_inner, double tol, unsigned long long * its_rows, hipStream_t st)
+{
+  if(I == 0 || max_inner == 0) return SPD_OK;
+  if(chunk == 0 || R == 0 || R > CH_TILE || spd_cdiv(I, chunk) > 0x7FFFFFFFull ||
+      (prox != DN_PROX_NONNEG && prox != DN_PROX_NONE)) {
+    spd_set_error(SPD_EINPUT, "chunked ADMM: unsupported shape (rows %llu, rank %u, "
+        "chunk %llu, prox %d)", (unsigned long long) I, R, (unsigned long long) chunk, prox);
+    return SPD_EINPUT;
+  }
+  uint32_t const mi = (max_inner > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t) max_inner;
+  if(mdtype == SPD_F32) {
+    return admm_chunked_t<float>((float const *) M, ginv, rho, I, R, chunk, P, D, prox, mi,
+        tol, its_rows, st);
+  }
+  return admm_chunked_t<double>((double const *) M, ginv, rho, I, R, chunk, P, D, prox, mi,
+      tol, its_rows, st);
+}
+
 int dn_cpd_norm(double const * const * aTa, int nmodes, uint32_t R,
     double const * lambda, double * out, hipStream_t st)
 {
@@ -1372,6 +1509,17 @@ extern "C" int spd_dense_apply(void const * M, int mdtype, double const * ginv,
 {
   SPD_TRY(spd_init());
   return dn_apply(M, mdtype, ginv, I, (uint32_t) R, X, colnorm, spd_hs(nullptr));
+}
+
+extern "C" int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,
+    double const * rho, uint64_t I, uint64_t R, uint64_t chunk, double * P, double * D,
+    int prox, uint64_t max_inner, double tol, uint64_t * its_rows)
+{
+  SPD_TRY(spd_init());
+  hipStream_t const st = spd_hs(nullptr);
+  SPD_TRY(spd_zero_async(its_rows, sizeof(uint64_t), st));
+  return dn_admm_chunked(M, mdtype, ginv, rho, I, (uint32_t) R, chunk, P, D, prox, max_inner,
+      tol, (unsigned long long *) its_rows, st);
 }
 
 extern "C" int spd_dense_admm_step(void const * M, int mdtype, double const * ginv,

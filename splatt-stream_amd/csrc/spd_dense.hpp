@@ -58,6 +58,15 @@ int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const 
     uint64_t I, uint32_t R, double * P, double * D, double * aux, double * init, int prox,
     uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st);
 
+// ADMM in independent chunks of `chunk` rows for a row-separable prox
+// (DN_PROX_NONNEG or none; src/cpd/admm.c:355-400): each chunk iterates to
+// its own convergence, up to max_inner times, one block per chunk, no host
+// synchronisation. *its_rows (device, zeroed by the caller) accumulates
+// iterations x rows per chunk. Ranks up to 3072.
+int dn_admm_chunked(void const * M, int mdtype, double const * ginv, double const * rho,
+    uint64_t I, uint32_t R, uint64_t chunk, double * P, double * D, int prox,
+    uint64_t max_inner, double tol, unsigned long long * its_rows, hipStream_t st);
+
 // out (R x R, full) = X^T Y for two I x R matrices.
 int dn_xgram(double const * X, double const * Y, uint64_t I, uint32_t R,
     double * out, hipStream_t st);

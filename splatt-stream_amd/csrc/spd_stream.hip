@@ -533,6 +533,9 @@ int StreamState::slice(uint32_t const * const * ind_in, double const * vals_b, u
           fabs(delta - prev_delta));
     }
     if(outer > 0 && fabs(delta - prev_delta) < cfg.tol) {
+      if(cfg.verbosity > 1) {
+        printf("  converged in: %llu\n", (unsigned long long) (outer + 1));
+      }
       ++outer;
       break;
     }
@@ -648,6 +651,58 @@ extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
 
   StreamState s;
   SPD_TRY(s.init(N, cfg, caps, st));
+
+  // cpd_error (src/cpd/cpd.c:580-656) of slices [b0, b1) against the factors
+  // (time rows gtime[b0 .. b1), the other modes' current rows); wd are the
+  // dims of that tensor, whose smallest mode takes the MTTKRP. Squared.
+  double * M2 = nullptr;
+  {
+    uint64_t mr = T;
+    for(int m=0; m < N; ++m) if(dims[m] > mr) mr = dims[m];
+    SPD_TRY(mem.get((void **) &M2, (mr ? mr : 1) * R * sizeof(double)));
+  }
+  double * const d_inner = s.sc + 4;
+  auto window_errsq = [&](uint64_t b0, uint64_t b1, uint64_t const * wd,
+      double * out) -> int {
+    int small = 0;
+    for(int m=1; m < N; ++m) if(wd[m] < wd[small]) small = m;
+    uint64_t const x0 = bstart[b0];
+    uint64_t const x1 = bend[b1 - 1];
+    uint64_t const nt = b1 - b0;
+    double const * fm[SPD_MAX_MODES];
+    for(int m=0; m < N; ++m) fm[m] = (m == t) ? gtime : s.Unew[m];
+    uint64_t frows = 0;
+    if(small == t) {
+      // the stream is sorted by time: one single-row reduction per slice
+      frows = nt;
+      SPD_TRY(spd_zero_async(M2, nt * R * sizeof(double), st));
+      for(uint64_t b=b0; b < b1; ++b) {
+        uint32_t const * ib[SPD_MAX_MODES];
+        for(int m=0; m < N; ++m) ib[m] = d_ind[m] + bstart[b];
+        SPD_TRY(coo_mttkrp(ib, d_vals + bstart[b], bend[b] - bstart[b], N, t, fm, R, 1,
+            M2 + (b - b0) * R, st));
+      }
+    } else {
+      frows = s.rows[small];
+      uint32_t const * ib[SPD_MAX_MODES];
+      for(int m=0; m < N; ++m) ib[m] = d_ind[m] + x0;
+      SPD_TRY(coo_mttkrp(ib, d_vals + x0, x1 - x0, N, small, fm, R, wd[small], M2, st));
+    }
+    double const * fsm = (small == t) ? gtime + b0 * R : s.Unew[small];
+    SPD_TRY(dn_innerprod(fsm, M2, SPD_F64, frows, R, s.lam1, d_inner, st));
+    SPD_TRY(dn_gram(gtime + b0 * R, nt, R, nullptr, nullptr, s.tgram, st));
+    double const * grams[SPD_MAX_MODES];
+    for(int m=0; m < N; ++m) grams[m] = (m == t) ? s.tgram : s.aTa[m];
+    SPD_TRY(dn_cpd_norm(grams, N, R, s.lam1, d_inner + 1, st));
+    double h[2];
+    SPD_TRY(spd_d2h(h, d_inner, sizeof(h)));
+    double xn = 0.;
+    for(uint64_t x=x0; x < x1; ++x) xn += vals[x] * vals[x];
+    double const e = sqrt(xn + h[1] - 2. * h[0]) / sqrt(xn);
+    *out = e * e;
+    return SPD_OK;
+  };
+
   for(uint64_t b=0; b < T; ++b) {
     uint32_t const * ib[SPD_MAX_MODES];
     for(int m=0; m < N; ++m) {
@@ -664,11 +719,31 @@ extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
     if(res->local_err) res->local_err[b] = le;
     if(res->batch_seconds) res->batch_seconds[b] = secs;
     if(res->outer_its) res->outer_its[b] = outer;
+
+    // checkpoints (StreamCPD.cc:517-528)
+    uint64_t const done = b + 1;
+    double ge = -1., l10 = -1., ce = -1.;
+    if(cfg.check_interval > 0 && (done % cfg.check_interval == 0 || bend[b] == nnz)) {
+      uint64_t wd[SPD_MAX_MODES];
+      for(int m=0; m < N; ++m) wd[m] = (m == t) ? done : s.rows[m];
+      SPD_TRY(window_errsq(0, done, wd, &ge));
+      uint64_t const b0 = (10 < done) ? done - 10 : 0;
+      wd[t] = done - b0;
+      SPD_TRY(window_errsq(b0, done, wd, &l10));
+      if(cfg.refit) {
+        wd[t] = done;
+        ce = cfg.refit(cfg.refit_ctx, N, bend[b], wd, inds, vals, R);
+        if(isnan(ce)) ce = -1.;
+      }
+    }
+    if(res->global_err) res->global_err[b] = ge;
+    if(res->local10_err) res->local10_err[b] = l10;
+    if(res->cpd_err) res->cpd_err[b] = ce;
     if(cfg.verbosity > 0) {
       printf("batch %5llu: %7llu nnz (%0.5fs) (%0.3e NNZ/s) cpd: %+0.5f global: "
-          "%+0.5f local-1: %+0.5f local-10: %+0.5f\n", (unsigned long long) (b + 1),
+          "%+0.5f local-1: %+0.5f local-10: %+0.5f\n", (unsigned long long) done,
           (unsigned long long) (bend[b] - bstart[b]), secs,
-          (double) (bend[b] - bstart[b]) / secs, -1., -1., le, -1.);
+          (double) (bend[b] - bstart[b]) / secs, ce, ge, le, l10);
       fflush(stdout);
     }
   }
@@ -682,42 +757,8 @@ extern "C" int spd_stream_cpd(int N, uint64_t nnz, uint64_t const * dims,
   // final error of the whole stream (StreamCPD.cc:549-556)
   res->nbatches = T;
   for(int m=0; m < N; ++m) res->dims[m] = (m == t) ? T : s.rows[m];
-  double * const d_inner = s.sc + 4;
   if(res->final_err && T > 0) {
-    int small = 0;
-    uint64_t fd[SPD_MAX_MODES];
-    for(int m=0; m < N; ++m) fd[m] = dims[m];
-    for(int m=1; m < N; ++m) if(fd[m] < fd[small]) small = m;
-    double const * fm[SPD_MAX_MODES];
-    for(int m=0; m < N; ++m) fm[m] = (m == t) ? gtime : s.Unew[m];
-    uint64_t const frows = (small == t) ? T : s.rows[small];
-    double * M2 = nullptr;
-    SPD_TRY(mem.get((void **) &M2, fd[small] * R * sizeof(double)));
-    if(small == t) {
-      // the stream is sorted by time: one single-row reduction per slice
-      SPD_TRY(spd_zero_async(M2, fd[small] * R * sizeof(double), st));
-      for(uint64_t b=0; b < T; ++b) {
-        uint32_t const * ib[SPD_MAX_MODES];
-        for(int m=0; m < N; ++m) ib[m] = d_ind[m] + bstart[b];
-        SPD_TRY(coo_mttkrp(ib, d_vals + bstart[b], bend[b] - bstart[b], N, t, fm, R, 1,
-            M2 + b * R, st));
-      }
-    } else {
-      uint32_t const * ib[SPD_MAX_MODES];
-      for(int m=0; m < N; ++m) ib[m] = d_ind[m];
-      SPD_TRY(coo_mttkrp(ib, d_vals, bend[T - 1], N, small, fm, R, fd[small], M2, st));
-    }
-    SPD_TRY(dn_innerprod(fm[small], M2, SPD_F64, frows, R, s.lam1, d_inner, st));
-    double const * grams[SPD_MAX_MODES];
-    SPD_TRY(dn_gram(gtime, T, R, nullptr, nullptr, s.tgram, st));
-    for(int m=0; m < N; ++m) grams[m] = (m == t) ? s.tgram : s.aTa[m];
-    SPD_TRY(dn_cpd_norm(grams, N, R, s.lam1, d_inner + 1, st));
-    double h[2];
-    SPD_TRY(spd_d2h(h, d_inner, sizeof(h)));
-    double xn = 0.;
-    for(uint64_t x=0; x < bend[T - 1]; ++x) xn += vals[x] * vals[x];
-    double const e = sqrt(xn + h[1] - 2. * h[0]) / sqrt(xn);
-    *res->final_err = e * e;
+    SPD_TRY(window_errsq(0, T, dims, res->final_err));
   }
   for(int m=0; m < N; ++m) {
     if(res->factors[m] == nullptr) continue;

@@ -49,6 +49,7 @@ int spl_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
   out->kind = SPD_CON_NONE;
   out->prox = SPD_PROX_NONE;
   out->diag_add = 0.;
+  out->chunk = 0;
   if(con->gram_func == splatt_frob_gram) {
     out->diag_add = *((val_t const *) con->data);
   } else if(con->gram_func != NULL) {
@@ -81,10 +82,7 @@ int spl_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
   /* src/cpd/admm.c:356-366: row-separable constraints are solved in
    * independent chunks of `chunk` rows */
   if(con->hints.row_separable && chunk > 0) {
-    fprintf(stderr, "SPLATT: chunked ADMM (constraint '%s', %" SPLATT_PF_IDX
-        " rows per chunk) is not available; set chunk_sizes[%" SPLATT_PF_IDX
-        "] = 0 to solve the mode as one block.\n", con->description, chunk, mode);
-    return -1;
+    out->chunk = chunk;
   }
   return 0;
 }

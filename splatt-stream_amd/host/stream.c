@@ -10,6 +10,7 @@
  * (spd_stream_cpd).
  */
 #include "host.h"
+#include <math.h>
 
 /* src/cpd/stream/stream.cc:22-40 */
 static void p_print_stream_stats(char const * fname, idx_t rank, idx_t stream_mode,
@@ -38,6 +39,9 @@ void splatt_amd_free_stream_result(splatt_amd_stream_result * r)
   free(r->local_err);
   free(r->batch_seconds);
   free(r->outer_its);
+  free(r->global_err);
+  free(r->local10_err);
+  free(r->cpd_err);
   free(r);
 }
 
@@ -47,9 +51,46 @@ static void p_fill_rand(void * ctx, double * vals, uint64_t n)
   spl_fill_rand(vals, n);
 }
 
+/* StreamCPD::compute_cpd_errorsq (StreamCPD.cc:89-106): a CSF with the
+ * default options, factors from rand() (splatt_alloc_cpd), splatt_cpd with
+ * the default options; the relative squared error 1 - fit. */
+static double p_refit(void * ctx, int nmodes, uint64_t nnz, uint64_t const * dims,
+    uint32_t const * const * inds, double const * vals, uint64_t rank)
+{
+  (void) ctx;
+  sptensor_t * tt = splatt_tt_alloc(nnz, (idx_t) nmodes);
+  if(tt == NULL) {
+    return NAN;
+  }
+  for(int m=0; m < nmodes; ++m) {
+    tt->dims[m] = dims[m];
+    for(uint64_t x=0; x < nnz; ++x) {
+      tt->ind[m][x] = inds[m][x];
+    }
+  }
+  memcpy(tt->vals, vals, nnz * sizeof(val_t));
+  double * opts = splatt_default_opts();
+  splatt_csf * csf = splatt_csf_alloc(tt, opts);
+  splatt_tt_free(tt);
+  double err = NAN;
+  if(csf != NULL) {
+    splatt_kruskal * k = splatt_alloc_cpd(csf, rank);
+    if(k != NULL && splatt_cpd(csf, rank, NULL, NULL, k) == SPLATT_SUCCESS) {
+      err = 1. - k->fit;
+    }
+    if(k != NULL) {
+      splatt_free_cpd(k);
+    }
+    splatt_free_csf(csf, opts);
+  }
+  splatt_free_opts(opts);
+  return err;
+}
+
 splatt_amd_stream_result * splatt_amd_cpd_stream_coo(
     sptensor_t const * const tt, splatt_idx_t rank, splatt_idx_t stream_mode,
-    double forget, splatt_cpd_opts * const cpd_options,
+    double forget, splatt_idx_t check_interval, int refit,
+    splatt_cpd_opts * const cpd_options,
     splatt_global_opts const * const global_options)
 {
   idx_t const N = tt->nmodes;
@@ -99,6 +140,8 @@ splatt_amd_stream_result * splatt_amd_cpd_stream_coo(
   cfg.inner_tol = copts->inner_tolerance;
   cfg.verbosity = (int) gopts->verbosity;
   cfg.fill_rand = p_fill_rand;
+  cfg.check_interval = check_interval;
+  cfg.refit = refit ? p_refit : NULL;
   int ok = 1;
   for(idx_t m=0; m < N; ++m) {
     if(spl_map_constraint(copts->constraints[m], m, 0, &cfg.con[m]) != 0) {
@@ -173,6 +216,9 @@ splatt_amd_stream_result * splatt_amd_cpd_stream_coo(
     res->local_err = calloc(T ? T : 1, sizeof(double));
     res->batch_seconds = calloc(T ? T : 1, sizeof(double));
     res->outer_its = calloc(T ? T : 1, sizeof(idx_t));
+    res->global_err = calloc(T ? T : 1, sizeof(double));
+    res->local10_err = calloc(T ? T : 1, sizeof(double));
+    res->cpd_err = calloc(T ? T : 1, sizeof(double));
 
     spd_stream_result out;
     memset(&out, 0, sizeof(out));
@@ -181,6 +227,9 @@ splatt_amd_stream_result * splatt_amd_cpd_stream_coo(
     out.batch_seconds = res->batch_seconds;
     out.outer_its = (uint64_t *) res->outer_its;
     out.final_err = &res->final_err;
+    out.global_err = res->global_err;
+    out.local10_err = res->local10_err;
+    out.cpd_err = res->cpd_err;
     int rc = spd_stream_cpd((int) N, nnz, tt->dims, (uint32_t const * const *) sind,
         svals, &cfg, &out);
     if(rc != SPD_OK) {
@@ -253,7 +302,14 @@ splatt_error_type splatt_cpd_stream(char const * const filename,
   if(tt == NULL) {
     ret = SPLATT_ERROR_BADINPUT;
   } else {
-    res = splatt_amd_cpd_stream_coo(tt, rank, stream_mode, forget, copts, gopts);
+    /* the reference prints every batch line and the outer iterations'
+     * deltas (StreamCPD.cc:472-477, :530-534) whatever the verbosity;
+     * CHECK_ERR_INTERVAL is 100 (StreamCPD.cc:23-25) */
+    splatt_global_opts loud = *gopts;
+    if(loud.verbosity < SPLATT_VERBOSITY_HIGH) {
+      loud.verbosity = SPLATT_VERBOSITY_HIGH;
+    }
+    res = splatt_amd_cpd_stream_coo(tt, rank, stream_mode, forget, 100, 1, copts, &loud);
     if(res == NULL) {
       ret = SPLATT_ERROR_BADINPUT;
     }

@@ -93,7 +93,8 @@ class StreamResult(C.Structure):
                 ("factors", f64p * MAX_NMODES),
                 ("iperm", u64p * MAX_NMODES),
                 ("local_err", f64p), ("batch_seconds", f64p),
-                ("outer_its", u64p), ("final_err", C.c_double)]
+                ("outer_its", u64p), ("final_err", C.c_double),
+                ("global_err", f64p), ("local10_err", f64p), ("cpd_err", f64p)]
 
 
 class MatrixT(C.Structure):
@@ -210,10 +211,15 @@ def lib():
     L.spd_dense_admm_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    L.spd_dense_admm_chunked.restype = C.c_int
+    L.spd_dense_admm_chunked.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
+                                         C.c_void_p, C.c_int, C.c_uint64, C.c_double,
+                                         C.c_void_p]
     L.splatt_amd_cpd_stream_coo.restype = C.POINTER(StreamResult)
     L.splatt_amd_cpd_stream_coo.argtypes = [C.POINTER(SplattCoord), C.c_uint64, C.c_uint64,
-                                            C.c_double, C.POINTER(CpdOpts),
-                                            C.POINTER(GlobalOpts)]
+                                            C.c_double, C.c_uint64, C.c_int,
+                                            C.POINTER(CpdOpts), C.POINTER(GlobalOpts)]
     L.splatt_amd_free_stream_result.argtypes = [C.POINTER(StreamResult)]
     L.splatt_cpd_stream.restype = C.c_int
     L.splatt_cpd_stream.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_double,
@@ -495,9 +501,11 @@ def coord_from_arrays(inds, vals, dims):
 
 
 def run_stream(inds, vals, dims, rank, stream_mode, forget=0.95, max_its=5, seed=1,
-               frob=None, verbosity=0):
+               frob=None, verbosity=0, check_interval=0, refit=False):
     """splatt_amd_cpd_stream_coo after srand(seed); frob registers Frobenius
-    regularisation on the streamed mode. Returns a dict of numpy results."""
+    regularisation on the streamed mode; check_interval/refit drive the
+    checkpoint errors (the reference uses 100 and a refit). Returns a dict of
+    numpy results."""
     L = lib()
     tt = coord_from_arrays(inds, vals, dims)
     co = L.splatt_alloc_cpd_opts()
@@ -508,7 +516,8 @@ def run_stream(inds, vals, dims, rank, stream_mode, forget=0.95, max_its=5, seed
     go = L.splatt_alloc_global_opts()
     go[0].verbosity = verbosity
     srand(seed)
-    r = L.splatt_amd_cpd_stream_coo(tt, rank, stream_mode, forget, co, go)
+    r = L.splatt_amd_cpd_stream_coo(tt, rank, stream_mode, forget, check_interval,
+                                    1 if refit else 0, co, go)
     out = None
     if r:
         R = r[0]
@@ -523,7 +532,10 @@ def run_stream(inds, vals, dims, rank, stream_mode, forget=0.95, max_its=5, seed
                    local_err=np.ctypeslib.as_array(R.local_err, shape=(max(nb, 1),))[:nb].copy(),
                    batch_seconds=np.ctypeslib.as_array(R.batch_seconds, shape=(max(nb, 1),))[:nb].copy(),
                    outer_its=np.ctypeslib.as_array(R.outer_its, shape=(max(nb, 1),))[:nb].copy(),
-                   final_err=float(R.final_err))
+                   final_err=float(R.final_err),
+                   global_err=np.ctypeslib.as_array(R.global_err, shape=(max(nb, 1),))[:nb].copy(),
+                   local10_err=np.ctypeslib.as_array(R.local10_err, shape=(max(nb, 1),))[:nb].copy(),
+                   cpd_err=np.ctypeslib.as_array(R.cpd_err, shape=(max(nb, 1),))[:nb].copy())
         L.splatt_amd_free_stream_result(r)
     L.splatt_free_cpd_opts(co)
     L.splatt_free_global_opts(go)

@@ -283,10 +283,53 @@ def test_cpd_admm_maxcolnorm(gpu_available):
     _model_close(res, facs, lam, 1e-5)
 
 
-def test_cpd_rejects_chunked_nonneg(gpu_available):
+@pytest.mark.parametrize("chunk", [50, 7, 0])
+def test_cpd_nonneg_chunked(gpu_available, chunk):
+    # nonneg on modes 0 and 2 with the reference's default of 50 rows per
+    # ADMM chunk, a ragged 7, and 0 (one block); each chunk converges on its
+    # own, so the iteration counts of the chunks differ
     def configure(L, co):
-        modes = np.array([0], dtype=np.uint64)
-        L.splatt_register_nonneg(co, modes.ctypes.data_as(ps.u64p), 1)
-    cs = _twomode(path=tensor_path("small.tns"))
-    res = ps.run_cpd(cs, 3, max_its=2, configure=configure)
-    assert res["rc"] != ps.SUCCESS
+        modes = np.array([0, 2], dtype=np.uint64)
+        L.splatt_register_nonneg(co, modes.ctypes.data_as(ps.u64p), 2)
+        for m in range(3):
+            co[0].chunk_sizes[m] = chunk
+    res, facs, lam, fits = _run_both(tensor_path("med.tns"), 5, 3,
+                                     cons=[(2, 3, 0.0, chunk), (1, 0, 0.0), (2, 3, 0.0, chunk)],
+                                     unconstrained=False, configure=configure)
+    for m in (0, 2):
+        assert np.all(res["factors"][m] >= 0)
+    assert abs(res["fit"] - fits[-1]) <= 1e-6 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-5)
+
+
+def test_admm_chunked_matches_oracle(gpu_available):
+    # the chunk kernel alone: rank 72 keeps the inverse in global memory,
+    # FP32 MTTKRP input, 33-row chunks
+    I, R, chunk = 300, 72, 33
+    rng = np.random.default_rng(31)
+    A = rng.standard_normal((3 * R, R))
+    G = A.T @ A + 0.5 * np.eye(R)
+    M = (rng.standard_normal((I, R)) * 4.0).astype(np.float32)
+    P0 = np.abs(rng.standard_normal((I, R)))
+    D0 = rng.standard_normal((I, R)) * 0.1
+    wantP, wantD, wtot = orc.admm_chunked(3, M.astype(np.float64), P0, D0, G, chunk, 50, 1e-2)
+    L = ps.lib()
+    RP = _rp(R)
+    grams = [G, np.ones((R, R))]
+    bufs = [_dev(g) for g in grams]
+    arr = (C.c_void_p * 2)(*[b.ptr for b in bufs])
+    ginv = ps.DevBuf(RP * RP * 8)
+    rho = ps.DevBuf(8)
+    info = ps.DevBuf(8)
+    # two modes, mode 1 solved: the normal matrix is aTa[0] = G
+    assert L.spd_dense_chol_inv(arr, 2, 1, R, 0.0, 1, ginv.ptr, rho.ptr, info.ptr) == 0
+    dM, dP, dD = _dev(M), _dev(P0), _dev(D0)
+    cnt = ps.DevBuf(8)
+    assert L.spd_dense_admm_chunked(dM.ptr, 1, ginv.ptr, rho.ptr, I, R, chunk, dP.ptr, dD.ptr,
+                                    3, 50, C.c_double(1e-2), cnt.ptr) == 0
+    P = dP.to_host((I, R), np.float64)
+    D = dD.to_host((I, R), np.float64)
+    assert int(cnt.to_host((1,), np.uint64)[0]) == wtot
+    tol = 1e-8
+    assert np.allclose(P, wantP, rtol=tol, atol=tol * np.max(np.abs(wantP)))
+    assert np.allclose(D, wantD, rtol=tol, atol=tol * np.max(np.abs(wantP)))

@@ -49,14 +49,16 @@ def _stream(T, dims, per_slice, seed, zipf=None):
     return inds, vals, full
 
 
-def _oracle(inds, vals, dims, stream_mode, R, forget, max_its, seed, frob):
+def _oracle(inds, vals, dims, stream_mode, R, forget, max_its, seed, frob,
+            check_interval=0, refit=False):
     coo = orc.Coo.from_arrays(inds, vals, dims)
     iperm = orc.stream_prepare(coo, stream_mode)
     N = len(dims)
     cons = [(2, 1, 0.0)] * N
     cons[stream_mode] = (1, 0, frob)
     libc.srand(seed)
-    res = orc.stream_cpd(coo, stream_mode, R, forget, max_its, cons)
+    res = orc.stream_cpd(coo, stream_mode, R, forget, max_its, cons,
+                         check_interval=check_interval, refit=refit)
     res["iperm"] = iperm
     return res
 
@@ -120,6 +122,28 @@ def test_stream_unconstrained_time_mode(gpu_available):
     _compare(dev, ref, 2)
 
 
+@pytest.mark.parametrize("d0", [40, 6])
+def test_stream_checkpoints_match_oracle(gpu_available, d0):
+    # every 5 slices and after the last: errors of all slices so far and of
+    # the last 10 (d0 = 6 puts the MTTKRP of both windows on mode 0, d0 = 40
+    # on the time mode), and the refit, whose rand() draws move the later
+    # slices' new rows
+    inds, vals, dims = _stream(14, [d0, 30], (20, 60), seed=23 + d0)
+    dev = ps.run_stream(inds, vals, dims, 8, 2, forget=0.95, max_its=5, seed=3, frob=1e-2,
+                        check_interval=5, refit=True)
+    ref = _oracle(inds, vals, dims, 2, 8, 0.95, 5, 3, 1e-2, check_interval=5, refit=True)
+    _compare(dev, ref, 2)
+    at = [4, 9, 13]
+    for key in ("global_err", "local10_err", "cpd_err"):
+        assert [b for b in range(14) if ref[key][b] != -1.0] == at, key
+        assert np.array_equal(dev[key] == -1.0, ref[key] == -1.0), key
+    for key in ("global_err", "local10_err"):
+        assert np.allclose(dev[key], ref[key], rtol=1e-8, atol=1e-12), key
+    # 200-iteration ALS on each side: agreement to 1e-6
+    assert np.allclose(dev["cpd_err"], ref["cpd_err"], rtol=1e-6, atol=1e-9)
+    assert abs(dev["global_err"][13] - dev["final_err"]) <= 1e-12
+
+
 def test_cpd_stream_file_entry(gpu_available, tmp_path, capfd):
     inds, vals, dims = _stream(5, [20, 15], (20, 40), seed=17)
     path = tmp_path / "s.tns"
@@ -142,6 +166,16 @@ def test_cpd_stream_file_entry(gpu_available, tmp_path, capfd):
     assert rc == ps.SUCCESS
     out = capfd.readouterr().out
     assert "final-err:" in out
+    # the reference's batch line carries the checkpoint errors after the
+    # last slice (a refit there draws rand() after every factor row)
+    last = [l for l in out.splitlines() if l.startswith("batch ")]
+    assert len(last) == 5
+    f = last[-1].split()
+    got = {f[i][:-1]: float(f[i + 1]) for i in range(len(f) - 1)
+           if f[i] in ("cpd:", "global:", "local-1:", "local-10:")}
+    assert got["cpd"] >= 0 and got["local-10"] >= 0
+    assert abs(got["global"] - ref["final_err"]) <= 1e-5
+    assert all(" global: -1.00000 " in l for l in last[:-1])
     ref = _oracle(inds, vals, dims, 2, 4, 0.95, 3, 1, 0.0)
     line = [l for l in out.splitlines() if l.startswith("final-err:")][0]
     assert abs(float(line.split()[1]) - ref["final_err"]) <= 1e-5

@@ -177,6 +177,11 @@ def test_cpd_exact_low_rank():
 #                        below 1 the constraint is inactive: the limit is
 #                        G^{-1} M; otherwise every column norm is <= 1.
 #   or_prox              closed forms of the three operators.
+#   or_admm_chunked      equals or_admm run on each chunk of rows on its own
+#                        (ragged last chunk, summed iteration count); one
+#                        chunk of at least I rows is the one-block ADMM; a
+#                        chunked nonneg CPD meets the nonneg least-squares
+#                        optimality conditions.
 #   or_cpd_als_con       frob with multiplier 0 equals the unconstrained
 #                        solve without normalisation; with multiplier mu the
 #                        solved factor satisfies (G + mu I) U^T = M^T.
@@ -252,6 +257,54 @@ def test_admm_stops_on_tolerance():
     M = rng.standard_normal((I, R))
     P, D, its, rho = orc.admm(3, M, np.zeros((I, R)), np.zeros((I, R)), G, 50, 1e-2)
     assert 1 <= its < 50
+
+
+@pytest.mark.parametrize("chunk", [7, 10, 50, 1000])
+def test_admm_chunked_is_independent_chunks(chunk):
+    """Each chunk of rows (the last ragged) runs its own ADMM to its own
+    convergence from the same G + rho I; a chunk at least I rows long is the
+    one-block ADMM; the count is iterations x rows summed over chunks."""
+    R, I = 5, 43
+    G = _spd(R, 12)
+    rng = np.random.default_rng(13)
+    M = rng.standard_normal((I, R)) * 2.0
+    P0 = np.abs(rng.standard_normal((I, R)))
+    D0 = rng.standard_normal((I, R)) * 0.1
+    P, D, tot = orc.admm_chunked(3, M, P0, D0, G, chunk, 50, 1e-2)
+    want_tot = 0
+    for s0 in range(0, I, chunk):
+        s1 = min(s0 + chunk, I)
+        Pc, Dc, its, rho = orc.admm(3, M[s0:s1], P0[s0:s1], D0[s0:s1], G, 50, 1e-2)
+        assert np.array_equal(P[s0:s1], Pc) and np.array_equal(D[s0:s1], Dc)
+        want_tot += its * (s1 - s0)
+    assert tot == want_tot
+    assert np.all(P >= 0)
+    if chunk >= I:
+        Pf, Df, its, rho = orc.admm(3, M, P0, D0, G, 50, 1e-2)
+        assert np.array_equal(P, Pf) and tot == its * I
+
+
+def test_cpd_nonneg_chunked_kkt():
+    """Nonneg CPD with chunks of 4 rows: a few sweeps keep every factor
+    non-negative and each chunk's ADMM (given enough inner iterations and a
+    tight tolerance) reaches the nonneg least-squares optimum of its rows."""
+    dims = [9, 8, 7]
+    inds, vals = _tensor(dims, 200, 21)
+    c = orc.Coo.from_arrays(inds, vals, dims)
+    csfs = [orc.Csf(c, orc.mode_order(dims, 3, m)) for m in range(3)]
+    init = [np.abs(f) for f in _init(dims, 3, 23)]
+    cons = [(2, 3, 0.0, 4)] * 3
+    f, lam, fits = orc.cpd_als_con(csfs, [0, 1, 2], init, 3, 0.0, cons, False,
+                                   max_inner=5000, inner_tol=0.0)
+    assert all(np.all(x >= 0) for x in f)
+    # the last mode's update: nonneg least squares against the other factors
+    G = (f[0].T @ f[0]) * (f[1].T @ f[1])
+    X = _dense(inds, vals, dims)
+    Mm = np.einsum("ijk,ir,jr->kr", X, f[0], f[1])
+    g = f[2] @ G - Mm
+    scale = np.max(np.abs(Mm))
+    assert np.all(g[f[2] == 0] >= -1e-6 * scale)
+    assert np.max(np.abs(g[f[2] > 1e-9])) <= 1e-6 * scale
 
 
 def test_cpd_frob_closed_form():

@@ -15,7 +15,14 @@
                      (c) on an exactly rank-2 stream every per-slice error
                      and final-err stay at a few percent (an unfitted model
                      sits near 1); (d) final-err equals the
-                     error of the returned model on the whole stream.
+                     error of the returned model on the whole stream;
+                     (e) checkpoint errors (all slices so far, the last 10)
+                     equal the dense squared error of that window against
+                     the model after the slice (truncated-stream runs give
+                     that model); (f) the refit equals CPD-ALS with the
+                     default options from rand() values drawn right after
+                     the slice, and consumes exactly one draw pair per
+                     factor entry of the window.
 """
 import ctypes
 
@@ -171,3 +178,77 @@ def test_stream_fits_exact_rank2():
     # percent; a model that had not learned the structure sits near 1
     assert np.max(res["local_err"]) < 0.06
     assert res["final_err"] < 0.05
+
+
+def _truncate(coo, k, t=2):
+    """The prepared stream's first k slices as a stream of its own (labels
+    kept: they are first-appearance labels already)."""
+    inds = coo.inds()
+    keep = inds[t] < k
+    sub = [i[keep] for i in inds]
+    dims = [int(i.max()) + 1 for i in sub]
+    dims[t] = k
+    return orc.Coo.from_arrays(sub, coo.vals()[keep], dims), sub, coo.vals()[keep], dims
+
+
+def test_stream_checkpoint_errors_dense():
+    """global / local-10 errors at checkpoints equal the dense squared error
+    of the window against the model after that slice."""
+    inds, vals, dims = _stream_tensor(25, 10, 8, 17)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    orc.stream_prepare(coo, 2)
+    R = 3
+    libc.srand(3)
+    full = orc.stream_cpd(coo, 2, R, 0.95, 5, CONS, check_interval=12)
+    at = {11, 23, 24}                       # slices 12, 24 and the last one
+    for b in range(25):
+        for key in ("global_err", "local10_err"):
+            assert (full[key][b] >= 0) == (b in at), (key, b)
+        assert full["cpd_err"][b] == -1.0
+    assert abs(full["global_err"][24] - full["final_err"]) <= 1e-13
+    for k in (12, 24, 25):
+        sub_coo, sub, sv, sd = _truncate(coo, k)
+        libc.srand(3)
+        tr = orc.stream_cpd(sub_coo, 2, R, 0.95, 5, CONS)
+        assert tr["nbatches"] == k
+        A, B, Tm = tr["factors"][0], tr["factors"][1], tr["time"]
+        for b0, key in ((0, "global_err"), (max(0, k - 10), "local10_err")):
+            w = sub[2] >= b0
+            X = _dense([sub[0][w], sub[1][w], sub[2][w] - b0], sv[w], [sd[0], sd[1], k - b0])
+            Z = np.einsum("ir,jr,kr->ijk", A, B, Tm[b0:k])
+            want = np.sum((X - Z) ** 2) / np.sum(X ** 2)
+            assert abs(full[key][k - 1] - want) <= 1e-10 * want, (k, key)
+
+
+def test_stream_refit_window_options_and_draws():
+    """The refit at a checkpoint is CPD-ALS (tolerance 1e-5, 200 iterations)
+    of the slices so far, started from rand() values drawn right after that
+    slice for every mode in order; later rows are drawn after those."""
+    inds, vals, dims = _stream_tensor(15, 9, 7, 21)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    orc.stream_prepare(coo, 2)
+    R = 3
+    libc.srand(7)
+    full = orc.stream_cpd(coo, 2, R, 0.95, 5, CONS, check_interval=6, refit=True)
+    after_full = libc.rand()
+    assert [b for b in range(15) if full["cpd_err"][b] != -1.0] == [5, 11, 14]
+    # the refit at slice 6: the rand() state is that after slices 0..5
+    sub_coo, sub, sv, sd = _truncate(coo, 6)
+    libc.srand(7)
+    orc.stream_cpd(sub_coo, 2, R, 0.95, 5, CONS)
+    start = [orc.fill_rand(d * R).reshape(d, R) for d in sd]
+    csf = orc.Csf(sub_coo, orc.mode_order(sd, 0))
+    _, _, fits = orc.cpd_als([csf], [0, 0, 0], start, 200, 1e-5)
+    assert 0 < len(fits) <= 200
+    assert abs(full["cpd_err"][5] - (1.0 - fits[-1])) <= 1e-12
+    assert 0.0 < full["cpd_err"][5] < 1.0
+    # draws: the same stream without refits, then one refit's worth of
+    # draws per checkpoint, leaves rand() at the same place
+    libc.srand(7)
+    plain = orc.stream_cpd(coo, 2, R, 0.95, 5, CONS, check_interval=6)
+    assert np.array_equal(plain["local_err"][:6], full["local_err"][:6])
+    for k in (6, 12, 15):
+        _, _, _, kd = _truncate(coo, k)
+        for _ in range(2 * R * sum(kd)):
+            libc.rand()
+    assert libc.rand() == after_full
