@@ -126,6 +126,9 @@ int spd_f32_to_f64(float const * src, double * dst, uint64_t n, void * stream);
  *   admm_step: one ADMM inner iteration; sums needs 4 + R doubles:
  *             ||P||^2, ||P-aux||^2, ||P-init||^2, ||D||^2, then the column
  *             norms used by the prox.
+ *   admm_solve: the ADMM inner loop to convergence (or max_inner); aux and
+ *             init are I x R work matrices, sums 4 + R doubles, ctl[1] the
+ *             iterations run.
  *   admm_chunked: ADMM in independent chunks of `chunk` rows (nonneg or no
  *             prox) to convergence; *its_rows = sum of iterations x rows. */
 int spd_dense_gram(double * X, uint64_t I, uint64_t R, double const * div,
@@ -137,6 +140,9 @@ int spd_dense_apply(void const * M, int mdtype, double const * ginv,
 int spd_dense_admm_step(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint64_t R, double * P, double * aux, double * D,
     double * init, int prox, double * sums);
+int spd_dense_admm_solve(void const * M, int mdtype, double const * ginv,
+    double const * rho, uint64_t I, uint64_t R, double * P, double * D, double * aux,
+    double * init, int prox, uint64_t max_inner, double tol, double * sums, int * ctl);
 int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint64_t R, uint64_t chunk, double * P, double * D,
     int prox, uint64_t max_inner, double tol, uint64_t * its_rows);

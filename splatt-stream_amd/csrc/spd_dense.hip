@@ -602,35 +602,53 @@ k_apply_mfma(ApplyArgs<TM> a)
   }
 }
 
-// ---- ADMM with the convergence test on the device (R <= 64) -------------
+// ---- ADMM with the iterate held as the pre-prox matrix (R <= 64) -----------
 //
-// One inner iteration (src/cpd/admm.c:240-308) in at most four launches and
-// no host round trip: pass 0 forms aux = (M + rho (P + D)) ginv and the
-// column sums of squares the prox needs; a one-block kernel turns them into
-// norms; pass 1 recomputes aux (the product is cheap on MFMA and saves
-// storing it), applies the prox, updates P and D and sums the residual
-// terms; a one-block kernel tests convergence and raises ctl[0]. Every
-// kernel returns at once when ctl[0] is set, so the host can queue all
-// max_inner iterations.
-
+// The reference's inner iteration k (src/cpd/admm.c:240-308) computes
+// aux_k = (M + rho (P_{k-1} + D_{k-1})) G^{-1}, P'_k = aux_k - D_{k-1},
+// P_k = prox(P'_k) and D_k = D_{k-1} + P_k - aux_k = P_k - P'_k. So the
+// state after iteration k is P'_k alone (plus the prox's column norms):
+// P_k = prox(P'_k), D_k = P_k - P'_k. Pass j reads M and P'_{j-1} (the
+// state it updates) and P'_{j-2} (for the convergence terms of iteration
+// j-1, which need P_{j-2} and D_{j-2}), and writes P'_j: four matrix passes
+// per iteration instead of the eight of a solve/prox/dual sweep pair. The
+// first two passes take P_0 and D_0 as given. Iteration j-1's test is known
+// after pass j; when it converges, pass j's output is discarded and P, D are
+// rebuilt from P'_{j-1}.
 template<typename TM>
-struct AdmmArgs {
+struct Admm2Args {
   TM const * M;
   double const * ginv;      // RP x RP
   uint64_t I;
   uint32_t R;
-  double * P;
-  double * D;
   double const * rho;
-  double const * norms;     // pass 1, max-column-norm prox
   int prox;
-  double * part;            // pass 0: RP per block; pass 1: 4 per block
-  int const * ctl;          // ctl[0] != 0: converged
+  double const * P0;        // explicit state (P_0, D_0)
+  double const * D0;
+  double const * cur;       // P'_{j-1}, or null: the state is (P0, D0)
+  double const * cur_n;     // its prox column norms
+  int check;                // accumulate the terms of iteration j-1
+  double const * prev;      // P'_{j-2}, or null: (P0, D0)
+  double const * prev_n;
+  double * out;             // P'_j
+  double * part;            // per block: RP column sums, then 4 check sums
+  int const * ctl;          // ctl[0] != 0: converged, nothing to do
 };
 
-template<int TT, typename TM, int PASS>
+__device__ __forceinline__ double dn_prox_apply(double p, int prox, double nc)
+{
+  if(prox == DN_PROX_NONNEG) {
+    p = (p > 0.) ? p : 0.;
+  } else if(prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG) {
+    if(prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) p = 0.;
+    if(!(nc <= 1.)) p /= nc;
+  }
+  return p;
+}
+
+template<int TT, typename TM>
 __global__ void __launch_bounds__(DN_THREADS)
-k_admm_mfma(AdmmArgs<TM> a)
+k_admm_pass(Admm2Args<TM> a)
 {
   if(a.ctl[0] != 0) return;
   constexpr int RP = TT * 16;
@@ -655,9 +673,12 @@ k_admm_mfma(AdmmArgs<TM> a)
   double pn = 0., pr = 0., dr = 0., dn = 0.;
   int const lr = lane / RP;
   int const lc = lane - lr * RP;
+  double const ncur_in = (a.cur != nullptr && (uint32_t) lc < R) ? a.cur_n[lc] : 1.;
+  double const nprev_in = (a.check && a.prev != nullptr && (uint32_t) lc < R) ? a.prev_n[lc] : 1.;
   uint64_t const ngroups = (I + 15) / 16;
   for(uint64_t gi = (uint64_t) blockIdx.x * 4 + wave; gi < ngroups; gi += (uint64_t) gridDim.x * 4) {
     uint64_t const row0 = gi * 16;
+    // stage M + rho (P + D) of iteration j-1; the test terms of j-1
     #pragma unroll
     for(int rr=0; rr < 16; rr += RPI) {
       int const r = rr + lr;
@@ -666,7 +687,33 @@ k_admm_mfma(AdmmArgs<TM> a)
         double v = 0.;
         if(row < I && (uint32_t) lc < R) {
           uint64_t const idx = row * R + lc;
-          v = (double) a.M[idx] + rho * (a.P[idx] + a.D[idx]);
+          double P, D;
+          if(a.cur != nullptr) {
+            double const p1 = a.cur[idx];
+            P = dn_prox_apply(p1, prox, ncur_in);
+            D = P - p1;
+          } else {
+            P = a.P0[idx];
+            D = a.D0[idx];
+          }
+          v = (double) a.M[idx] + rho * (P + D);
+          if(a.check) {
+            double Pp, Dp;
+            if(a.prev != nullptr) {
+              double const p2 = a.prev[idx];
+              Pp = dn_prox_apply(p2, prox, nprev_in);
+              Dp = Pp - p2;
+            } else {
+              Pp = a.P0[idx];
+              Dp = a.D0[idx];
+            }
+            double const q = D - Dp;
+            double const w = P - Pp;
+            pn += P * P;
+            pr += q * q;
+            dr += w * w;
+            dn += D * D;
+          }
         }
         s_m[wave][r * LD + lc] = v;
       }
@@ -684,106 +731,119 @@ k_admm_mfma(AdmmArgs<TM> a)
       }
     }
     __builtin_amdgcn_wave_barrier();
+    // P'_j = aux_j - D_{j-1}; column sums of what the prox will measure
     #pragma unroll
     for(int t=0; t < TT; ++t) {
       uint32_t const col = t * 16 + cc;
+      double const ncur = (a.cur != nullptr && col < R) ? a.cur_n[col] : 1.;
       #pragma unroll
       for(int r=0; r < 4; ++r) {
         uint64_t const row = row0 + kk + 4 * r;
         if(row < I && col < R) {
           uint64_t const idx = row * R + col;
-          double const x = acc[t][r];
-          double const d_old = a.D[idx];
-          double p = x - d_old;
-          if(PASS == 0) {
-            double const q = (prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) ? 0. : p;
-            cs[t] += q * q;
+          double D;
+          if(a.cur != nullptr) {
+            double const p1 = a.cur[idx];
+            D = dn_prox_apply(p1, prox, ncur) - p1;
           } else {
-            if(prox == DN_PROX_NONNEG) {
-              p = (p > 0.) ? p : 0.;
-            } else if(prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG) {
-              if(prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) p = 0.;
-              double const nc = a.norms[col];
-              if(!(nc <= 1.)) p /= nc;
-            }
-            double const p_old = a.P[idx];
-            double const d_new = d_old + (p - x);
-            a.P[idx] = p;
-            a.D[idx] = d_new;
-            double const q = p - x;
-            double const w = p - p_old;
-            pn += p * p;
-            pr += q * q;
-            dr += w * w;
-            dn += d_new * d_new;
+            D = a.D0[idx];
           }
+          double const p = acc[t][r] - D;
+          a.out[idx] = p;
+          double const q = (prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) ? 0. : p;
+          cs[t] += q * q;
         }
       }
     }
   }
-  // block partials in a fixed order
-  if(PASS == 0) {
-    #pragma unroll
-    for(int t=0; t < TT; ++t) {
-      cs[t] += __shfl_xor(cs[t], 16, 64);
-      cs[t] += __shfl_xor(cs[t], 32, 64);
-    }
-    if(kk == 0) {
-      #pragma unroll
-      for(int t=0; t < TT; ++t) s_red[wave][t * 16 + cc] = cs[t];
-    }
-    __syncthreads();
-    for(int e=threadIdx.x; e < RP; e += DN_THREADS) {
-      a.part[(uint64_t) blockIdx.x * RP + e] = ((s_red[0][e] + s_red[1][e]) + s_red[2][e]) + s_red[3][e];
-    }
-  } else {
-    s_red[0][threadIdx.x] = pn;
-    s_red[1][threadIdx.x] = pr;
-    s_red[2][threadIdx.x] = dr;
-    s_red[3][threadIdx.x] = dn;
-    __syncthreads();
-    for(int h=DN_THREADS / 2; h > 0; h >>= 1) {
-      if((int) threadIdx.x < h) {
-        #pragma unroll
-        for(int k=0; k < 4; ++k) s_red[k][threadIdx.x] += s_red[k][threadIdx.x + h];
-      }
-      __syncthreads();
-    }
-    if(threadIdx.x < 4) a.part[(uint64_t) blockIdx.x * 4 + threadIdx.x] = s_red[threadIdx.x][0];
+  // block partials in a fixed order: RP column sums, then the four terms
+  double * const part = a.part + (uint64_t) blockIdx.x * (RP + 4);
+  #pragma unroll
+  for(int t=0; t < TT; ++t) {
+    cs[t] += __shfl_xor(cs[t], 16, 64);
+    cs[t] += __shfl_xor(cs[t], 32, 64);
   }
+  if(kk == 0) {
+    #pragma unroll
+    for(int t=0; t < TT; ++t) s_red[wave][t * 16 + cc] = cs[t];
+  }
+  __syncthreads();
+  for(int e=threadIdx.x; e < RP; e += DN_THREADS) {
+    part[e] = ((s_red[0][e] + s_red[1][e]) + s_red[2][e]) + s_red[3][e];
+  }
+  __syncthreads();
+  s_red[0][threadIdx.x] = pn;
+  s_red[1][threadIdx.x] = pr;
+  s_red[2][threadIdx.x] = dr;
+  s_red[3][threadIdx.x] = dn;
+  __syncthreads();
+  for(int h=DN_THREADS / 2; h > 0; h >>= 1) {
+    if((int) threadIdx.x < h) {
+      #pragma unroll
+      for(int k=0; k < 4; ++k) s_red[k][threadIdx.x] += s_red[k][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if(threadIdx.x < 4) part[RP + threadIdx.x] = s_red[threadIdx.x][0];
 }
 
-// norms[c] = sqrt(sum_b part[b * ld + c]) unless converged. One block.
+// After pass j: norms[c] = sqrt(column total) of P'_j; with check, the test
+// of iteration j-1 (sums[0..3] = ||P||^2, ||P - aux||^2, ||P - P_prev||^2,
+// ||D||^2). ctl[1] = index of the state that is final so far (j, or j-1 on
+// convergence, which sets ctl[0]). One block.
 __global__ void __launch_bounds__(DN_THREADS)
-k_admm_norms(double const * __restrict__ part, uint32_t nb, uint32_t R,
-    uint32_t ld, double * __restrict__ norms, int const * __restrict__ ctl)
+k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R, uint32_t RP,
+    double * __restrict__ norms, double * __restrict__ sums, int * __restrict__ ctl,
+    double tol, int check, int j)
 {
   if(ctl[0] != 0) return;
   __shared__ double s[DN_THREADS / 32][32];
+  __shared__ double s4[DN_THREADS / 4][4];
+  __shared__ double tot[4];
+  uint32_t const ld = RP + 4;
   for(uint32_t c0=0; c0 < R; c0 += 32) {
     uint32_t const c = c0 + threadIdx.x % 32;
     double const v = block_partsum<32>(part, nb, ld, c < R, c, s);
     if(threadIdx.x < 32 && c < R) norms[c] = sqrt(v);
   }
-}
-
-// sums = block totals; ctl[1] counts iterations; ctl[0] set on convergence.
-__global__ void __launch_bounds__(DN_THREADS)
-k_admm_check(double const * __restrict__ part, uint32_t nb,
-    double * __restrict__ sums, int * __restrict__ ctl, double tol)
-{
-  if(ctl[0] != 0) return;
-  __shared__ double s[DN_THREADS / 4][4];
-  __shared__ double tot[4];
-  double const v = block_partsum<4>(part, nb, 4, true, threadIdx.x % 4, s);
-  if(threadIdx.x < 4) {
-    sums[threadIdx.x] = v;
-    tot[threadIdx.x] = v;
-  }
+  double const v = block_partsum<4>(part, nb, ld, true, RP + threadIdx.x % 4, s4);
+  if(threadIdx.x < 4) tot[threadIdx.x] = v;
   __syncthreads();
   if(threadIdx.x == 0) {
-    ctl[1] += 1;
-    if(tot[1] <= tol * tot[0] && tot[2] <= tol * tot[3]) ctl[0] = 1;
+    bool conv = false;
+    if(check) {
+      for(int k=0; k < 4; ++k) sums[k] = tot[k];
+      conv = (tot[1] <= tol * tot[0] && tot[2] <= tol * tot[3]);
+    }
+    if(conv) {
+      ctl[0] = 1;
+      ctl[1] = j - 1;
+    } else {
+      ctl[1] = j;
+    }
+  }
+}
+
+struct Bufs3 { double * b[3]; double * n[3]; };
+
+// P = prox(P'_k), D = P - P'_k for the final state k = ctl[1] (k >= 1).
+__global__ void k_admm_final(Bufs3 bf, double * __restrict__ P, double * __restrict__ D,
+    uint64_t n, uint32_t R, int prox, int const * __restrict__ ctl)
+{
+  int const k = ctl[1];
+  double const * const src = bf.b[k % 3];
+  double const * const nrm = bf.n[k % 3];
+  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  uint32_t c = (uint32_t) (e % R);
+  uint32_t const sc = (uint32_t) (stride % R);
+  for(; e < n; e += stride) {
+    double const p1 = src[e];
+    double const p = dn_prox_apply(p1, prox, nrm[c]);
+    P[e] = p;
+    D[e] = p - p1;
+    c += sc;
+    if(c >= R) c -= R;
   }
 }
 
@@ -1336,64 +1396,67 @@ static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_
   return SPD_OK;
 }
 
-template<int TT, typename TM, int PASS>
-static int launch_admm(AdmmArgs<TM> const & a, uint32_t nb, hipStream_t st)
+template<int TT, typename TM>
+static int launch_admm2(Admm2Args<TM> const & a, uint32_t nb, hipStream_t st)
 {
-  hipLaunchKernelGGL((k_admm_mfma<TT, TM, PASS>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+  hipLaunchKernelGGL((k_admm_pass<TT, TM>), dim3(nb), dim3(DN_THREADS), 0, st, a);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
 
-template<typename TM, int PASS>
-static int dispatch_admm(AdmmArgs<TM> const & a, uint32_t TT, uint32_t nb, hipStream_t st)
-{
-  switch(TT) {
-  case 1: return launch_admm<1, TM, PASS>(a, nb, st);
-  case 2: return launch_admm<2, TM, PASS>(a, nb, st);
-  case 3: return launch_admm<3, TM, PASS>(a, nb, st);
-  default: return launch_admm<4, TM, PASS>(a, nb, st);
-  }
-}
-
+// The inner loop as passes over the pre-prox iterate (k_admm_pass): P'_j
+// goes to bufs[j % 3] = {P, aux, init}; P and D hold P_0 and D_0 until pass
+// 3 first overwrites P, and receive the final state at the end.
 template<typename TM>
-static int admm_solve_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
-    uint32_t R, double * P, double * D, int prox, uint64_t max_inner, double tol,
-    double * sums, int * ctl, hipStream_t st)
+static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
+    uint32_t R, double * P, double * D, double * aux, double * init, int prox,
+    uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
 {
   uint32_t const RP = dn_rpad(R);
   uint32_t const nb = apply_blocks(I);
-  bool const need_norms = (prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG);
   void * pc = nullptr;
-  void * ps = nullptr;
-  SPD_TRY(spd_scratch(SLOT_COL, (size_t) nb * RP * sizeof(double), &pc));
-  SPD_TRY(spd_scratch(SLOT_GRAM, (size_t) nb * 4 * sizeof(double), &ps));
+  size_t const part_n = (size_t) nb * (RP + 4);
+  SPD_TRY(spd_scratch(SLOT_COL, (part_n + 3 * (size_t) RP) * sizeof(double), &pc));
+  double * const part = (double *) pc;
+  Bufs3 bf;
+  bf.b[0] = P;
+  bf.b[1] = aux;
+  bf.b[2] = init;
+  for(int k=0; k < 3; ++k) bf.n[k] = part + part_n + (size_t) k * RP;
   hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, st, ctl);
   SPD_LAUNCH_CHECK();
-  AdmmArgs<TM> a = {};
+  Admm2Args<TM> a = {};
   a.M = M;
   a.ginv = ginv;
   a.I = I;
   a.R = R;
-  a.P = P;
-  a.D = D;
   a.rho = rho;
-  a.norms = sums + 4;
   a.prox = prox;
+  a.P0 = P;
+  a.D0 = D;
+  a.part = part;
   a.ctl = ctl;
-  for(uint64_t k=0; k < max_inner; ++k) {
-    if(need_norms) {
-      a.part = (double *) pc;
-      SPD_TRY((dispatch_admm<TM, 0>(a, RP / 16, nb, st)));
-      hipLaunchKernelGGL(k_admm_norms, dim3(1), dim3(DN_THREADS), 0, st, (double const *) pc, nb,
-          R, RP, sums + 4, (int const *) ctl);
-      SPD_LAUNCH_CHECK();
+  for(uint64_t j=1; j <= max_inner; ++j) {
+    a.cur = (j >= 2) ? bf.b[(j - 1) % 3] : nullptr;
+    a.cur_n = (j >= 2) ? bf.n[(j - 1) % 3] : nullptr;
+    a.check = (j >= 2);
+    a.prev = (j >= 3) ? bf.b[(j - 2) % 3] : nullptr;
+    a.prev_n = (j >= 3) ? bf.n[(j - 2) % 3] : nullptr;
+    a.out = bf.b[j % 3];
+    switch(RP / 16) {
+    case 1: SPD_TRY((launch_admm2<1, TM>(a, nb, st))); break;
+    case 2: SPD_TRY((launch_admm2<2, TM>(a, nb, st))); break;
+    case 3: SPD_TRY((launch_admm2<3, TM>(a, nb, st))); break;
+    default: SPD_TRY((launch_admm2<4, TM>(a, nb, st))); break;
     }
-    a.part = (double *) ps;
-    SPD_TRY((dispatch_admm<TM, 1>(a, RP / 16, nb, st)));
-    hipLaunchKernelGGL(k_admm_check, dim3(1), dim3(DN_THREADS), 0, st, (double const *) ps, nb,
-        sums, ctl, tol);
+    hipLaunchKernelGGL(k_admm_pass_reduce, dim3(1), dim3(DN_THREADS), 0, st,
+        (double const *) part, nb, R, RP, bf.n[j % 3], sums, ctl, tol, a.check, (int) j);
     SPD_LAUNCH_CHECK();
   }
+  uint64_t const n = I * R;
+  hipLaunchKernelGGL(k_admm_final, dim3(dn_blocks(n, DN_THREADS * 8)), dim3(DN_THREADS), 0, st,
+      bf, P, D, n, R, prox, (int const *) ctl);
+  SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
 
@@ -1404,11 +1467,11 @@ int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const 
   if(I == 0 || max_inner == 0) return SPD_OK;
   if(R <= 64) {
     if(mdtype == SPD_F32) {
-      return admm_solve_t<float>((float const *) M, ginv, rho, I, R, P, D, prox, max_inner,
-          tol, sums, ctl, st);
+      return admm_solve2_t<float>((float const *) M, ginv, rho, I, R, P, D, aux, init, prox,
+          max_inner, tol, sums, ctl, st);
     }
-    return admm_solve_t<double>((double const *) M, ginv, rho, I, R, P, D, prox, max_inner,
-        tol, sums, ctl, st);
+    return admm_solve2_t<double>((double const *) M, ginv, rho, I, R, P, D, aux, init, prox,
+        max_inner, tol, sums, ctl, st);
   }
   // ranks above 64: one step at a time, the test on the host
   int hctl[2] = {0, 0};
@@ -1509,6 +1572,15 @@ extern "C" int spd_dense_apply(void const * M, int mdtype, double const * ginv,
 {
   SPD_TRY(spd_init());
   return dn_apply(M, mdtype, ginv, I, (uint32_t) R, X, colnorm, spd_hs(nullptr));
+}
+
+extern "C" int spd_dense_admm_solve(void const * M, int mdtype, double const * ginv,
+    double const * rho, uint64_t I, uint64_t R, double * P, double * D, double * aux,
+    double * init, int prox, uint64_t max_inner, double tol, double * sums, int * ctl)
+{
+  SPD_TRY(spd_init());
+  return dn_admm_solve(M, mdtype, ginv, rho, I, (uint32_t) R, P, D, aux, init, prox,
+      max_inner, tol, sums, ctl, spd_hs(nullptr));
 }
 
 extern "C" int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,

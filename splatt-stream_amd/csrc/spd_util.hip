@@ -3,6 +3,7 @@
 // exclusive scan.
 #include "spd_internal.hpp"
 #include <stdarg.h>
+#include <stdlib.h>
 #include <string.h>
 #include <mutex>
 
@@ -23,12 +24,8 @@ void spd_set_error(int code, const char * fmt, ...)
 
 extern "C" const char * spd_last_error(void) { return g_err; }
 
-extern "C" int spd_init(void)
+static int init_runtime(void)
 {
-  std::lock_guard<std::mutex> lk(g_init_mu);
-  if(g_inited) {
-    return SPD_OK;
-  }
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if(e != hipSuccess || ndev < 1) {
@@ -39,6 +36,27 @@ extern "C" int spd_init(void)
   SPD_HIP(hipSetDevice(0));
   // default priority, non-blocking w.r.t. the legacy null stream
   SPD_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  return SPD_OK;
+}
+
+extern "C" int spd_init(void)
+{
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if(g_inited) {
+    return SPD_OK;
+  }
+  // The HIP runtime's start-up draws from (or reseeds) the C library's
+  // rand() generator. Callers seed rand() to fix the factor initialisation
+  // (the reference draws factors from rand(), src/util.c:15-40), and the
+  // first device call may come after that seed, so the start-up runs on a
+  // private generator state and the caller's state is put back untouched.
+  static int32_t s_rand_state[32];
+  char * const prev = initstate(1u, (char *) s_rand_state, sizeof(s_rand_state));
+  int const rc = init_runtime();
+  setstate(prev);
+  if(rc != SPD_OK) {
+    return rc;
+  }
   g_inited = 1;
   return SPD_OK;
 }

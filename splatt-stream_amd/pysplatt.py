@@ -211,6 +211,11 @@ def lib():
     L.spd_dense_admm_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    L.spd_dense_admm_solve.restype = C.c_int
+    L.spd_dense_admm_solve.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                       C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int, C.c_uint64,
+                                       C.c_double, C.c_void_p, C.c_void_p]
     L.spd_dense_admm_chunked.restype = C.c_int
     L.spd_dense_admm_chunked.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,

@@ -188,6 +188,44 @@ def test_admm_step_matches_oracle(gpu_available, R, prox):
                                np.sum(D * D)], rtol=1e-12)
 
 
+@pytest.mark.parametrize("R,I", [(5, 1), (16, 37), (33, 500), (64, 3000)])
+@pytest.mark.parametrize("prox", [0, 1, 2, 3])
+@pytest.mark.parametrize("tol,max_inner", [(1e-2, 50), (7e-2, 25), (0.0, 6)])
+def test_admm_solve_matches_oracle(gpu_available, R, I, prox, tol, max_inner):
+    # the whole inner loop: same iterations, P and D at 1e-8
+    rng = np.random.default_rng(R * 7 + prox + int(tol * 1000))
+    A = rng.standard_normal((3 * R, R))
+    G = A.T @ A + 0.5 * np.eye(R)
+    M = rng.standard_normal((I, R)) * 4.0
+    P0 = rng.standard_normal((I, R))
+    D0 = rng.standard_normal((I, R)) * 0.1
+    wantP, wantD, wits, wrho = orc.admm(prox, M, P0, D0, G, max_inner, tol)
+    L = ps.lib()
+    RP = _rp(R)
+    bufs = [_dev(G), _dev(np.ones((R, R)))]
+    arr = (C.c_void_p * 2)(*[b.ptr for b in bufs])
+    ginv = ps.DevBuf(RP * RP * 8)
+    rho = ps.DevBuf(8)
+    info = ps.DevBuf(8)
+    assert L.spd_dense_chol_inv(arr, 2, 1, R, 0.0, 1, ginv.ptr, rho.ptr, info.ptr) == 0
+    dM, dP, dD = _dev(M), _dev(P0), _dev(D0)
+    aux = ps.DevBuf(I * R * 8)
+    init = ps.DevBuf(I * R * 8)
+    sums = ps.DevBuf((4 + R) * 8)
+    ctl = ps.DevBuf(8)
+    assert L.spd_dense_admm_solve(dM.ptr, 0, ginv.ptr, rho.ptr, I, R, dP.ptr, dD.ptr, aux.ptr,
+                                  init.ptr, prox, max_inner, C.c_double(tol), sums.ptr,
+                                  ctl.ptr) == 0
+    c = ctl.to_host((2,), np.int32)
+    assert int(c[1]) == wits
+    P = dP.to_host((I, R), np.float64)
+    D = dD.to_host((I, R), np.float64)
+    t = 1e-8
+    scale = max(np.max(np.abs(wantP)), np.max(np.abs(wantD)))
+    assert np.allclose(P, wantP, rtol=t, atol=t * scale)
+    assert np.allclose(D, wantD, rtol=t, atol=t * scale)
+
+
 # ---- splatt_cpd ---------------------------------------------------------------
 
 def _twomode(path=None, coo=None, alloc=ps.CSF_TWOMODE):

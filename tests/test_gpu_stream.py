@@ -8,6 +8,8 @@ iteration counts agree too; values are compared at 1e-8 relative.
 """
 import ctypes
 import os
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -166,6 +168,9 @@ def test_cpd_stream_file_entry(gpu_available, tmp_path, capfd):
     assert rc == ps.SUCCESS
     out = capfd.readouterr().out
     assert "final-err:" in out
+    ref = _oracle(inds, vals, dims, 2, 4, 0.95, 3, 1, 0.0)
+    line = [l for l in out.splitlines() if l.startswith("final-err:")][0]
+    assert abs(float(line.split()[1]) - ref["final_err"]) <= 1e-5
     # the reference's batch line carries the checkpoint errors after the
     # last slice (a refit there draws rand() after every factor row)
     last = [l for l in out.splitlines() if l.startswith("batch ")]
@@ -176,9 +181,6 @@ def test_cpd_stream_file_entry(gpu_available, tmp_path, capfd):
     assert got["cpd"] >= 0 and got["local-10"] >= 0
     assert abs(got["global"] - ref["final_err"]) <= 1e-5
     assert all(" global: -1.00000 " in l for l in last[:-1])
-    ref = _oracle(inds, vals, dims, 2, 4, 0.95, 3, 1, 0.0)
-    line = [l for l in out.splitlines() if l.startswith("final-err:")][0]
-    assert abs(float(line.split()[1]) - ref["final_err"]) <= 1e-5
     m1 = np.loadtxt(tmp_path / "mode1.mat")
     m3 = np.loadtxt(tmp_path / "mode3.mat")
     assert m1.shape == (dims[0], 4) and m3.shape == (dims[2], 4)
@@ -187,3 +189,26 @@ def test_cpd_stream_file_entry(gpu_available, tmp_path, capfd):
     want1[ref["iperm"][0][:ref["dims"][0]].astype(int)] = ref["factors"][0]
     assert np.allclose(m1, want1, rtol=1e-7, atol=1e-8)
     assert np.allclose(m3, ref["time"], rtol=1e-7, atol=1e-8)
+
+
+def test_stream_first_device_call_keeps_rand(gpu_available):
+    # In a fresh process the stream is the first device call, after the
+    # caller's srand(): the runtime's start-up must not move the rand()
+    # sequence the new factor rows are drawn from.
+    here = os.path.dirname(os.path.abspath(__file__))
+    code = (
+        "import sys\n"
+        "sys.path[:0] = [%r, %r, %r]\n"
+        "import numpy as np, ctypes\n"
+        "import test_gpu_stream as t, pysplatt as ps\n"
+        "inds, vals, dims = t._stream(4, [300, 200], (100, 300), seed=41)\n"
+        "dev = ps.run_stream(inds, vals, dims, 8, 2, forget=0.95, max_its=3, seed=5, frob=1e-2)\n"
+        "ref = t._oracle(inds, vals, dims, 2, 8, 0.95, 3, 5, 1e-2)\n"
+        "print(float(np.max(np.abs(dev['local_err'] - ref['local_err']))))\n"
+    ) % (here, os.path.join(os.path.dirname(here), "splatt-stream_amd"),
+         os.path.join(os.path.dirname(here), "oracle"))
+    env = dict(os.environ)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert float(r.stdout.strip().splitlines()[-1]) <= 1e-10
