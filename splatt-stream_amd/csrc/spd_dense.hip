@@ -395,22 +395,27 @@ k_chol_inv_reg(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int
   __shared__ double s_diag[64];
   __shared__ double s_rho;
   int const i = threadIdx.x;
+  bool const irow = i < (int) R;
   double a[RP];
   #pragma unroll
-  for(int k=0; k < RP; ++k) {
-    double h;
-    if(i < (int) R && k < (int) R) {
-      int const lo = i < k ? i : k;
-      int const hi = i < k ? k : i;
-      h = 1.;
-      for(int m=0; m < nmodes; ++m) {
-        if(m != mode) h *= g.aTa[m][lo * R + hi];
+  for(int k=0; k < RP; ++k) a[k] = (irow && k < (int) R) ? 1. : ((k == i) ? 1. : 0.);
+  // row i of the Hadamard product, one Gram at a time so that the RP loads
+  // of a Gram are all in flight together
+  for(int m=0; m < nmodes; ++m) {
+    if(m == mode) continue;
+    double const * const gm = g.aTa[m];
+    #pragma unroll
+    for(int k=0; k < RP; ++k) {
+      if(irow && k < (int) R) {
+        int const lo = i < k ? i : k;
+        int const hi = i < k ? k : i;
+        a[k] *= gm[lo * R + hi];
       }
-      if(k == i) h += diag_add;
-    } else {
-      h = (k == i) ? 1. : 0.;
     }
-    a[k] = h;
+  }
+  #pragma unroll
+  for(int k=0; k < RP; ++k) {
+    if(irow && k == i) a[k] += diag_add;
   }
   if(admm) {
     double d = 0.;
@@ -625,13 +630,12 @@ struct Admm2Args {
   int prox;
   double const * P0;        // explicit state (P_0, D_0)
   double const * D0;
-  double const * cur;       // P'_{j-1}, or null: the state is (P0, D0)
+  double const * cur;       // P'_{j-1} (states >= 1)
   double const * cur_n;     // its prox column norms
-  int check;                // accumulate the terms of iteration j-1
-  double const * prev;      // P'_{j-2}, or null: (P0, D0)
+  double const * prev;      // P'_{j-2} (test against a state >= 1)
   double const * prev_n;
   double * out;             // P'_j
-  double * part;            // per block: RP column sums, then 4 check sums
+  double * part;            // per block: RP column sums, then 4 test sums
   int const * ctl;          // ctl[0] != 0: converged, nothing to do
 };
 
@@ -646,7 +650,11 @@ __device__ __forceinline__ double dn_prox_apply(double p, int prox, double nc)
   return p;
 }
 
-template<int TT, typename TM>
+// MODE 0: pass 1 (state P_0, D_0; no test). MODE 1: pass 2 (state P'_1,
+// test of iteration 1 against P_0, D_0). MODE 2: pass j >= 3 (state
+// P'_{j-1}, test against P'_{j-2}). Block partials: column sums of squares
+// of P'_j and the four test terms.
+template<int TT, typename TM, int MODE>
 __global__ void __launch_bounds__(DN_THREADS)
 k_admm_pass(Admm2Args<TM> a)
 {
@@ -654,9 +662,14 @@ k_admm_pass(Admm2Args<TM> a)
   constexpr int RP = TT * 16;
   constexpr int LD = RP + 1;
   constexpr int RPI = (64 / RP) > 0 ? 64 / RP : 1;
+  constexpr int NQ = (16 + RPI - 1) / RPI;     // staged entries per lane
+  constexpr int E = RP + 4;                    // partials per block
+  // s_w holds each wave's 16-row tile during the sweep, then the block's
+  // partial sums, then the last block's wave totals (one after the other)
+  constexpr int WS = 4 * 16 * LD;
+  static_assert(WS >= 4 * DN_THREADS && WS >= 4 * E, "LDS work area too small");
   __shared__ double s_g[RP * RP];
-  __shared__ double s_m[4][16 * LD];
-  __shared__ double s_red[4][RP > 4 * 64 ? RP : 4 * 64];
+  __shared__ double s_w[WS];
   int const lane = threadIdx.x & 63;
   int const wave = threadIdx.x >> 6;
   int const kk = lane >> 4;
@@ -673,50 +686,67 @@ k_admm_pass(Admm2Args<TM> a)
   double pn = 0., pr = 0., dr = 0., dn = 0.;
   int const lr = lane / RP;
   int const lc = lane - lr * RP;
-  double const ncur_in = (a.cur != nullptr && (uint32_t) lc < R) ? a.cur_n[lc] : 1.;
-  double const nprev_in = (a.check && a.prev != nullptr && (uint32_t) lc < R) ? a.prev_n[lc] : 1.;
+  bool const lok = (lr < RPI) && ((uint32_t) lc < R);
+  double const ncur = (MODE >= 1 && lok) ? a.cur_n[lc] : 1.;
+  double const nprev = (MODE == 2 && lok) ? a.prev_n[lc] : 1.;
   uint64_t const ngroups = (I + 15) / 16;
   for(uint64_t gi = (uint64_t) blockIdx.x * 4 + wave; gi < ngroups; gi += (uint64_t) gridDim.x * 4) {
     uint64_t const row0 = gi * 16;
-    // stage M + rho (P + D) of iteration j-1; the test terms of j-1
+    // stage M + rho (P + D) of the state j-1; D kept for the output; the
+    // test terms of iteration j-1. Every load of the group is issued first.
+    double mv[NQ], cv[NQ], xv[NQ], yv[NQ];
     #pragma unroll
-    for(int rr=0; rr < 16; rr += RPI) {
-      int const r = rr + lr;
-      if(lr < RPI && r < 16) {
-        uint64_t const row = row0 + r;
-        double v = 0.;
-        if(row < I && (uint32_t) lc < R) {
-          uint64_t const idx = row * R + lc;
-          double P, D;
-          if(a.cur != nullptr) {
-            double const p1 = a.cur[idx];
-            P = dn_prox_apply(p1, prox, ncur_in);
-            D = P - p1;
-          } else {
-            P = a.P0[idx];
-            D = a.D0[idx];
-          }
-          v = (double) a.M[idx] + rho * (P + D);
-          if(a.check) {
-            double Pp, Dp;
-            if(a.prev != nullptr) {
-              double const p2 = a.prev[idx];
-              Pp = dn_prox_apply(p2, prox, nprev_in);
-              Dp = Pp - p2;
-            } else {
-              Pp = a.P0[idx];
-              Dp = a.D0[idx];
-            }
-            double const q = D - Dp;
-            double const w = P - Pp;
-            pn += P * P;
-            pr += q * q;
-            dr += w * w;
-            dn += D * D;
-          }
-        }
-        s_m[wave][r * LD + lc] = v;
+    for(int q=0; q < NQ; ++q) {
+      int const r = q * RPI + lr;
+      uint64_t const row = row0 + r;
+      bool const ok = lok && r < 16 && row < I;
+      uint64_t const idx = ok ? row * R + lc : 0;
+      mv[q] = ok ? (double) a.M[idx] : 0.;
+      if(MODE == 0) {
+        cv[q] = ok ? a.P0[idx] : 0.;
+        xv[q] = ok ? a.D0[idx] : 0.;
+      } else if(MODE == 1) {
+        cv[q] = ok ? a.cur[idx] : 0.;
+        xv[q] = ok ? a.P0[idx] : 0.;
+        yv[q] = ok ? a.D0[idx] : 0.;
+      } else {
+        cv[q] = ok ? a.cur[idx] : 0.;
+        xv[q] = ok ? a.prev[idx] : 0.;
       }
+    }
+    double dv[NQ];
+    #pragma unroll
+    for(int q=0; q < NQ; ++q) {
+      int const r = q * RPI + lr;
+      uint64_t const row = row0 + r;
+      bool const ok = lok && r < 16 && row < I;
+      double P, D;
+      if(MODE == 0) {
+        P = cv[q];
+        D = xv[q];
+      } else {
+        P = dn_prox_apply(cv[q], prox, ncur);
+        D = P - cv[q];
+      }
+      double const v = ok ? mv[q] + rho * (P + D) : 0.;
+      if(MODE >= 1 && ok) {
+        double Pp, Dp;
+        if(MODE == 2) {
+          Pp = dn_prox_apply(xv[q], prox, nprev);
+          Dp = Pp - xv[q];
+        } else {
+          Pp = xv[q];
+          Dp = yv[q];
+        }
+        double const x = D - Dp;
+        double const w = P - Pp;
+        pn += P * P;
+        pr += x * x;
+        dr += w * w;
+        dn += D * D;
+      }
+      dv[q] = ok ? D : 0.;
+      if(lr < RPI && r < 16) s_w[wave * 16 * LD + r * LD + lc] = v;
     }
     __builtin_amdgcn_wave_barrier();
     d4 acc[TT];
@@ -724,40 +754,41 @@ k_admm_pass(Admm2Args<TM> a)
     for(int t=0; t < TT; ++t) acc[t] = (d4) 0.;
     #pragma unroll
     for(int k0=0; k0 < RP; k0 += 4) {
-      double const av = s_m[wave][cc * LD + k0 + kk];
+      double const av = s_w[wave * 16 * LD + cc * LD + k0 + kk];
       #pragma unroll
       for(int t=0; t < TT; ++t) {
         acc[t] = mfma16(av, s_g[(k0 + kk) * RP + t * 16 + cc], acc[t]);
       }
     }
     __builtin_amdgcn_wave_barrier();
+    // D of the state, in the layout of the product
+    #pragma unroll
+    for(int q=0; q < NQ; ++q) {
+      int const r = q * RPI + lr;
+      if(lr < RPI && r < 16) s_w[wave * 16 * LD + r * LD + lc] = dv[q];
+    }
+    __builtin_amdgcn_wave_barrier();
     // P'_j = aux_j - D_{j-1}; column sums of what the prox will measure
     #pragma unroll
     for(int t=0; t < TT; ++t) {
       uint32_t const col = t * 16 + cc;
-      double const ncur = (a.cur != nullptr && col < R) ? a.cur_n[col] : 1.;
       #pragma unroll
       for(int r=0; r < 4; ++r) {
-        uint64_t const row = row0 + kk + 4 * r;
+        int const rl = kk + 4 * r;
+        uint64_t const row = row0 + rl;
         if(row < I && col < R) {
-          uint64_t const idx = row * R + col;
-          double D;
-          if(a.cur != nullptr) {
-            double const p1 = a.cur[idx];
-            D = dn_prox_apply(p1, prox, ncur) - p1;
-          } else {
-            D = a.D0[idx];
-          }
-          double const p = acc[t][r] - D;
-          a.out[idx] = p;
+          double const p = acc[t][r] - s_w[wave * 16 * LD + rl * LD + col];
+          a.out[row * R + col] = p;
           double const q = (prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) ? 0. : p;
           cs[t] += q * q;
         }
       }
     }
+    __builtin_amdgcn_wave_barrier();
   }
   // block partials in a fixed order: RP column sums, then the four terms
-  double * const part = a.part + (uint64_t) blockIdx.x * (RP + 4);
+  __syncthreads();
+  double * const part = a.part + (uint64_t) blockIdx.x * E;
   #pragma unroll
   for(int t=0; t < TT; ++t) {
     cs[t] += __shfl_xor(cs[t], 16, 64);
@@ -765,55 +796,72 @@ k_admm_pass(Admm2Args<TM> a)
   }
   if(kk == 0) {
     #pragma unroll
-    for(int t=0; t < TT; ++t) s_red[wave][t * 16 + cc] = cs[t];
+    for(int t=0; t < TT; ++t) s_w[wave * DN_THREADS + t * 16 + cc] = cs[t];
   }
   __syncthreads();
   for(int e=threadIdx.x; e < RP; e += DN_THREADS) {
-    part[e] = ((s_red[0][e] + s_red[1][e]) + s_red[2][e]) + s_red[3][e];
+    part[e] = ((s_w[e] + s_w[DN_THREADS + e]) + s_w[2 * DN_THREADS + e]) + s_w[3 * DN_THREADS + e];
   }
   __syncthreads();
-  s_red[0][threadIdx.x] = pn;
-  s_red[1][threadIdx.x] = pr;
-  s_red[2][threadIdx.x] = dr;
-  s_red[3][threadIdx.x] = dn;
+  s_w[threadIdx.x] = pn;
+  s_w[DN_THREADS + threadIdx.x] = pr;
+  s_w[2 * DN_THREADS + threadIdx.x] = dr;
+  s_w[3 * DN_THREADS + threadIdx.x] = dn;
   __syncthreads();
   for(int h=DN_THREADS / 2; h > 0; h >>= 1) {
     if((int) threadIdx.x < h) {
       #pragma unroll
-      for(int k=0; k < 4; ++k) s_red[k][threadIdx.x] += s_red[k][threadIdx.x + h];
+      for(int k=0; k < 4; ++k) s_w[k * DN_THREADS + threadIdx.x] += s_w[k * DN_THREADS + threadIdx.x + h];
     }
     __syncthreads();
   }
-  if(threadIdx.x < 4) part[RP + threadIdx.x] = s_red[threadIdx.x][0];
+  if(threadIdx.x < 4) part[RP + threadIdx.x] = s_w[threadIdx.x * DN_THREADS];
 }
 
-// After pass j: norms[c] = sqrt(column total) of P'_j; with check, the test
-// of iteration j-1 (sums[0..3] = ||P||^2, ||P - aux||^2, ||P - P_prev||^2,
-// ||D||^2). ctl[1] = index of the state that is final so far (j, or j-1 on
-// convergence, which sets ctl[0]). One block.
+// After pass j (one block): the block partials added in block order per
+// thread, wave butterflies and waves in order; norms of P'_j for the next
+// prox, the test of iteration j-1, and ctl (ctl[1] = index of the state
+// that is final so far: j, or j-1 on convergence, which sets ctl[0]).
+template<int E>
 __global__ void __launch_bounds__(DN_THREADS)
-k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R, uint32_t RP,
+k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
     double * __restrict__ norms, double * __restrict__ sums, int * __restrict__ ctl,
     double tol, int check, int j)
 {
   if(ctl[0] != 0) return;
-  __shared__ double s[DN_THREADS / 32][32];
-  __shared__ double s4[DN_THREADS / 4][4];
-  __shared__ double tot[4];
-  uint32_t const ld = RP + 4;
-  for(uint32_t c0=0; c0 < R; c0 += 32) {
-    uint32_t const c = c0 + threadIdx.x % 32;
-    double const v = block_partsum<32>(part, nb, ld, c < R, c, s);
-    if(threadIdx.x < 32 && c < R) norms[c] = sqrt(v);
+  constexpr int RP = E - 4;
+  __shared__ double s_w[DN_THREADS / 64][E];
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  double acc[E];
+  #pragma unroll
+  for(int e=0; e < E; ++e) acc[e] = 0.;
+  for(uint32_t b=threadIdx.x; b < nb; b += DN_THREADS) {
+    double const * const pb = part + (uint64_t) b * E;
+    #pragma unroll
+    for(int e=0; e < E; ++e) acc[e] += pb[e];
   }
-  double const v = block_partsum<4>(part, nb, ld, true, RP + threadIdx.x % 4, s4);
-  if(threadIdx.x < 4) tot[threadIdx.x] = v;
+  #pragma unroll
+  for(int e=0; e < E; ++e) {
+    double v = acc[e];
+    #pragma unroll
+    for(int off=32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if(lane == 0) s_w[wave][e] = v;
+  }
   __syncthreads();
-  if(threadIdx.x == 0) {
+  if(threadIdx.x < RP) {
+    int const e = threadIdx.x;
+    if(e < (int) R) norms[e] = sqrt(((s_w[0][e] + s_w[1][e]) + s_w[2][e]) + s_w[3][e]);
+  } else if(threadIdx.x == RP) {
+    double t[4];
+    for(int k=0; k < 4; ++k) {
+      int const e = RP + k;
+      t[k] = ((s_w[0][e] + s_w[1][e]) + s_w[2][e]) + s_w[3][e];
+    }
     bool conv = false;
     if(check) {
-      for(int k=0; k < 4; ++k) sums[k] = tot[k];
-      conv = (tot[1] <= tol * tot[0] && tot[2] <= tol * tot[3]);
+      for(int k=0; k < 4; ++k) sums[k] = t[k];
+      conv = (t[1] <= tol * t[0] && t[2] <= tol * t[3]);
     }
     if(conv) {
       ctl[0] = 1;
@@ -1397,9 +1445,20 @@ static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_
 }
 
 template<int TT, typename TM>
-static int launch_admm2(Admm2Args<TM> const & a, uint32_t nb, hipStream_t st)
+static int launch_admm2(Admm2Args<TM> const & a, int j, uint32_t nb, double * norms,
+    double * sums, int * ctl, double tol, hipStream_t st)
 {
-  hipLaunchKernelGGL((k_admm_pass<TT, TM>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+  constexpr int E = TT * 16 + 4;
+  if(j == 1) {
+    hipLaunchKernelGGL((k_admm_pass<TT, TM, 0>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+  } else if(j == 2) {
+    hipLaunchKernelGGL((k_admm_pass<TT, TM, 1>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((k_admm_pass<TT, TM, 2>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+  }
+  SPD_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_admm_pass_reduce<E>), dim3(1), dim3(DN_THREADS), 0, st,
+      (double const *) a.part, nb, a.R, norms, sums, ctl, tol, j >= 2 ? 1 : 0, j);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
@@ -1413,7 +1472,9 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
     uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
 {
   uint32_t const RP = dn_rpad(R);
-  uint32_t const nb = apply_blocks(I);
+  // one block per 64 rows, at most six resident blocks per CU's worth
+  uint32_t nb = (uint32_t) (spd_cdiv(I, 64) < 1536 ? spd_cdiv(I, 64) : 1536);
+  if(nb < 1) nb = 1;
   void * pc = nullptr;
   size_t const part_n = (size_t) nb * (RP + 4);
   SPD_TRY(spd_scratch(SLOT_COL, (part_n + 3 * (size_t) RP) * sizeof(double), &pc));
@@ -1439,19 +1500,16 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
   for(uint64_t j=1; j <= max_inner; ++j) {
     a.cur = (j >= 2) ? bf.b[(j - 1) % 3] : nullptr;
     a.cur_n = (j >= 2) ? bf.n[(j - 1) % 3] : nullptr;
-    a.check = (j >= 2);
     a.prev = (j >= 3) ? bf.b[(j - 2) % 3] : nullptr;
     a.prev_n = (j >= 3) ? bf.n[(j - 2) % 3] : nullptr;
     a.out = bf.b[j % 3];
+    double * const nrm = bf.n[j % 3];
     switch(RP / 16) {
-    case 1: SPD_TRY((launch_admm2<1, TM>(a, nb, st))); break;
-    case 2: SPD_TRY((launch_admm2<2, TM>(a, nb, st))); break;
-    case 3: SPD_TRY((launch_admm2<3, TM>(a, nb, st))); break;
-    default: SPD_TRY((launch_admm2<4, TM>(a, nb, st))); break;
+    case 1: SPD_TRY((launch_admm2<1, TM>(a, (int) j, nb, nrm, sums, ctl, tol, st))); break;
+    case 2: SPD_TRY((launch_admm2<2, TM>(a, (int) j, nb, nrm, sums, ctl, tol, st))); break;
+    case 3: SPD_TRY((launch_admm2<3, TM>(a, (int) j, nb, nrm, sums, ctl, tol, st))); break;
+    default: SPD_TRY((launch_admm2<4, TM>(a, (int) j, nb, nrm, sums, ctl, tol, st))); break;
     }
-    hipLaunchKernelGGL(k_admm_pass_reduce, dim3(1), dim3(DN_THREADS), 0, st,
-        (double const *) part, nb, R, RP, bf.n[j % 3], sums, ctl, tol, a.check, (int) j);
-    SPD_LAUNCH_CHECK();
   }
   uint64_t const n = I * R;
   hipLaunchKernelGGL(k_admm_final, dim3(dn_blocks(n, DN_THREADS * 8)), dim3(DN_THREADS), 0, st,

@@ -41,10 +41,13 @@ struct CooArgs {
 // COO MTTKRP: per nonzero v * prod_{m != mode} U_m[ind_m] (modes ascending,
 // as the reference multiplies), added to the output row. L lanes per
 // nonzero, NC column chunks of L. A group takes CO_CHUNK consecutive
-// nonzeros and adds runs of equal output rows before one atomic update per
-// run (a slice is sorted by the mode after the streamed one, so that mode's
-// rows come in runs).
+// nonzeros: its first lanes load their values and indices once (coalesced),
+// the products of CO_BATCH nonzeros are formed with every row load issued
+// before any is used, then runs of equal output rows are added up and
+// written with one atomic update per run (a slice is sorted by the mode
+// after the streamed one, so that mode's rows come in runs).
 #define CO_CHUNK 16
+#define CO_BATCH 8
 template<int L, int NC>
 __global__ void __launch_bounds__(CO_THREADS)
 k_coo_mttkrp(CooArgs a)
@@ -56,7 +59,10 @@ k_coo_mttkrp(CooArgs a)
   int const wave = threadIdx.x >> 6;
   int const c = lane % L;
   int const g = lane / L;
+  int const base = g * L;
   uint32_t const R = a.R;
+  int const N = a.N;
+  int const mode = a.mode;
   uint64_t const group = ((uint64_t) blockIdx.x * W + wave) * G + g;
   uint64_t const ngroups = (uint64_t) gridDim.x * W * G;
   double acc[NC];
@@ -64,61 +70,86 @@ k_coo_mttkrp(CooArgs a)
   for(int k=0; k < NC; ++k) acc[k] = 0.;
   uint32_t cur = 0xFFFFFFFFu;
   for(uint64_t n0 = group * CO_CHUNK; n0 < a.nnz; n0 += ngroups * CO_CHUNK) {
-    uint64_t const n1 = (n0 + CO_CHUNK < a.nnz) ? n0 + CO_CHUNK : a.nnz;
-    for(uint64_t n=n0; n < n1; ++n) {
-      double const v = a.vals[n];
-      double p[NC];
-      #pragma unroll
-      for(int k=0; k < NC; ++k) p[k] = v;
+    uint32_t const cnt = (n0 + CO_CHUNK < a.nnz) ? CO_CHUNK : (uint32_t) (a.nnz - n0);
+    // lane c < cnt of the group holds nonzero n0 + c
+    double myv = 0.;
+    uint32_t myo = 0;
+    uint32_t myi[SPD_MAX_MODES];
+    #pragma unroll
+    for(int m=0; m < SPD_MAX_MODES; ++m) myi[m] = 0;
+    if((uint32_t) c < cnt) {
+      myv = a.vals[n0 + c];
       #pragma unroll
       for(int m=0; m < SPD_MAX_MODES; ++m) {
-        if(m >= a.N) break;
-        if(m == a.mode) continue;
-        double const * row = a.mats[m] + (uint64_t) a.ind[m][n] * R;
-        #pragma unroll
-        for(int k=0; k < NC; ++k) {
-          uint32_t const col = c + k * L;
-          if(col < R) p[k] *= row[col];
-        }
+        if(m < N && m != mode) myi[m] = a.ind[m][n0 + c];
       }
-      if(!a.reduce) {
-        uint32_t const orow = a.ind[a.mode][n];
-        if(orow != cur) {
-          if(cur != 0xFFFFFFFFu) {
-            double * o = a.out + (uint64_t) cur * R;
-            #pragma unroll
-            for(int k=0; k < NC; ++k) {
-              uint32_t const col = c + k * L;
-              if(col < R) unsafeAtomicAdd(o + col, acc[k]);
-            }
-          }
+      if(!a.reduce) myo = a.ind[mode][n0 + c];
+    }
+    for(uint32_t k0=0; k0 < cnt; k0 += CO_BATCH) {
+      double p[CO_BATCH][NC];
+      uint32_t orow[CO_BATCH];
+      #pragma unroll
+      for(int k=0; k < CO_BATCH; ++k) {
+        int const src = base + (int) k0 + k;
+        bool const live = (k0 + k) < cnt;
+        double const v = __shfl(myv, src, 64);
+        orow[k] = __shfl(myo, src, 64);
+        #pragma unroll
+        for(int q=0; q < NC; ++q) p[k][q] = v;
+        #pragma unroll
+        for(int m=0; m < SPD_MAX_MODES; ++m) {
+          if(m >= N) break;
+          if(m == mode) continue;
+          uint32_t const im = __shfl(myi[m], src, 64);
+          double const * const row = a.mats[m] + (uint64_t) im * R;
           #pragma unroll
-          for(int k=0; k < NC; ++k) acc[k] = 0.;
-          cur = orow;
+          for(int q=0; q < NC; ++q) {
+            uint32_t const col = c + q * L;
+            if(live && col < R) p[k][q] *= row[col];
+          }
         }
       }
       #pragma unroll
-      for(int k=0; k < NC; ++k) acc[k] += p[k];
+      for(int k=0; k < CO_BATCH; ++k) {
+        if(k0 + k >= cnt) break;
+        if(!a.reduce) {
+          if(orow[k] != cur) {
+            if(cur != 0xFFFFFFFFu) {
+              double * const o = a.out + (uint64_t) cur * R;
+              #pragma unroll
+              for(int q=0; q < NC; ++q) {
+                uint32_t const col = c + q * L;
+                if(col < R) unsafeAtomicAdd(o + col, acc[q]);
+              }
+            }
+            #pragma unroll
+            for(int q=0; q < NC; ++q) acc[q] = 0.;
+            cur = orow[k];
+          }
+        }
+        #pragma unroll
+        for(int q=0; q < NC; ++q) acc[q] += p[k][q];
+      }
     }
     if(!a.reduce && cur != 0xFFFFFFFFu) {
-      double * o = a.out + (uint64_t) cur * R;
+      double * const o = a.out + (uint64_t) cur * R;
       #pragma unroll
-      for(int k=0; k < NC; ++k) {
-        uint32_t const col = c + k * L;
-        if(col < R) unsafeAtomicAdd(o + col, acc[k]);
-        acc[k] = 0.;
+      for(int q=0; q < NC; ++q) {
+        uint32_t const col = c + q * L;
+        if(col < R) unsafeAtomicAdd(o + col, acc[q]);
+        acc[q] = 0.;
       }
       cur = 0xFFFFFFFFu;
     }
   }
   if(a.reduce) {
     #pragma unroll
-    for(int k=0; k < NC; ++k) s_red[wave * G + g][c + k * L] = acc[k];
+    for(int q=0; q < NC; ++q) s_red[wave * G + g][c + q * L] = acc[q];
     __syncthreads();
     for(uint32_t col=threadIdx.x; col < R; col += CO_THREADS) {
-      double s = 0.;
-      for(int q=0; q < W * G; ++q) s += s_red[q][col];
-      a.part[(uint64_t) blockIdx.x * R + col] = s;
+      double sum = 0.;
+      for(int w=0; w < W * G; ++w) sum += s_red[w][col];
+      a.part[(uint64_t) blockIdx.x * R + col] = sum;
     }
   }
 }
