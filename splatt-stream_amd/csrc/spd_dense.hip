@@ -40,21 +40,30 @@ static uint32_t dn_blocks(uint64_t work_items, uint64_t per_block)
 }
 
 // Sums of block partials, part[b * ld + src], b < nb, for up to CW entries
-// at once by one block: thread t takes entry t % CW and every G-th partial
-// starting at t / CW (G = DN_THREADS / CW); the G partial sums are then added
-// in a fixed order. Returns the total in threads t < CW (src of the entry is
-// given by those threads). Deterministic for fixed nb and CW.
-template<int CW>
+// at once by one block of NT threads: thread t takes entry t % CW and every
+// G-th partial starting at t / CW (G = NT / CW), four loads in flight; the
+// G partial sums are then added in a fixed order. Returns the total in
+// threads t < CW (src of the entry is given by those threads).
+// Deterministic for fixed nb, CW and NT.
+template<int CW, int NT = DN_THREADS>
 __device__ __forceinline__ double block_partsum(double const * __restrict__ part,
     uint32_t nb, uint32_t ld, bool valid, uint32_t src, double (*s)[CW])
 {
-  constexpr int G = DN_THREADS / CW;
+  constexpr int G = NT / CW;
   int const t = threadIdx.x;
   int const e = t % CW;
   int const q = t / CW;
   double v = 0.;
   if(q < G && valid) {
-    for(uint32_t b=q; b < nb; b += G) v += part[(uint64_t) b * ld + src];
+    uint32_t b = q;
+    for(; b + 3 * G < nb; b += 4 * G) {
+      double const x0 = part[(uint64_t) b * ld + src];
+      double const x1 = part[(uint64_t) (b + G) * ld + src];
+      double const x2 = part[(uint64_t) (b + 2 * G) * ld + src];
+      double const x3 = part[(uint64_t) (b + 3 * G) * ld + src];
+      v += ((x0 + x1) + (x2 + x3));
+    }
+    for(; b < nb; b += G) v += part[(uint64_t) b * ld + src];
   }
   if(q < G) s[q][e] = v;
   __syncthreads();
@@ -165,18 +174,19 @@ k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
 // Sum of the block partials into the full R x R: the upper triangle
 // mirrored (sym) or every entry as computed (cross products). 32 entries
 // per block.
-__global__ void __launch_bounds__(DN_THREADS)
+#define GR_RED_THREADS 1024
+__global__ void __launch_bounds__(GR_RED_THREADS)
 k_gram_reduce(double const * __restrict__ part, uint32_t nb,
     uint32_t R, uint32_t ld, double * __restrict__ out, int sym)
 {
-  __shared__ double s[DN_THREADS / 32][32];
+  __shared__ double s[GR_RED_THREADS / 32][32];
   uint32_t const e = blockIdx.x * 32 + threadIdx.x % 32;
   bool const valid = e < R * R;
   uint32_t const i = valid ? e / R : 0;
   uint32_t const j = valid ? e - i * R : 0;
   uint32_t const a = (sym && j < i) ? j : i;
   uint32_t const b = (sym && j < i) ? i : j;
-  double const tot = block_partsum<32>(part, nb, ld * ld, valid, a * ld + b, s);
+  double const tot = block_partsum<32, GR_RED_THREADS>(part, nb, ld * ld, valid, a * ld + b, s);
   if(threadIdx.x < 32 && valid) out[e] = tot;
 }
 
@@ -338,10 +348,12 @@ k_chol_inv(GramSrc g, int nmodes, int mode, uint32_t R, uint32_t RP,
   }
   __syncthreads();
 
-  // right-looking Cholesky on the lower triangle; thread t owns rows t+64k
+  // right-looking Cholesky on the lower triangle; thread t owns rows t+64k.
+  // The first bad pivot is kept in a register and stored once at the end.
+  int bad = 0;
   for(uint32_t j=0; j < R; ++j) {
     double const d = A[j * R + j];
-    if(t == 0 && !(d > 0.) && *info == 0) *info = (int) j + 1;
+    if(t == 0 && !(d > 0.) && bad == 0) bad = (int) j + 1;
     double const ljj = sqrt(d);
     __syncthreads();
     for(uint32_t i=j + 1 + t; i < R; i += 64) A[i * R + j] /= ljj;
@@ -353,6 +365,7 @@ k_chol_inv(GramSrc g, int nmodes, int mode, uint32_t R, uint32_t RP,
     }
     __syncthreads();
   }
+  if(t == 0 && bad != 0 && *info == 0) *info = bad;
   // L^{-1}, column c by thread c (forward substitution against e_c)
   for(uint32_t c=t; c < R; c += 64) {
     Li[c * R + c] = 1. / A[c * R + c];
@@ -379,107 +392,106 @@ k_chol_inv(GramSrc g, int nmodes, int mode, uint32_t R, uint32_t RP,
   }
 }
 
-// The same for R <= RP <= 64 with the rows in registers: thread i holds row
-// i of the matrix (identity beyond R), column broadcasts go through lane
-// shuffles and LDS, and every loop is unrolled, so nothing waits on
-// dependent LDS round trips. The arithmetic (operations and their order)
-// is that of k_chol_inv, so the two give the same bits. Launched with
-// exactly RP threads: thread i owns row i, and rows i >= RP do not exist
-// in sL or ginv.
+// The same for R <= RP <= 64 by Gauss-Jordan elimination in place (no
+// pivoting: G is symmetric positive definite, and its pivots are the
+// squares of the Cholesky diagonal, so a non-positive one is reported the
+// same way). Every step updates all RP x RP entries spread over the 64
+// threads, so the work per thread is RP^3/64 independent LDS updates rather
+// than a chain of dependent ones. The result is written from its upper
+// triangle, so it is exactly symmetric.
 template<int RP>
 __global__ void __launch_bounds__(64)
-k_chol_inv_reg(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
+k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
     double * __restrict__ ginv, double * __restrict__ rho_out, int * __restrict__ info)
 {
-  __shared__ double sL[RP][RP + 1];
-  __shared__ double s_diag[64];
+  __shared__ double A[RP][RP + 1];
+  __shared__ double colj[RP];
   __shared__ double s_rho;
-  int const i = threadIdx.x;
-  bool const irow = i < (int) R;
-  double a[RP];
+  constexpr int NE = RP * RP / 64;
+  int const t = threadIdx.x;
+  int const iR = (int) R;
+  // the Hadamard product, one Gram at a time with all of a thread's loads
+  // of that Gram in flight together (the Grams were just written by other
+  // kernels and come from far away)
+  double hv[NE];
   #pragma unroll
-  for(int k=0; k < RP; ++k) a[k] = (irow && k < (int) R) ? 1. : ((k == i) ? 1. : 0.);
-  // row i of the Hadamard product, one Gram at a time so that the RP loads
-  // of a Gram are all in flight together
+  for(int q=0; q < NE; ++q) {
+    int const e = t + 64 * q;
+    int const r = e / RP;
+    int const k = e - r * RP;
+    hv[q] = (r < iR && k < iR) ? 1. : ((k == r) ? 1. : 0.);
+  }
   for(int m=0; m < nmodes; ++m) {
     if(m == mode) continue;
     double const * const gm = g.aTa[m];
+    double gv[NE];
     #pragma unroll
-    for(int k=0; k < RP; ++k) {
-      if(irow && k < (int) R) {
-        int const lo = i < k ? i : k;
-        int const hi = i < k ? k : i;
-        a[k] *= gm[lo * R + hi];
-      }
+    for(int q=0; q < NE; ++q) {
+      int const e = t + 64 * q;
+      int const r = e / RP;
+      int const k = e - r * RP;
+      bool const ok = (r < iR && k < iR);
+      int const lo = r < k ? r : k;
+      int const hi = r < k ? k : r;
+      gv[q] = gm[ok ? lo * R + hi : 0];     // straight-line: every load issued at once
+    }
+    #pragma unroll
+    for(int q=0; q < NE; ++q) {
+      int const e = t + 64 * q;
+      int const r = e / RP;
+      int const k = e - r * RP;
+      if(r < iR && k < iR) hv[q] *= gv[q];
     }
   }
   #pragma unroll
-  for(int k=0; k < RP; ++k) {
-    if(irow && k == i) a[k] += diag_add;
+  for(int q=0; q < NE; ++q) {
+    int const e = t + 64 * q;
+    int const r = e / RP;
+    int const k = e - r * RP;
+    A[r][k] = (r < iR && k == r) ? hv[q] + diag_add : hv[q];
   }
+  __syncthreads();
   if(admm) {
-    double d = 0.;
-    #pragma unroll
-    for(int k=0; k < RP; ++k) if(k == i) d = a[k];
-    s_diag[i] = d;
-    __syncthreads();
-    if(i == 0) {
+    if(t == 0) {
       double tr = 0.;
-      for(uint32_t q=0; q < R; ++q) tr += s_diag[q];
+      for(int q=0; q < iR; ++q) tr += A[q][q];
       double const rho = tr / (double) R;
       *rho_out = rho;
       s_rho = rho;
     }
     __syncthreads();
-    double const rho = s_rho;
-    if(i < (int) R) {
-      #pragma unroll
-      for(int k=0; k < RP; ++k) if(k == i) a[k] += rho;
-    }
+    if(t < iR) A[t][t] += s_rho;
+    __syncthreads();
   }
-  // right-looking Cholesky: column j of L, then the trailing update of row i
-  #pragma unroll
+  // the first bad pivot is kept in a register and stored once at the end
+  int bad = 0;
   for(int j=0; j < RP; ++j) {
-    double const ajj = __shfl(a[j], j, 64);
-    if(i == 0 && j < (int) R && !(ajj > 0.) && *info == 0) *info = j + 1;
-    double const ljj = sqrt(ajj);
-    double const lij = a[j] / ljj;
-    if(i == j) a[j] = ljj;
-    else if(i > j) a[j] = lij;
+    double const d = A[j][j];
+    if(t == 0 && j < iR && !(d > 0.) && bad == 0) bad = j + 1;
+    double const p = 1. / d;
+    if(t < RP) colj[t] = A[t][j];
+    __syncthreads();
+    if(t < RP) A[j][t] = (t == j) ? p : A[j][t] * p;
+    __syncthreads();
     #pragma unroll
-    for(int k=j + 1; k < RP; ++k) {
-      double const lkj = __shfl(lij, k, 64);
-      if(i >= k) a[k] -= lij * lkj;
+    for(int q=0; q < NE; ++q) {
+      int const e = t + 64 * q;
+      int const i = e / RP;
+      int const k = e - i * RP;
+      double const ci = colj[i];
+      double const ajk = A[j][k];
+      double const aik = A[i][k];
+      double const v = (k == j) ? -ci * p : aik - ci * ajk;
+      if(i != j) A[i][k] = v;
     }
+    __syncthreads();
   }
-  #pragma unroll
-  for(int k=0; k < RP; ++k) sL[i][k] = (k <= i) ? a[k] : 0.;
-  __syncthreads();
-  // column i of L^{-1} (forward substitution against e_i)
-  double x[RP];
-  #pragma unroll
-  for(int r=0; r < RP; ++r) {
-    double sum = 0.;
-    #pragma unroll
-    for(int k=0; k < r; ++k) sum += sL[r][k] * x[k];
-    double const lrr = sL[r][r];
-    x[r] = (r < i) ? 0. : ((r == i) ? 1. / lrr : -sum / lrr);
-  }
-  __syncthreads();
-  #pragma unroll
-  for(int k=0; k < RP; ++k) sL[k][i] = x[k];
-  __syncthreads();
-  // G^{-1} = L^{-T} L^{-1}: entry (q, i) for q <= i from columns q and i
-  #pragma unroll
-  for(int q=0; q < RP; ++q) {
-    double sum = 0.;
-    #pragma unroll
-    for(int k=0; k < RP; ++k) sum += sL[k][q] * x[k];
-    if(q <= i) {
-      bool const in = (i < (int) R) && (q < (int) R);
-      ginv[q * RP + i] = in ? sum : 0.;
-      ginv[i * RP + q] = in ? sum : 0.;
-    }
+  if(t == 0 && bad != 0 && *info == 0) *info = bad;
+  for(int e=t; e < RP * RP; e += 64) {
+    int const q = e / RP;
+    int const i = e - q * RP;
+    bool const in = (i < iR) && (q < iR);
+    ginv[e] = in ? ((q <= i) ? A[q][i] : A[i][q]) : 0.;
   }
 }
 
@@ -635,7 +647,7 @@ struct Admm2Args {
   double const * prev;      // P'_{j-2} (test against a state >= 1)
   double const * prev_n;
   double * out;             // P'_j
-  double * part;            // per block: RP column sums, then 4 test sums
+  double * part;            // RP column sums, then 4 test sums, each over the blocks
   int const * ctl;          // ctl[0] != 0: converged, nothing to do
 };
 
@@ -788,7 +800,9 @@ k_admm_pass(Admm2Args<TM> a)
   }
   // block partials in a fixed order: RP column sums, then the four terms
   __syncthreads();
-  double * const part = a.part + (uint64_t) blockIdx.x * E;
+  // entry-major: part[e * gridDim.x + block], so the reduction reads rows
+  double * const part = a.part + blockIdx.x;
+  uint64_t const pstride = gridDim.x;
   #pragma unroll
   for(int t=0; t < TT; ++t) {
     cs[t] += __shfl_xor(cs[t], 16, 64);
@@ -800,7 +814,8 @@ k_admm_pass(Admm2Args<TM> a)
   }
   __syncthreads();
   for(int e=threadIdx.x; e < RP; e += DN_THREADS) {
-    part[e] = ((s_w[e] + s_w[DN_THREADS + e]) + s_w[2 * DN_THREADS + e]) + s_w[3 * DN_THREADS + e];
+    part[(uint64_t) e * pstride] =
+        ((s_w[e] + s_w[DN_THREADS + e]) + s_w[2 * DN_THREADS + e]) + s_w[3 * DN_THREADS + e];
   }
   __syncthreads();
   s_w[threadIdx.x] = pn;
@@ -815,53 +830,44 @@ k_admm_pass(Admm2Args<TM> a)
     }
     __syncthreads();
   }
-  if(threadIdx.x < 4) part[RP + threadIdx.x] = s_w[threadIdx.x * DN_THREADS];
+  if(threadIdx.x < 4) part[(uint64_t) (RP + threadIdx.x) * pstride] = s_w[threadIdx.x * DN_THREADS];
 }
 
-// After pass j (one block): the block partials added in block order per
-// thread, wave butterflies and waves in order; norms of P'_j for the next
-// prox, the test of iteration j-1, and ctl (ctl[1] = index of the state
-// that is final so far: j, or j-1 on convergence, which sets ctl[0]).
+// After pass j (one block of RD_THREADS): the block partials (entry-major)
+// are added by wave w for entries w, w + 16, ...: lane l sums blocks l,
+// l + 64, ... in order, then a butterfly. Norms of P'_j for the next prox,
+// the test of iteration j-1, and ctl (ctl[1] = index of the state that is
+// final so far: j, or j-1 on convergence, which sets ctl[0]). The code is
+// kept short: a single block runs it and its instruction fetches are cold.
+#define RD_THREADS 1024
 template<int E>
-__global__ void __launch_bounds__(DN_THREADS)
+__global__ void __launch_bounds__(RD_THREADS)
 k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
     double * __restrict__ norms, double * __restrict__ sums, int * __restrict__ ctl,
     double tol, int check, int j)
 {
   if(ctl[0] != 0) return;
   constexpr int RP = E - 4;
-  __shared__ double s_w[DN_THREADS / 64][E];
+  constexpr int NW = RD_THREADS / 64;
+  __shared__ double s_t[E];
   int const lane = threadIdx.x & 63;
   int const wave = threadIdx.x >> 6;
-  double acc[E];
-  #pragma unroll
-  for(int e=0; e < E; ++e) acc[e] = 0.;
-  for(uint32_t b=threadIdx.x; b < nb; b += DN_THREADS) {
-    double const * const pb = part + (uint64_t) b * E;
-    #pragma unroll
-    for(int e=0; e < E; ++e) acc[e] += pb[e];
-  }
-  #pragma unroll
-  for(int e=0; e < E; ++e) {
-    double v = acc[e];
-    #pragma unroll
+  for(int e=wave; e < E; e += NW) {
+    double const * const pe = part + (uint64_t) e * nb;
+    double v = 0.;
+    #pragma unroll 4
+    for(uint32_t b=lane; b < nb; b += 64) v += pe[b];
     for(int off=32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if(lane == 0) s_w[wave][e] = v;
+    if(lane == 0) s_t[e] = v;
   }
   __syncthreads();
   if(threadIdx.x < RP) {
-    int const e = threadIdx.x;
-    if(e < (int) R) norms[e] = sqrt(((s_w[0][e] + s_w[1][e]) + s_w[2][e]) + s_w[3][e]);
+    if(threadIdx.x < (int) R) norms[threadIdx.x] = sqrt(s_t[threadIdx.x]);
   } else if(threadIdx.x == RP) {
-    double t[4];
-    for(int k=0; k < 4; ++k) {
-      int const e = RP + k;
-      t[k] = ((s_w[0][e] + s_w[1][e]) + s_w[2][e]) + s_w[3][e];
-    }
     bool conv = false;
     if(check) {
-      for(int k=0; k < 4; ++k) sums[k] = t[k];
-      conv = (t[1] <= tol * t[0] && t[2] <= tol * t[3]);
+      for(int k=0; k < 4; ++k) sums[k] = s_t[RP + k];
+      conv = (s_t[RP + 1] <= tol * s_t[RP] && s_t[RP + 2] <= tol * s_t[RP + 3]);
     }
     if(conv) {
       ctl[0] = 1;
@@ -1218,7 +1224,7 @@ static int gram_impl(double * X, double const * Y, uint64_t I, uint32_t R,
         (double const *) X, sym ? (double const *) X : Y, I, R, rpb, (double *) part);
     SPD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 32)),
-        dim3(DN_THREADS), 0, st, (double const *) part, nb, R, R, out, sym);
+        dim3(GR_RED_THREADS), 0, st, (double const *) part, nb, R, R, out, sym);
     SPD_LAUNCH_CHECK();
     return SPD_OK;
   }
@@ -1235,7 +1241,7 @@ static int gram_impl(double * X, double const * Y, uint64_t I, uint32_t R,
   default: SPD_TRY(launch_gram_mfma<4>(X, I, R, div, X32, (double *) part, nb, st, Y)); break;
   }
   hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned) spd_cdiv((uint64_t) R * R, 32)),
-      dim3(DN_THREADS), 0, st, (double const *) part, nb, R, RP, out, sym);
+      dim3(GR_RED_THREADS), 0, st, (double const *) part, nb, R, RP, out, sym);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
@@ -1265,13 +1271,13 @@ int dn_chol_inv(double const * const * aTa, int nmodes, int mode, uint32_t R,
   uint32_t const RP = dn_rpad(R);
   if(RP <= 64) {
     switch(RP) {
-    case 16: hipLaunchKernelGGL((k_chol_inv_reg<16>), dim3(1), dim3(16), 0, st, g, nmodes, mode,
+    case 16: hipLaunchKernelGGL((k_gj_inv<16>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
-    case 32: hipLaunchKernelGGL((k_chol_inv_reg<32>), dim3(1), dim3(32), 0, st, g, nmodes, mode,
+    case 32: hipLaunchKernelGGL((k_gj_inv<32>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
-    case 48: hipLaunchKernelGGL((k_chol_inv_reg<48>), dim3(1), dim3(48), 0, st, g, nmodes, mode,
+    case 48: hipLaunchKernelGGL((k_gj_inv<48>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
-    default: hipLaunchKernelGGL((k_chol_inv_reg<64>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+    default: hipLaunchKernelGGL((k_gj_inv<64>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
     }
     SPD_LAUNCH_CHECK();
@@ -1457,7 +1463,7 @@ static int launch_admm2(Admm2Args<TM> const & a, int j, uint32_t nb, double * no
     hipLaunchKernelGGL((k_admm_pass<TT, TM, 2>), dim3(nb), dim3(DN_THREADS), 0, st, a);
   }
   SPD_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_admm_pass_reduce<E>), dim3(1), dim3(DN_THREADS), 0, st,
+  hipLaunchKernelGGL((k_admm_pass_reduce<E>), dim3(1), dim3(RD_THREADS), 0, st,
       (double const *) a.part, nb, a.R, norms, sums, ctl, tol, j >= 2 ? 1 : 0, j);
   SPD_LAUNCH_CHECK();
   return SPD_OK;

@@ -155,24 +155,24 @@ k_coo_mttkrp(CooArgs a)
 }
 
 // out[c] = sum_b part[b * R + c]: one block per 32 columns, the partials
-// split over 8 thread groups and added in a fixed order.
-__global__ void __launch_bounds__(256)
+// split over 32 thread groups and added in a fixed order.
+__global__ void __launch_bounds__(1024)
 k_rowsum(double const * __restrict__ part, uint32_t nb, uint32_t R,
     double * __restrict__ out)
 {
-  __shared__ double s[8][32];
+  __shared__ double s[32][33];
   int const t = threadIdx.x;
   uint32_t const c = blockIdx.x * 32 + t % 32;
   int const q = t / 32;
   double v = 0.;
   if(c < R) {
-    for(uint32_t b=q; b < nb; b += 8) v += part[(uint64_t) b * R + c];
+    for(uint32_t b=q; b < nb; b += 32) v += part[(uint64_t) b * R + c];
   }
   s[q][t % 32] = v;
   __syncthreads();
   if(t < 32 && c < R) {
     double tot = 0.;
-    for(int k=0; k < 8; ++k) tot += s[k][t];
+    for(int k=0; k < 32; ++k) tot += s[k][t];
     out[c] = tot;
   }
 }
@@ -271,6 +271,36 @@ __global__ void k_fill_f64(double * __restrict__ x, uint64_t n, double v)
   if(e < n) x[e] = v;
 }
 
+// Sort keys (one mode's labels) and positions for a stable re-sort.
+__global__ void k_key_pos(uint32_t const * __restrict__ ind, uint64_t n,
+    uint64_t * __restrict__ key, uint32_t * __restrict__ pos)
+{
+  uint64_t x = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  for(; x < n; x += stride) {
+    key[x] = ind[x];
+    pos[x] = (uint32_t) x;
+  }
+}
+
+struct SliceCopy { uint32_t const * in[SPD_MAX_MODES]; uint32_t * out[SPD_MAX_MODES]; };
+
+// out[m][x] = in[m][perm[x]] for the modes given, vals likewise.
+__global__ void k_gather_slice(SliceCopy c, int N, double const * __restrict__ vals,
+    double * __restrict__ ovals, uint32_t const * __restrict__ perm, uint64_t n)
+{
+  uint64_t x = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  for(; x < n; x += stride) {
+    uint32_t const p = perm[x];
+    #pragma unroll
+    for(int m=0; m < SPD_MAX_MODES; ++m) {
+      if(m < N && c.out[m] != nullptr) c.out[m][x] = c.in[m][p];
+    }
+    ovals[x] = vals[p];
+  }
+}
+
 namespace {
 
 template<int L, int NC>
@@ -310,7 +340,7 @@ int coo_mttkrp(uint32_t const * const * ind, double const * vals,
   }
   uint64_t const per_block = (uint64_t) (CO_THREADS / L) * CO_CHUNK;
   uint64_t nb64 = spd_cdiv(nnz > 0 ? nnz : 1, per_block);
-  uint32_t const cap = a.reduce ? 256u : 8192u;
+  uint32_t const cap = a.reduce ? 2048u : 8192u;
   uint32_t const nb = (uint32_t) (nb64 < cap ? nb64 : cap);
   void * part = nullptr;
   if(a.reduce) {
@@ -328,7 +358,7 @@ int coo_mttkrp(uint32_t const * const * ind, double const * vals,
     else SPD_TRY((launch_coo<64, 4>(a, nb, st)));
   }
   if(a.reduce) {
-    hipLaunchKernelGGL(k_rowsum, dim3((unsigned) spd_cdiv(R, 32)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_rowsum, dim3((unsigned) spd_cdiv(R, 32)), dim3(1024), 0, st,
         (double const *) part, nb, R, out);
     SPD_LAUNCH_CHECK();
   }
@@ -515,6 +545,57 @@ int StreamState::slice(uint32_t const * const * ind_in, double const * vals_b, u
   double const * mats[SPD_MAX_MODES];
   for(int m=0; m < N; ++m) mats[m] = (m == t) ? time_row : Unew[m];
 
+  // copies of the slice sorted by each mode after the first one following
+  // the streamed mode, so that their MTTKRP outputs come in runs of rows
+  // and add up before one atomic per run
+  int const lead = (t + 1) % N;
+  if(bn > sort_cap) {
+    uint64_t nc = sort_cap ? sort_cap : 4096;
+    while(nc < bn) nc *= 2;
+    SPD_HIP(hipStreamSynchronize(st));
+    SpdMem fresh;
+    for(int k=0; k < 2; ++k) {
+      SPD_TRY(fresh.get((void **) &sk[k], nc * sizeof(uint64_t)));
+      SPD_TRY(fresh.get((void **) &sv[k], nc * sizeof(uint32_t)));
+    }
+    for(int m=0; m < N; ++m) {
+      by_vals[m] = nullptr;
+      for(int k=0; k < N; ++k) by_ind[m][k] = nullptr;
+      if(m == t || m == lead) continue;
+      SPD_TRY(fresh.get((void **) &by_vals[m], nc * sizeof(double)));
+      for(int k=0; k < N; ++k) {
+        if(k != t) SPD_TRY(fresh.get((void **) &by_ind[m][k], nc * sizeof(uint32_t)));
+      }
+    }
+    sort_mem.ptrs.swap(fresh.ptrs);
+    sort_cap = nc;
+  }
+  uint32_t const * ind_m[SPD_MAX_MODES][SPD_MAX_MODES];
+  double const * vals_m[SPD_MAX_MODES];
+  for(int m=0; m < N; ++m) {
+    vals_m[m] = vals_b;
+    for(int k=0; k < N; ++k) ind_m[m][k] = ind_b[k];
+    if(m == t || m == lead || bn < 2) continue;
+    unsigned const grid = (unsigned) (spd_cdiv(bn, 256) < 4096 ? spd_cdiv(bn, 256) : 4096);
+    hipLaunchKernelGGL(k_key_pos, dim3(grid), dim3(256), 0, st, ind_b[m], bn, sk[0], sv[0]);
+    SPD_LAUNCH_CHECK();
+    int bits = 1;
+    while(bits < 32 && (1ull << bits) < bd[m]) ++bits;
+    int which = 0;
+    SPD_TRY(spd_radix_sort(sk[0], sv[0], sk[1], sv[1], bn, bits, &which, st));
+    SliceCopy cp = {};
+    for(int k=0; k < N; ++k) {
+      if(k == t) continue;
+      cp.in[k] = ind_b[k];
+      cp.out[k] = by_ind[m][k];
+      ind_m[m][k] = by_ind[m][k];
+    }
+    hipLaunchKernelGGL(k_gather_slice, dim3(grid), dim3(256), 0, st, cp, N, vals_b,
+        by_vals[m], (uint32_t const *) sv[which], bn);
+    SPD_LAUNCH_CHECK();
+    vals_m[m] = by_vals[m];
+  }
+
   double prev_delta = 0.;
   uint64_t outer = 0;
   for(outer=0; outer < cfg.max_its; ++outer) {
@@ -529,7 +610,7 @@ int StreamState::slice(uint32_t const * const * ind_in, double const * vals_b, u
     for(int m=0; m < N; ++m) {
       if(m == t) continue;
       uint64_t const I = bd[m];
-      SPD_TRY(coo_mttkrp(ind_b, vals_b, bn, N, m, mats, R, I, Mbuf, st));
+      SPD_TRY(coo_mttkrp(ind_m[m], vals_m[m], bn, N, m, mats, R, I, Mbuf, st));
       // add_historical (StreamCPD.cc:223-294)
       HistSrc hs = {};
       int nc = 0;

@@ -53,6 +53,16 @@ struct StreamState {
   int * ctl = nullptr;
   uint32_t * d_zero_ind = nullptr;
   uint64_t zero_cap = 0;
+  // the slice re-sorted by each mode whose rows do not come in runs in the
+  // slice's own order (every mode but the streamed one and the one after
+  // it): by_ind[m][k] is mode k's index array of that copy, by_vals[m] its
+  // values; sort keys and payloads; sized for sort_cap nonzeros
+  uint32_t * by_ind[SPD_MAX_MODES][SPD_MAX_MODES] = {};
+  double * by_vals[SPD_MAX_MODES] = {};
+  uint64_t * sk[2] = {};
+  uint32_t * sv[2] = {};
+  uint64_t sort_cap = 0;
+  SpdMem sort_mem;
   std::vector<double> hrand;
 
   // device bytes init() allocates for these capacities

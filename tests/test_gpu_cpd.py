@@ -72,7 +72,7 @@ def _grams(R, n, seed):
     return out
 
 
-@pytest.mark.parametrize("R", [3, 16, 33, 64, 80])
+@pytest.mark.parametrize("R", [3, 16, 32, 33, 48, 64, 80])
 @pytest.mark.parametrize("admm", [0, 1])
 def test_chol_inv(gpu_available, R, admm):
     L = ps.lib()
