@@ -271,6 +271,9 @@ int spd_server_close(spd_server * sv, int64_t id);
 void spd_server_stats(spd_server * sv, size_t * used, size_t * budget,
     int * admitted, int * waiting);
 void spd_server_free(spd_server * sv);
+/* The values a server stream seeded with `seed` draws for its first n new
+ * factor entries (for tests). */
+void spd_serve_rand_fill(unsigned seed, double * vals, uint64_t n);
 
 #ifdef __cplusplus
 }

@@ -410,37 +410,40 @@ k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
   constexpr int NE = RP * RP / 64;
   int const t = threadIdx.x;
   int const iR = (int) R;
-  // the Hadamard product, one Gram at a time with all of a thread's loads
-  // of that Gram in flight together (the Grams were just written by other
-  // kernels and come from far away)
-  double hv[NE];
+  // the Hadamard product into A, one Gram at a time in chunks of 16 entries
+  // per thread whose loads are all in flight together (the Grams were just
+  // written by other kernels and come from far away)
   #pragma unroll
   for(int q=0; q < NE; ++q) {
     int const e = t + 64 * q;
     int const r = e / RP;
     int const k = e - r * RP;
-    hv[q] = (r < iR && k < iR) ? 1. : ((k == r) ? 1. : 0.);
+    A[r][k] = (r < iR && k < iR) ? 1. : ((k == r) ? 1. : 0.);
   }
+  constexpr int CH = (NE < 16) ? NE : 16;
   for(int m=0; m < nmodes; ++m) {
     if(m == mode) continue;
     double const * const gm = g.aTa[m];
-    double gv[NE];
     #pragma unroll
-    for(int q=0; q < NE; ++q) {
-      int const e = t + 64 * q;
-      int const r = e / RP;
-      int const k = e - r * RP;
-      bool const ok = (r < iR && k < iR);
-      int const lo = r < k ? r : k;
-      int const hi = r < k ? k : r;
-      gv[q] = gm[ok ? lo * R + hi : 0];     // straight-line: every load issued at once
-    }
-    #pragma unroll
-    for(int q=0; q < NE; ++q) {
-      int const e = t + 64 * q;
-      int const r = e / RP;
-      int const k = e - r * RP;
-      if(r < iR && k < iR) hv[q] *= gv[q];
+    for(int q0=0; q0 < NE; q0 += CH) {
+      double gv[CH];
+      #pragma unroll
+      for(int q=0; q < CH; ++q) {
+        int const e = t + 64 * (q0 + q);
+        int const r = e / RP;
+        int const k = e - r * RP;
+        bool const ok = (r < iR && k < iR);
+        int const lo = r < k ? r : k;
+        int const hi = r < k ? k : r;
+        gv[q] = gm[ok ? lo * R + hi : 0];
+      }
+      #pragma unroll
+      for(int q=0; q < CH; ++q) {
+        int const e = t + 64 * (q0 + q);
+        int const r = e / RP;
+        int const k = e - r * RP;
+        if(r < iR && k < iR) A[r][k] *= gv[q];
+      }
     }
   }
   #pragma unroll
@@ -448,7 +451,7 @@ k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
     int const e = t + 64 * q;
     int const r = e / RP;
     int const k = e - r * RP;
-    A[r][k] = (r < iR && k == r) ? hv[q] + diag_add : hv[q];
+    if(r < iR && k == r) A[r][k] += diag_add;
   }
   __syncthreads();
   if(admm) {

@@ -62,8 +62,15 @@ struct SpdScratchCtx {
 int spd_scratch(int slot, size_t bytes, void ** ptr);
 // Route this thread's scratch requests to ctx (nullptr: the library's own).
 void spd_scratch_bind(SpdScratchCtx * ctx);
-// Free a context's buffers (after its stream has drained).
+// Free a context's buffers (stream-ordered on its stream).
 void spd_scratch_release(SpdScratchCtx * ctx);
+
+// Stream-ordered device allocation from the device's memory pool: no
+// device-wide synchronisation (hipFree waits for the whole device), the
+// memory is usable by work queued on st after the call and is returned
+// after the work queued before the free.
+int spd_malloc_on(void ** ptr, size_t bytes, hipStream_t st);
+int spd_free_on(void * ptr, hipStream_t st);
 
 // Zero fill with a kernel on stream st.
 int spd_zero_async(void * ptr, size_t bytes, hipStream_t st);

@@ -71,26 +71,74 @@ struct Stream {
   bool has_kept = false;
 };
 
-void fill_stream_rand(void * ctx, double * vals, uint64_t n)
+// n factor values from a glibc generator state (src/util.c:15-40: the
+// magnitude from one draw, the sign from the next). The draws are those of
+// random_r: for the additive feedback generator (TYPE_3, a 128-byte state)
+// the step is inlined here, with the two state pointers kept in registers
+// (random_r's own call costs several times more); other types call it.
+void fill_rand_state(struct random_data * rd, double * vals, uint64_t n)
 {
-  Stream * s = (Stream *) ctx;
-  // src/util.c:15-40: magnitude from one draw, sign from the next
+  if(rd->rand_type != 3) {
+    for(uint64_t i=0; i < n; ++i) {
+      int32_t a = 0, b = 0;
+      random_r(rd, &a);
+      random_r(rd, &b);
+      double v = 3.0 * ((double) a / (double) RAND_MAX);
+      if(b % 2 == 0) v = -v;
+      vals[i] = v;
+    }
+    return;
+  }
+  int32_t * fptr = rd->fptr;
+  int32_t * rptr = rd->rptr;
+  int32_t * const state = rd->state;
+  int32_t * const end = rd->end_ptr;
+  auto draw = [&]() -> int32_t {
+    uint32_t const val = (uint32_t) *fptr + (uint32_t) *rptr;
+    *fptr = (int32_t) val;
+    ++fptr;
+    if(fptr >= end) {
+      fptr = state;
+      ++rptr;
+    } else {
+      ++rptr;
+      if(rptr >= end) rptr = state;
+    }
+    return (int32_t) (val >> 1);
+  };
   for(uint64_t i=0; i < n; ++i) {
-    int32_t a = 0, b = 0;
-    random_r(&s->rd, &a);
-    random_r(&s->rd, &b);
+    int32_t const a = draw();
+    int32_t const b = draw();
     double v = 3.0 * ((double) a / (double) RAND_MAX);
     if(b % 2 == 0) v = -v;
     vals[i] = v;
   }
+  rd->fptr = fptr;
+  rd->rptr = rptr;
+}
+
+void fill_stream_rand(void * ctx, double * vals, uint64_t n)
+{
+  fill_rand_state(&((Stream *) ctx)->rd, vals, n);
 }
 
 }  // namespace
+
+// A worker's HIP stream and pinned staging for slice uploads (reused by
+// every slice the worker runs, so copies are asynchronous and no stream is
+// created per slice or per tensor stream).
+struct WorkerRes {
+  hipStream_t hs = nullptr;
+  uint32_t * hind[SPD_MAX_MODES] = {};
+  double * hv = nullptr;
+  uint64_t cap = 0;
+};
 
 struct spd_server {
   size_t budget = 0;
   size_t used = 0;
   int max_inflight = 1;
+  std::vector<WorkerRes> wres;
   std::mutex mu;
   std::condition_variable cv_work;
   std::condition_variable cv_done;
@@ -119,32 +167,45 @@ void admit_waiting(spd_server * sv)
   }
 }
 
-void release_stream(spd_server * sv, Stream * s)
+// Device part of releasing an idle stream: its factors to the host, its
+// device memory back to the pool (stream-ordered). Runs without the server
+// lock: nothing else touches an idle, closing stream (it is marked busy).
+void release_device(Stream * s)
 {
-  // caller holds sv->mu; the stream is idle
-  if(s->allocated) {
-    if(s->state != nullptr) {
-      uint32_t const R = s->state->R;
-      for(int m=0; m < s->N; ++m) {
-        if(m == s->cfg.stream_mode) continue;
-        s->kept_rows[m] = s->state->rows[m];
-        s->kept[m].resize(s->kept_rows[m] * R);
-        if(hipMemcpyAsync(s->kept[m].data(), s->state->Unew[m],
-              s->kept_rows[m] * R * sizeof(double), hipMemcpyDeviceToHost, s->hs) != hipSuccess) {
-          s->kept_rows[m] = 0;
-        }
+  if(!s->allocated) return;
+  if(s->state != nullptr) {
+    uint32_t const R = s->state->R;
+    for(int m=0; m < s->N; ++m) {
+      if(m == s->cfg.stream_mode) continue;
+      s->kept_rows[m] = s->state->rows[m];
+      s->kept[m].resize(s->kept_rows[m] * R);
+      if(hipMemcpyAsync(s->kept[m].data(), s->state->Unew[m],
+            s->kept_rows[m] * R * sizeof(double), hipMemcpyDeviceToHost, s->hs) != hipSuccess) {
+        s->kept_rows[m] = 0;
       }
-      (void) hipStreamSynchronize(s->hs);
-      s->has_kept = true;
     }
-    spd_scratch_release(&s->scr);
-    delete s->state;
-    s->state = nullptr;
-    for(int m=0; m < s->N; ++m) spd_free(s->d_ind[m]);
-    spd_free(s->d_vals);
-    (void) hipStreamDestroy(s->hs);
-    s->allocated = false;
+    (void) hipStreamSynchronize(s->hs);
+    s->has_kept = true;
   }
+  // stream-ordered frees on the worker stream the tensor stream last ran
+  // on (its work is complete)
+  spd_scratch_release(&s->scr);
+  delete s->state;
+  s->state = nullptr;
+  for(int m=0; m < s->N; ++m) {
+    (void) spd_free_on(s->d_ind[m], s->hs);
+    s->d_ind[m] = nullptr;
+  }
+  (void) spd_free_on(s->d_vals, s->hs);
+  s->d_vals = nullptr;
+  s->slice_cap = 0;
+  s->allocated = false;
+}
+
+// Budget part (caller holds sv->mu): the footprint returns, waiting
+// streams are admitted.
+void release_budget(spd_server * sv, Stream * s)
+{
   if(s->admitted) {
     sv->used -= s->footprint;
     s->admitted = false;
@@ -152,37 +213,116 @@ void release_stream(spd_server * sv, Stream * s)
   admit_waiting(sv);
 }
 
+// Stable order of a slice by the modes after the streamed one (cyclically),
+// the order the reference's parser gives (ParserSimple.cc:13-80). When the
+// capacities' bits fit 64, an LSD radix sort of packed keys (11-bit digits,
+// stable); otherwise a comparison sort.
+static void slice_order(Slice const * sl, int N, int t, uint64_t const * cap,
+    std::vector<uint32_t> & order)
+{
+  uint64_t const n = sl->nnz;
+  int keys[SPD_MAX_MODES];
+  int kb[SPD_MAX_MODES];
+  int nk = 0;
+  int tot = 0;
+  for(int k=1; k < N; ++k) {
+    int const m = (t + k) % N;
+    int b = 1;
+    while(b < 63 && (1ull << b) < cap[m]) ++b;
+    keys[nk] = m;
+    kb[nk++] = b;
+    tot += b;
+  }
+  order.resize(n);
+  if(tot > 64) {
+    for(uint64_t x=0; x < n; ++x) order[x] = (uint32_t) x;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      for(int q=0; q < nk; ++q) {
+        uint64_t const ia = sl->ind[keys[q]][a];
+        uint64_t const ib = sl->ind[keys[q]][b];
+        if(ia != ib) return ia < ib;
+      }
+      return false;
+    });
+    return;
+  }
+  // packed key: the first sort mode most significant; indices beyond a
+  // capacity are caught by the caller's range check
+  std::vector<uint64_t> key(n), key2(n);
+  std::vector<uint32_t> ord2(n);
+  for(uint64_t x=0; x < n; ++x) {
+    uint64_t kv = 0;
+    for(int q=0; q < nk; ++q) {
+      uint64_t const v = sl->ind[keys[q]][x] & ((kb[q] >= 64) ? ~0ull : ((1ull << kb[q]) - 1));
+      kv = (kb[q] >= 64) ? v : ((kv << kb[q]) | v);
+    }
+    key[x] = kv;
+    order[x] = (uint32_t) x;
+  }
+  constexpr int DB = 11;
+  constexpr uint32_t NB = 1u << DB;
+  std::vector<uint64_t> cnt(NB);
+  for(int sh=0; sh < tot; sh += DB) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for(uint64_t x=0; x < n; ++x) ++cnt[(key[x] >> sh) & (NB - 1)];
+    uint64_t run = 0;
+    for(uint32_t d=0; d < NB; ++d) {
+      uint64_t const c = cnt[d];
+      cnt[d] = run;
+      run += c;
+    }
+    for(uint64_t x=0; x < n; ++x) {
+      uint64_t const pos = cnt[(key[x] >> sh) & (NB - 1)]++;
+      key2[pos] = key[x];
+      ord2[pos] = order[x];
+    }
+    key.swap(key2);
+    order.swap(ord2);
+  }
+}
+
 // Host part of a slice: order by the modes after the streamed one
 // (cyclically), relabel by first appearance, upload. Device part: the step.
-int run_slice(Stream * s, Slice * sl)
+int run_slice(Stream * s, Slice * sl, WorkerRes & w)
 {
   int const N = s->N;
   int const t = s->cfg.stream_mode;
   uint64_t const n = sl->nnz;
-  if(!s->allocated) {
-    SPD_HIP(hipStreamCreateWithFlags(&s->hs, hipStreamNonBlocking));
-    s->scr.st = s->hs;
-    s->allocated = true;
-    spd_scratch_bind(&s->scr);
-    s->state = new StreamState();
-    SPD_TRY(s->state->init(N, s->cfg, s->cap, s->hs));
-  }
+  // this worker's stream runs the slice; the tensor stream's earlier work
+  // (on any worker stream) is complete, and so are the worker's earlier
+  // copies out of its staging buffers
+  SPD_HIP(hipStreamSynchronize(w.hs));
+  s->hs = w.hs;
+  s->scr.st = w.hs;
   spd_scratch_bind(&s->scr);
-  std::vector<uint32_t> order(n);
-  for(uint64_t x=0; x < n; ++x) order[x] = (uint32_t) x;
-  int keys[SPD_MAX_MODES];
-  int nk = 0;
-  for(int k=1; k < N; ++k) keys[nk++] = (t + k) % N;
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-    for(int q=0; q < nk; ++q) {
-      uint64_t const ia = sl->ind[keys[q]][a];
-      uint64_t const ib = sl->ind[keys[q]][b];
-      if(ia != ib) return ia < ib;
+  if(!s->allocated) {
+    s->allocated = true;
+    s->state = new StreamState();
+    SPD_TRY(s->state->init(N, s->cfg, s->cap, w.hs, 1));
+  } else {
+    s->state->set_stream(w.hs);
+  }
+  if(n > w.cap) {
+    uint64_t nc = w.cap ? w.cap : 4096;
+    while(nc < n) nc *= 2;
+    for(int m=0; m < SPD_MAX_MODES; ++m) {
+      if(w.hind[m]) (void) hipHostFree(w.hind[m]);
+      w.hind[m] = nullptr;
     }
-    return false;
-  });
-  std::vector<uint32_t> hind[SPD_MAX_MODES];
-  std::vector<double> hv(n);
+    if(w.hv) (void) hipHostFree(w.hv);
+    w.hv = nullptr;
+    w.cap = 0;
+    for(int m=0; m < N; ++m) {
+      if(m == t) continue;
+      SPD_HIP(hipHostMalloc((void **) &w.hind[m], nc * sizeof(uint32_t), hipHostMallocDefault));
+    }
+    SPD_HIP(hipHostMalloc((void **) &w.hv, nc * sizeof(double), hipHostMallocDefault));
+    w.cap = nc;
+  }
+  std::vector<uint32_t> order;
+  slice_order(sl, N, t, s->cap, order);
+  uint32_t * const * hind = w.hind;
+  double * const hv = w.hv;
   double xn = 0.;
   for(uint64_t x=0; x < n; ++x) {
     hv[x] = sl->vals[order[x]];
@@ -191,7 +331,6 @@ int run_slice(Stream * s, Slice * sl)
   for(int m=0; m < N; ++m) {
     s->state->batch_max[m] = 0;
     if(m == t) continue;
-    hind[m].resize(n);
     for(uint64_t x=0; x < n; ++x) {
       uint64_t const o = sl->ind[m][order[x]];
       if(o >= s->cap[m]) {
@@ -208,23 +347,21 @@ int run_slice(Stream * s, Slice * sl)
   if(n > s->slice_cap) {
     uint64_t nc = s->slice_cap ? s->slice_cap : 4096;
     while(nc < n) nc *= 2;
-    SPD_HIP(hipStreamSynchronize(s->hs));
     for(int m=0; m < N; ++m) {
-      spd_free(s->d_ind[m]);
+      (void) spd_free_on(s->d_ind[m], s->hs);
       s->d_ind[m] = nullptr;
-      if(m != t) SPD_TRY(spd_malloc((void **) &s->d_ind[m], nc * sizeof(uint32_t)));
+      if(m != t) SPD_TRY(spd_malloc_on((void **) &s->d_ind[m], nc * sizeof(uint32_t), s->hs));
     }
-    spd_free(s->d_vals);
-    SPD_TRY(spd_malloc((void **) &s->d_vals, nc * sizeof(double)));
+    (void) spd_free_on(s->d_vals, s->hs);
+    SPD_TRY(spd_malloc_on((void **) &s->d_vals, nc * sizeof(double), s->hs));
     s->slice_cap = nc;
   }
   for(int m=0; m < N; ++m) {
     if(m == t) continue;
-    SPD_HIP(hipMemcpyAsync(s->d_ind[m], hind[m].data(), n * sizeof(uint32_t),
+    SPD_HIP(hipMemcpyAsync(s->d_ind[m], hind[m], n * sizeof(uint32_t),
         hipMemcpyHostToDevice, s->hs));
   }
-  SPD_HIP(hipMemcpyAsync(s->d_vals, hv.data(), n * sizeof(double), hipMemcpyHostToDevice,
-      s->hs));
+  SPD_HIP(hipMemcpyAsync(s->d_vals, hv, n * sizeof(double), hipMemcpyHostToDevice, s->hs));
   uint32_t const * ind[SPD_MAX_MODES];
   for(int m=0; m < N; ++m) ind[m] = s->d_ind[m];
   SPD_TRY(s->state->slice(ind, s->d_vals, n, xn, &sl->local_err, &sl->outer, &sl->gpu_s));
@@ -237,7 +374,7 @@ int run_slice(Stream * s, Slice * sl)
   return SPD_OK;
 }
 
-void worker_main(spd_server * sv)
+void worker_main(spd_server * sv, int wi)
 {
   (void) hipSetDevice(0);
   std::unique_lock<std::mutex> lk(sv->mu);
@@ -251,21 +388,31 @@ void worker_main(spd_server * sv)
     Slice * sl = s->queue.front();
     s->queue.pop_front();
     lk.unlock();
-    int const rc = run_slice(s, sl);
+    int const rc = run_slice(s, sl, sv->wres[wi]);
     spd_scratch_bind(nullptr);
     double const now = sst_wall_s();
     lk.lock();
     sl->t_done = now;
     sl->status = rc;
     if(rc != SPD_OK) snprintf(sl->err, sizeof(sl->err), "%s", spd_last_error());
-    s->busy = false;
-    --sv->pending;
     if(!s->queue.empty()) {
+      s->busy = false;
+      --sv->pending;
       sv->ready.push_back(id);
       sv->cv_work.notify_one();
     } else if(s->closing) {
-      release_stream(sv, s);
+      // the copy-back and frees run outside the lock; the stream stays busy
+      // and its slice pending until they are done
+      lk.unlock();
+      release_device(s);
+      lk.lock();
+      s->busy = false;
+      --sv->pending;
+      release_budget(sv, s);
       sv->cv_work.notify_all();
+    } else {
+      s->busy = false;
+      --sv->pending;
     }
     sv->cv_done.notify_all();
   }
@@ -289,7 +436,17 @@ extern "C" int spd_server_create(size_t budget, int max_inflight, spd_server ** 
   spd_server * sv = new spd_server();
   sv->budget = budget;
   sv->max_inflight = max_inflight;
-  for(int w=0; w < max_inflight; ++w) sv->workers.emplace_back(worker_main, sv);
+  sv->wres.resize(max_inflight);
+  for(int w=0; w < max_inflight; ++w) {
+    // default priority, non-blocking w.r.t. the legacy null stream
+    if(hipStreamCreateWithFlags(&sv->wres[w].hs, hipStreamNonBlocking) != hipSuccess) {
+      for(int k=0; k < w; ++k) (void) hipStreamDestroy(sv->wres[k].hs);
+      delete sv;
+      spd_set_error(SPD_EHIP, "serve: cannot create worker streams");
+      return SPD_EHIP;
+    }
+  }
+  for(int w=0; w < max_inflight; ++w) sv->workers.emplace_back(worker_main, sv, w);
   *out = sv;
   return SPD_OK;
 }
@@ -447,20 +604,27 @@ extern "C" int spd_server_factors(spd_server * sv, int64_t id, int mode, double 
 
 extern "C" int spd_server_close(spd_server * sv, int64_t id)
 {
-  std::lock_guard<std::mutex> lk(sv->mu);
+  std::unique_lock<std::mutex> lk(sv->mu);
   auto it = sv->streams.find(id);
   if(it == sv->streams.end()) {
     spd_set_error(SPD_EINPUT, "serve: bad stream");
     return SPD_EINPUT;
   }
   Stream * s = it->second;
+  if(s->closing) return SPD_OK;
   s->closing = true;
   if(s->queue.empty() && !s->busy) {
-    release_stream(sv, s);
     for(auto w = sv->waiting.begin(); w != sv->waiting.end(); ++w) {
       if(*w == id) { sv->waiting.erase(w); break; }
     }
+    s->busy = true;
+    lk.unlock();
+    release_device(s);
+    lk.lock();
+    s->busy = false;
+    release_budget(sv, s);
     sv->cv_work.notify_all();
+    sv->cv_done.notify_all();
   }
   return SPD_OK;
 }
@@ -493,9 +657,27 @@ extern "C" void spd_server_free(spd_server * sv)
   for(auto & kv : sv->streams) {
     kv.second->closing = true;
     kv.second->busy = false;
-    release_stream(sv, kv.second);
+    release_device(kv.second);
+    release_budget(sv, kv.second);
     delete kv.second;
   }
   for(Slice * sl : sv->slices) delete sl;
+  for(WorkerRes & w : sv->wres) {
+    (void) hipStreamSynchronize(w.hs);
+    for(int m=0; m < SPD_MAX_MODES; ++m) {
+      if(w.hind[m]) (void) hipHostFree(w.hind[m]);
+    }
+    if(w.hv) (void) hipHostFree(w.hv);
+    (void) hipStreamDestroy(w.hs);
+  }
   delete sv;
+}
+
+// The per-stream generator alone (tests compare it with glibc's random_r).
+extern "C" void spd_serve_rand_fill(unsigned seed, double * vals, uint64_t n)
+{
+  struct random_data rd = {};
+  char st[128] = {};
+  initstate_r(seed, st, sizeof(st), &rd);
+  fill_rand_state(&rd, vals, n);
 }

@@ -435,9 +435,20 @@ size_t StreamState::footprint(int N, int t, uint64_t R, uint64_t const * cap)
   return b;
 }
 
-int StreamState::init(int nmodes, spd_stream_cfg const & c, uint64_t const * caps,
-    hipStream_t stream)
+void StreamState::set_stream(hipStream_t stream)
 {
+  st = stream;
+  if(mem.st) mem.st = stream;
+  if(sort_mem.st) sort_mem.st = stream;
+}
+
+int StreamState::init(int nmodes, spd_stream_cfg const & c, uint64_t const * caps,
+    hipStream_t stream, int ordered)
+{
+  if(ordered) {
+    mem.st = stream;
+    sort_mem.st = stream;
+  }
   N = nmodes;
   cfg = c;
   t = c.stream_mode;
@@ -552,8 +563,9 @@ int StreamState::slice(uint32_t const * const * ind_in, double const * vals_b, u
   if(bn > sort_cap) {
     uint64_t nc = sort_cap ? sort_cap : 4096;
     while(nc < bn) nc *= 2;
-    SPD_HIP(hipStreamSynchronize(st));
     SpdMem fresh;
+    fresh.st = sort_mem.st;
+    if(!fresh.st) SPD_HIP(hipStreamSynchronize(st));
     for(int k=0; k < 2; ++k) {
       SPD_TRY(fresh.get((void **) &sk[k], nc * sizeof(uint64_t)));
       SPD_TRY(fresh.get((void **) &sv[k], nc * sizeof(uint32_t)));

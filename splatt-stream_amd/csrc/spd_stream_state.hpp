@@ -5,14 +5,22 @@
 #include "spd_internal.hpp"
 #include <vector>
 
-// Device allocations freed together.
+// Device allocations freed together; stream-ordered on st when it is set
+// (allocation and free then never wait for the rest of the device).
 struct SpdMem {
   std::vector<void *> ptrs;
-  ~SpdMem() { for(void * p : ptrs) spd_free(p); }
+  hipStream_t st = nullptr;
+  ~SpdMem() {
+    for(void * p : ptrs) {
+      if(st) (void) spd_free_on(p, st);
+      else spd_free(p);
+    }
+  }
   int get(void ** out, size_t bytes) {
     *out = nullptr;
     if(bytes == 0) bytes = 8;
-    SPD_TRY(spd_malloc(out, bytes));
+    if(st) SPD_TRY(spd_malloc_on(out, bytes, st));
+    else SPD_TRY(spd_malloc(out, bytes));
     ptrs.push_back(*out);
     return SPD_OK;
   }
@@ -67,7 +75,12 @@ struct StreamState {
 
   // device bytes init() allocates for these capacities
   static size_t footprint(int N, int t, uint64_t R, uint64_t const * cap);
-  int init(int nmodes, spd_stream_cfg const & c, uint64_t const * caps, hipStream_t stream);
+  // ordered != 0: allocate and free in stream order on the stream in use
+  int init(int nmodes, spd_stream_cfg const & c, uint64_t const * caps, hipStream_t stream,
+      int ordered = 0);
+  // Run later slices on another stream (the work queued so far on the old
+  // one must be complete).
+  void set_stream(hipStream_t stream);
   // One time slice: ind[m] (device, the slice's nonzeros, labels of every
   // mode but the streamed one) and vals (device); xnorm = sum of squared
   // values. batch_max[] must hold the slice's largest label per mode.

@@ -86,6 +86,31 @@ extern "C" int spd_malloc(void ** ptr, size_t bytes)
   return SPD_OK;
 }
 
+int spd_malloc_on(void ** ptr, size_t bytes, hipStream_t st)
+{
+  *ptr = nullptr;
+  if(bytes == 0) {
+    bytes = 256;
+  }
+  hipError_t e = hipMallocAsync(ptr, bytes, st);
+  if(e != hipSuccess) {
+    *ptr = nullptr;
+    (void) hipGetLastError();
+    spd_set_error(SPD_ENOMEM, "hipMallocAsync(%zu) failed: %s", bytes,
+        hipGetErrorString(e));
+    return SPD_ENOMEM;
+  }
+  return SPD_OK;
+}
+
+int spd_free_on(void * ptr, hipStream_t st)
+{
+  if(ptr != nullptr) {
+    SPD_HIP(hipFreeAsync(ptr, st));
+  }
+  return SPD_OK;
+}
+
 extern "C" int spd_free(void * ptr)
 {
   if(ptr != nullptr) {
@@ -208,9 +233,11 @@ void spd_scratch_bind(SpdScratchCtx * ctx)
 void spd_scratch_release(SpdScratchCtx * ctx)
 {
   if(ctx == nullptr) return;
-  if(ctx->st) (void) hipStreamSynchronize(ctx->st);
   for(int k=0; k < SPD_NSCRATCH; ++k) {
-    if(ctx->p[k]) (void) hipFree(ctx->p[k]);
+    if(ctx->p[k]) {
+      if(ctx->st) (void) spd_free_on(ctx->p[k], ctx->st);
+      else (void) hipFree(ctx->p[k]);
+    }
     ctx->p[k] = nullptr;
     ctx->bytes[k] = 0;
   }
@@ -225,16 +252,23 @@ int spd_scratch(int slot, size_t bytes, void ** ptr)
   SpdScratchCtx * c = t_ctx ? t_ctx : &g_ctx_lib;
   hipStream_t const st = (c == &g_ctx_lib) ? g_stream : c->st;
   if(c->bytes[slot] < bytes || c->p[slot] == nullptr) {
+    bool const own = (c != &g_ctx_lib);
     if(c->p[slot] != nullptr) {
-      // the stream may still use the old buffer
-      SPD_HIP(hipStreamSynchronize(st));
-      SPD_HIP(hipFree(c->p[slot]));
+      // the stream may still use the old buffer: a context with its own
+      // stream frees in stream order, the library's waits for its stream
+      if(own) {
+        SPD_TRY(spd_free_on(c->p[slot], st));
+      } else {
+        SPD_HIP(hipStreamSynchronize(st));
+        SPD_HIP(hipFree(c->p[slot]));
+      }
       c->p[slot] = nullptr;
       c->bytes[slot] = 0;
     }
     size_t want = bytes < 4096 ? 4096 : bytes;
     want += want / 8;
-    SPD_TRY(spd_malloc(&c->p[slot], want));
+    if(own) SPD_TRY(spd_malloc_on(&c->p[slot], want, st));
+    else SPD_TRY(spd_malloc(&c->p[slot], want));
     c->bytes[slot] = want;
   }
   *ptr = c->p[slot];
