@@ -181,6 +181,53 @@ def bench_cpd_c1(threads, seed=20261203):
                 fit_abs_diff=abs(res["fit"] - float(fits[-1])))
 
 
+def bench_cpd_c3(nnz, R=64, its=25, seed=20261208):
+    """CPD-ALS rank 64, 25 iterations, 4-mode 500000x300000x100000x2000 with
+    nnz Zipf(1.1) nonzeros in the first three modes and uniform in the
+    fourth, values U(-1, 1), two-CSF allocation. MTTKRP in FP32 (Grams,
+    solves and the fit in FP64), then the same iterations from the same start
+    with FP64 MTTKRP: the fit and every factor (scaled by lambda) of the FP32
+    run are compared with the FP64 run."""
+    inds, vals, dims = ps.synth_coord([500_000, 300_000, 100_000, 2_000], nnz,
+                                      zipf=[1.1, 1.1, 1.1, 0.0], vals="symmetric",
+                                      compact=False, seed=seed)
+    opts = ps.Opts(csf_alloc=ps.CSF_TWOMODE)
+    t0 = time.perf_counter()
+    cs = ps.CsfSet.convert(inds, vals, opts)
+    build_s = time.perf_counter() - t0
+    del inds, vals
+    L = ps.lib()
+    out = {}
+    try:
+        for name, prec in (("f32", 1), ("f64", 0)):
+            L.splatt_amd_set_mttkrp_precision(prec)
+            ps.run_cpd(cs, R, max_its=1, tol=0.0, seed=1)     # warm-up
+            t0 = time.perf_counter()
+            res = ps.run_cpd(cs, R, max_its=its, tol=0.0, seed=1)
+            secs = time.perf_counter() - t0
+            if res["rc"] != ps.SUCCESS:
+                raise RuntimeError("splatt_cpd failed: " + ps.last_error())
+            out[name] = (res, secs)
+    finally:
+        L.splatt_amd_set_mttkrp_precision(0)
+        cs.free()
+    r32, s32 = out["f32"]
+    r64, s64 = out["f64"]
+    ferr = []
+    for m in range(4):
+        a = r32["factors"][m] * r32["lam"]
+        b = r64["factors"][m] * r64["lam"]
+        ferr.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+    return dict(workload="CPD-ALS rank %d, %d iterations, 500000x300000x100000x2000, %d nnz "
+                         "Zipf(1.1) x3 + uniform, values U(-1,1), two-CSF allocation; FP32 "
+                         "MTTKRP vs an all-FP64 run from the same start" % (R, its, nnz),
+                csf_build_s=build_s, f32_s=s32, f32_ms_per_iteration=1e3 * s32 / its,
+                f64_s=s64, f64_ms_per_iteration=1e3 * s64 / its,
+                fit_f32=r32["fit"], fit_f64=r64["fit"],
+                fit_abs_diff=abs(r32["fit"] - r64["fit"]),
+                factor_rel_err_vs_f64=ferr)
+
+
 def stream_c4_slices(nslices, seed=20261206):
     """The first nslices time slices of the streaming workload: 500000 x
     200000 non-stream dims, per-slice nnz lognormal (median 2e5, sigma 1)
@@ -332,8 +379,9 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=64)
     ap.add_argument("--serve-slices", type=int, default=3)
     ap.add_argument("--serve-rate", type=float, default=500.0)
-    ap.add_argument("--serve-inflight", type=int, default=8)
-    ap.add_argument("--only", choices=["cpd", "stream", "serve"], default=None,
+    ap.add_argument("--serve-inflight", type=int, default=16)
+    ap.add_argument("--c3-nnz", type=int, default=100_000_000)
+    ap.add_argument("--only", choices=["cpd", "c3", "stream", "serve"], default=None,
                     help="run one secondary workload alone and print its JSON (profiling)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a streaming kernel of known traffic (FP64->FP32 "
@@ -349,6 +397,8 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         if args.only == "cpd":
             out = bench_cpd_c1(threads)
+        elif args.only == "c3":
+            out = bench_cpd_c3(args.c3_nnz)
         elif args.only == "stream":
             out = bench_stream_c4(args.stream_slices,
                                   0 if args.skip_cpu else args.stream_cpu_slices)
@@ -427,6 +477,11 @@ def main():
             % (c1["gpu_ms_per_iteration"], c1["cpu_ms_per_iteration"], threads,
                c1["fit_abs_diff"]))
         extra["cpd_c1"] = c1
+        c3 = bench_cpd_c3(args.c3_nnz)
+        log("CPD c3: FP32 %.1f ms/it, FP64 %.1f ms/it, fit diff %.2e, factor errors %s"
+            % (c3["f32_ms_per_iteration"], c3["f64_ms_per_iteration"], c3["fit_abs_diff"],
+               ", ".join("%.1e" % e for e in c3["factor_rel_err_vs_f64"])))
+        extra["cpd_c3"] = c3
         c4 = bench_stream_c4(args.stream_slices, 0 if args.skip_cpu else args.stream_cpu_slices)
         log("stream c4: %.2f slices/s, p50 %.4fs p99 %.4fs" % (
             c4["slices_per_s"], c4["p50_slice_s"], c4["p99_slice_s"]))

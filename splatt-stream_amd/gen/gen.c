@@ -32,16 +32,19 @@ static inline double u01(uint64_t * s)
   return (double) (sm64(s) >> 11) * (1.0 / 9007199254740992.0);
 }
 
-/* LSD radix sort of 64-bit keys, 16-bit digits, OpenMP-parallel with
- * per-thread histograms (stable, so the result is thread-count independent). */
-static void sort_u64(uint64_t * a, uint64_t n, int bits)
+/* Linear cell keys: 128 bits, so tensors with more than 2^64 cells work. */
+typedef unsigned __int128 gkey;
+
+/* LSD radix sort of keys, 16-bit digits, OpenMP-parallel with per-thread
+ * histograms (stable, so the result is thread-count independent). */
+static void sort_keys(gkey * a, uint64_t n, int bits)
 {
   if(n < 2) return;
-  uint64_t * b = malloc(n * sizeof(uint64_t));
+  gkey * b = malloc(n * sizeof(gkey));
   int const T = omp_get_max_threads();
   uint64_t * cnt = malloc((size_t) T * 65536 * sizeof(uint64_t));
-  uint64_t * src = a;
-  uint64_t * dst = b;
+  gkey * src = a;
+  gkey * dst = b;
   for(int sh=0; sh < bits; sh += 16) {
     #pragma omp parallel num_threads(T)
     {
@@ -50,7 +53,7 @@ static void sort_u64(uint64_t * a, uint64_t n, int bits)
       uint64_t const hi = n * (uint64_t) (t + 1) / T;
       uint64_t * c = cnt + (size_t) t * 65536;
       memset(c, 0, 65536 * sizeof(uint64_t));
-      for(uint64_t i=lo; i < hi; ++i) c[(src[i] >> sh) & 0xFFFF]++;
+      for(uint64_t i=lo; i < hi; ++i) c[(uint64_t) (src[i] >> sh) & 0xFFFF]++;
       #pragma omp barrier
       #pragma omp single
       {
@@ -63,11 +66,11 @@ static void sort_u64(uint64_t * a, uint64_t n, int bits)
           }
         }
       }
-      for(uint64_t i=lo; i < hi; ++i) dst[c[(src[i] >> sh) & 0xFFFF]++] = src[i];
+      for(uint64_t i=lo; i < hi; ++i) dst[c[(uint64_t) (src[i] >> sh) & 0xFFFF]++] = src[i];
     }
-    uint64_t * t = src; src = dst; dst = t;
+    gkey * t = src; src = dst; dst = t;
   }
-  if(src != a) memcpy(a, src, n * sizeof(uint64_t));
+  if(src != a) memcpy(a, src, n * sizeof(gkey));
   free(cnt);
   free(b);
 }
@@ -108,11 +111,14 @@ int splatt_gen_coord(int nmodes, uint64_t const * dims, double const * zipf_s,
 {
   unsigned __int128 cells = 1;
   for(int m=0; m < nmodes; ++m) cells *= dims[m];
-  if((unsigned __int128) nnz > cells || cells >= ((unsigned __int128) 1 << 64)) {
+  for(int m=0; m < nmodes; ++m) {
+    if(dims[m] >= ((uint64_t) 1 << 62)) return -1;
+  }
+  if((unsigned __int128) nnz > cells) {
     return -1;
   }
   int bits = 0;
-  while(bits < 64 && ((unsigned __int128) 1 << bits) < cells) ++bits;
+  while(bits < 127 && ((unsigned __int128) 1 << bits) < cells) ++bits;
 
   uint64_t s_coord = seed ^ 0x1234567890ABCDEFull;
   uint64_t s_perm = seed ^ 0x0F1E2D3C4B5A6978ull;
@@ -146,13 +152,13 @@ int splatt_gen_coord(int nmodes, uint64_t const * dims, double const * zipf_s,
    * only on (seed, round, chunk), sorts the new keys, and merges them into
    * the sorted set of distinct keys. The size of the next round follows the
    * fraction of new distinct keys the last round produced. */
-  uint64_t * keys = malloc((nnz + 1) * sizeof(uint64_t));
+  gkey * keys = malloc((nnz + 1) * sizeof(gkey));
   uint64_t have = 0;
   double yield_est = 1.0;
   for(uint64_t round=0; have < nnz; ++round) {
     uint64_t const deficit = nnz - have;
     uint64_t want = (uint64_t) ((double) deficit / yield_est * 1.05) + 1024;
-    uint64_t * fresh = malloc(want * sizeof(uint64_t));
+    gkey * fresh = malloc(want * sizeof(gkey));
     uint64_t const nchunks = (want + GEN_CHUNK - 1) / GEN_CHUNK;
     #pragma omp parallel for schedule(dynamic, 1)
     for(uint64_t ch=0; ch < nchunks; ++ch) {
@@ -160,23 +166,23 @@ int splatt_gen_coord(int nmodes, uint64_t const * dims, double const * zipf_s,
       uint64_t const lo = ch * GEN_CHUNK;
       uint64_t const hi = (lo + GEN_CHUNK < want) ? lo + GEN_CHUNK : want;
       for(uint64_t i=lo; i < hi; ++i) {
-        uint64_t k = 0;
+        gkey k = 0;
         for(int m=0; m < nmodes; ++m) {
           k = k * dims[m] + draw(tabs + m, dims[m], zipf_s[m], &st);
         }
         fresh[i] = k;
       }
     }
-    sort_u64(fresh, want, bits);
+    sort_keys(fresh, want, bits);
     uint64_t u = 0;
     for(uint64_t i=0; i < want; ++i) {
       if(u == 0 || fresh[i] != fresh[u-1]) fresh[u++] = fresh[i];
     }
     /* merge into the sorted set of distinct keys */
-    uint64_t * merged = malloc((have + u) * sizeof(uint64_t));
+    gkey * merged = malloc((have + u) * sizeof(gkey));
     uint64_t i = 0, j = 0, w = 0;
     while(i < have || j < u) {
-      uint64_t v;
+      gkey v;
       if(j >= u || (i < have && keys[i] <= fresh[j])) {
         v = keys[i++];
         if(j < u && fresh[j] == v) ++j;
@@ -208,9 +214,9 @@ int splatt_gen_coord(int nmodes, uint64_t const * dims, double const * zipf_s,
     free(drop);
   }
   for(uint64_t i=nnz; i-- > 0; ) {
-    uint64_t k = keys[i];
+    gkey k = keys[i];
     for(int m=nmodes; m-- > 0; ) {
-      inds[m][i] = k % dims[m];
+      inds[m][i] = (uint64_t) (k % dims[m]);
       k /= dims[m];
     }
   }
