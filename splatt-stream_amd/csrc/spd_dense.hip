@@ -442,17 +442,11 @@ k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
         int const e = t + 64 * (q0 + q);
         int const r = e / RP;
         int const k = e - r * RP;
-        if(r < iR && k < iR) A[r][k] *= gv[q];
+        A[r][k] *= (r < iR && k < iR) ? gv[q] : 1.;   // branch-free: x * 1 == x
       }
     }
   }
-  #pragma unroll
-  for(int q=0; q < NE; ++q) {
-    int const e = t + 64 * q;
-    int const r = e / RP;
-    int const k = e - r * RP;
-    if(r < iR && k == r) A[r][k] += diag_add;
-  }
+  if(t < iR) A[t][t] += diag_add;
   __syncthreads();
   if(admm) {
     if(t == 0) {
@@ -468,24 +462,48 @@ k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
   }
   // the first bad pivot is kept in a register and stored once at the end
   int bad = 0;
+  // Step j: colj = column j (pivot entry zeroed); row j scaled by p = 1/d
+  // with p in the pivot; column j cleared elsewhere. Then every entry takes
+  // the same update A[i][k] -= colj[i] * A[j][k]: row j passes unchanged
+  // (colj[j] = 0), column j becomes -colj[i] * p, the rest is eliminated.
+  // The update runs in chunks whose LDS reads are all issued before use.
+  constexpr int UC = (NE < 16) ? NE : 16;
   for(int j=0; j < RP; ++j) {
     double const d = A[j][j];
     if(t == 0 && j < iR && !(d > 0.) && bad == 0) bad = j + 1;
     double const p = 1. / d;
-    if(t < RP) colj[t] = A[t][j];
+    double cj = 0.;
+    if(t < RP) cj = A[t][j];
     __syncthreads();
-    if(t < RP) A[j][t] = (t == j) ? p : A[j][t] * p;
+    if(t < RP) {
+      colj[t] = (t == j) ? 0. : cj;
+      if(t == j) {
+        A[j][j] = p;
+      } else {
+        A[j][t] = A[j][t] * p;
+        A[t][j] = 0.;
+      }
+    }
     __syncthreads();
     #pragma unroll
-    for(int q=0; q < NE; ++q) {
-      int const e = t + 64 * q;
-      int const i = e / RP;
-      int const k = e - i * RP;
-      double const ci = colj[i];
-      double const ajk = A[j][k];
-      double const aik = A[i][k];
-      double const v = (k == j) ? -ci * p : aik - ci * ajk;
-      if(i != j) A[i][k] = v;
+    for(int q0=0; q0 < NE; q0 += UC) {
+      double ci[UC], ajk[UC], aik[UC];
+      #pragma unroll
+      for(int q=0; q < UC; ++q) {
+        int const e = t + 64 * (q0 + q);
+        int const i = e / RP;
+        int const k = e - i * RP;
+        ci[q] = colj[i];
+        ajk[q] = A[j][k];
+        aik[q] = A[i][k];
+      }
+      #pragma unroll
+      for(int q=0; q < UC; ++q) {
+        int const e = t + 64 * (q0 + q);
+        int const i = e / RP;
+        int const k = e - i * RP;
+        A[i][k] = aik[q] - ci[q] * ajk[q];
+      }
     }
     __syncthreads();
   }
