@@ -3,29 +3,27 @@
 // (src/mttkrp.c:1287-1343 and the kernels at :390-1278).
 //
 // Work decomposition. The nonzeros (leaf level, in CSF order) are cut into
-// items of S consecutive nonzeros. A group of L lanes owns one item; each
-// lane holds V consecutive columns of a factor row, so one group covers
-// L*V columns (a second grid dimension tiles wider ranks). 64/L groups
-// share a wavefront. Items are cut without regard to fiber or slice
-// boundaries, so a giant Zipf slice is spread over many groups and every
-// group does the same amount of leaf work.
+// chunks of 256 consecutive nonzeros without regard to fiber or slice
+// boundaries, so a giant Zipf slice is spread over many chunks and every
+// chunk is the same amount of leaf work. A wavefront owns a chunk; it is
+// split between 64/L groups of L lanes, each lane holding V consecutive
+// columns of a factor row (a second grid dimension tiles wider ranks).
+// Before walking, the wave stages the chunk's structure (node rows, node
+// starts, values) into LDS with coalesced loads, so the walk never waits
+// on a dependent global load: it reads its records from LDS and keeps a
+// batch of factor-row gathers in flight.
 //
-// A group walks its item like the reference's DFS, but iteratively: it
-// keeps the current node and that node's child end at every level; leaf
-// nonzeros accumulate v * U_{N-1}[j] into acc[N-2]; when a node completes,
-// its accumulator is multiplied by the node's factor row and added to the
-// parent's accumulator (root output), or flushed to the output row
-// (internal/leaf output). Partial nodes at item edges are folded the same
-// way (the products are linear).
-//
-// Root output needs no atomics in the walker: a slice that starts and ends
-// inside one item is stored; the partial sums of a slice cut by item edges
-// go to a carry buffer (two rows per item) that a fix-up kernel reduces, so
-// a Zipf slice spread over thousands of items costs a few dozen atomics.
-// Internal and leaf outputs scatter to rows shared by many fibers; the rows
-// with the most contributions (top-K, chosen once per tensor and level) are
-// accumulated per block in LDS with LDS atomics and written once per block,
-// the remaining (cold) rows take global atomics.
+// 3-mode root output (k_root3) follows the reference's CSF factoring: the
+// leaf sum of a fiber is multiplied by the fiber's row once per nonzero
+// gather pair. Every other output (k_walks) multiplies, per nonzero, the
+// rows of all levels but the output level and sums runs of equal output
+// rows (contiguous in CSF order) in registers. Root rows are unique per
+// node: runs closed inside a group's range are stored, runs cut by range
+// edges are added atomically. Internal and leaf outputs scatter to rows
+// shared by many nodes; the rows with the most contributions (top-K,
+// chosen once per tensor and level) are accumulated per block in LDS with
+// LDS atomics and written once per block, the remaining (cold) rows take
+// global atomics.
 #include "spd_internal.hpp"
 #include "spd_csf_dev.hpp"
 #include <stdlib.h>
@@ -80,280 +78,7 @@ vadd_atomic(T * base, uint32_t row, uint32_t R, uint32_t col, bool ok,
   }
 }
 
-template<typename T>
-struct WalkArgs {
-  uint32_t const * fptr[SPD_MAX_MODES];
-  uint32_t const * fids[SPD_MAX_MODES];
-  T const * vals;
-  T const * um[SPD_MAX_MODES];    // factor of the mode stored at each level
-  T * out;
-  uint32_t const * plan;          // per item: node at levels 0..N-2
-  uint8_t const * head;           // per item: 1 if its first slice began earlier
-  T * carry;                      // root: 2 partial rows per item (R wide)
-  uint32_t * carry_row;           // root: output row of each partial, or NONE
-  int32_t const * hot_slot;       // internal/leaf: LDS slot of an output row, -1 cold
-  uint32_t const * hot_rows;      // internal/leaf: output row of each LDS slot
-  uint32_t nhot;                  // internal/leaf: rows privatised in LDS
-  uint64_t nnz;
-  uint32_t R;
-  uint32_t S;
-  uint32_t nitems;
-  int depth;                      // output level (internal kernels)
-  int rows_are_positions;         // root row = slice position (untiled host CSF)
-};
-
 #define CARRY_NONE 0xFFFFFFFFu
-
-// Output of a completed or partial node (internal/leaf kernels): rows with
-// an LDS slot accumulate there with LDS atomics, the rest go to HBM.
-template<typename T, int V, int L>
-__device__ __forceinline__ void
-flush_priv(WalkArgs<T> const & a, T * lds, uint32_t row, uint32_t col, uint32_t c,
-    bool cok, typename VecT<T, V>::type v)
-{
-  int32_t const slot = (a.nhot > 0) ? a.hot_slot[row] : -1;
-  if(slot >= 0 && (uint32_t) slot < a.nhot) {
-    if(!cok) return;
-    T * p = lds + (uint32_t) slot * (L * V) + c * V;
-    if constexpr (V == 1) {
-      atomicAdd(p, v);
-    } else {
-      #pragma unroll
-      for(int k=0; k < V; ++k) atomicAdd(p + k, v[k]);
-    }
-  } else {
-    vadd_atomic<T, V>(a.out, row, a.R, col, cok, v);
-  }
-}
-
-template<typename T, int N, int KIND, int V, int L>
-__device__ __forceinline__ void
-walk_item(WalkArgs<T> const & a, uint32_t item, uint32_t c, T * lds)
-{
-  typedef typename VecT<T, V>::type VT;
-  constexpr int NL = N - 1;        // internal levels 0..N-2
-  uint32_t const col = blockIdx.y * (L * V) + c * V;
-  bool const cok = col < a.R;
-  uint32_t const R = a.R;
-
-  uint64_t const n0 = (uint64_t) item * a.S;
-  uint64_t n1 = n0 + a.S;
-  if(n1 > a.nnz) n1 = a.nnz;
-
-  uint32_t node[NL];
-  uint32_t cend[NL];
-  #pragma unroll
-  for(int l=0; l < NL; ++l) {
-    node[l] = a.plan[(uint64_t) item * NL + l];
-  }
-  #pragma unroll
-  for(int l=0; l < NL; ++l) {
-    cend[l] = a.fptr[l][node[l] + 1];
-  }
-  bool head_partial = a.head[item] != 0;
-  int const depth = (KIND == KIND_ROOT) ? 0 : ((KIND == KIND_LEAF) ? N - 1 : a.depth);
-  uint32_t carry0 = CARRY_NONE;   // root: row of the completed partial slice
-  uint32_t carry1 = CARRY_NONE;   // root: row of the unfinished last slice
-
-  VT acc[NL];
-  #pragma unroll
-  for(int l=0; l < NL; ++l) acc[l] = vzero<VT>();
-
-  // pre[l] = product of factor rows of the current nodes at levels 0..l,
-  // kept for levels below the output level (internal/leaf outputs).
-  VT pre[NL];
-  if constexpr (KIND != KIND_ROOT) {
-    #pragma unroll
-    for(int l=0; l < NL; ++l) {
-      if(l < depth) {
-        uint32_t const row = (l == 0) ? (a.rows_are_positions ? node[0] : a.fids[0][node[0]])
-                                      : a.fids[l][node[l]];
-        VT const r = vload<T, V>(a.um[l], row, R, col, cok);
-        pre[l] = (l == 0) ? r : pre[l > 0 ? l - 1 : 0] * r;
-      }
-    }
-  }
-
-  uint64_t n = n0;
-  int stop = NL;   // levels >= stop hold nothing pending at the end
-  while(true) {
-    uint64_t const e = (cend[NL - 1] < n1) ? cend[NL - 1] : n1;
-    if constexpr (KIND == KIND_LEAF) {
-      VT const h = pre[NL - 1];
-      for(; n < e; ++n) {
-        T const v = a.vals[n];
-        uint32_t const j = a.fids[N - 1][n];
-        flush_priv<T, V, L>(a, lds, j, col, c, cok, v * h);
-      }
-    } else {
-      T const * __restrict__ ul = a.um[N - 1];
-      uint32_t const * __restrict__ lf = a.fids[N - 1];
-      T const * __restrict__ vv = a.vals;
-      VT s = acc[NL - 1];
-      for(; n + 4 <= e; n += 4) {
-        T const v0 = vv[n], v1 = vv[n + 1], v2 = vv[n + 2], v3 = vv[n + 3];
-        uint32_t const j0 = lf[n], j1 = lf[n + 1], j2 = lf[n + 2], j3 = lf[n + 3];
-        VT const b0 = vload<T, V>(ul, j0, R, col, cok);
-        VT const b1 = vload<T, V>(ul, j1, R, col, cok);
-        VT const b2 = vload<T, V>(ul, j2, R, col, cok);
-        VT const b3 = vload<T, V>(ul, j3, R, col, cok);
-        s += v0 * b0;
-        s += v1 * b1;
-        s += v2 * b2;
-        s += v3 * b3;
-      }
-      for(; n < e; ++n) {
-        T const v = vv[n];
-        uint32_t const j = lf[n];
-        s += v * vload<T, V>(ul, j, R, col, cok);
-      }
-      acc[NL - 1] = s;
-    }
-    if(n < cend[NL - 1]) {
-      // the item ends inside this leaf fiber
-      stop = NL;
-      break;
-    }
-    // the leaf fiber node[NL-1] is complete: walk up through completed nodes
-    bool go = true;
-    int top = NL - 1;
-    #pragma unroll
-    for(int l=NL - 1; l >= 0; --l) {
-      if(go) {
-        top = l;
-        if constexpr (KIND == KIND_ROOT) {
-          if(l == 0) {
-            uint32_t const row = a.rows_are_positions ? node[0] : a.fids[0][node[0]];
-            if(head_partial) {
-              // began in an earlier item: hand the partial to the fix-up
-              vstore<T, V>(a.carry, 2 * item, R, col, cok, acc[0]);
-              carry0 = row;
-            } else {
-              vstore<T, V>(a.out, row, R, col, cok, acc[0]);
-            }
-            head_partial = false;
-            acc[0] = vzero<VT>();
-            node[0] += 1;
-            go = false;
-          } else {
-            VT const r = vload<T, V>(a.um[l], a.fids[l][node[l]], R, col, cok);
-            acc[l - 1] += acc[l] * r;
-            acc[l] = vzero<VT>();
-            node[l] += 1;
-            go = (node[l] == cend[l - 1]);
-          }
-        } else if constexpr (KIND == KIND_INTL) {
-          if(l > depth) {
-            VT const r = vload<T, V>(a.um[l], a.fids[l][node[l]], R, col, cok);
-            acc[l - 1] += acc[l] * r;
-            acc[l] = vzero<VT>();
-          } else if(l == depth) {
-            flush_priv<T, V, L>(a, lds, a.fids[l][node[l]], col, c, cok,
-                acc[l] * pre[l > 0 ? l - 1 : 0]);
-            acc[l] = vzero<VT>();
-          }
-          node[l] += 1;
-          go = (l > 0) && (node[l] == cend[l > 0 ? l - 1 : 0]);
-        } else {
-          node[l] += 1;
-          go = (l > 0) && (node[l] == cend[l > 0 ? l - 1 : 0]);
-        }
-      }
-    }
-    if(n >= n1) {
-      stop = top;
-      break;
-    }
-    // refresh child ends (and prefix rows) of the levels whose node moved
-    #pragma unroll
-    for(int l=0; l < NL; ++l) {
-      if(l >= top) {
-        cend[l] = a.fptr[l][node[l] + 1];
-      }
-    }
-    if constexpr (KIND != KIND_ROOT) {
-      #pragma unroll
-      for(int l=0; l < NL; ++l) {
-        if(l >= top && l < depth) {
-          uint32_t const row = (l == 0) ? (a.rows_are_positions ? node[0] : a.fids[0][node[0]])
-                                        : a.fids[l][node[l]];
-          VT const r = vload<T, V>(a.um[l], row, R, col, cok);
-          pre[l] = (l == 0) ? r : pre[l > 0 ? l - 1 : 0] * r;
-        }
-      }
-    }
-  }
-
-  // flush what is pending at the item end: levels < stop hold partial sums
-  if constexpr (KIND == KIND_ROOT) {
-    if(stop >= 1) {
-      #pragma unroll
-      for(int l=NL - 1; l >= 1; --l) {
-        if(l < stop) {
-          VT const r = vload<T, V>(a.um[l], a.fids[l][node[l]], R, col, cok);
-          acc[l - 1] += acc[l] * r;
-        }
-      }
-      carry1 = a.rows_are_positions ? node[0] : a.fids[0][node[0]];
-      vstore<T, V>(a.carry, 2 * item + 1, R, col, cok, acc[0]);
-    }
-    if(c == 0 && blockIdx.y == 0) {
-      a.carry_row[2 * item] = carry0;
-      a.carry_row[2 * item + 1] = carry1;
-    }
-  } else if constexpr (KIND == KIND_INTL) {
-    if(stop > depth) {
-      #pragma unroll
-      for(int l=NL - 1; l >= 1; --l) {
-        if(l < stop && l > depth) {
-          VT const r = vload<T, V>(a.um[l], a.fids[l][node[l]], R, col, cok);
-          acc[l - 1] += acc[l] * r;
-        }
-      }
-      #pragma unroll
-      for(int l=0; l < NL; ++l) {
-        if(l == depth) {
-          flush_priv<T, V, L>(a, lds, a.fids[l][node[l]], col, c, cok,
-              acc[l] * pre[l > 0 ? l - 1 : 0]);
-        }
-      }
-    }
-  }
-}
-
-template<typename T, int N, int KIND, int V, int L>
-__global__ void __launch_bounds__(MK_THREADS)
-k_walk(WalkArgs<T> a)
-{
-  constexpr int G = 64 / L;
-  constexpr int CW = L * V;        // columns of this column tile
-  int const lane = threadIdx.x & 63;
-  uint32_t const c = lane % L;
-  uint32_t const g = lane / L;
-  uint32_t const item = ((uint32_t) blockIdx.x * (MK_THREADS / 64) + (threadIdx.x >> 6)) * G + g;
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  T * lds = (T *) lds_raw;
-  if constexpr (KIND != KIND_ROOT) {
-    for(uint32_t x=threadIdx.x; x < a.nhot * CW; x += MK_THREADS) lds[x] = (T) 0;
-    __syncthreads();
-  }
-  if(item < a.nitems) {
-    walk_item<T, N, KIND, V, L>(a, item, c, lds);
-  }
-  if constexpr (KIND != KIND_ROOT) {
-    __syncthreads();
-    // hot rows accumulated by this block go to HBM once per block
-    for(uint32_t x=threadIdx.x; x < a.nhot * CW; x += MK_THREADS) {
-      uint32_t const slot = x / CW;
-      uint32_t const cc = x % CW;
-      uint32_t const colx = blockIdx.y * CW + cc;
-      T const v = lds[x];
-      if(colx < a.R && v != (T) 0) {
-        vatomic(a.out + (uint64_t) a.hot_rows[slot] * a.R + colx, v);
-      }
-    }
-  }
-}
 
 // Root fix-up: sums the partial rows of slices that spanned several items.
 // Each group of L lanes owns a chunk of FIX_CHUNK consecutive carry entries;
@@ -731,6 +456,297 @@ k_root3(Root3Args<T> a)
   }
 }
 
+// ---- staged walk: internal and leaf outputs, and roots of 4+ modes --------
+//
+// out[r] = sum over nonzeros n whose output-level node has row r of
+//          v_n * prod_{levels l != output} U_l[row of n's level-l node].
+// CSF order keeps the nonzeros of one output-level node contiguous, so a
+// group sums a run of equal output rows in registers and flushes it once.
+//
+// A wave owns chunks of 256 consecutive nonzeros (persistent blocks stride
+// over the chunks). Per chunk it stages into LDS, with coalesced loads, the
+// row ids and child pointers of every node of every level that overlaps the
+// chunk. Node starts are resolved bottom-up (a node starts where its first
+// child starts), marked per position and turned into local node indices with
+// wave prefix sums; each position then gets a record: the row of each
+// gathered level, the output row (and its LDS slot, if privatised) and the
+// value. Groups of L lanes then walk S = 256 / (64/L) positions each in
+// batches of B: B records are read from LDS, B x (N-1) factor-row gathers
+// are in flight at once, and no step waits on a dependent global load.
+//
+// Root outputs (rows unique per node): runs closed inside a group's range
+// are stored, the first and last run of a range are added atomically (the
+// output is zeroed first). Internal and leaf outputs: rows repeat across
+// nodes, so every run is added, in LDS for the most frequent rows (flushed
+// once per block) and with global atomics for the rest.
+#define WS_WAVES 4
+#define WS_WN 256
+#define WS_SENT 0x7FFFFFFFu
+#define WS_BLOCKS 1024
+#define WS_HOT_BYTES (32 * 1024)
+
+template<typename T>
+struct WsArgs {
+  uint32_t const * fptr[SPD_MAX_MODES];
+  uint32_t const * fids[SPD_MAX_MODES];
+  uint32_t nfibs[SPD_MAX_MODES];
+  T const * vals;
+  T const * gm[SPD_MAX_MODES];    // factor of the g-th gathered level, levels in order
+  T * out;
+  uint32_t const * plan;          // per chunk: node at levels 0..N-2 holding its first nonzero
+  int32_t const * hot_slot;       // internal/leaf: LDS slot of an output row (>= nhot: cold)
+  uint32_t const * hot_rows;      // internal/leaf: output row of each LDS slot
+  uint32_t nhot;
+  uint64_t nnz;
+  uint32_t nchunks;
+  uint32_t R;
+  int depth;                      // output level (internal kernels)
+  int rows_are_positions;         // level-0 row = node index
+};
+
+template<typename T, int V>
+__device__ __forceinline__ typename VecT<T, V>::type
+vload_c(T const * __restrict__ base, uint32_t row, uint32_t R, uint32_t colc)
+{
+  typedef typename VecT<T, V>::type VT;
+  return *(VT const *) (base + (uint64_t) row * R + colc);
+}
+
+template<typename T, int N, int KIND, int V, int L, int B>
+__global__ void __launch_bounds__(WS_WAVES * 64)
+k_walks(WsArgs<T> a)
+{
+  typedef typename VecT<T, V>::type VT;
+  constexpr int NL = N - 1;                   // internal levels 0..N-2
+  constexpr int NG = N - 1;                   // gathered levels per nonzero
+  constexpr int G = 64 / L;
+  constexpr int S = WS_WN / G;
+  constexpr int CW = L * V;
+  constexpr int PER = WS_WN / 64;
+  constexpr int HS = (KIND == KIND_ROOT) ? 1 : WS_WN;
+  __shared__ uint32_t s_row[WS_WAVES][NL][WS_WN];   // row id of local node k
+  // child pointers, then node start positions, then gathered row ids
+  __shared__ uint32_t s_tmp[WS_WAVES][NL][WS_WN];
+  __shared__ uint8_t s_mk[WS_WAVES][NL][WS_WN];     // node-start marks
+  __shared__ uint32_t s_orow[WS_WAVES][WS_WN];      // output row per position
+  __shared__ int32_t s_osl[WS_WAVES][HS];           // its LDS slot
+  __shared__ int32_t s_nsl[WS_WAVES][HS];           // slot of output-level node k
+  __shared__ T s_val[WS_WAVES][WS_WN];
+  extern __shared__ __align__(16) unsigned char ws_dyn[];
+  T * hot = (T *) ws_dyn;
+
+  int const lane = threadIdx.x & 63;
+  int const wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  uint32_t const c = lane % L;
+  uint32_t const g = lane / L;
+  uint32_t const R = a.R;
+  uint32_t const col = blockIdx.y * CW + c * V;
+  bool const cok = col < R;
+  uint32_t const colc = cok ? col : 0u;      // lanes past the last column read column 0
+  int const d = (KIND == KIND_ROOT) ? 0 : ((KIND == KIND_LEAF) ? NL : a.depth);
+  uint32_t const nhot = (KIND == KIND_ROOT) ? 0u : a.nhot;
+
+  if constexpr (KIND != KIND_ROOT) {
+    for(uint32_t x=threadIdx.x; x < nhot * CW; x += WS_WAVES * 64) hot[x] = (T) 0;
+    __syncthreads();
+  }
+
+  for(uint32_t ch = blockIdx.x * WS_WAVES + wave; ch < a.nchunks; ch += gridDim.x * WS_WAVES) {
+    uint64_t const W0 = (uint64_t) ch * WS_WN;
+    uint64_t const W1 = (W0 + WS_WN < a.nnz) ? W0 + WS_WN : a.nnz;
+    uint32_t const wn = (uint32_t) (W1 - W0);
+    // nodes overlapping the chunk: from the node holding W0 to the node
+    // holding the next chunk's first nonzero (at most wn of them)
+    uint32_t a0[NL], cnt[NL];
+    #pragma unroll
+    for(int l=0; l < NL; ++l) {
+      a0[l] = a.plan[(uint64_t) ch * NL + l];
+      uint32_t const last = (ch + 1 < a.nchunks) ? a.plan[(uint64_t) (ch + 1) * NL + l]
+                                                 : a.nfibs[l] - 1;
+      uint32_t const n = last - a0[l] + 1;
+      cnt[l] = n < wn ? n : wn;
+    }
+    #pragma unroll
+    for(int l=0; l < NL; ++l) {
+      for(uint32_t k=lane; k < cnt[l]; k += 64) {
+        uint32_t const node = a0[l] + k;
+        uint32_t const row = (l == 0 && a.rows_are_positions) ? node : a.fids[l][node];
+        s_row[wave][l][k] = row;
+        s_tmp[wave][l][k] = a.fptr[l][node];
+        if constexpr (KIND == KIND_INTL) {
+          if(l == d) s_nsl[wave][k] = (nhot > 0) ? a.hot_slot[row] : -1;
+        }
+      }
+      for(uint32_t p=lane; p < WS_WN; p += 64) s_mk[wave][l][p] = 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // node starts, bottom-up; local node k >= 1 starts inside the chunk
+    // (node 0 holds W0) or at or after W1 (then WS_SENT)
+    for(uint32_t k=lane; k < cnt[NL - 1]; k += 64) {
+      uint32_t const st = s_tmp[wave][NL - 1][k];
+      uint32_t pos = WS_SENT;
+      if(k > 0 && (uint64_t) st > W0 && (uint64_t) st < W1) {
+        pos = (uint32_t) (st - W0);
+        s_mk[wave][NL - 1][pos] = 1;
+      }
+      s_tmp[wave][NL - 1][k] = pos;
+    }
+    __builtin_amdgcn_wave_barrier();
+    #pragma unroll
+    for(int l=NL - 2; l >= 0; --l) {
+      for(uint32_t k=lane; k < cnt[l]; k += 64) {
+        uint32_t const ci = s_tmp[wave][l][k] - a0[l + 1];   // first child, local
+        uint32_t pos = WS_SENT;
+        if(k > 0 && ci < cnt[l + 1]) pos = s_tmp[wave][l + 1][ci];
+        if(pos < wn) s_mk[wave][l][pos] = 1;
+        s_tmp[wave][l][k] = pos;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    // local node index of every position at every level
+    uint32_t idx[NL][PER];
+    #pragma unroll
+    for(int l=0; l < NL; ++l) {
+      uint32_t m[PER];
+      uint32_t sum = 0;
+      #pragma unroll
+      for(int q=0; q < PER; ++q) {
+        m[q] = s_mk[wave][l][lane * PER + q];
+        sum += m[q];
+      }
+      uint32_t pre = wave_incl_scan(sum) - sum;
+      #pragma unroll
+      for(int q=0; q < PER; ++q) {
+        pre += m[q];
+        idx[l][q] = pre;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // records
+    #pragma unroll
+    for(int q=0; q < PER; ++q) {
+      uint32_t const p = lane * PER + q;
+      if(p < wn) {
+        uint32_t const leaf = a.fids[N - 1][W0 + p];
+        T const v = a.vals[W0 + p];
+        uint32_t orow = leaf;
+        int32_t osl = -1;
+        #pragma unroll
+        for(int l=0; l < NL; ++l) {
+          uint32_t const r = s_row[wave][l][idx[l][q]];
+          if(l == d) {
+            orow = r;
+            if constexpr (KIND == KIND_INTL) osl = s_nsl[wave][idx[l][q]];
+          } else {
+            s_tmp[wave][l - (l > d ? 1 : 0)][p] = r;
+          }
+        }
+        if constexpr (KIND == KIND_LEAF) {
+          osl = (nhot > 0) ? a.hot_slot[leaf] : -1;
+        } else {
+          s_tmp[wave][NG - 1][p] = leaf;
+        }
+        s_orow[wave][p] = orow;
+        if constexpr (KIND != KIND_ROOT) s_osl[wave][p] = osl;
+        s_val[wave][p] = v;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- walk this group's positions ----
+    uint32_t const k0 = g * S;
+    uint32_t const k1 = (k0 + S < wn) ? k0 + S : wn;
+    if(k0 < k1) {
+      uint32_t cur = s_orow[wave][k0];
+      int32_t csl = -1;
+      if constexpr (KIND != KIND_ROOT) csl = s_osl[wave][k0];
+      VT acc = vzero<VT>();
+      bool first = true;
+      auto flush = [&](bool edge) {
+        if constexpr (KIND == KIND_ROOT) {
+          if(edge) vadd_atomic<T, V>(a.out, cur, R, col, cok, acc);
+          else vstore<T, V>(a.out, cur, R, col, cok, acc);
+        } else {
+          if(csl >= 0 && (uint32_t) csl < nhot) {
+            if(cok) {
+              T * p = hot + (uint32_t) csl * CW + c * V;
+              if constexpr (V == 1) {
+                atomicAdd(p, acc);
+              } else {
+                #pragma unroll
+                for(int k=0; k < V; ++k) atomicAdd(p + k, acc[k]);
+              }
+            }
+          } else {
+            vadd_atomic<T, V>(a.out, cur, R, col, cok, acc);
+          }
+        }
+      };
+      auto step = [&](uint32_t orow, int32_t osl, VT prod) {
+        if(orow != cur) {
+          flush(first);
+          first = false;
+          cur = orow;
+          csl = osl;
+          acc = vzero<VT>();
+        }
+        acc += prod;
+      };
+      uint32_t k = k0;
+      for(; k + B <= k1; k += B) {
+        uint32_t gid[B][NG];
+        uint32_t orw[B];
+        int32_t osl[B];
+        T v[B];
+        #pragma unroll
+        for(int j=0; j < B; ++j) {
+          #pragma unroll
+          for(int q=0; q < NG; ++q) gid[j][q] = s_tmp[wave][q][k + j];
+          orw[j] = s_orow[wave][k + j];
+          osl[j] = (KIND != KIND_ROOT) ? s_osl[wave][(KIND != KIND_ROOT) ? k + j : 0] : -1;
+          v[j] = s_val[wave][k + j];
+        }
+        VT b[B][NG];
+        #pragma unroll
+        for(int j=0; j < B; ++j) {
+          #pragma unroll
+          for(int q=0; q < NG; ++q) b[j][q] = vload_c<T, V>(a.gm[q], gid[j][q], R, colc);
+        }
+        #pragma unroll
+        for(int j=0; j < B; ++j) {
+          VT pr = v[j] * b[j][0];
+          #pragma unroll
+          for(int q=1; q < NG; ++q) pr *= b[j][q];
+          step(orw[j], osl[j], pr);
+        }
+      }
+      for(; k < k1; ++k) {
+        VT pr = s_val[wave][k] * vload_c<T, V>(a.gm[0], s_tmp[wave][0][k], R, colc);
+        #pragma unroll
+        for(int q=1; q < NG; ++q) pr *= vload_c<T, V>(a.gm[q], s_tmp[wave][q][k], R, colc);
+        step(s_orow[wave][k], (KIND != KIND_ROOT) ? s_osl[wave][(KIND != KIND_ROOT) ? k : 0] : -1, pr);
+      }
+      flush(true);
+    }
+    // the next chunk's staging overwrites these records
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  if constexpr (KIND != KIND_ROOT) {
+    __syncthreads();
+    // privatised rows accumulated by this block go to HBM once per block
+    for(uint32_t x=threadIdx.x; x < nhot * CW; x += WS_WAVES * 64) {
+      uint32_t const slot = x / CW;
+      uint32_t const cc = x % CW;
+      uint32_t const colx = blockIdx.y * CW + cc;
+      T const v = hot[x];
+      if(colx < R && v != (T) 0) {
+        vatomic(a.out + (uint64_t) a.hot_rows[slot] * R + colx, v);
+      }
+    }
+  }
+}
+
 // ---- plans ----------------------------------------------------------------
 
 struct MttkrpPlan {
@@ -942,59 +958,50 @@ static int get_plan(spd_csf * c, uint32_t S, MttkrpPlan ** out, hipStream_t st)
 
 // ---- dispatch -------------------------------------------------------------
 
-// LDS for privatised output rows, per block.
-#define HOT_LDS_BYTES (48 * 1024)
-
 template<typename T, int N, int KIND, int V, int L>
-static int launch_walk(WalkArgs<T> a, hipStream_t st)
+static int launch_walks(WsArgs<T> a, hipStream_t st)
 {
-  constexpr int G = 64 / L;
   constexpr int CW = L * V;
-  uint32_t const groups_per_block = (MK_THREADS / 64) * G;
-  dim3 grid((unsigned) spd_cdiv(a.nitems, groups_per_block),
-            (unsigned) spd_cdiv(a.R, (uint64_t) CW));
+  constexpr int B = (N <= 4) ? 8 : 4;
+  uint32_t const ny = (uint32_t) spd_cdiv(a.R, (uint64_t) CW);
+  uint64_t nb = spd_cdiv(a.nchunks, WS_WAVES);
+  if(nb > WS_BLOCKS) nb = WS_BLOCKS;
+  if(nb == 0) return SPD_OK;
   size_t shmem = 0;
   if constexpr (KIND != KIND_ROOT) {
-    uint32_t const cap = HOT_LDS_BYTES / (CW * sizeof(T));
+    uint32_t const cap = WS_HOT_BYTES / (CW * sizeof(T));
     if(a.nhot > cap) a.nhot = cap;
     shmem = (size_t) a.nhot * CW * sizeof(T);
   }
-  hipLaunchKernelGGL((k_walk<T, N, KIND, V, L>), grid, dim3(MK_THREADS), shmem, st, a);
+  hipLaunchKernelGGL((k_walks<T, N, KIND, V, L, B>), dim3((unsigned) nb, ny),
+      dim3(WS_WAVES * 64), shmem, st, a);
   SPD_LAUNCH_CHECK();
-  if constexpr (KIND == KIND_ROOT) {
-    uint32_t const nent = 2 * a.nitems;
-    uint32_t const chunks = (uint32_t) spd_cdiv(nent, FIX_CHUNK);
-    dim3 g2((unsigned) spd_cdiv(chunks, groups_per_block), grid.y);
-    hipLaunchKernelGGL((k_root_fixup<T, V, L>), g2, dim3(MK_THREADS), 0, st,
-        (T const *) a.carry, (uint32_t const *) a.carry_row, nent, a.out, a.R);
-    SPD_LAUNCH_CHECK();
-  }
   return SPD_OK;
 }
 
 template<typename T, int N, int KIND>
-static int dispatch_vl(WalkArgs<T> const & a, hipStream_t st)
+static int dispatch_vl(WsArgs<T> const & a, hipStream_t st)
 {
   uint32_t const R = a.R;
   if constexpr (sizeof(T) == 4) {
     if(R % 4 == 0 && R >= 8) {
       uint32_t const lanes = R / 4;
-      if(lanes <= 2) return launch_walk<T, N, KIND, 4, 2>(a, st);
-      if(lanes <= 4) return launch_walk<T, N, KIND, 4, 4>(a, st);
-      if(lanes <= 8) return launch_walk<T, N, KIND, 4, 8>(a, st);
-      return launch_walk<T, N, KIND, 4, 16>(a, st);
+      if(lanes <= 2) return launch_walks<T, N, KIND, 4, 2>(a, st);
+      if(lanes <= 4) return launch_walks<T, N, KIND, 4, 4>(a, st);
+      if(lanes <= 8) return launch_walks<T, N, KIND, 4, 8>(a, st);
+      return launch_walks<T, N, KIND, 4, 16>(a, st);
     }
   } else {
     if(R % 2 == 0 && R >= 8) {
       uint32_t const lanes = R / 2;
-      if(lanes <= 4) return launch_walk<T, N, KIND, 2, 4>(a, st);
-      if(lanes <= 8) return launch_walk<T, N, KIND, 2, 8>(a, st);
-      if(lanes <= 16) return launch_walk<T, N, KIND, 2, 16>(a, st);
-      return launch_walk<T, N, KIND, 2, 32>(a, st);
+      if(lanes <= 4) return launch_walks<T, N, KIND, 2, 4>(a, st);
+      if(lanes <= 8) return launch_walks<T, N, KIND, 2, 8>(a, st);
+      if(lanes <= 16) return launch_walks<T, N, KIND, 2, 16>(a, st);
+      return launch_walks<T, N, KIND, 2, 32>(a, st);
     }
   }
-  if(R <= 16) return launch_walk<T, N, KIND, 1, 16>(a, st);
-  return launch_walk<T, N, KIND, 1, 64>(a, st);
+  if(R <= 16) return launch_walks<T, N, KIND, 1, 16>(a, st);
+  return launch_walks<T, N, KIND, 1, 64>(a, st);
 }
 
 template<typename T, int V, int L>
@@ -1061,7 +1068,7 @@ static int dispatch_root3(Root3Args<T> const & a, uint32_t nblocks, int V, int L
 }
 
 template<typename T, int KIND>
-static int dispatch_n(int N, WalkArgs<T> const & a, hipStream_t st)
+static int dispatch_n(int N, WsArgs<T> const & a, hipStream_t st)
 {
   switch(N) {
   case 3: return dispatch_vl<T, 3, KIND>(a, st);
@@ -1074,24 +1081,6 @@ static int dispatch_n(int N, WalkArgs<T> const & a, hipStream_t st)
   }
   spd_set_error(SPD_EINPUT, "unsupported number of modes %d", N);
   return SPD_EINPUT;
-}
-
-// Groups to give each CU several waves even at modest nnz, but never
-// fewer than 64 nonzeros per item.
-static uint32_t choose_S(uint64_t nnz, uint32_t R, int dtype)
-{
-  uint32_t lanes_per_row = (dtype == SPD_F32) ? (R % 4 == 0 && R >= 8 ? R / 4 : 16)
-                                              : (R % 2 == 0 && R >= 8 ? R / 2 : 16);
-  if(lanes_per_row > 64) lanes_per_row = 64;
-  uint32_t L = 1;
-  while(L < lanes_per_row) L <<= 1;
-  uint32_t const G = 64 / L;
-  uint64_t const target_groups = 256ull * 16 * G;   // 16 waves per CU
-  uint64_t S = nnz / target_groups;
-  if(S < 64) S = 64;
-  if(S > 2048) S = 2048;
-  S = (S + 3) & ~3ull;
-  return (uint32_t) S;
 }
 
 // Kernel timing: when enabled, a pair of HIP events is recorded around every
@@ -1216,48 +1205,39 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
       c->dims[c->perm[2]] * R * sizeof(T) <= lim) {
     return run_root3<T>(c, R, mats, out, rows_pos, st, dtype);
   }
-  uint32_t const S = choose_S(c->nnz, (uint32_t) R, dtype);
   MttkrpPlan * plan = nullptr;
   // plans are built on the library stream; make the caller's stream wait
-  SPD_TRY(get_plan(c, S, &plan, spd_hs(nullptr)));
-  if(st != spd_hs(nullptr)) {
-    SPD_HIP(hipStreamSynchronize(spd_hs(nullptr)));
+  SPD_TRY(get_plan(c, WS_WN, &plan, spd_hs(nullptr)));
+  WsArgs<T> a = {};
+  for(int l=0; l < N - 1; ++l) {
+    a.fptr[l] = c->u.fptr[l];
+    a.nfibs[l] = (uint32_t) c->u.nfibs[l];
   }
-  WalkArgs<T> a = {};
-  for(int l=0; l < N - 1; ++l) a.fptr[l] = c->u.fptr[l];
-  for(int l=0; l < N; ++l) {
-    a.fids[l] = c->u.fids[l];
-    a.um[l] = (T const *) mats[c->perm[l]];
+  for(int l=0; l < N; ++l) a.fids[l] = c->u.fids[l];
+  // gathered levels: every level but the output level, in level order
+  for(int l=0, q=0; l < N; ++l) {
+    if(l != depth) a.gm[q++] = (T const *) mats[c->perm[l]];
   }
   a.vals = (sizeof(T) == 4) ? (T const *) c->vals32 : (T const *) c->u.vals;
   a.out = (T *) out;
   a.plan = plan->nodes;
-  a.head = plan->head;
   a.nnz = c->nnz;
+  a.nchunks = plan->nitems;
   a.R = (uint32_t) R;
-  a.S = S;
-  a.nitems = plan->nitems;
   a.depth = depth;
   a.rows_are_positions = rows_pos;
-  if(depth == 0) {
-    void * cv = nullptr;
-    void * cr = nullptr;
-    SPD_TRY(spd_scratch(2, 2ull * plan->nitems * R * sizeof(T), &cv));
-    SPD_TRY(spd_scratch(3, 2ull * plan->nitems * sizeof(uint32_t), &cr));
-    a.carry = (T *) cv;
-    a.carry_row = (uint32_t *) cr;
-  } else {
+  if(depth > 0) {
     HotPlan * hp = nullptr;
     SPD_TRY(get_hot(c, depth, &hp, spd_hs(nullptr)));
-    if(st != spd_hs(nullptr)) {
-      SPD_HIP(hipStreamSynchronize(spd_hs(nullptr)));
-    }
     // privatise rows that collect at least two contributions
     uint32_t nhot = 0;
     while(nhot < hp->counts.size() && hp->counts[nhot] >= 2) ++nhot;
     a.hot_slot = hp->rank;
     a.hot_rows = hp->rows;
     a.nhot = nhot;
+  }
+  if(st != spd_hs(nullptr)) {
+    SPD_HIP(hipStreamSynchronize(spd_hs(nullptr)));
   }
   SPD_TRY(spd_zero_async(out, c->dims[mode] * R * sizeof(T), st));
   int const slot = (int) (g_ev_count % SPD_NEV);

@@ -7,6 +7,7 @@ tests do. It raises if the native library is missing: there is no fallback.
 """
 import ctypes as C
 import os
+import time
 
 import numpy as np
 
@@ -460,7 +461,8 @@ def run_cpd(csfset, rank, max_its=20, tol=0.0, seed=1, configure=None,
             verbosity=0, max_inner=None, inner_tol=None):
     """splatt_alloc_cpd (after srand(seed)) + splatt_cpd through the C API.
     configure(L, opts_ptr) may register constraints. Returns a dict with the
-    initial factors, the result factors, lambda, fit and the return code."""
+    initial factors, the result factors, lambda, fit, the return code and
+    cpd_s, the wall time of the splatt_cpd call alone."""
     L = lib()
     srand(seed)
     K = L.splatt_alloc_cpd(csfset.ptr, rank)
@@ -479,7 +481,9 @@ def run_cpd(csfset, rank, max_its=20, tol=0.0, seed=1, configure=None,
         configure(L, co)
     go = L.splatt_alloc_global_opts()
     go[0].verbosity = verbosity
+    t0 = time.perf_counter()
     rc = L.splatt_cpd(csfset.ptr, rank, co, go, K)
+    cpd_s = time.perf_counter() - t0
     out = [np.ctypeslib.as_array(K[0].factors[m], shape=(dims[m], rank)).copy()
            for m in range(N)]
     lam = np.ctypeslib.as_array(K[0].lambda_, shape=(rank,)).copy()
@@ -487,7 +491,7 @@ def run_cpd(csfset, rank, max_its=20, tol=0.0, seed=1, configure=None,
     L.splatt_free_cpd_opts(co)
     L.splatt_free_global_opts(go)
     L.splatt_free_cpd(K)
-    return dict(init=init, factors=out, lam=lam, fit=fit, rc=rc, dims=dims)
+    return dict(init=init, factors=out, lam=lam, fit=fit, rc=rc, dims=dims, cpd_s=cpd_s)
 
 
 def coord_from_arrays(inds, vals, dims):
