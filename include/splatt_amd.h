@@ -26,6 +26,20 @@ char const * splatt_amd_last_error(void);
  * values, factors and accumulation; Grams, Cholesky and fit stay FP64). */
 splatt_error_type splatt_amd_set_mttkrp_precision(int precision);
 
+#define SPLATT_AMD_LAYOUT_AUTO 0
+#define SPLATT_AMD_LAYOUT_CSF 1
+
+/* How MTTKRP reaches internal and leaf output modes (no counterpart in the
+ * reference, whose mttkrp_csf, src/mttkrp.c:1287-1343, always walks the
+ * CSF with locks or privatisation). SPLATT_AMD_LAYOUT_AUTO (default): on
+ * first use for a mode, a copy of the nonzeros ordered by that mode's row is
+ * built on the device (4 bytes per other mode, 4 for the row and 12 for the
+ * FP64 and FP32 values, per nonzero) when free device memory allows, and
+ * the mode is computed like a root mode, without atomics on its rows.
+ * SPLATT_AMD_LAYOUT_CSF: always walk the CSF, adding to output rows with
+ * atomics (hot rows privatised in LDS). Results agree to rounding. */
+splatt_error_type splatt_amd_set_mttkrp_layout(int layout);
+
 /* MTTKRP with operands already in GPU memory. d_mats[m] are device pointers
  * to row-major dims[m] x ncolumns arrays of `precision` type; d_out
  * (dims[mode] x ncolumns) is overwritten. The CSF must come from this

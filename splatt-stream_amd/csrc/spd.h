@@ -108,6 +108,12 @@ int spd_mttkrp_timing(int enable);
 double spd_mttkrp_last_kernel_ms(void);
 int spd_mttkrp_kernel_times(double * ms, int max);
 
+/* Internal and leaf outputs: enable = 1 (default) walks a row-sorted copy of
+ * the CSF, built on the device on first use for each output level when
+ * device memory allows; enable = 0 always walks the CSF itself, adding to
+ * the output rows atomically. */
+int spd_mttkrp_set_copies(int enable);
+
 /* ---- dense / CPD ------------------------------------------------------- */
 
 /* Convert FP64 device array to FP32 and back. */

@@ -40,6 +40,7 @@ struct spd_csf {
   float * vals32;                  // FP32 copy of u.vals, made on demand
   MttkrpPlan * plans;              // launch plans, one per item size
   struct HotPlan * hot;            // output-row rankings, one per level
+  struct RowCopy * copies;         // row-sorted copies, one per output level
 };
 
 void spd_csf_drop_plans(spd_csf * c);

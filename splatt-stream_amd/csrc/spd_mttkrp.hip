@@ -475,10 +475,21 @@ k_root3(Root3Args<T> a)
 // are in flight at once, and no step waits on a dependent global load.
 //
 // Root outputs (rows unique per node): runs closed inside a group's range
-// are stored, the first and last run of a range are added atomically (the
-// output is zeroed first). Internal and leaf outputs: rows repeat across
-// nodes, so every run is added, in LDS for the most frequent rows (flushed
-// once per block) and with global atomics for the rest.
+// are stored; the first and last run of a range go to a carry buffer (two
+// rows per range) that k_root_fixup sums in order, so a heavy row cut into
+// thousands of ranges costs a few dozen atomics. Internal and leaf outputs:
+// rows repeat across nodes, so every run is added, in LDS for the most
+// frequent rows (flushed once per block) and with global atomics for the
+// rest.
+//
+// Row-sorted copies. For an internal or leaf output the nonzeros can also
+// be walked in the order of their output row: KIND_EXPAND writes each
+// position's record (gathered rows and output row) to HBM, a stable radix
+// sort by output row orders them, and KIND_COPY walks that copy with the
+// root rules (runs are whole output rows: stores and carries, no atomics on
+// the output rows).
+#define KIND_EXPAND 3
+#define KIND_COPY 4
 #define WS_WAVES 4
 #define WS_WN 256
 #define WS_SENT 0x7FFFFFFFu
@@ -491,16 +502,22 @@ struct WsArgs {
   uint32_t const * fids[SPD_MAX_MODES];
   uint32_t nfibs[SPD_MAX_MODES];
   T const * vals;
-  T const * gm[SPD_MAX_MODES];    // factor of the g-th gathered level, levels in order
+  T const * gm[SPD_MAX_MODES];    // factor of the q-th gathered level, levels in order
   T * out;
   uint32_t const * plan;          // per chunk: node at levels 0..N-2 holding its first nonzero
   int32_t const * hot_slot;       // internal/leaf: LDS slot of an output row (>= nhot: cold)
   uint32_t const * hot_rows;      // internal/leaf: output row of each LDS slot
+  T * carry;                      // root/copy: 2 partial rows per range (R wide)
+  uint32_t * carry_row;           // root/copy: output row of each partial, or CARRY_NONE
+  uint32_t * xg[SPD_MAX_MODES];   // expand: gathered rows per position (q-th level)
+  uint32_t * xrow;                // expand: output row per position
+  uint32_t const * cg[SPD_MAX_MODES];   // copy: gathered rows in copy order
+  uint32_t const * crow;          // copy: output rows (ascending)
   uint32_t nhot;
   uint64_t nnz;
   uint32_t nchunks;
   uint32_t R;
-  int depth;                      // output level (internal kernels)
+  int depth;                      // output level (internal, leaf and expand kernels)
   int rows_are_positions;         // level-0 row = node index
 };
 
@@ -512,26 +529,138 @@ vload_c(T const * __restrict__ base, uint32_t row, uint32_t R, uint32_t colc)
   return *(VT const *) (base + (uint64_t) row * R + colc);
 }
 
+// Walk positions [k0, k1) of the staged chunk: s_gid[q][k] gathered rows,
+// s_orow[k] output rows, s_osl[k] LDS slots (internal/leaf), s_val[k]
+// values. KIND is KIND_ROOT (runs unique per row: stores and carries) or
+// KIND_INTL/KIND_LEAF (every run added).
+template<typename T, int NG, int KIND, int V, int L, int B>
+__device__ __forceinline__ void
+walk_range(WsArgs<T> const & a, uint32_t const (*s_gid)[WS_WN], uint32_t const * s_orow,
+    int32_t const * s_osl, T const * s_val, T * hot, uint32_t nhot, uint32_t k0, uint32_t k1,
+    uint32_t item, uint32_t c, uint32_t col, uint32_t colc, bool cok)
+{
+  typedef typename VecT<T, V>::type VT;
+  constexpr int CW = L * V;
+  uint32_t const R = a.R;
+  uint32_t carry0 = CARRY_NONE;
+  uint32_t carry1 = CARRY_NONE;
+  if(k0 < k1) {
+    uint32_t cur = s_orow[k0];
+    int32_t csl = -1;
+    if constexpr (KIND != KIND_ROOT) csl = s_osl[k0];
+    VT acc = vzero<VT>();
+    bool first = true;
+    auto flush = [&]() {
+      if constexpr (KIND == KIND_ROOT) {
+        if(first) {
+          // may have begun in an earlier range
+          carry0 = cur;
+          vstore<T, V>(a.carry, 2 * item, R, col, cok, acc);
+        } else {
+          vstore<T, V>(a.out, cur, R, col, cok, acc);
+        }
+      } else {
+        if(csl >= 0 && (uint32_t) csl < nhot) {
+          if(cok) {
+            T * p = hot + (uint32_t) csl * CW + c * V;
+            if constexpr (V == 1) {
+              atomicAdd(p, acc);
+            } else {
+              #pragma unroll
+              for(int k=0; k < V; ++k) atomicAdd(p + k, acc[k]);
+            }
+          }
+        } else {
+          vadd_atomic<T, V>(a.out, cur, R, col, cok, acc);
+        }
+      }
+    };
+    auto step = [&](uint32_t orow, int32_t osl, VT prod) {
+      if(orow != cur) {
+        flush();
+        first = false;
+        cur = orow;
+        csl = osl;
+        acc = vzero<VT>();
+      }
+      acc += prod;
+    };
+    uint32_t k = k0;
+    for(; k + B <= k1; k += B) {
+      uint32_t gid[B][NG];
+      uint32_t orw[B];
+      int32_t osl[B];
+      T v[B];
+      #pragma unroll
+      for(int j=0; j < B; ++j) {
+        #pragma unroll
+        for(int q=0; q < NG; ++q) gid[j][q] = s_gid[q][k + j];
+        orw[j] = s_orow[k + j];
+        osl[j] = -1;
+        if constexpr (KIND != KIND_ROOT) osl[j] = s_osl[k + j];
+        v[j] = s_val[k + j];
+      }
+      VT b[B][NG];
+      #pragma unroll
+      for(int j=0; j < B; ++j) {
+        #pragma unroll
+        for(int q=0; q < NG; ++q) b[j][q] = vload_c<T, V>(a.gm[q], gid[j][q], R, colc);
+      }
+      #pragma unroll
+      for(int j=0; j < B; ++j) {
+        VT pr = v[j] * b[j][0];
+        #pragma unroll
+        for(int q=1; q < NG; ++q) pr *= b[j][q];
+        step(orw[j], osl[j], pr);
+      }
+    }
+    for(; k < k1; ++k) {
+      VT pr = s_val[k] * vload_c<T, V>(a.gm[0], s_gid[0][k], R, colc);
+      #pragma unroll
+      for(int q=1; q < NG; ++q) pr *= vload_c<T, V>(a.gm[q], s_gid[q][k], R, colc);
+      int32_t osl = -1;
+      if constexpr (KIND != KIND_ROOT) osl = s_osl[k];
+      step(s_orow[k], osl, pr);
+    }
+    if constexpr (KIND == KIND_ROOT) {
+      // the last run may continue in the next range
+      carry1 = cur;
+      vstore<T, V>(a.carry, 2 * item + 1, R, col, cok, acc);
+    } else {
+      flush();
+    }
+  }
+  if constexpr (KIND == KIND_ROOT) {
+    if(c == 0 && blockIdx.y == 0) {
+      a.carry_row[2 * item] = carry0;
+      a.carry_row[2 * item + 1] = carry1;
+    }
+  }
+}
+
 template<typename T, int N, int KIND, int V, int L, int B>
 __global__ void __launch_bounds__(WS_WAVES * 64)
 k_walks(WsArgs<T> a)
 {
-  typedef typename VecT<T, V>::type VT;
   constexpr int NL = N - 1;                   // internal levels 0..N-2
   constexpr int NG = N - 1;                   // gathered levels per nonzero
   constexpr int G = 64 / L;
   constexpr int S = WS_WN / G;
   constexpr int CW = L * V;
   constexpr int PER = WS_WN / 64;
-  constexpr int HS = (KIND == KIND_ROOT) ? 1 : WS_WN;
-  __shared__ uint32_t s_row[WS_WAVES][NL][WS_WN];   // row id of local node k
+  constexpr bool STRUCT = (KIND != KIND_COPY);          // stages the CSF structure
+  constexpr bool PRIV = (KIND == KIND_INTL || KIND == KIND_LEAF);
+  constexpr int WALK = (KIND == KIND_COPY) ? KIND_ROOT : KIND;
+  constexpr int NLS = STRUCT ? NL : 1;
+  constexpr int HS = PRIV ? WS_WN : 1;
+  __shared__ uint32_t s_row[WS_WAVES][NLS][WS_WN];  // row id of local node k
   // child pointers, then node start positions, then gathered row ids
   __shared__ uint32_t s_tmp[WS_WAVES][NL][WS_WN];
-  __shared__ uint8_t s_mk[WS_WAVES][NL][WS_WN];     // node-start marks
+  __shared__ uint8_t s_mk[WS_WAVES][NLS][WS_WN];    // node-start marks
   __shared__ uint32_t s_orow[WS_WAVES][WS_WN];      // output row per position
   __shared__ int32_t s_osl[WS_WAVES][HS];           // its LDS slot
   __shared__ int32_t s_nsl[WS_WAVES][HS];           // slot of output-level node k
-  __shared__ T s_val[WS_WAVES][WS_WN];
+  __shared__ T s_val[WS_WAVES][KIND == KIND_EXPAND ? 1 : WS_WN];
   extern __shared__ __align__(16) unsigned char ws_dyn[];
   T * hot = (T *) ws_dyn;
 
@@ -539,14 +668,13 @@ k_walks(WsArgs<T> a)
   int const wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   uint32_t const c = lane % L;
   uint32_t const g = lane / L;
-  uint32_t const R = a.R;
   uint32_t const col = blockIdx.y * CW + c * V;
-  bool const cok = col < R;
+  bool const cok = col < a.R;
   uint32_t const colc = cok ? col : 0u;      // lanes past the last column read column 0
   int const d = (KIND == KIND_ROOT) ? 0 : ((KIND == KIND_LEAF) ? NL : a.depth);
-  uint32_t const nhot = (KIND == KIND_ROOT) ? 0u : a.nhot;
+  uint32_t const nhot = PRIV ? a.nhot : 0u;
 
-  if constexpr (KIND != KIND_ROOT) {
+  if constexpr (PRIV) {
     for(uint32_t x=threadIdx.x; x < nhot * CW; x += WS_WAVES * 64) hot[x] = (T) 0;
     __syncthreads();
   }
@@ -555,184 +683,135 @@ k_walks(WsArgs<T> a)
     uint64_t const W0 = (uint64_t) ch * WS_WN;
     uint64_t const W1 = (W0 + WS_WN < a.nnz) ? W0 + WS_WN : a.nnz;
     uint32_t const wn = (uint32_t) (W1 - W0);
-    // nodes overlapping the chunk: from the node holding W0 to the node
-    // holding the next chunk's first nonzero (at most wn of them)
-    uint32_t a0[NL], cnt[NL];
-    #pragma unroll
-    for(int l=0; l < NL; ++l) {
-      a0[l] = a.plan[(uint64_t) ch * NL + l];
-      uint32_t const last = (ch + 1 < a.nchunks) ? a.plan[(uint64_t) (ch + 1) * NL + l]
-                                                 : a.nfibs[l] - 1;
-      uint32_t const n = last - a0[l] + 1;
-      cnt[l] = n < wn ? n : wn;
-    }
-    #pragma unroll
-    for(int l=0; l < NL; ++l) {
-      for(uint32_t k=lane; k < cnt[l]; k += 64) {
-        uint32_t const node = a0[l] + k;
-        uint32_t const row = (l == 0 && a.rows_are_positions) ? node : a.fids[l][node];
-        s_row[wave][l][k] = row;
-        s_tmp[wave][l][k] = a.fptr[l][node];
-        if constexpr (KIND == KIND_INTL) {
-          if(l == d) s_nsl[wave][k] = (nhot > 0) ? a.hot_slot[row] : -1;
+    if constexpr (!STRUCT) {
+      // a row-sorted copy: records are stored as they are walked
+      #pragma unroll
+      for(int q=0; q < NG; ++q) {
+        for(uint32_t p=lane; p < wn; p += 64) s_tmp[wave][q][p] = a.cg[q][W0 + p];
+      }
+      for(uint32_t p=lane; p < wn; p += 64) {
+        s_orow[wave][p] = a.crow[W0 + p];
+        s_val[wave][p] = a.vals[W0 + p];
+      }
+    } else {
+      // nodes overlapping the chunk: from the node holding W0 to the node
+      // holding the next chunk's first nonzero (at most wn of them)
+      uint32_t a0[NL], cnt[NL];
+      #pragma unroll
+      for(int l=0; l < NL; ++l) {
+        a0[l] = a.plan[(uint64_t) ch * NL + l];
+        uint32_t const last = (ch + 1 < a.nchunks) ? a.plan[(uint64_t) (ch + 1) * NL + l]
+                                                   : a.nfibs[l] - 1;
+        uint32_t const n = last - a0[l] + 1;
+        cnt[l] = n < wn ? n : wn;
+      }
+      #pragma unroll
+      for(int l=0; l < NL; ++l) {
+        for(uint32_t k=lane; k < cnt[l]; k += 64) {
+          uint32_t const node = a0[l] + k;
+          uint32_t const row = (l == 0 && a.rows_are_positions) ? node : a.fids[l][node];
+          s_row[wave][l][k] = row;
+          s_tmp[wave][l][k] = a.fptr[l][node];
+          if constexpr (KIND == KIND_INTL) {
+            if(l == d) s_nsl[wave][k] = (nhot > 0) ? a.hot_slot[row] : -1;
+          }
         }
-      }
-      for(uint32_t p=lane; p < WS_WN; p += 64) s_mk[wave][l][p] = 0;
-    }
-    __builtin_amdgcn_wave_barrier();
-    // node starts, bottom-up; local node k >= 1 starts inside the chunk
-    // (node 0 holds W0) or at or after W1 (then WS_SENT)
-    for(uint32_t k=lane; k < cnt[NL - 1]; k += 64) {
-      uint32_t const st = s_tmp[wave][NL - 1][k];
-      uint32_t pos = WS_SENT;
-      if(k > 0 && (uint64_t) st > W0 && (uint64_t) st < W1) {
-        pos = (uint32_t) (st - W0);
-        s_mk[wave][NL - 1][pos] = 1;
-      }
-      s_tmp[wave][NL - 1][k] = pos;
-    }
-    __builtin_amdgcn_wave_barrier();
-    #pragma unroll
-    for(int l=NL - 2; l >= 0; --l) {
-      for(uint32_t k=lane; k < cnt[l]; k += 64) {
-        uint32_t const ci = s_tmp[wave][l][k] - a0[l + 1];   // first child, local
-        uint32_t pos = WS_SENT;
-        if(k > 0 && ci < cnt[l + 1]) pos = s_tmp[wave][l + 1][ci];
-        if(pos < wn) s_mk[wave][l][pos] = 1;
-        s_tmp[wave][l][k] = pos;
+        for(uint32_t p=lane; p < WS_WN; p += 64) s_mk[wave][l][p] = 0;
       }
       __builtin_amdgcn_wave_barrier();
-    }
-    // local node index of every position at every level
-    uint32_t idx[NL][PER];
-    #pragma unroll
-    for(int l=0; l < NL; ++l) {
-      uint32_t m[PER];
-      uint32_t sum = 0;
-      #pragma unroll
-      for(int q=0; q < PER; ++q) {
-        m[q] = s_mk[wave][l][lane * PER + q];
-        sum += m[q];
+      // node starts, bottom-up; local node k >= 1 starts inside the chunk
+      // (node 0 holds W0) or at or after W1 (then WS_SENT)
+      for(uint32_t k=lane; k < cnt[NL - 1]; k += 64) {
+        uint32_t const st = s_tmp[wave][NL - 1][k];
+        uint32_t pos = WS_SENT;
+        if(k > 0 && (uint64_t) st > W0 && (uint64_t) st < W1) {
+          pos = (uint32_t) (st - W0);
+          s_mk[wave][NL - 1][pos] = 1;
+        }
+        s_tmp[wave][NL - 1][k] = pos;
       }
-      uint32_t pre = wave_incl_scan(sum) - sum;
+      __builtin_amdgcn_wave_barrier();
       #pragma unroll
-      for(int q=0; q < PER; ++q) {
-        pre += m[q];
-        idx[l][q] = pre;
+      for(int l=NL - 2; l >= 0; --l) {
+        for(uint32_t k=lane; k < cnt[l]; k += 64) {
+          uint32_t const ci = s_tmp[wave][l][k] - a0[l + 1];   // first child, local
+          uint32_t pos = WS_SENT;
+          if(k > 0 && ci < cnt[l + 1]) pos = s_tmp[wave][l + 1][ci];
+          if(pos < wn) s_mk[wave][l][pos] = 1;
+          s_tmp[wave][l][k] = pos;
+        }
+        __builtin_amdgcn_wave_barrier();
       }
-    }
-    __builtin_amdgcn_wave_barrier();
-    // records
-    #pragma unroll
-    for(int q=0; q < PER; ++q) {
-      uint32_t const p = lane * PER + q;
-      if(p < wn) {
-        uint32_t const leaf = a.fids[N - 1][W0 + p];
-        T const v = a.vals[W0 + p];
-        uint32_t orow = leaf;
-        int32_t osl = -1;
+      // local node index of every position at every level
+      uint32_t idx[NL][PER];
+      #pragma unroll
+      for(int l=0; l < NL; ++l) {
+        uint32_t m[PER];
+        uint32_t sum = 0;
         #pragma unroll
-        for(int l=0; l < NL; ++l) {
-          uint32_t const r = s_row[wave][l][idx[l][q]];
-          if(l == d) {
-            orow = r;
-            if constexpr (KIND == KIND_INTL) osl = s_nsl[wave][idx[l][q]];
-          } else {
-            s_tmp[wave][l - (l > d ? 1 : 0)][p] = r;
-          }
+        for(int q=0; q < PER; ++q) {
+          m[q] = s_mk[wave][l][lane * PER + q];
+          sum += m[q];
         }
-        if constexpr (KIND == KIND_LEAF) {
-          osl = (nhot > 0) ? a.hot_slot[leaf] : -1;
-        } else {
-          s_tmp[wave][NG - 1][p] = leaf;
+        uint32_t pre = wave_incl_scan(sum) - sum;
+        #pragma unroll
+        for(int q=0; q < PER; ++q) {
+          pre += m[q];
+          idx[l][q] = pre;
         }
-        s_orow[wave][p] = orow;
-        if constexpr (KIND != KIND_ROOT) s_osl[wave][p] = osl;
-        s_val[wave][p] = v;
       }
-    }
-    __builtin_amdgcn_wave_barrier();
-
-    // ---- walk this group's positions ----
-    uint32_t const k0 = g * S;
-    uint32_t const k1 = (k0 + S < wn) ? k0 + S : wn;
-    if(k0 < k1) {
-      uint32_t cur = s_orow[wave][k0];
-      int32_t csl = -1;
-      if constexpr (KIND != KIND_ROOT) csl = s_osl[wave][k0];
-      VT acc = vzero<VT>();
-      bool first = true;
-      auto flush = [&](bool edge) {
-        if constexpr (KIND == KIND_ROOT) {
-          if(edge) vadd_atomic<T, V>(a.out, cur, R, col, cok, acc);
-          else vstore<T, V>(a.out, cur, R, col, cok, acc);
-        } else {
-          if(csl >= 0 && (uint32_t) csl < nhot) {
-            if(cok) {
-              T * p = hot + (uint32_t) csl * CW + c * V;
-              if constexpr (V == 1) {
-                atomicAdd(p, acc);
-              } else {
-                #pragma unroll
-                for(int k=0; k < V; ++k) atomicAdd(p + k, acc[k]);
-              }
+      __builtin_amdgcn_wave_barrier();
+      // records
+      bool const leaf_out = (KIND == KIND_LEAF) || (KIND == KIND_EXPAND && d == NL);
+      #pragma unroll
+      for(int q=0; q < PER; ++q) {
+        uint32_t const p = lane * PER + q;
+        if(p < wn) {
+          uint32_t const leaf = a.fids[N - 1][W0 + p];
+          uint32_t orow = leaf;
+          int32_t osl = -1;
+          #pragma unroll
+          for(int l=0; l < NL; ++l) {
+            uint32_t const r = s_row[wave][l][idx[l][q]];
+            int const gq = l - (l > d ? 1 : 0);
+            if(l == d) {
+              orow = r;
+              if constexpr (KIND == KIND_INTL) osl = s_nsl[wave][idx[l][q]];
+            } else if constexpr (KIND == KIND_EXPAND) {
+              a.xg[gq][W0 + p] = r;
+            } else {
+              s_tmp[wave][gq][p] = r;
             }
+          }
+          if(leaf_out) {
+            if constexpr (KIND == KIND_LEAF) osl = (nhot > 0) ? a.hot_slot[leaf] : -1;
+          } else if constexpr (KIND == KIND_EXPAND) {
+            a.xg[NG - 1][W0 + p] = leaf;
           } else {
-            vadd_atomic<T, V>(a.out, cur, R, col, cok, acc);
+            s_tmp[wave][NG - 1][p] = leaf;
+          }
+          if constexpr (KIND == KIND_EXPAND) {
+            a.xrow[W0 + p] = orow;
+          } else {
+            s_orow[wave][p] = orow;
+            if constexpr (PRIV) s_osl[wave][p] = osl;
+            s_val[wave][p] = a.vals[W0 + p];
           }
         }
-      };
-      auto step = [&](uint32_t orow, int32_t osl, VT prod) {
-        if(orow != cur) {
-          flush(first);
-          first = false;
-          cur = orow;
-          csl = osl;
-          acc = vzero<VT>();
-        }
-        acc += prod;
-      };
-      uint32_t k = k0;
-      for(; k + B <= k1; k += B) {
-        uint32_t gid[B][NG];
-        uint32_t orw[B];
-        int32_t osl[B];
-        T v[B];
-        #pragma unroll
-        for(int j=0; j < B; ++j) {
-          #pragma unroll
-          for(int q=0; q < NG; ++q) gid[j][q] = s_tmp[wave][q][k + j];
-          orw[j] = s_orow[wave][k + j];
-          osl[j] = (KIND != KIND_ROOT) ? s_osl[wave][(KIND != KIND_ROOT) ? k + j : 0] : -1;
-          v[j] = s_val[wave][k + j];
-        }
-        VT b[B][NG];
-        #pragma unroll
-        for(int j=0; j < B; ++j) {
-          #pragma unroll
-          for(int q=0; q < NG; ++q) b[j][q] = vload_c<T, V>(a.gm[q], gid[j][q], R, colc);
-        }
-        #pragma unroll
-        for(int j=0; j < B; ++j) {
-          VT pr = v[j] * b[j][0];
-          #pragma unroll
-          for(int q=1; q < NG; ++q) pr *= b[j][q];
-          step(orw[j], osl[j], pr);
-        }
       }
-      for(; k < k1; ++k) {
-        VT pr = s_val[wave][k] * vload_c<T, V>(a.gm[0], s_tmp[wave][0][k], R, colc);
-        #pragma unroll
-        for(int q=1; q < NG; ++q) pr *= vload_c<T, V>(a.gm[q], s_tmp[wave][q][k], R, colc);
-        step(s_orow[wave][k], (KIND != KIND_ROOT) ? s_osl[wave][(KIND != KIND_ROOT) ? k : 0] : -1, pr);
-      }
-      flush(true);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if constexpr (KIND != KIND_EXPAND) {
+      uint32_t const k0 = g * S;
+      uint32_t const k1 = (k0 + S < wn) ? k0 + S : wn;
+      walk_range<T, NG, WALK, V, L, B>(a, s_tmp[wave], s_orow[wave], s_osl[wave], s_val[wave],
+          hot, nhot, k0, k1, ch * G + g, c, col, colc, cok);
     }
     // the next chunk's staging overwrites these records
     __builtin_amdgcn_wave_barrier();
   }
 
-  if constexpr (KIND != KIND_ROOT) {
+  if constexpr (PRIV) {
     __syncthreads();
     // privatised rows accumulated by this block go to HBM once per block
     for(uint32_t x=threadIdx.x; x < nhot * CW; x += WS_WAVES * 64) {
@@ -740,11 +819,34 @@ k_walks(WsArgs<T> a)
       uint32_t const cc = x % CW;
       uint32_t const colx = blockIdx.y * CW + cc;
       T const v = hot[x];
-      if(colx < R && v != (T) 0) {
-        vatomic(a.out + (uint64_t) a.hot_rows[slot] * R + colx, v);
+      if(colx < a.R && v != (T) 0) {
+        vatomic(a.out + (uint64_t) a.hot_rows[slot] * a.R + colx, v);
       }
     }
   }
+}
+
+// Row-sorted copy: position i of the copy is CSF position perm[i].
+__global__ void k_copy_gather(uint32_t const * __restrict__ perm, uint64_t n, int ng,
+    DevPtrs32 xg, DevPtrs32 cg, double const * __restrict__ vals,
+    double * __restrict__ cv64, float * __restrict__ cv32)
+{
+  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  for(; i < n; i += stride) {
+    uint32_t const p = perm[i];
+    for(int q=0; q < ng; ++q) cg.p[q][i] = xg.p[q][p];
+    double const v = vals[p];
+    cv64[i] = v;
+    cv32[i] = (float) v;
+  }
+}
+
+__global__ void k_u64_to_u32(uint64_t const * __restrict__ k, uint64_t n, uint32_t * __restrict__ o)
+{
+  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  for(; i < n; i += stride) o[i] = (uint32_t) k[i];
 }
 
 // ---- plans ----------------------------------------------------------------
@@ -811,6 +913,18 @@ struct HotPlan {
   HotPlan * next;
 };
 
+// A row-sorted copy of a CSF for one output level: the nonzeros ordered by
+// output row (CSF order within a row), with the row of every other level.
+struct RowCopy {
+  int level;
+  int rows_pos;                  // level-0 rows were node positions
+  uint32_t * cg[SPD_MAX_MODES];  // gathered rows, levels other than `level` in order
+  uint32_t * crow;               // output rows, ascending
+  double * v64;
+  float * v32;
+  RowCopy * next;
+};
+
 void spd_csf_drop_plans(spd_csf * c)
 {
   MttkrpPlan * p = c->plans;
@@ -831,6 +945,17 @@ void spd_csf_drop_plans(spd_csf * c)
     h = nx;
   }
   c->hot = nullptr;
+  RowCopy * r = c->copies;
+  while(r) {
+    RowCopy * nx = r->next;
+    for(int q=0; q < SPD_MAX_MODES; ++q) spd_free(r->cg[q]);
+    spd_free(r->crow);
+    spd_free(r->v64);
+    spd_free(r->v32);
+    delete r;
+    r = nx;
+  }
+  c->copies = nullptr;
 }
 
 __global__ void k_count_rows(uint32_t const * __restrict__ ids, uint64_t n,
@@ -968,7 +1093,7 @@ static int launch_walks(WsArgs<T> a, hipStream_t st)
   if(nb > WS_BLOCKS) nb = WS_BLOCKS;
   if(nb == 0) return SPD_OK;
   size_t shmem = 0;
-  if constexpr (KIND != KIND_ROOT) {
+  if constexpr (KIND == KIND_INTL || KIND == KIND_LEAF) {
     uint32_t const cap = WS_HOT_BYTES / (CW * sizeof(T));
     if(a.nhot > cap) a.nhot = cap;
     shmem = (size_t) a.nhot * CW * sizeof(T);
@@ -976,6 +1101,16 @@ static int launch_walks(WsArgs<T> a, hipStream_t st)
   hipLaunchKernelGGL((k_walks<T, N, KIND, V, L, B>), dim3((unsigned) nb, ny),
       dim3(WS_WAVES * 64), shmem, st, a);
   SPD_LAUNCH_CHECK();
+  if constexpr (KIND == KIND_ROOT || KIND == KIND_COPY) {
+    constexpr uint32_t G = 64 / L;
+    uint32_t const nent = 2 * a.nchunks * G;
+    uint32_t const chunks = (uint32_t) spd_cdiv(nent, FIX_CHUNK);
+    uint32_t const groups_per_block = (MK_THREADS / 64) * G;
+    dim3 g2((unsigned) spd_cdiv(chunks, groups_per_block), ny);
+    hipLaunchKernelGGL((k_root_fixup<T, V, L>), g2, dim3(MK_THREADS), 0, st,
+        (T const *) a.carry, (uint32_t const *) a.carry_row, nent, a.out, a.R);
+    SPD_LAUNCH_CHECK();
+  }
   return SPD_OK;
 }
 
@@ -1188,6 +1323,143 @@ static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
   return rc;
 }
 
+static int g_rowcopy = 1;
+
+extern "C" int spd_mttkrp_set_copies(int enable)
+{
+  g_rowcopy = enable ? 1 : 0;
+  return SPD_OK;
+}
+
+__global__ void k_sort_init(uint32_t const * __restrict__ row, uint64_t n,
+    uint64_t * __restrict__ key, uint32_t * __restrict__ val)
+{
+  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  for(; i < n; i += stride) {
+    key[i] = row[i];
+    val[i] = (uint32_t) i;
+  }
+}
+
+template<int N>
+static int launch_expand(WsArgs<double> const & a, hipStream_t st)
+{
+  uint64_t nb = spd_cdiv(a.nchunks, WS_WAVES);
+  if(nb > WS_BLOCKS) nb = WS_BLOCKS;
+  if(nb == 0) return SPD_OK;
+  hipLaunchKernelGGL((k_walks<double, N, KIND_EXPAND, 1, 64, 1>), dim3((unsigned) nb),
+      dim3(WS_WAVES * 64), 0, st, a);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+static int bits_of(uint64_t v)
+{
+  int b = 1;
+  while(b < 64 && (v >> b) != 0) ++b;
+  return b;
+}
+
+// The row-sorted copy of c for output level `level`, built on first use on
+// stream st. *out is null (and the call succeeds) when device memory would
+// run short: the caller then walks the CSF itself.
+static int get_copy(spd_csf * c, int level, int rows_pos, RowCopy ** out, hipStream_t st)
+{
+  *out = nullptr;
+  for(RowCopy * r = c->copies; r; r = r->next) {
+    if(r->level == level && r->rows_pos == rows_pos) { *out = r; return SPD_OK; }
+  }
+  int const N = c->nmodes;
+  int const NG = N - 1;
+  uint64_t const n = c->nnz;
+  if(n == 0 || n >= 0xFFFFFFFFull) return SPD_OK;
+  size_t const keep = (size_t) n * (NG * 4 + 4 + 8 + 4);
+  size_t const temp = (size_t) n * (NG * 4 + 4 + 2 * 8 + 2 * 4);
+  size_t fr = 0, tot = 0;
+  SPD_HIP(hipMemGetInfo(&fr, &tot));
+  if(fr < 2 * (keep + temp) + (1ull << 30)) return SPD_OK;
+  MttkrpPlan * plan = nullptr;
+  SPD_TRY(get_plan(c, WS_WN, &plan, st));
+  RowCopy * r = new RowCopy();
+  r->level = level;
+  r->rows_pos = rows_pos;
+  uint32_t * xg[SPD_MAX_MODES] = {};
+  uint32_t * xrow = nullptr;
+  uint64_t * k0 = nullptr; uint64_t * k1 = nullptr;
+  uint32_t * v0 = nullptr; uint32_t * v1 = nullptr;
+  int rc = SPD_OK;
+  for(int q=0; q < NG && rc == SPD_OK; ++q) rc = spd_malloc((void **) &xg[q], n * 4);
+  for(int q=0; q < NG && rc == SPD_OK; ++q) rc = spd_malloc((void **) &r->cg[q], n * 4);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &xrow, n * 4);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &r->crow, n * 4);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &r->v64, n * 8);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &r->v32, n * 4);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &k0, n * 8);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &k1, n * 8);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &v0, n * 4);
+  if(rc == SPD_OK) rc = spd_malloc((void **) &v1, n * 4);
+  if(rc == SPD_OK) {
+    WsArgs<double> a = {};
+    for(int l=0; l < N - 1; ++l) {
+      a.fptr[l] = c->u.fptr[l];
+      a.nfibs[l] = (uint32_t) c->u.nfibs[l];
+    }
+    for(int l=0; l < N; ++l) a.fids[l] = c->u.fids[l];
+    for(int q=0; q < NG; ++q) a.xg[q] = xg[q];
+    a.xrow = xrow;
+    a.plan = plan->nodes;
+    a.nnz = n;
+    a.nchunks = plan->nitems;
+    a.depth = level;
+    a.rows_are_positions = rows_pos;
+    switch(N) {
+    case 3: rc = launch_expand<3>(a, st); break;
+    case 4: rc = launch_expand<4>(a, st); break;
+    case 5: rc = launch_expand<5>(a, st); break;
+    case 6: rc = launch_expand<6>(a, st); break;
+    case 7: rc = launch_expand<7>(a, st); break;
+    case 8: rc = launch_expand<8>(a, st); break;
+    default: rc = SPD_EINPUT; break;
+    }
+  }
+  int which = 0;
+  if(rc == SPD_OK) {
+    unsigned const grid = (unsigned) (spd_cdiv(n, 256) < 8192 ? spd_cdiv(n, 256) : 8192);
+    hipLaunchKernelGGL(k_sort_init, dim3(grid), dim3(256), 0, st, xrow, n, k0, v0);
+    if(hipGetLastError() != hipSuccess) rc = SPD_EHIP;
+    if(rc == SPD_OK) rc = spd_radix_sort(k0, v0, k1, v1, n, bits_of(c->dims[c->perm[level]]),
+        &which, st);
+  }
+  if(rc == SPD_OK) {
+    unsigned const grid = (unsigned) (spd_cdiv(n, 256) < 8192 ? spd_cdiv(n, 256) : 8192);
+    DevPtrs32 src = {}, dst = {};
+    for(int q=0; q < NG; ++q) {
+      src.p[q] = xg[q];
+      dst.p[q] = r->cg[q];
+    }
+    hipLaunchKernelGGL(k_u64_to_u32, dim3(grid), dim3(256), 0, st, which ? k1 : k0, n, r->crow);
+    hipLaunchKernelGGL(k_copy_gather, dim3(grid), dim3(256), 0, st, which ? v1 : v0, n, NG,
+        src, dst, c->u.vals, r->v64, r->v32);
+    if(hipGetLastError() != hipSuccess) rc = SPD_EHIP;
+  }
+  if(rc == SPD_OK) rc = (hipStreamSynchronize(st) == hipSuccess) ? SPD_OK : SPD_EHIP;
+  for(int q=0; q < NG; ++q) spd_free(xg[q]);
+  spd_free(xrow);
+  spd_free(k0); spd_free(k1); spd_free(v0); spd_free(v1);
+  if(rc != SPD_OK) {
+    for(int q=0; q < SPD_MAX_MODES; ++q) spd_free(r->cg[q]);
+    spd_free(r->crow); spd_free(r->v64); spd_free(r->v32);
+    delete r;
+    if(rc == SPD_EHIP) spd_set_error(SPD_EHIP, "row-sorted copy build failed");
+    return rc;
+  }
+  r->next = c->copies;
+  c->copies = r;
+  *out = r;
+  return SPD_OK;
+}
+
 template<typename T>
 static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
     void * out, int rows_pos, hipStream_t st, int dtype)
@@ -1226,7 +1498,25 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   a.R = (uint32_t) R;
   a.depth = depth;
   a.rows_are_positions = rows_pos;
-  if(depth > 0) {
+  RowCopy * cp = nullptr;
+  if(depth > 0 && g_rowcopy) SPD_TRY(get_copy(c, depth, rows_pos, &cp, spd_hs(nullptr)));
+  if(cp != nullptr) {
+    for(int q=0; q < N - 1; ++q) a.cg[q] = cp->cg[q];
+    a.crow = cp->crow;
+    a.vals = (sizeof(T) == 4) ? (T const *) cp->v32 : (T const *) cp->v64;
+  }
+  if(depth == 0 || cp != nullptr) {
+    // two partial rows per range of 256 / (64 / L) positions
+    int V = 1, L = 16;
+    lane_layout((uint32_t) R, dtype, &V, &L);
+    uint64_t const nent = 2ull * plan->nitems * (64 / L);
+    void * cv = nullptr;
+    void * cr = nullptr;
+    SPD_TRY(spd_scratch(2, nent * R * sizeof(T), &cv));
+    SPD_TRY(spd_scratch(3, nent * sizeof(uint32_t), &cr));
+    a.carry = (T *) cv;
+    a.carry_row = (uint32_t *) cr;
+  } else {
     HotPlan * hp = nullptr;
     SPD_TRY(get_hot(c, depth, &hp, spd_hs(nullptr)));
     // privatise rows that collect at least two contributions
@@ -1244,6 +1534,7 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   if(g_timing) SPD_HIP(hipEventRecord(g_ev[slot][0], st));
   int rc;
   if(depth == 0) rc = dispatch_n<T, KIND_ROOT>(N, a, st);
+  else if(cp != nullptr) rc = dispatch_n<T, KIND_COPY>(N, a, st);
   else if(depth == N - 1) rc = dispatch_n<T, KIND_LEAF>(N, a, st);
   else rc = dispatch_n<T, KIND_INTL>(N, a, st);
   if(g_timing) {

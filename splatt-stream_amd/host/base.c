@@ -57,6 +57,15 @@ splatt_error_type splatt_amd_set_mttkrp_precision(int precision)
   return SPLATT_SUCCESS;
 }
 
+splatt_error_type splatt_amd_set_mttkrp_layout(int layout)
+{
+  if(layout != SPLATT_AMD_LAYOUT_AUTO && layout != SPLATT_AMD_LAYOUT_CSF) {
+    return SPLATT_ERROR_BADINPUT;
+  }
+  spd_mttkrp_set_copies(layout == SPLATT_AMD_LAYOUT_AUTO);
+  return SPLATT_SUCCESS;
+}
+
 /* src/util.c:15-24: magnitude 3*rand()/RAND_MAX, negated when a second
  * rand() is even. Drawn on the host so the sequence is glibc's. */
 val_t spl_rand_val(void)

@@ -26,6 +26,7 @@ CSF_ONEMODE, CSF_TWOMODE, CSF_ALLMODE = 0, 1, 2
 NOTILE, DENSETILE = 0, 1
 SUCCESS = 1
 F64, F32 = 0, 1
+LAYOUT_AUTO, LAYOUT_CSF = 0, 1
 
 u64p = C.POINTER(C.c_uint64)
 f64p = C.POINTER(C.c_double)
@@ -161,6 +162,8 @@ def lib():
     L.splatt_amd_last_error.restype = C.c_char_p
     L.splatt_amd_set_mttkrp_precision.restype = C.c_int
     L.splatt_amd_set_mttkrp_precision.argtypes = [C.c_int]
+    L.splatt_amd_set_mttkrp_layout.restype = C.c_int
+    L.splatt_amd_set_mttkrp_layout.argtypes = [C.c_int]
     L.splatt_amd_mttkrp_dev.restype = C.c_int
     L.splatt_amd_mttkrp_dev.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(SplattCsf),
                                         C.POINTER(C.c_void_p), C.c_void_p, C.c_int,

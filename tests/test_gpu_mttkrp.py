@@ -5,7 +5,8 @@ tests/mttkrp_test.c:36-88).
 Tolerances (normwise relative, ||got - ref||_F / ||ref||_F):
   FP64 path 1e-12, FP32 path 1e-5  -- as stated for this path.
 Covered: every mode of every allocation scheme (root, internal and leaf
-kernels), 3- to 8-mode tensors, tiled tensors, ranks that exercise every
+kernels; internal and leaf outputs both over row-sorted copies and by the
+CSF walk with atomics), 3- to 8-mode tensors, tiled tensors, ranks that exercise every
 lane layout (1, 3, 8, 16, 32, 64, 100), Zipf-skewed tensors whose heavy
 slices span many work items, the device-pointer entry point, and at a
 larger size the mode-independence of <X, [[A_1..A_N]]> = sum(M_m * A_m).
@@ -34,16 +35,23 @@ def _set_precision(p):
     assert ps.lib().splatt_amd_set_mttkrp_precision(p) == ps.SUCCESS
 
 
+def _set_layout(x):
+    assert ps.lib().splatt_amd_set_mttkrp_layout(x) == ps.SUCCESS
+
+
 @pytest.fixture(autouse=True)
 def _reset_precision():
     yield
     ps.lib().splatt_amd_set_mttkrp_precision(ps.F64)
+    ps.lib().splatt_amd_set_mttkrp_layout(ps.LAYOUT_AUTO)
 
 
 @pytest.mark.parametrize("name", DATASETS)
 @pytest.mark.parametrize("kind", [ps.CSF_ONEMODE, ps.CSF_TWOMODE, ps.CSF_ALLMODE])
-def test_fixtures_fp64(gpu_available, name, kind):
-    print("case", name, kind, flush=True)
+@pytest.mark.parametrize("layout", [ps.LAYOUT_AUTO, ps.LAYOUT_CSF])
+def test_fixtures_fp64(gpu_available, name, kind, layout):
+    print("case", name, kind, layout, flush=True)
+    _set_layout(layout)
     coo = orc.Coo.read(tensor_path(name))
     s = ps.CsfSet.convert(coo.inds(), coo.vals(), ps.Opts(csf_alloc=kind))
     mats = [ps.synth_matrix(d, 16, 100 + m) for m, d in enumerate(coo.dims)]
@@ -52,8 +60,10 @@ def test_fixtures_fp64(gpu_available, name, kind):
 
 
 @pytest.mark.parametrize("name", ["med.tns", "med4.tns", "med5.tns"])
-def test_fixtures_fp32(gpu_available, name):
-    print("case fp32", name, flush=True)
+@pytest.mark.parametrize("layout", [ps.LAYOUT_AUTO, ps.LAYOUT_CSF])
+def test_fixtures_fp32(gpu_available, name, layout):
+    print("case fp32", name, layout, flush=True)
+    _set_layout(layout)
     coo = orc.Coo.read(tensor_path(name))
     s = ps.CsfSet.convert(coo.inds(), coo.vals(), ps.Opts(csf_alloc=ps.CSF_ONEMODE))
     mats = [ps.synth_matrix(d, 32, 200 + m).astype(np.float32) for m, d in enumerate(coo.dims)]
@@ -65,8 +75,10 @@ def test_fixtures_fp32(gpu_available, name):
 
 @pytest.mark.parametrize("R", [1, 3, 8, 16, 32, 64, 100])
 @pytest.mark.parametrize("prec", [ps.F64, ps.F32])
-def test_ranks_and_layouts(gpu_available, R, prec):
-    print("case rank", R, prec, flush=True)
+@pytest.mark.parametrize("layout", [ps.LAYOUT_AUTO, ps.LAYOUT_CSF])
+def test_ranks_and_layouts(gpu_available, R, prec, layout):
+    print("case rank", R, prec, layout, flush=True)
+    _set_layout(layout)
     inds, vals, dims = ps.synth_coord([300, 200, 100], 30000, zipf=[1.2, 1.1, 0.0],
                                       vals="lognormal", seed=R)
     coo = orc.Coo.from_arrays(inds, vals, dims)
@@ -83,8 +95,10 @@ def test_ranks_and_layouts(gpu_available, R, prec):
 
 
 @pytest.mark.parametrize("nmodes", [4, 5, 6, 8])
-def test_higher_order(gpu_available, nmodes):
-    print("case nmodes", nmodes, flush=True)
+@pytest.mark.parametrize("layout", [ps.LAYOUT_AUTO, ps.LAYOUT_CSF])
+def test_higher_order(gpu_available, nmodes, layout):
+    print("case nmodes", nmodes, layout, flush=True)
+    _set_layout(layout)
     dims = [40, 30, 20, 12, 9, 7, 5, 4][:nmodes]
     inds, vals, d2 = ps.synth_coord(dims, 40000, zipf=[1.1] * nmodes, seed=nmodes)
     coo = orc.Coo.from_arrays(inds, vals, d2)
@@ -106,7 +120,9 @@ def test_tiled(gpu_available, tilelevel):
         assert rel(s.mttkrp(mode, mats), orc.mttkrp_coo(coo, mats, mode)) <= TOL64
 
 
-def test_heavy_zipf_slices(gpu_available):
+@pytest.mark.parametrize("layout", [ps.LAYOUT_AUTO, ps.LAYOUT_CSF])
+def test_heavy_zipf_slices(gpu_available, layout):
+    _set_layout(layout)
     # a few slices hold most nonzeros, so they span many work items and are
     # completed through the partial (atomic) flushes
     inds, vals, dims = ps.synth_coord([2000, 1500, 1000], 1000000, zipf=[1.6, 1.6, 1.6],
