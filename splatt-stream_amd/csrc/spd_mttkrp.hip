@@ -236,8 +236,10 @@ k_root3(Root3Args<T> a)
   constexpr int PER = R3_WN / 64;             // staged nonzeros per lane
   // per nonzero: byte offset of its B row, byte offset of its fiber's A row,
   // local slice index, and (FP32) the value's bits; FP64 values in s_v64
-  __shared__ uint4 s_rec[R3_WAVES][R3_WN];
-  __shared__ T s_v64[sizeof(T) == 8 ? R3_WAVES : 1][sizeof(T) == 8 ? R3_WN : 1];
+  // records of group g start at g * (S + 1): the groups' concurrent reads of
+  // their own records fall in different LDS banks
+  __shared__ uint4 s_rec[R3_WAVES][R3_WN + G];
+  __shared__ T s_v64[sizeof(T) == 8 ? R3_WAVES : 1][sizeof(T) == 8 ? R3_WN + G : 1];
   __shared__ uint32_t s_arow[R3_WAVES][R3_WN + 1];   // A row offset of local fiber k
   __shared__ uint16_t s_fi[R3_WAVES][R3_WN];     // fiber-start marks
   __shared__ uint16_t s_si[R3_WAVES][R3_WN];     // slice-start marks
@@ -330,9 +332,9 @@ k_root3(Root3Args<T> a)
         r.w = __float_as_uint(v);
       } else {
         r.w = 0;
-        s_v64[wave][p] = v;
+        s_v64[wave][p + p / S] = v;
       }
-      s_rec[wave][p] = r;
+      s_rec[wave][p + p / S] = r;
     }
   }
   for(uint32_t e=threadIdx.x; e < NE; e += R3_WAVES * 64) s_crow[e] = CARRY_NONE;
@@ -351,13 +353,13 @@ k_root3(Root3Args<T> a)
     bool head_partial = a.head[item] != 0;
     uint32_t const k0 = (uint32_t) (n0 - W0);
     uint32_t const k1 = (uint32_t) (n1 - W0);
-    uint32_t cur_s = s_rec[wave][k0].z;
+    uint32_t cur_s = s_rec[wave][k0 + k0 / S].z;
     VT accs = vzero<VT>();
     auto rec_val = [&](uint4 const & r, uint32_t k) -> T {
       if constexpr (sizeof(T) == 4) {
         return __uint_as_float(r.w);
       } else {
-        return s_v64[wave][k];
+        return s_v64[wave][k + k / S];
       }
     };
     auto step = [&](uint4 const & r, T v, VT b, VT av) {
@@ -382,7 +384,7 @@ k_root3(Root3Args<T> a)
         VT b[R3_BATCH];
         VT av[R3_BATCH];
         #pragma unroll
-        for(int j=0; j < R3_BATCH; ++j) r[j] = s_rec[wave][k + j];
+        for(int j=0; j < R3_BATCH; ++j) r[j] = s_rec[wave][k + j + (k + j) / S];
         #pragma unroll
         for(int j=0; j < R3_BATCH; ++j) {
           b[j] = *(VT const *) (Bb + (r[j].x + colb));
@@ -392,7 +394,7 @@ k_root3(Root3Args<T> a)
         for(int j=0; j < R3_BATCH; ++j) step(r[j], rec_val(r[j], k + j), b[j], av[j]);
       } else {
         for(uint32_t kk=k; kk < k1; ++kk) {
-          uint4 const r = s_rec[wave][kk];
+          uint4 const r = s_rec[wave][kk + kk / S];
           VT const b = *(VT const *) (Bb + (r.x + colb));
           VT const av = *(VT const *) (Ab + (r.y + colb));
           step(r, rec_val(r, kk), b, av);
@@ -403,7 +405,7 @@ k_root3(Root3Args<T> a)
     // slice (or the tensor's end)
     bool complete = (n1 == a.nnz);
     if(!complete && n1 < W1) {
-      complete = s_rec[wave][k1].z != cur_s;
+      complete = s_rec[wave][k1 + k1 / S].z != cur_s;
     } else if(!complete) {
       // the item ends at the wave's end: the slice is complete iff the
       // next slice starts exactly there
