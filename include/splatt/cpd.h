@@ -93,6 +93,15 @@ splatt_error_type splatt_register_frob(
     splatt_idx_t const * const modes_included,
     splatt_idx_t const num_modes);
 
+/* include/splatt/cpd.h:332-336 (src/cpd/constraints/lasso.c:80-107): L1
+ * regularisation, solved by ADMM with soft thresholding at multiplier/rho;
+ * row-separable, so rows are solved in independent chunks by default. */
+splatt_error_type splatt_register_lasso(
+    splatt_cpd_opts * opts,
+    splatt_val_t const multiplier,
+    splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes);
+
 /* Options and constraint objects: include/splatt/cpd.h:372-418. */
 splatt_cpd_opts * splatt_alloc_cpd_opts(void);
 void splatt_free_cpd_opts(splatt_cpd_opts * const opts);

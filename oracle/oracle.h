@@ -102,10 +102,11 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
  * (src/cpd/constraints/frob.c:25-36), 2 = ADMM (src/cpd/admm.c:346-406,
  * one chunk) with prox 1 = max-col-norm (constraints/maxcolnorm.c:32-61),
  * 2 = non-negative max-col-norm (maxcolnorm-nonneg.c:32-66),
- * 3 = non-negative (nonneg.c), 0 = none. chunk > 0 runs ADMM in
- * independent chunks of that many rows (admm.c:355-400; the caller passes
- * 0 unless the prox is row-separable). */
-typedef struct { int kind; int prox; double diag_add; uint64_t chunk; } or_con;
+ * 3 = non-negative (nonneg.c), 4 = L1 soft threshold at lam / rho
+ * (lasso.c:30-52), 0 = none. chunk > 0 runs ADMM in independent chunks of
+ * that many rows (admm.c:355-400; the caller passes 0 unless the prox is
+ * row-separable). */
+typedef struct { int kind; int prox; double diag_add; uint64_t chunk; double lam; } or_con;
 
 /* or_cpd_als with per-mode constraints; `unconstrained` is the reference's
  * ws->unconstrained (every constraint UNCONSTRAINED: normalise columns into
@@ -122,17 +123,18 @@ int or_cpd_als_con(or_csf * const * csfs, uint64_t const * mode_csf,
  * of inner iterations; *rho_out receives rho when not NULL. */
 uint64_t or_admm(int prox, double const * M, double * P, double * D,
     double * G, uint64_t I, uint64_t R, uint64_t max_inner, double inner_tol,
-    double * rho_out);
+    double * rho_out, double lam);
 
 /* The same with the rows split into chunks of `chunk` rows (the last one
  * ragged), each iterated to its own convergence (src/cpd/admm.c:355-400).
  * Returns the sum over chunks of iterations x rows. */
 uint64_t or_admm_chunked(int prox, double const * M, double * P, double * D,
     double * G, uint64_t I, uint64_t R, uint64_t chunk, uint64_t max_inner,
-    double inner_tol);
+    double inner_tol, double lam);
 
-/* The proximal operators (row-major I x R, in place). */
-void or_prox(int prox, double * P, uint64_t I, uint64_t R);
+/* The proximal operators (row-major I x R, in place); thr is the lasso
+ * threshold lam / rho (unused by the others). */
+void or_prox(int prox, double * P, uint64_t I, uint64_t R, double thr);
 
 /* ---- streaming (the files under src/cpd/stream) --------------------- */
 

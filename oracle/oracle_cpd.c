@@ -116,9 +116,23 @@ void or_mat_normalize(double * A, uint64_t I, uint64_t R, double * lambda)
 
 /* src/cpd/constraints/maxcolnorm.c:32-61, maxcolnorm-nonneg.c:32-66,
  * nonneg.c: column norms capped at 1 (after clamping negatives for the
- * non-negative variant); plain clamping for nonneg. */
-void or_prox(int prox, double * P, uint64_t I, uint64_t R)
+ * non-negative variant); plain clamping for nonneg; lasso.c:30-52: entries
+ * above thr lose thr, entries below -thr gain thr, the rest become 0. */
+void or_prox(int prox, double * P, uint64_t I, uint64_t R, double thr)
 {
+  if(prox == 4) {
+    for(uint64_t x=0; x < I * R; ++x) {
+      double const v = P[x];
+      if(v > thr) {
+        P[x] = v - thr;
+      } else if(v < -thr) {
+        P[x] = v + thr;
+      } else {
+        P[x] = 0.;
+      }
+    }
+    return;
+  }
   if(prox == 3) {
     for(uint64_t x=0; x < I * R; ++x) {
       P[x] = (P[x] > 0.) ? P[x] : 0.;
@@ -152,7 +166,7 @@ void or_prox(int prox, double * P, uint64_t I, uint64_t R)
  * the Cholesky factor of G + rho I. */
 static uint64_t p_admm_iterate(int prox, double const * M, double * P, double * D,
     double const * L, double rho, uint64_t I, uint64_t R, uint64_t max_inner,
-    double inner_tol)
+    double inner_tol, double lam)
 {
   double * aux = malloc((I ? I : 1) * R * sizeof(double));
   double * init = malloc((I ? I : 1) * R * sizeof(double));
@@ -166,7 +180,7 @@ static uint64_t p_admm_iterate(int prox, double const * M, double * P, double * 
     for(uint64_t x=0; x < I * R; ++x) {
       P[x] = aux[x] - D[x];
     }
-    or_prox(prox, P, I, R);
+    or_prox(prox, P, I, R, lam / rho);
     double dual_norm = 0.;
     for(uint64_t x=0; x < I * R; ++x) {
       D[x] += P[x] - aux[x];
@@ -208,18 +222,18 @@ static double p_admm_setup(double * G, uint64_t R)
 
 uint64_t or_admm(int prox, double const * M, double * P, double * D,
     double * G, uint64_t I, uint64_t R, uint64_t max_inner, double inner_tol,
-    double * rho_out)
+    double * rho_out, double lam)
 {
   double const rho = p_admm_setup(G, R);
   if(rho_out) {
     *rho_out = rho;
   }
-  return p_admm_iterate(prox, M, P, D, G, rho, I, R, max_inner, inner_tol);
+  return p_admm_iterate(prox, M, P, D, G, rho, I, R, max_inner, inner_tol, lam);
 }
 
 uint64_t or_admm_chunked(int prox, double const * M, double * P, double * D,
     double * G, uint64_t I, uint64_t R, uint64_t chunk, uint64_t max_inner,
-    double inner_tol)
+    double inner_tol, double lam)
 {
   double const rho = p_admm_setup(G, R);
   /* src/cpd/admm.c:355-400: chunks of `chunk` rows, the last one ragged,
@@ -234,7 +248,7 @@ uint64_t or_admm_chunked(int prox, double const * M, double * P, double * D,
     uint64_t const stop = (c == nchunks - 1) ? I : (c + 1) * chunk;
     uint64_t const off = start * R;
     uint64_t const its = p_admm_iterate(prox, M + off, P + off, D + off, G, rho,
-        stop - start, R, max_inner, inner_tol);
+        stop - start, R, max_inner, inner_tol, lam);
     total += its * (stop - start);
   }
   return total;
@@ -309,10 +323,10 @@ int or_cpd_als_con(or_csf * const * csfs, uint64_t const * mode_csf,
       }
       if(cons[m].kind == 2 && cons[m].chunk > 0) {
         or_admm_chunked(cons[m].prox, mttkrp, factors[m], duals[m], gram, I, R,
-            cons[m].chunk, max_inner, inner_tol);
+            cons[m].chunk, max_inner, inner_tol, cons[m].lam);
       } else if(cons[m].kind == 2) {
         or_admm(cons[m].prox, mttkrp, factors[m], duals[m], gram, I, R,
-            max_inner, inner_tol, NULL);
+            max_inner, inner_tol, NULL, cons[m].lam);
       } else {
         or_cholesky(gram, R);
         memcpy(factors[m], mttkrp, I * R * sizeof(double));

@@ -229,13 +229,14 @@ def cpd_als(csfs, mode_csf, factors, max_its, tol, nthreads=1):
 
 class _OrCon(C.Structure):
     _fields_ = [("kind", C.c_int), ("prox", C.c_int), ("diag_add", C.c_double),
-                ("chunk", C.c_uint64)]
+                ("chunk", C.c_uint64), ("lam", C.c_double)]
 
 
 def cpd_als_con(csfs, mode_csf, factors, max_its, tol, cons, unconstrained,
                 max_inner=50, inner_tol=1e-2, nthreads=1):
     """or_cpd_als_con on copies of `factors`. cons: list of (kind, prox,
-    diag_add[, chunk]) per mode. Returns (factors, lambda, fits)."""
+    diag_add[, chunk[, lam]]) per mode (lam: the lasso multiplier). Returns
+    (factors, lambda, fits)."""
     N = len(factors)
     R = factors[0].shape[1]
     facs = [np.ascontiguousarray(f, dtype=np.float64).copy() for f in factors]
@@ -248,6 +249,7 @@ def cpd_als_con(csfs, mode_csf, factors, max_its, tol, cons, unconstrained,
     for m, c in enumerate(cons):
         carr[m].kind, carr[m].prox, carr[m].diag_add = c[0], c[1], c[2]
         carr[m].chunk = c[3] if len(c) > 3 else 0
+        carr[m].lam = c[4] if len(c) > 4 else 0.0
     L = lib()
     L.or_cpd_als_con.restype = C.c_int
     its = L.or_cpd_als_con(cptrs, _p(mc, C.c_uint64), C.c_uint64(N), C.c_uint64(R),
@@ -258,7 +260,7 @@ def cpd_als_con(csfs, mode_csf, factors, max_its, tol, cons, unconstrained,
     return facs, lam, fits[:its]
 
 
-def admm(prox, M, P, D, G, max_inner, inner_tol):
+def admm(prox, M, P, D, G, max_inner, inner_tol, lam=0.0):
     """or_admm on copies; returns (P, D, iterations, rho)."""
     M = np.ascontiguousarray(M, dtype=np.float64)
     P = np.ascontiguousarray(P, dtype=np.float64).copy()
@@ -270,11 +272,12 @@ def admm(prox, M, P, D, G, max_inner, inner_tol):
     L.or_admm.restype = C.c_uint64
     its = L.or_admm(C.c_int(prox), _p(M, C.c_double), _p(P, C.c_double),
                     _p(D, C.c_double), _p(G, C.c_double), C.c_uint64(I), C.c_uint64(R),
-                    C.c_uint64(max_inner), C.c_double(inner_tol), C.byref(rho))
+                    C.c_uint64(max_inner), C.c_double(inner_tol), C.byref(rho),
+                    C.c_double(lam))
     return P, D, int(its), rho.value
 
 
-def admm_chunked(prox, M, P, D, G, chunk, max_inner, inner_tol):
+def admm_chunked(prox, M, P, D, G, chunk, max_inner, inner_tol, lam=0.0):
     """or_admm_chunked on copies; returns (P, D, iterations x rows summed)."""
     M = np.ascontiguousarray(M, dtype=np.float64)
     P = np.ascontiguousarray(P, dtype=np.float64).copy()
@@ -286,14 +289,15 @@ def admm_chunked(prox, M, P, D, G, chunk, max_inner, inner_tol):
     tot = L.or_admm_chunked(C.c_int(prox), _p(M, C.c_double), _p(P, C.c_double),
                             _p(D, C.c_double), _p(G, C.c_double), C.c_uint64(I),
                             C.c_uint64(R), C.c_uint64(chunk), C.c_uint64(max_inner),
-                            C.c_double(inner_tol))
+                            C.c_double(inner_tol), C.c_double(lam))
     return P, D, int(tot)
 
 
-def prox(kind, P):
+def prox(kind, P, thr=0.0):
     P = np.ascontiguousarray(P, dtype=np.float64).copy()
     I, R = P.shape
-    lib().or_prox(C.c_int(kind), _p(P, C.c_double), C.c_uint64(I), C.c_uint64(R))
+    lib().or_prox(C.c_int(kind), _p(P, C.c_double), C.c_uint64(I), C.c_uint64(R),
+                  C.c_double(thr))
     return P
 
 

@@ -399,7 +399,7 @@ int or_stream_cpd(or_coo const * tt, or_stream_cfg const * cfg,
         p_form_gram(aTa, N, m, R, cfg->cons[m].diag_add, gram);
         if(cfg->cons[m].kind == 2) {
           or_admm(cfg->cons[m].prox, mttkrp, snew[m].v, sdual[m].v, gram, I, R,
-              max_inner, inner_tol, NULL);
+              max_inner, inner_tol, NULL, cfg->cons[m].lam);
         } else {
           or_cholesky(gram, R);
           memcpy(snew[m].v, mttkrp, I * R * sizeof(double));

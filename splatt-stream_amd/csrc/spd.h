@@ -148,10 +148,11 @@ int spd_dense_admm_step(void const * M, int mdtype, double const * ginv,
     double * init, int prox, double * sums);
 int spd_dense_admm_solve(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint64_t R, double * P, double * D, double * aux,
-    double * init, int prox, uint64_t max_inner, double tol, double * sums, int * ctl);
+    double * init, int prox, double lam, uint64_t max_inner, double tol, double * sums,
+    int * ctl);
 int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint64_t R, uint64_t chunk, double * P, double * D,
-    int prox, uint64_t max_inner, double tol, uint64_t * its_rows);
+    int prox, double lam, uint64_t max_inner, double tol, uint64_t * its_rows);
 
 /* ---- CPD-ALS (src/cpd/cpd.c:161-332, src/cpd/admm.c:316-406) ---------- */
 
@@ -163,6 +164,7 @@ int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,
 #define SPD_PROX_MAXCOLNORM 1
 #define SPD_PROX_MAXCOLNORM_NONNEG 2
 #define SPD_PROX_NONNEG 3
+#define SPD_PROX_LASSO 4   /* soft threshold at lam / rho */
 
 typedef struct {
   int kind;          /* SPD_CON_* */
@@ -170,6 +172,7 @@ typedef struct {
   double diag_add;   /* Frobenius regularisation added to the Gram diagonal */
   uint64_t chunk;    /* ADMM rows per independent chunk (row-separable prox
                         only, src/cpd/admm.c:355-400); 0: one block */
+  double lam;        /* SPD_PROX_LASSO: the L1 multiplier */
 } spd_con;
 
 typedef struct {

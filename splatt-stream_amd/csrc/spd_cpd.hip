@@ -146,7 +146,7 @@ extern "C" int spd_cpd_als(spd_csf * const * csfs, int const * mode_csf,
         SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
         SPD_TRY(spd_zero_async(its_rows, sizeof(unsigned long long), st));
         SPD_TRY(dn_admm_chunked(Mbuf, dt, ginv, d_rho, dims[m], R, con.chunk, U[m], dual[m],
-            con.prox, cfg->max_inner, cfg->inner_tol, its_rows, st));
+            con.prox, con.lam, cfg->max_inner, cfg->inner_tol, its_rows, st));
         if(cfg->verbosity > 1) {
           unsigned long long hr = 0;
           SPD_TRY(spd_d2h(&hr, its_rows, sizeof(hr)));
@@ -156,7 +156,7 @@ extern "C" int spd_cpd_als(spd_csf * const * csfs, int const * mode_csf,
       } else if(con.kind == SPD_CON_ADMM) {
         SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
         SPD_TRY(dn_admm_solve(Mbuf, dt, ginv, d_rho, dims[m], R, U[m], dual[m], aux, init,
-            con.prox, cfg->max_inner, cfg->inner_tol, d_sums, ctl, st));
+            con.prox, con.lam, cfg->max_inner, cfg->inner_tol, d_sums, ctl, st));
         if(cfg->verbosity > 1) {
           int hc[2];
           SPD_TRY(spd_d2h(hc, ctl, sizeof(hc)));

@@ -661,6 +661,7 @@ struct Admm2Args {
   uint32_t R;
   double const * rho;
   int prox;
+  double lam;               // lasso multiplier
   double const * P0;        // explicit state (P_0, D_0)
   double const * D0;
   double const * cur;       // P'_{j-1} (states >= 1)
@@ -672,10 +673,14 @@ struct Admm2Args {
   int const * ctl;          // ctl[0] != 0: converged, nothing to do
 };
 
+// nc: the column's norm (max-col-norm variants) or the lasso threshold
+// lam / rho (src/cpd/constraints/lasso.c:30-52)
 __device__ __forceinline__ double dn_prox_apply(double p, int prox, double nc)
 {
   if(prox == DN_PROX_NONNEG) {
     p = (p > 0.) ? p : 0.;
+  } else if(prox == DN_PROX_LASSO) {
+    p = (p > nc) ? p - nc : ((p < -nc) ? p + nc : 0.);
   } else if(prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG) {
     if(prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) p = 0.;
     if(!(nc <= 1.)) p /= nc;
@@ -720,8 +725,9 @@ k_admm_pass(Admm2Args<TM> a)
   int const lr = lane / RP;
   int const lc = lane - lr * RP;
   bool const lok = (lr < RPI) && ((uint32_t) lc < R);
-  double const ncur = (MODE >= 1 && lok) ? a.cur_n[lc] : 1.;
-  double const nprev = (MODE == 2 && lok) ? a.prev_n[lc] : 1.;
+  double const thr = a.lam / rho;
+  double const ncur = (prox == DN_PROX_LASSO) ? thr : ((MODE >= 1 && lok) ? a.cur_n[lc] : 1.);
+  double const nprev = (prox == DN_PROX_LASSO) ? thr : ((MODE == 2 && lok) ? a.prev_n[lc] : 1.);
   uint64_t const ngroups = (I + 15) / 16;
   for(uint64_t gi = (uint64_t) blockIdx.x * 4 + wave; gi < ngroups; gi += (uint64_t) gridDim.x * 4) {
     uint64_t const row0 = gi * 16;
@@ -903,9 +909,11 @@ struct Bufs3 { double * b[3]; double * n[3]; };
 
 // P = prox(P'_k), D = P - P'_k for the final state k = ctl[1] (k >= 1).
 __global__ void k_admm_final(Bufs3 bf, double * __restrict__ P, double * __restrict__ D,
-    uint64_t n, uint32_t R, int prox, int const * __restrict__ ctl)
+    uint64_t n, uint32_t R, int prox, double lam, double const * __restrict__ rho,
+    int const * __restrict__ ctl)
 {
   int const k = ctl[1];
+  double const thr = lam / *rho;
   double const * const src = bf.b[k % 3];
   double const * const nrm = bf.n[k % 3];
   uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -914,7 +922,7 @@ __global__ void k_admm_final(Bufs3 bf, double * __restrict__ P, double * __restr
   uint32_t const sc = (uint32_t) (stride % R);
   for(; e < n; e += stride) {
     double const p1 = src[e];
-    double const p = dn_prox_apply(p1, prox, nrm[c]);
+    double const p = dn_prox_apply(p1, prox, (prox == DN_PROX_LASSO) ? thr : nrm[c]);
     P[e] = p;
     D[e] = p - p1;
     c += sc;
@@ -940,7 +948,7 @@ template<typename TM, bool GL>
 __global__ void __launch_bounds__(DN_THREADS)
 k_admm_chunk(TM const * __restrict__ M, double const * __restrict__ ginv, uint32_t RP,
     double const * __restrict__ rho_p, uint64_t I, uint32_t R, uint64_t chunk,
-    double * __restrict__ P, double * __restrict__ D, int prox, uint32_t max_inner,
+    double * __restrict__ P, double * __restrict__ D, int prox, double lam, uint32_t max_inner,
     double tol, unsigned long long * __restrict__ its_rows)
 {
   __shared__ double s_aux[CH_TILE];
@@ -958,6 +966,7 @@ k_admm_chunk(TM const * __restrict__ M, double const * __restrict__ ginv, uint32
   }
   uint32_t const TR = CH_TILE / R;
   double const rho = *rho_p;
+  double const thr = lam / rho;
   uint32_t it = 0;
   while(it < max_inner) {
     double pn = 0., pr = 0., dr = 0., dn = 0.;
@@ -982,7 +991,11 @@ k_admm_chunk(TM const * __restrict__ M, double const * __restrict__ ginv, uint32
         double const p0 = P[base + e];
         double const d0 = D[base + e];
         double p = x - d0;
-        if(prox == DN_PROX_NONNEG) p = (p > 0.) ? p : 0.;
+        if(prox == DN_PROX_NONNEG) {
+          p = (p > 0.) ? p : 0.;
+        } else if(prox == DN_PROX_LASSO) {
+          p = (p > thr) ? p - thr : ((p < -thr) ? p + thr : 0.);
+        }
         double const d = d0 + (p - x);
         P[base + e] = p;
         D[base + e] = d;
@@ -1087,9 +1100,10 @@ __global__ void k_admm_sub(double const * __restrict__ aux, double const * __res
 __global__ void __launch_bounds__(DN_THREADS)
 k_admm_post(double * __restrict__ P, double const * __restrict__ aux,
     double * __restrict__ D, double const * __restrict__ init,
-    double const * __restrict__ norms, int prox, uint64_t n, uint32_t R,
-    double * __restrict__ part)
+    double const * __restrict__ norms, int prox, double lam, double const * __restrict__ rho,
+    uint64_t n, uint32_t R, double * __restrict__ part)
 {
+  double const thr = lam / *rho;
   __shared__ double s[4][DN_THREADS];
   uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
@@ -1100,6 +1114,8 @@ k_admm_post(double * __restrict__ P, double const * __restrict__ aux,
     double p = P[e];
     if(prox == DN_PROX_NONNEG) {
       p = (p > 0.) ? p : 0.;
+    } else if(prox == DN_PROX_LASSO) {
+      p = (p > thr) ? p - thr : ((p < -thr) ? p + thr : 0.);
     } else if(prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG) {
       if(prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) p = 0.;
       double const nc = norms[c];
@@ -1406,7 +1422,7 @@ int dn_apply(void const * M, int mdtype, double const * ginv, uint64_t I,
 
 template<typename TM>
 static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
-    uint32_t R, double * P, double * aux, double * D, double * init, int prox,
+    uint32_t R, double * P, double * aux, double * D, double * init, int prox, double lam,
     double * sums, hipStream_t st)
 {
   if(I == 0) return SPD_OK;
@@ -1463,7 +1479,7 @@ static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_
   void * part2 = nullptr;
   SPD_TRY(spd_scratch(SLOT_GRAM, (size_t) nb2 * 4 * sizeof(double), &part2));
   hipLaunchKernelGGL(k_admm_post, dim3(nb2), dim3(DN_THREADS), 0, st, P,
-      (double const *) aux, D, (double const *) init, (double const *) norms, prox,
+      (double const *) aux, D, (double const *) init, (double const *) norms, prox, lam, rho,
       n, R, (double *) part2);
   SPD_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum4, dim3(1), dim3(DN_THREADS), 0, st, (double const *) part2, nb2, sums);
@@ -1495,7 +1511,7 @@ static int launch_admm2(Admm2Args<TM> const & a, int j, uint32_t nb, double * no
 // 3 first overwrites P, and receive the final state at the end.
 template<typename TM>
 static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
-    uint32_t R, double * P, double * D, double * aux, double * init, int prox,
+    uint32_t R, double * P, double * D, double * aux, double * init, int prox, double lam,
     uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
 {
   uint32_t const RP = dn_rpad(R);
@@ -1520,6 +1536,7 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
   a.R = R;
   a.rho = rho;
   a.prox = prox;
+  a.lam = lam;
   a.P0 = P;
   a.D0 = D;
   a.part = part;
@@ -1540,28 +1557,28 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
   }
   uint64_t const n = I * R;
   hipLaunchKernelGGL(k_admm_final, dim3(dn_blocks(n, DN_THREADS * 8)), dim3(DN_THREADS), 0, st,
-      bf, P, D, n, R, prox, (int const *) ctl);
+      bf, P, D, n, R, prox, lam, rho, (int const *) ctl);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
 
 int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const * rho,
     uint64_t I, uint32_t R, double * P, double * D, double * aux, double * init, int prox,
-    uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
+    double lam, uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
 {
   if(I == 0 || max_inner == 0) return SPD_OK;
   if(R <= 64) {
     if(mdtype == SPD_F32) {
       return admm_solve2_t<float>((float const *) M, ginv, rho, I, R, P, D, aux, init, prox,
-          max_inner, tol, sums, ctl, st);
+          lam, max_inner, tol, sums, ctl, st);
     }
     return admm_solve2_t<double>((double const *) M, ginv, rho, I, R, P, D, aux, init, prox,
-        max_inner, tol, sums, ctl, st);
+        lam, max_inner, tol, sums, ctl, st);
   }
   // ranks above 64: one step at a time, the test on the host
   int hctl[2] = {0, 0};
   for(uint64_t k=0; k < max_inner; ++k) {
-    SPD_TRY(dn_admm_step(M, mdtype, ginv, rho, I, R, P, aux, D, init, prox, sums, st));
+    SPD_TRY(dn_admm_step(M, mdtype, ginv, rho, I, R, P, aux, D, init, prox, lam, sums, st));
     double s4[4];
     SPD_HIP(hipMemcpyAsync(s4, sums, sizeof(s4), hipMemcpyDeviceToHost, st));
     SPD_HIP(hipStreamSynchronize(st));
@@ -1578,50 +1595,52 @@ int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const 
 
 int dn_admm_step(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint32_t R, double * P, double * aux,
-    double * D, double * init, int prox, double * sums, hipStream_t st)
+    double * D, double * init, int prox, double lam, double * sums, hipStream_t st)
 {
   if(mdtype == SPD_F32) {
-    return admm_t<float>((float const *) M, ginv, rho, I, R, P, aux, D, init, prox, sums, st);
+    return admm_t<float>((float const *) M, ginv, rho, I, R, P, aux, D, init, prox, lam, sums,
+        st);
   }
-  return admm_t<double>((double const *) M, ginv, rho, I, R, P, aux, D, init, prox, sums, st);
+  return admm_t<double>((double const *) M, ginv, rho, I, R, P, aux, D, init, prox, lam, sums,
+      st);
 }
 
 template<typename TM>
 static int admm_chunked_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
-    uint32_t R, uint64_t chunk, double * P, double * D, int prox, uint32_t max_inner,
+    uint32_t R, uint64_t chunk, double * P, double * D, int prox, double lam, uint32_t max_inner,
     double tol, unsigned long long * its_rows, hipStream_t st)
 {
   uint64_t const nchunks = spd_cdiv(I, chunk);
   uint32_t const RP = dn_rpad(R);
   if(R <= 64) {
     hipLaunchKernelGGL((k_admm_chunk<TM, true>), dim3((unsigned) nchunks), dim3(DN_THREADS),
-        0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, max_inner, tol, its_rows);
+        0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, lam, max_inner, tol, its_rows);
   } else {
     hipLaunchKernelGGL((k_admm_chunk<TM, false>), dim3((unsigned) nchunks), dim3(DN_THREADS),
-        0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, max_inner, tol, its_rows);
+        0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, lam, max_inner, tol, its_rows);
   }
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
 
 int dn_admm_chunked(void const * M, int mdtype, double const * ginv, double const * rho,
-    uint64_t I, uint32_t R, uint64_t chunk, double * P, double * D, int prox,
+    uint64_t I, uint32_t R, uint64_t chunk, double * P, double * D, int prox, double lam,
     uint64_t max_inner, double tol, unsigned long long * its_rows, hipStream_t st)
 {
   if(I == 0 || max_inner == 0) return SPD_OK;
   if(chunk == 0 || R == 0 || R > CH_TILE || spd_cdiv(I, chunk) > 0x7FFFFFFFull ||
-      (prox != DN_PROX_NONNEG && prox != DN_PROX_NONE)) {
+      (prox != DN_PROX_NONNEG && prox != DN_PROX_LASSO && prox != DN_PROX_NONE)) {
     spd_set_error(SPD_EINPUT, "chunked ADMM: unsupported shape (rows %llu, rank %u, "
         "chunk %llu, prox %d)", (unsigned long long) I, R, (unsigned long long) chunk, prox);
     return SPD_EINPUT;
   }
   uint32_t const mi = (max_inner > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t) max_inner;
   if(mdtype == SPD_F32) {
-    return admm_chunked_t<float>((float const *) M, ginv, rho, I, R, chunk, P, D, prox, mi,
-        tol, its_rows, st);
+    return admm_chunked_t<float>((float const *) M, ginv, rho, I, R, chunk, P, D, prox, lam,
+        mi, tol, its_rows, st);
   }
-  return admm_chunked_t<double>((double const *) M, ginv, rho, I, R, chunk, P, D, prox, mi,
-      tol, its_rows, st);
+  return admm_chunked_t<double>((double const *) M, ginv, rho, I, R, chunk, P, D, prox, lam,
+      mi, tol, its_rows, st);
 }
 
 int dn_cpd_norm(double const * const * aTa, int nmodes, uint32_t R,
@@ -1661,22 +1680,23 @@ extern "C" int spd_dense_apply(void const * M, int mdtype, double const * ginv,
 
 extern "C" int spd_dense_admm_solve(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint64_t R, double * P, double * D, double * aux,
-    double * init, int prox, uint64_t max_inner, double tol, double * sums, int * ctl)
+    double * init, int prox, double lam, uint64_t max_inner, double tol, double * sums,
+    int * ctl)
 {
   SPD_TRY(spd_init());
-  return dn_admm_solve(M, mdtype, ginv, rho, I, (uint32_t) R, P, D, aux, init, prox,
+  return dn_admm_solve(M, mdtype, ginv, rho, I, (uint32_t) R, P, D, aux, init, prox, lam,
       max_inner, tol, sums, ctl, spd_hs(nullptr));
 }
 
 extern "C" int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint64_t R, uint64_t chunk, double * P, double * D,
-    int prox, uint64_t max_inner, double tol, uint64_t * its_rows)
+    int prox, double lam, uint64_t max_inner, double tol, uint64_t * its_rows)
 {
   SPD_TRY(spd_init());
   hipStream_t const st = spd_hs(nullptr);
   SPD_TRY(spd_zero_async(its_rows, sizeof(uint64_t), st));
-  return dn_admm_chunked(M, mdtype, ginv, rho, I, (uint32_t) R, chunk, P, D, prox, max_inner,
-      tol, (unsigned long long *) its_rows, st);
+  return dn_admm_chunked(M, mdtype, ginv, rho, I, (uint32_t) R, chunk, P, D, prox, lam,
+      max_inner, tol, (unsigned long long *) its_rows, st);
 }
 
 extern "C" int spd_dense_admm_step(void const * M, int mdtype, double const * ginv,
@@ -1684,6 +1704,6 @@ extern "C" int spd_dense_admm_step(void const * M, int mdtype, double const * gi
     double * init, int prox, double * sums)
 {
   SPD_TRY(spd_init());
-  return dn_admm_step(M, mdtype, ginv, rho, I, (uint32_t) R, P, aux, D, init, prox,
+  return dn_admm_step(M, mdtype, ginv, rho, I, (uint32_t) R, P, aux, D, init, prox, 0.,
       sums, spd_hs(nullptr));
 }

@@ -11,6 +11,7 @@
 #define DN_PROX_MAXCOLNORM 1
 #define DN_PROX_MAXCOLNORM_NONNEG 2
 #define DN_PROX_NONNEG 3
+#define DN_PROX_LASSO 4
 
 // out (R x R, full, symmetric) = X^T X. When div != nullptr the columns of X
 // are first divided in place by div[r] (X[:,r] /= div[r], the reference's
@@ -43,10 +44,10 @@ int dn_apply(void const * M, int mdtype, double const * ginv, uint64_t I,
 //   init = P; aux = (M + rho (P + D)) ginv; P = prox(aux - D); D += P - aux;
 // sums[0..3] = ||P||^2, ||P - aux||^2, ||P - init||^2, ||D||^2 (after the
 // update); sums[4 .. 4+R) receive the column norms the prox used, so sums
-// must hold 4 + R doubles.
+// must hold 4 + R doubles. lam: the lasso multiplier (threshold lam / rho).
 int dn_admm_step(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint32_t R, double * P, double * aux,
-    double * D, double * init, int prox, double * sums, hipStream_t st);
+    double * D, double * init, int prox, double lam, double * sums, hipStream_t st);
 
 // The whole ADMM inner loop (up to max_inner iterations, src/cpd/admm.c:
 // 240-308): P and D are updated in place; sums (4 + R doubles) ends with the
@@ -56,15 +57,15 @@ int dn_admm_step(void const * M, int mdtype, double const * ginv,
 // used only above rank 64.
 int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const * rho,
     uint64_t I, uint32_t R, double * P, double * D, double * aux, double * init, int prox,
-    uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st);
+    double lam, uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st);
 
 // ADMM in independent chunks of `chunk` rows for a row-separable prox
-// (DN_PROX_NONNEG or none; src/cpd/admm.c:355-400): each chunk iterates to
+// (DN_PROX_NONNEG, DN_PROX_LASSO or none; src/cpd/admm.c:355-400): each chunk iterates to
 // its own convergence, up to max_inner times, one block per chunk, no host
 // synchronisation. *its_rows (device, zeroed by the caller) accumulates
 // iterations x rows per chunk. Ranks up to 3072.
 int dn_admm_chunked(void const * M, int mdtype, double const * ginv, double const * rho,
-    uint64_t I, uint32_t R, uint64_t chunk, double * P, double * D, int prox,
+    uint64_t I, uint32_t R, uint64_t chunk, double * P, double * D, int prox, double lam,
     uint64_t max_inner, double tol, unsigned long long * its_rows, hipStream_t st);
 
 // out (R x R, full) = X^T Y for two I x R matrices.

@@ -641,7 +641,7 @@ int StreamState::slice(uint32_t const * const * ind_in, double const * vals_b, u
       if(con.kind == SPD_CON_ADMM) {
         SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
         SPD_TRY(dn_admm_solve(Mbuf, SPD_F64, ginv, d_rho, I, R, Unew[m], Dual[m], aux,
-            init_buf, con.prox, cfg.max_inner, cfg.inner_tol, d_sums, ctl, st));
+            init_buf, con.prox, con.lam, cfg.max_inner, cfg.inner_tol, d_sums, ctl, st));
       } else {
         SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 0, ginv, d_rho, info, st));
         SPD_TRY(dn_apply(Mbuf, SPD_F64, ginv, I, R, Unew[m], nullptr, st));

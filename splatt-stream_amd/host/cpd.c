@@ -50,6 +50,7 @@ int spl_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
   out->prox = SPD_PROX_NONE;
   out->diag_add = 0.;
   out->chunk = 0;
+  out->lam = 0.;
   if(con->gram_func == splatt_frob_gram) {
     out->diag_add = *((val_t const *) con->data);
   } else if(con->gram_func != NULL) {
@@ -74,6 +75,9 @@ int spl_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
     out->prox = SPD_PROX_MAXCOLNORM_NONNEG;
   } else if(con->prox_func == splatt_nonneg_prox) {
     out->prox = SPD_PROX_NONNEG;
+  } else if(con->prox_func == splatt_lasso_prox) {
+    out->prox = SPD_PROX_LASSO;
+    out->lam = *((val_t const *) con->data);
   } else {
     fprintf(stderr, "SPLATT: proximity operator of '%s' is not supported.\n",
         con->description);

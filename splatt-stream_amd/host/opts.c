@@ -211,6 +211,29 @@ void splatt_nonneg_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
   }
 }
 
+/* src/cpd/constraints/lasso.c:30-52: soft thresholding at lambda / rho. */
+void splatt_lasso_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
+    idx_t const offset, void * data, val_t const rho, bool const should_parallelize)
+{
+  (void) offset; (void) should_parallelize;
+  val_t const mult = *((val_t const *) data) / rho;
+  for(idx_t x=0; x < nrows * ncols; ++x) {
+    val_t const v = primal[x];
+    if(v > mult) {
+      primal[x] = v - mult;
+    } else if(v < -mult) {
+      primal[x] = v + mult;
+    } else {
+      primal[x] = 0.;
+    }
+  }
+}
+
+static void p_lasso_free(void * data)
+{
+  free(data);
+}
+
 static splatt_error_type p_register_prox(splatt_cpd_opts * opts,
     splatt_idx_t const * modes, splatt_idx_t num,
     void (*prox)(val_t *, idx_t const, idx_t const, idx_t const, void *,
@@ -247,4 +270,29 @@ splatt_error_type splatt_register_nonneg(splatt_cpd_opts * opts,
 {
   return p_register_prox(opts, modes_included, num_modes,
       splatt_nonneg_prox, true, true, "NON-NEGATIVE");
+}
+
+/* src/cpd/constraints/lasso.c:80-107: one ADMM constraint per mode holding
+ * its own copy of the multiplier; row-separable and sparsity-inducing. */
+splatt_error_type splatt_register_lasso(splatt_cpd_opts * opts,
+    splatt_val_t const multiplier, splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes)
+{
+  for(idx_t m=0; m < num_modes; ++m) {
+    splatt_cpd_constraint * con = splatt_alloc_constraint(SPLATT_CON_ADMM);
+    con->prox_func = splatt_lasso_prox;
+    con->free_func = p_lasso_free;
+    con->hints.row_separable = true;
+    con->hints.sparsity_inducing = true;
+    snprintf(con->description, sizeof(con->description), "L1-REG (%0.1e)", multiplier);
+    val_t * mult = (val_t *) malloc(sizeof(*mult));
+    if(mult == NULL) {
+      splatt_free_constraint(con);
+      return SPLATT_ERROR_NOMEMORY;
+    }
+    *mult = multiplier;
+    con->data = mult;
+    splatt_register_constraint(opts, modes_included[m], con);
+  }
+  return SPLATT_SUCCESS;
 }

@@ -197,6 +197,8 @@ def lib():
     L.splatt_free_global_opts.argtypes = [C.POINTER(GlobalOpts)]
     L.splatt_register_frob.restype = C.c_int
     L.splatt_register_frob.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
+    L.splatt_register_lasso.restype = C.c_int
+    L.splatt_register_lasso.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
     for name in ("splatt_register_maxcolnorm", "splatt_register_maxcolnorm_nonneg",
                  "splatt_register_nonneg"):
         getattr(L, name).restype = C.c_int
@@ -218,13 +220,13 @@ def lib():
     L.spd_dense_admm_solve.restype = C.c_int
     L.spd_dense_admm_solve.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                        C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
-                                       C.c_void_p, C.c_void_p, C.c_int, C.c_uint64,
-                                       C.c_double, C.c_void_p, C.c_void_p]
+                                       C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                       C.c_uint64, C.c_double, C.c_void_p, C.c_void_p]
     L.spd_dense_admm_chunked.restype = C.c_int
     L.spd_dense_admm_chunked.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
-                                         C.c_void_p, C.c_int, C.c_uint64, C.c_double,
-                                         C.c_void_p]
+                                         C.c_void_p, C.c_int, C.c_double, C.c_uint64,
+                                         C.c_double, C.c_void_p]
     L.splatt_amd_cpd_stream_coo.restype = C.POINTER(StreamResult)
     L.splatt_amd_cpd_stream_coo.argtypes = [C.POINTER(SplattCoord), C.c_uint64, C.c_uint64,
                                             C.c_double, C.c_uint64, C.c_int,

@@ -189,17 +189,19 @@ def test_admm_step_matches_oracle(gpu_available, R, prox):
 
 
 @pytest.mark.parametrize("R,I", [(5, 1), (16, 37), (33, 500), (64, 3000)])
-@pytest.mark.parametrize("prox", [0, 1, 2, 3])
+@pytest.mark.parametrize("prox", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("tol,max_inner", [(1e-2, 50), (7e-2, 25), (0.0, 6)])
 def test_admm_solve_matches_oracle(gpu_available, R, I, prox, tol, max_inner):
-    # the whole inner loop: same iterations, P and D at 1e-8
+    # the whole inner loop: same iterations, P and D at 1e-8 (prox 4: lasso
+    # with multiplier 2)
     rng = np.random.default_rng(R * 7 + prox + int(tol * 1000))
     A = rng.standard_normal((3 * R, R))
     G = A.T @ A + 0.5 * np.eye(R)
     M = rng.standard_normal((I, R)) * 4.0
     P0 = rng.standard_normal((I, R))
     D0 = rng.standard_normal((I, R)) * 0.1
-    wantP, wantD, wits, wrho = orc.admm(prox, M, P0, D0, G, max_inner, tol)
+    lam = 2.0 if prox == 4 else 0.0
+    wantP, wantD, wits, wrho = orc.admm(prox, M, P0, D0, G, max_inner, tol, lam=lam)
     L = ps.lib()
     RP = _rp(R)
     bufs = [_dev(G), _dev(np.ones((R, R)))]
@@ -214,8 +216,8 @@ def test_admm_solve_matches_oracle(gpu_available, R, I, prox, tol, max_inner):
     sums = ps.DevBuf((4 + R) * 8)
     ctl = ps.DevBuf(8)
     assert L.spd_dense_admm_solve(dM.ptr, 0, ginv.ptr, rho.ptr, I, R, dP.ptr, dD.ptr, aux.ptr,
-                                  init.ptr, prox, max_inner, C.c_double(tol), sums.ptr,
-                                  ctl.ptr) == 0
+                                  init.ptr, prox, C.c_double(lam), max_inner, C.c_double(tol),
+                                  sums.ptr, ctl.ptr) == 0
     c = ctl.to_host((2,), np.int32)
     assert int(c[1]) == wits
     P = dP.to_host((I, R), np.float64)
@@ -340,9 +342,34 @@ def test_cpd_nonneg_chunked(gpu_available, chunk):
     _model_close(res, facs, lam, 1e-5)
 
 
-def test_admm_chunked_matches_oracle(gpu_available):
+@pytest.mark.parametrize("chunk", [50, 0])
+def test_cpd_lasso_matches_oracle(gpu_available, chunk):
+    # splatt_register_lasso on modes 0 and 2 (multiplier 1e-4 zeroes part of
+    # both factors), in chunks of 50 rows (the reference's default) or one
+    # block
+    lam_l1 = 1e-4
+
+    def configure(L, co):
+        modes = np.array([0, 2], dtype=np.uint64)
+        assert L.splatt_register_lasso(co, lam_l1, modes.ctypes.data_as(ps.u64p), 2) == \
+            ps.SUCCESS
+        for m in range(3):
+            co[0].chunk_sizes[m] = chunk
+    res, facs, lam, fits = _run_both(tensor_path("med.tns"), 5, 3,
+                                     cons=[(2, 4, 0.0, chunk, lam_l1), (1, 0, 0.0),
+                                           (2, 4, 0.0, chunk, lam_l1)],
+                                     unconstrained=False, configure=configure)
+    for m in (0, 2):
+        z = np.mean(facs[m] == 0)
+        assert 0.0 < z < 1.0
+    assert abs(res["fit"] - fits[-1]) <= 1e-6 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-5)
+
+
+@pytest.mark.parametrize("prox,lam", [(3, 0.0), (4, 2.0)])
+def test_admm_chunked_matches_oracle(gpu_available, prox, lam):
     # the chunk kernel alone: rank 72 keeps the inverse in global memory,
-    # FP32 MTTKRP input, 33-row chunks
+    # FP32 MTTKRP input, 33-row chunks; nonneg and lasso
     I, R, chunk = 300, 72, 33
     rng = np.random.default_rng(31)
     A = rng.standard_normal((3 * R, R))
@@ -350,7 +377,8 @@ def test_admm_chunked_matches_oracle(gpu_available):
     M = (rng.standard_normal((I, R)) * 4.0).astype(np.float32)
     P0 = np.abs(rng.standard_normal((I, R)))
     D0 = rng.standard_normal((I, R)) * 0.1
-    wantP, wantD, wtot = orc.admm_chunked(3, M.astype(np.float64), P0, D0, G, chunk, 50, 1e-2)
+    wantP, wantD, wtot = orc.admm_chunked(prox, M.astype(np.float64), P0, D0, G, chunk, 50,
+                                          1e-2, lam=lam)
     L = ps.lib()
     RP = _rp(R)
     grams = [G, np.ones((R, R))]
@@ -364,7 +392,8 @@ def test_admm_chunked_matches_oracle(gpu_available):
     dM, dP, dD = _dev(M), _dev(P0), _dev(D0)
     cnt = ps.DevBuf(8)
     assert L.spd_dense_admm_chunked(dM.ptr, 1, ginv.ptr, rho.ptr, I, R, chunk, dP.ptr, dD.ptr,
-                                    3, 50, C.c_double(1e-2), cnt.ptr) == 0
+                                    prox, C.c_double(lam), 50, C.c_double(1e-2),
+                                    cnt.ptr) == 0
     P = dP.to_host((I, R), np.float64)
     D = dD.to_host((I, R), np.float64)
     assert int(cnt.to_host((1,), np.uint64)[0]) == wtot

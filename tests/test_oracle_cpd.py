@@ -340,3 +340,43 @@ def test_cpd_frob_closed_form():
     G = (f1[0].T @ f1[0]) * (start[2].T @ start[2]) + mu * np.eye(3)
     M1 = np.einsum("ijk,ir,kr->jr", X, f1[0], start[2])
     assert np.allclose(f1[1] @ G, M1, rtol=1e-9, atol=1e-9)
+
+
+def test_lasso_prox_is_the_l1_proximal_point():
+    # x = prox(v) minimises (x - v)^2 / 2 + thr |x|: either x != 0 and
+    # x - v + thr sign(x) = 0, or x = 0 and |v| <= thr
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal((40, 7)) * 3.0
+    v[0, :3] = [1.5, -1.5, 0.0]                      # exactly at the threshold
+    thr = 1.5
+    x = orc.prox(4, v, thr)
+    nz = x != 0
+    assert np.all(np.abs(x[nz] - v[nz] + thr * np.sign(x[nz])) <= 1e-15 * np.abs(v[nz]).max())
+    assert np.all(np.abs(v[~nz]) <= thr)
+    assert np.all(np.sign(x[nz]) == np.sign(v[nz]))
+    assert 0 < nz.sum() < nz.size
+    # threshold 0 is the identity
+    assert np.array_equal(orc.prox(4, v, 0.0), v)
+
+
+def test_admm_lasso_meets_l1_optimality():
+    # ADMM (src/cpd/admm.c:240-308) with the lasso prox solves
+    # min_P tr(P G P^T) / 2 - tr(M^T P) + lam |P|_1: at convergence
+    # E = M - P G equals lam sign(P) where P != 0 and |E| <= lam where P = 0;
+    # lam = 0 reproduces the unconstrained run exactly
+    rng = np.random.default_rng(11)
+    I, R, lam = 60, 5, 2.0
+    A = rng.standard_normal((15, R))
+    G = A.T @ A + 0.5 * np.eye(R)
+    M = rng.standard_normal((I, R)) * 4.0
+    Z = np.zeros((I, R))
+    P, D, its, rho = orc.admm(4, M, Z, Z, G, 3000, 1e-14, lam=lam)
+    E = M - P @ G
+    nz = P != 0
+    assert 0 < nz.sum() < nz.size
+    assert np.max(np.abs(E[nz] - lam * np.sign(P[nz]))) <= 1e-5 * lam
+    assert np.max(np.abs(E[~nz])) <= lam * (1 + 1e-5)
+    P0, D0, its0, _ = orc.admm(0, M, Z, Z, G, 40, 1e-3)
+    P4, D4, its4, _ = orc.admm(4, M, Z, Z, G, 40, 1e-3, lam=0.0)
+    assert its0 == its4 and np.array_equal(P0, P4) and np.array_equal(D0, D4)
+
