@@ -93,6 +93,14 @@ splatt_error_type splatt_register_frob(
     splatt_idx_t const * const modes_included,
     splatt_idx_t const num_modes);
 
+/* include/splatt/cpd.h:279-282 (src/cpd/constraints/simplex.cc:167-191):
+ * every row of the factor on the probability simplex; row-separable, solved
+ * by chunked ADMM (this library needs chunk_sizes > 0 for it). */
+splatt_error_type splatt_register_rowsimp(
+    splatt_cpd_opts * opts,
+    splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes);
+
 /* include/splatt/cpd.h:332-336 (src/cpd/constraints/lasso.c:80-107): L1
  * regularisation, solved by ADMM with soft thresholding at multiplier/rho;
  * row-separable, so rows are solved in independent chunks by default. */

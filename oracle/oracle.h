@@ -103,7 +103,8 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
  * one chunk) with prox 1 = max-col-norm (constraints/maxcolnorm.c:32-61),
  * 2 = non-negative max-col-norm (maxcolnorm-nonneg.c:32-66),
  * 3 = non-negative (nonneg.c), 4 = L1 soft threshold at lam / rho
- * (lasso.c:30-52), 0 = none. chunk > 0 runs ADMM in independent chunks of
+ * (lasso.c:30-52), 5 = rows onto the probability simplex (simplex.cc:32-104),
+ * 0 = none. chunk > 0 runs ADMM in independent chunks of
  * that many rows (admm.c:355-400; the caller passes 0 unless the prox is
  * row-separable). */
 typedef struct { int kind; int prox; double diag_add; uint64_t chunk; double lam; } or_con;

@@ -114,12 +114,53 @@ void or_mat_normalize(double * A, uint64_t I, uint64_t R, double * lambda)
   }
 }
 
+static int p_cmp_desc(void const * a, void const * b)
+{
+  double const x = *(double const *) a;
+  double const y = *(double const *) b;
+  return (x < y) ? 1 : ((x > y) ? -1 : 0);
+}
+
+/* src/cpd/constraints/simplex.cc:32-66 (p_project_vector): sort a copy of
+ * the row into non-increasing order, accumulate from -1 until an entry no
+ * longer exceeds the running mean, theta = sum / count, row = max(row -
+ * theta, 0). */
+static void p_simplex_row(double * row, double * buf, uint64_t n)
+{
+  memcpy(buf, row, n * sizeof(double));
+  qsort(buf, n, sizeof(double), p_cmp_desc);
+  double run = -1.;
+  uint64_t piv;
+  for(piv=0; piv < n; ++piv) {
+    run += buf[piv];
+    double const v = buf[piv] - (run / (double) (piv + 1));
+    if(v <= 0.) {
+      run -= buf[piv];
+      break;
+    }
+  }
+  double const theta = run / (double) piv;
+  for(uint64_t j=0; j < n; ++j) {
+    row[j] -= theta;
+    row[j] = (row[j] > 0.) ? row[j] : 0.;
+  }
+}
+
 /* src/cpd/constraints/maxcolnorm.c:32-61, maxcolnorm-nonneg.c:32-66,
  * nonneg.c: column norms capped at 1 (after clamping negatives for the
  * non-negative variant); plain clamping for nonneg; lasso.c:30-52: entries
- * above thr lose thr, entries below -thr gain thr, the rest become 0. */
+ * above thr lose thr, entries below -thr gain thr, the rest become 0;
+ * prox 5 projects every row onto the probability simplex. */
 void or_prox(int prox, double * P, uint64_t I, uint64_t R, double thr)
 {
+  if(prox == 5) {
+    double * buf = malloc((R ? R : 1) * sizeof(double));
+    for(uint64_t i=0; i < I; ++i) {
+      p_simplex_row(P + i * R, buf, R);
+    }
+    free(buf);
+    return;
+  }
   if(prox == 4) {
     for(uint64_t x=0; x < I * R; ++x) {
       double const v = P[x];

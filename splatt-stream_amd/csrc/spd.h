@@ -165,6 +165,7 @@ int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,
 #define SPD_PROX_MAXCOLNORM_NONNEG 2
 #define SPD_PROX_NONNEG 3
 #define SPD_PROX_LASSO 4   /* soft threshold at lam / rho */
+#define SPD_PROX_ROWSIMP 5 /* rows onto the probability simplex (chunked ADMM) */
 
 typedef struct {
   int kind;          /* SPD_CON_* */

@@ -942,16 +942,23 @@ __global__ void k_ctl_reset(int * ctl)
 // entry the product with the inverse, P = prox(aux - D), D += P - aux and
 // the residual sums. Sums over the chunk are added in a fixed order
 // (thread, wave butterfly, waves in order). GL: the inverse (R <= 64) sits
-// in LDS. its_rows += iterations x rows of the chunk.
+// in LDS. SX: the prox projects every row onto the probability simplex
+// (src/cpd/constraints/simplex.cc:32-66): the tile's pre-prox rows are kept
+// in LDS, each value's place in its row's descending order is counted, the
+// row's threshold comes from the running sum in that order, and each entry
+// becomes max(p - theta, 0). its_rows += iterations x rows of the chunk.
 #define CH_TILE 3072
-template<typename TM, bool GL>
+template<typename TM, bool GL, bool SX>
 __global__ void __launch_bounds__(DN_THREADS)
 k_admm_chunk(TM const * __restrict__ M, double const * __restrict__ ginv, uint32_t RP,
     double const * __restrict__ rho_p, uint64_t I, uint32_t R, uint64_t chunk,
     double * __restrict__ P, double * __restrict__ D, int prox, double lam, uint32_t max_inner,
     double tol, unsigned long long * __restrict__ its_rows)
 {
-  __shared__ double s_aux[CH_TILE];
+  __shared__ double s_aux[CH_TILE];      // aux; with SX, then the rows in descending order
+  __shared__ double s_x[SX ? CH_TILE : 1];
+  __shared__ double s_p[SX ? CH_TILE : 1];
+  __shared__ double s_th[SX ? CH_TILE : 1];
   __shared__ double s_g[GL ? 64 * 64 : 1];
   __shared__ double s_w[DN_THREADS / 64][4];
   __shared__ int s_done;
@@ -979,19 +986,71 @@ k_admm_chunk(TM const * __restrict__ M, double const * __restrict__ ginv, uint32
         s_aux[e] = (double) M[base + e] + rho * (P[base + e] + D[base + e]);
       }
       __syncthreads();
+      if constexpr (SX) {
+        for(uint32_t e=t; e < ne; e += DN_THREADS) {
+          uint32_t const i = e / R;
+          uint32_t const j = e - i * R;
+          double const * const arow = s_aux + (uint64_t) i * R;
+          double x = 0.;
+          for(uint32_t k=0; k < R; ++k) {
+            double const g = GL ? s_g[k * R + j] : ginv[(uint64_t) k * RP + j];
+            x += arow[k] * g;
+          }
+          s_x[e] = x;
+          s_p[e] = x - D[base + e];
+        }
+        __syncthreads();
+        // place of each value in its row's non-increasing order (ties by
+        // column), written into the row's slot of s_aux
+        for(uint32_t e=t; e < ne; e += DN_THREADS) {
+          uint32_t const i = e / R;
+          uint32_t const j = e - i * R;
+          double const v = s_p[e];
+          uint32_t pos = 0;
+          for(uint32_t k=0; k < R; ++k) {
+            double const u = s_p[i * R + k];
+            pos += (u > v || (u == v && k < j)) ? 1u : 0u;
+          }
+          s_aux[i * R + pos] = v;
+        }
+        __syncthreads();
+        for(uint32_t i=t; i < (uint32_t) nr; i += DN_THREADS) {
+          double const * const b = s_aux + (uint64_t) i * R;
+          double run = -1.;
+          uint32_t piv;
+          for(piv=0; piv < R; ++piv) {
+            run += b[piv];
+            double const v = b[piv] - (run / (double) (piv + 1));
+            if(v <= 0.) {
+              run -= b[piv];
+              break;
+            }
+          }
+          s_th[i] = run / (double) piv;
+        }
+        __syncthreads();
+      }
       for(uint32_t e=t; e < ne; e += DN_THREADS) {
         uint32_t const i = e / R;
         uint32_t const j = e - i * R;
-        double const * const arow = s_aux + (uint64_t) i * R;
-        double x = 0.;
-        for(uint32_t k=0; k < R; ++k) {
-          double const g = GL ? s_g[k * R + j] : ginv[(uint64_t) k * RP + j];
-          x += arow[k] * g;
+        double x;
+        if constexpr (SX) {
+          x = s_x[e];
+        } else {
+          double const * const arow = s_aux + (uint64_t) i * R;
+          x = 0.;
+          for(uint32_t k=0; k < R; ++k) {
+            double const g = GL ? s_g[k * R + j] : ginv[(uint64_t) k * RP + j];
+            x += arow[k] * g;
+          }
         }
         double const p0 = P[base + e];
         double const d0 = D[base + e];
         double p = x - d0;
-        if(prox == DN_PROX_NONNEG) {
+        if constexpr (SX) {
+          p = s_p[e] - s_th[i];
+          p = (p > 0.) ? p : 0.;
+        } else if(prox == DN_PROX_NONNEG) {
           p = (p > 0.) ? p : 0.;
         } else if(prox == DN_PROX_LASSO) {
           p = (p > thr) ? p - thr : ((p < -thr) ? p + thr : 0.);
@@ -1566,6 +1625,10 @@ int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const 
     uint64_t I, uint32_t R, double * P, double * D, double * aux, double * init, int prox,
     double lam, uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
 {
+  if(prox == DN_PROX_ROWSIMP) {
+    spd_set_error(SPD_EINPUT, "the row-simplex prox runs in chunked ADMM only");
+    return SPD_EINPUT;
+  }
   if(I == 0 || max_inner == 0) return SPD_OK;
   if(R <= 64) {
     if(mdtype == SPD_F32) {
@@ -1597,6 +1660,10 @@ int dn_admm_step(void const * M, int mdtype, double const * ginv,
     double const * rho, uint64_t I, uint32_t R, double * P, double * aux,
     double * D, double * init, int prox, double lam, double * sums, hipStream_t st)
 {
+  if(prox == DN_PROX_ROWSIMP) {
+    spd_set_error(SPD_EINPUT, "the row-simplex prox runs in chunked ADMM only");
+    return SPD_EINPUT;
+  }
   if(mdtype == SPD_F32) {
     return admm_t<float>((float const *) M, ginv, rho, I, R, P, aux, D, init, prox, lam, sums,
         st);
@@ -1612,12 +1679,23 @@ static int admm_chunked_t(TM const * M, double const * ginv, double const * rho,
 {
   uint64_t const nchunks = spd_cdiv(I, chunk);
   uint32_t const RP = dn_rpad(R);
-  if(R <= 64) {
-    hipLaunchKernelGGL((k_admm_chunk<TM, true>), dim3((unsigned) nchunks), dim3(DN_THREADS),
-        0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, lam, max_inner, tol, its_rows);
+  bool const sx = (prox == DN_PROX_ROWSIMP);
+  if(R <= 64 && sx) {
+    hipLaunchKernelGGL((k_admm_chunk<TM, true, true>), dim3((unsigned) nchunks),
+        dim3(DN_THREADS), 0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, lam, max_inner, tol,
+        its_rows);
+  } else if(R <= 64) {
+    hipLaunchKernelGGL((k_admm_chunk<TM, true, false>), dim3((unsigned) nchunks),
+        dim3(DN_THREADS), 0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, lam, max_inner, tol,
+        its_rows);
+  } else if(sx) {
+    hipLaunchKernelGGL((k_admm_chunk<TM, false, true>), dim3((unsigned) nchunks),
+        dim3(DN_THREADS), 0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, lam, max_inner, tol,
+        its_rows);
   } else {
-    hipLaunchKernelGGL((k_admm_chunk<TM, false>), dim3((unsigned) nchunks), dim3(DN_THREADS),
-        0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, lam, max_inner, tol, its_rows);
+    hipLaunchKernelGGL((k_admm_chunk<TM, false, false>), dim3((unsigned) nchunks),
+        dim3(DN_THREADS), 0, st, M, ginv, RP, rho, I, R, chunk, P, D, prox, lam, max_inner, tol,
+        its_rows);
   }
   SPD_LAUNCH_CHECK();
   return SPD_OK;
@@ -1629,7 +1707,8 @@ int dn_admm_chunked(void const * M, int mdtype, double const * ginv, double cons
 {
   if(I == 0 || max_inner == 0) return SPD_OK;
   if(chunk == 0 || R == 0 || R > CH_TILE || spd_cdiv(I, chunk) > 0x7FFFFFFFull ||
-      (prox != DN_PROX_NONNEG && prox != DN_PROX_LASSO && prox != DN_PROX_NONE)) {
+      (prox != DN_PROX_NONNEG && prox != DN_PROX_LASSO && prox != DN_PROX_ROWSIMP &&
+       prox != DN_PROX_NONE)) {
     spd_set_error(SPD_EINPUT, "chunked ADMM: unsupported shape (rows %llu, rank %u, "
         "chunk %llu, prox %d)", (unsigned long long) I, R, (unsigned long long) chunk, prox);
     return SPD_EINPUT;

@@ -12,6 +12,7 @@
 #define DN_PROX_MAXCOLNORM_NONNEG 2
 #define DN_PROX_NONNEG 3
 #define DN_PROX_LASSO 4
+#define DN_PROX_ROWSIMP 5      // chunked ADMM only
 
 // out (R x R, full, symmetric) = X^T X. When div != nullptr the columns of X
 // are first divided in place by div[r] (X[:,r] /= div[r], the reference's
@@ -60,7 +61,8 @@ int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const 
     double lam, uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st);
 
 // ADMM in independent chunks of `chunk` rows for a row-separable prox
-// (DN_PROX_NONNEG, DN_PROX_LASSO or none; src/cpd/admm.c:355-400): each chunk iterates to
+// (DN_PROX_NONNEG, DN_PROX_LASSO, DN_PROX_ROWSIMP or none; src/cpd/admm.c:
+// 355-400): each chunk iterates to
 // its own convergence, up to max_inner times, one block per chunk, no host
 // synchronisation. *its_rows (device, zeroed by the caller) accumulates
 // iterations x rows per chunk. Ranks up to 3072.

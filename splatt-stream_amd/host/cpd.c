@@ -78,6 +78,13 @@ int spl_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
   } else if(con->prox_func == splatt_lasso_prox) {
     out->prox = SPD_PROX_LASSO;
     out->lam = *((val_t const *) con->data);
+  } else if(con->prox_func == splatt_rowsimp_prox) {
+    out->prox = SPD_PROX_ROWSIMP;
+    if(chunk == 0) {
+      fprintf(stderr, "SPLATT: '%s' (mode %" SPLATT_PF_IDX ") needs chunked ADMM "
+          "(chunk_sizes > 0) on the GPU.\n", con->description, mode);
+      return -1;
+    }
   } else {
     fprintf(stderr, "SPLATT: proximity operator of '%s' is not supported.\n",
         con->description);

@@ -11,6 +11,7 @@
  */
 #include "host.h"
 #include <math.h>
+#include <string.h>
 #include <time.h>
 #include <omp.h>
 
@@ -229,6 +230,47 @@ void splatt_lasso_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
   }
 }
 
+static int p_cmp_desc(void const * a, void const * b)
+{
+  val_t const x = *(val_t const *) a;
+  val_t const y = *(val_t const *) b;
+  return (x < y) ? 1 : ((x > y) ? -1 : 0);
+}
+
+/* src/cpd/constraints/simplex.cc:32-104: every row onto the probability
+ * simplex (sort a copy in non-increasing order, threshold from the running
+ * sum, clamp). */
+void splatt_rowsimp_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
+    idx_t const offset, void * data, val_t const rho, bool const should_parallelize)
+{
+  (void) offset; (void) data; (void) rho; (void) should_parallelize;
+  val_t * buf = (val_t *) malloc((ncols ? ncols : 1) * sizeof(*buf));
+  if(buf == NULL) {
+    return;
+  }
+  for(idx_t i=0; i < nrows; ++i) {
+    val_t * const row = primal + i * ncols;
+    memcpy(buf, row, ncols * sizeof(*buf));
+    qsort(buf, ncols, sizeof(*buf), p_cmp_desc);
+    val_t run = -1.;
+    idx_t piv;
+    for(piv=0; piv < ncols; ++piv) {
+      run += buf[piv];
+      val_t const v = buf[piv] - (run / (val_t) (piv + 1));
+      if(v <= 0.) {
+        run -= buf[piv];
+        break;
+      }
+    }
+    val_t const theta = run / (val_t) piv;
+    for(idx_t j=0; j < ncols; ++j) {
+      row[j] -= theta;
+      row[j] = (row[j] > 0.) ? row[j] : 0.;
+    }
+  }
+  free(buf);
+}
+
 static void p_lasso_free(void * data)
 {
   free(data);
@@ -295,4 +337,13 @@ splatt_error_type splatt_register_lasso(splatt_cpd_opts * opts,
     splatt_register_constraint(opts, modes_included[m], con);
   }
   return SPLATT_SUCCESS;
+}
+
+/* src/cpd/constraints/simplex.cc:167-191: row simplex, row-separable (its
+ * ADMM runs in chunks of rows; chunk_sizes must be positive). */
+splatt_error_type splatt_register_rowsimp(splatt_cpd_opts * opts,
+    splatt_idx_t const * const modes_included, splatt_idx_t const num_modes)
+{
+  return p_register_prox(opts, modes_included, num_modes,
+      splatt_rowsimp_prox, true, true, "ROW-SIMPLEX");
 }

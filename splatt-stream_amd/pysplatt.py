@@ -25,6 +25,7 @@ OPT_NTHREADS, OPT_TOLERANCE, OPT_REGULARIZE, OPT_NITER, OPT_VERBOSITY, \
 CSF_ONEMODE, CSF_TWOMODE, CSF_ALLMODE = 0, 1, 2
 NOTILE, DENSETILE = 0, 1
 SUCCESS = 1
+ERROR_BADINPUT = 2
 F64, F32 = 0, 1
 LAYOUT_AUTO, LAYOUT_CSF = 0, 1
 
@@ -200,7 +201,7 @@ def lib():
     L.splatt_register_lasso.restype = C.c_int
     L.splatt_register_lasso.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
     for name in ("splatt_register_maxcolnorm", "splatt_register_maxcolnorm_nonneg",
-                 "splatt_register_nonneg"):
+                 "splatt_register_nonneg", "splatt_register_rowsimp"):
         getattr(L, name).restype = C.c_int
         getattr(L, name).argtypes = [C.POINTER(CpdOpts), u64p, C.c_uint64]
     L.spd_dense_gram.restype = C.c_int

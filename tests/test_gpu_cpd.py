@@ -366,10 +366,40 @@ def test_cpd_lasso_matches_oracle(gpu_available, chunk):
     _model_close(res, facs, lam, 1e-5)
 
 
-@pytest.mark.parametrize("prox,lam", [(3, 0.0), (4, 2.0)])
+@pytest.mark.parametrize("chunk", [50, 7])
+def test_cpd_rowsimp_matches_oracle(gpu_available, chunk):
+    # splatt_register_rowsimp on mode 0: chunked ADMM whose prox projects
+    # every row onto the probability simplex
+    def configure(L, co):
+        modes = np.array([0], dtype=np.uint64)
+        assert L.splatt_register_rowsimp(co, modes.ctypes.data_as(ps.u64p), 1) == ps.SUCCESS
+        for m in range(3):
+            co[0].chunk_sizes[m] = chunk
+    res, facs, lam, fits = _run_both(tensor_path("med.tns"), 5, 3,
+                                     cons=[(2, 5, 0.0, chunk), (1, 0, 0.0), (1, 0, 0.0)],
+                                     unconstrained=False, configure=configure)
+    A = res["factors"][0]
+    assert np.all(A >= 0)
+    assert np.allclose(A.sum(axis=1), 1.0, atol=1e-6)
+    assert abs(res["fit"] - fits[-1]) <= 1e-6 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-5)
+
+
+def test_cpd_rowsimp_needs_chunks(gpu_available):
+    def configure(L, co):
+        modes = np.array([0], dtype=np.uint64)
+        L.splatt_register_rowsimp(co, modes.ctypes.data_as(ps.u64p), 1)
+        for m in range(3):
+            co[0].chunk_sizes[m] = 0
+    cs = _twomode(path=tensor_path("med.tns"))
+    res = ps.run_cpd(cs, 5, max_its=1, tol=0.0, seed=7, configure=configure)
+    assert res["rc"] == ps.ERROR_BADINPUT
+
+
+@pytest.mark.parametrize("prox,lam", [(3, 0.0), (4, 2.0), (5, 0.0)])
 def test_admm_chunked_matches_oracle(gpu_available, prox, lam):
     # the chunk kernel alone: rank 72 keeps the inverse in global memory,
-    # FP32 MTTKRP input, 33-row chunks; nonneg and lasso
+    # FP32 MTTKRP input, 33-row chunks; nonneg, lasso and row simplex
     I, R, chunk = 300, 72, 33
     rng = np.random.default_rng(31)
     A = rng.standard_normal((3 * R, R))

@@ -380,3 +380,26 @@ def test_admm_lasso_meets_l1_optimality():
     P4, D4, its4, _ = orc.admm(4, M, Z, Z, G, 40, 1e-3, lam=0.0)
     assert its0 == its4 and np.array_equal(P0, P4) and np.array_equal(D0, D4)
 
+
+def test_rowsimp_prox_is_the_simplex_projection():
+    # the Euclidean projection of v onto {x >= 0, sum x = 1} is
+    # max(v - theta, 0) with theta the root of sum max(v - theta, 0) = 1;
+    # theta is found here by bisection, independently of the sort
+    rng = np.random.default_rng(8)
+    V = rng.standard_normal((30, 6)) * 2.0
+    V[0] = [0.3, 0.3, 0.3, 0.3, -1.0, 2.0]             # ties
+    V[1] = [5.0, -3.0, 0.1, 0.2, 0.0, 0.0]             # one active entry
+    X = orc.prox(5, V)
+    for v, x in zip(V, X):
+        lo, hi = v.min() - 1.0, v.max()
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            if np.maximum(v - mid, 0.0).sum() > 1.0:
+                lo = mid
+            else:
+                hi = mid
+        want = np.maximum(v - 0.5 * (lo + hi), 0.0)
+        assert np.all(x >= 0) and abs(x.sum() - 1.0) <= 1e-12
+        assert np.allclose(x, want, atol=1e-12)
+    assert np.array_equal(orc.prox(5, np.array([[4.0]])), np.array([[1.0]]))
+
