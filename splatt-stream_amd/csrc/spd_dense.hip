@@ -395,19 +395,20 @@ k_chol_inv(GramSrc g, int nmodes, int mode, uint32_t R, uint32_t RP,
 // The same for R <= RP <= 64 by Gauss-Jordan elimination in place (no
 // pivoting: G is symmetric positive definite, and its pivots are the
 // squares of the Cholesky diagonal, so a non-positive one is reported the
-// same way). Every step updates all RP x RP entries spread over the 64
-// threads, so the work per thread is RP^3/64 independent LDS updates rather
+// same way). Every step updates all RP x RP entries spread over the NT
+// threads, so the work per thread is RP^3/NT independent LDS updates rather
 // than a chain of dependent ones. The result is written from its upper
 // triangle, so it is exactly symmetric.
-template<int RP>
-__global__ void __launch_bounds__(64)
+template<int RP, int NT>
+__global__ void __launch_bounds__(NT)
 k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
     double * __restrict__ ginv, double * __restrict__ rho_out, int * __restrict__ info)
 {
   __shared__ double A[RP][RP + 1];
   __shared__ double colj[RP];
   __shared__ double s_rho;
-  constexpr int NE = RP * RP / 64;
+  constexpr int NE = RP * RP / NT;
+  static_assert(NE * NT == RP * RP && RP <= NT, "entries must split evenly");
   int const t = threadIdx.x;
   int const iR = (int) R;
   // the Hadamard product into A, one Gram at a time in chunks of 16 entries
@@ -415,7 +416,7 @@ k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
   // written by other kernels and come from far away)
   #pragma unroll
   for(int q=0; q < NE; ++q) {
-    int const e = t + 64 * q;
+    int const e = t + NT * q;
     int const r = e / RP;
     int const k = e - r * RP;
     A[r][k] = (r < iR && k < iR) ? 1. : ((k == r) ? 1. : 0.);
@@ -426,26 +427,34 @@ k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
     double const * const gm = g.aTa[m];
     #pragma unroll
     for(int q0=0; q0 < NE; q0 += CH) {
+      // chunks past NE (when CH does not divide NE) are skipped at compile time
       double gv[CH];
       #pragma unroll
       for(int q=0; q < CH; ++q) {
-        int const e = t + 64 * (q0 + q);
-        int const r = e / RP;
-        int const k = e - r * RP;
-        bool const ok = (r < iR && k < iR);
-        int const lo = r < k ? r : k;
-        int const hi = r < k ? k : r;
-        gv[q] = gm[ok ? lo * R + hi : 0];
+        if(q0 + q < NE) {
+          int const e = t + NT * (q0 + q);
+          int const r = e / RP;
+          int const k = e - r * RP;
+          bool const ok = (r < iR && k < iR);
+          int const lo = r < k ? r : k;
+          int const hi = r < k ? k : r;
+          gv[q] = gm[ok ? lo * R + hi : 0];
+        }
       }
       #pragma unroll
       for(int q=0; q < CH; ++q) {
-        int const e = t + 64 * (q0 + q);
-        int const r = e / RP;
-        int const k = e - r * RP;
-        A[r][k] *= (r < iR && k < iR) ? gv[q] : 1.;   // branch-free: x * 1 == x
+        if(q0 + q < NE) {
+          int const e = t + NT * (q0 + q);
+          int const r = e / RP;
+          int const k = e - r * RP;
+          A[r][k] *= (r < iR && k < iR) ? gv[q] : 1.;   // branch-free: x * 1 == x
+        }
       }
     }
   }
+  // the diagonal entry (t, t) belongs to another thread of the staging
+  // (in another wave when NT > 64): wait for the Hadamard product first
+  __syncthreads();
   if(t < iR) A[t][t] += diag_add;
   __syncthreads();
   if(admm) {
@@ -485,30 +494,38 @@ k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
       }
     }
     __syncthreads();
+    // when RP divides NT a thread's entries share one column k = t % RP, so
+    // the row-j factor is read once per step (row j itself passes unchanged)
+    constexpr bool KF = (NT % RP) == 0;
+    double const ajk0 = KF ? A[j][t % RP] : 0.;
     #pragma unroll
     for(int q0=0; q0 < NE; q0 += UC) {
       double ci[UC], ajk[UC], aik[UC];
       #pragma unroll
       for(int q=0; q < UC; ++q) {
-        int const e = t + 64 * (q0 + q);
-        int const i = e / RP;
-        int const k = e - i * RP;
-        ci[q] = colj[i];
-        ajk[q] = A[j][k];
-        aik[q] = A[i][k];
+        if(q0 + q < NE) {
+          int const e = t + NT * (q0 + q);
+          int const i = e / RP;
+          int const k = e - i * RP;
+          ci[q] = colj[i];
+          ajk[q] = KF ? ajk0 : A[j][k];
+          aik[q] = A[i][k];
+        }
       }
       #pragma unroll
       for(int q=0; q < UC; ++q) {
-        int const e = t + 64 * (q0 + q);
-        int const i = e / RP;
-        int const k = e - i * RP;
-        A[i][k] = aik[q] - ci[q] * ajk[q];
+        if(q0 + q < NE) {
+          int const e = t + NT * (q0 + q);
+          int const i = e / RP;
+          int const k = e - i * RP;
+          A[i][k] = aik[q] - ci[q] * ajk[q];
+        }
       }
     }
     __syncthreads();
   }
   if(t == 0 && bad != 0 && *info == 0) *info = bad;
-  for(int e=t; e < RP * RP; e += 64) {
+  for(int e=t; e < RP * RP; e += NT) {
     int const q = e / RP;
     int const i = e - q * RP;
     bool const in = (i < iR) && (q < iR);
@@ -1367,13 +1384,13 @@ int dn_chol_inv(double const * const * aTa, int nmodes, int mode, uint32_t R,
   uint32_t const RP = dn_rpad(R);
   if(RP <= 64) {
     switch(RP) {
-    case 16: hipLaunchKernelGGL((k_gj_inv<16>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+    case 16: hipLaunchKernelGGL((k_gj_inv<16, 64>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
-    case 32: hipLaunchKernelGGL((k_gj_inv<32>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+    case 32: hipLaunchKernelGGL((k_gj_inv<32, 128>), dim3(1), dim3(128), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
-    case 48: hipLaunchKernelGGL((k_gj_inv<48>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+    case 48: hipLaunchKernelGGL((k_gj_inv<48, 128>), dim3(1), dim3(128), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
-    default: hipLaunchKernelGGL((k_gj_inv<64>), dim3(1), dim3(64), 0, st, g, nmodes, mode,
+    default: hipLaunchKernelGGL((k_gj_inv<64, 256>), dim3(1), dim3(256), 0, st, g, nmodes, mode,
                  R, diag_add, admm, ginv, rho, info); break;
     }
     SPD_LAUNCH_CHECK();
