@@ -137,7 +137,7 @@ k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
 
 #define R3_WAVES 4
 #define R3_WN 256                  // nonzeros per wave
-#define R3_BATCH 8
+#define R3_BATCH 6
 
 template<typename T>
 struct Root3Args {
@@ -225,7 +225,7 @@ combine_runs(uint32_t const * rows, T const * vals, int ld, uint32_t e0, uint32_
 }
 
 template<typename T, int V, int L>
-__global__ void __launch_bounds__(R3_WAVES * 64)
+__global__ void __launch_bounds__(R3_WAVES * 64, 5)
 k_root3(Root3Args<T> a)
 {
   typedef typename VecT<T, V>::type VT;
@@ -240,12 +240,22 @@ k_root3(Root3Args<T> a)
   // their own records fall in different LDS banks
   __shared__ uint4 s_rec[R3_WAVES][R3_WN + G];
   __shared__ T s_v64[sizeof(T) == 8 ? R3_WAVES : 1][sizeof(T) == 8 ? R3_WN + G : 1];
-  __shared__ uint32_t s_arow[R3_WAVES][R3_WN + 1];   // A row offset of local fiber k
-  __shared__ uint16_t s_fi[R3_WAVES][R3_WN];     // fiber-start marks
-  __shared__ uint16_t s_si[R3_WAVES][R3_WN];     // slice-start marks
   __shared__ uint32_t s_srow[R3_WAVES][R3_WN + 1];   // output row of slice S0 + k
   __shared__ uint32_t s_crow[NE];
-  __shared__ __align__(16) T s_cval[NE][CW];
+  // staging-only arrays share their LDS with the walk's partial rows (the
+  // block barrier between staging and walking separates the two uses):
+  // s_arow (A row offset of local fiber k), s_fi / s_si (fiber- and
+  // slice-start marks), then s_cval (two partial rows per item). With the
+  // batch of 6 this keeps five blocks (20 waves) per CU.
+  constexpr int SZ_AROW = R3_WAVES * (R3_WN + 1) * 4;
+  constexpr int SZ_MARK = R3_WAVES * R3_WN * 2;
+  constexpr int SZ_STAGE = SZ_AROW + 2 * SZ_MARK;
+  constexpr int SZ_WALK = NE * CW * (int) sizeof(T);
+  __shared__ __align__(16) unsigned char s_u[SZ_STAGE > SZ_WALK ? SZ_STAGE : SZ_WALK];
+  auto s_arow = reinterpret_cast<uint32_t (*)[R3_WN + 1]>(s_u);
+  auto s_fi = reinterpret_cast<uint16_t (*)[R3_WN]>(s_u + SZ_AROW);
+  auto s_si = reinterpret_cast<uint16_t (*)[R3_WN]>(s_u + SZ_AROW + SZ_MARK);
+  auto s_cval = reinterpret_cast<T (*)[CW]>(s_u);
   __shared__ uint32_t s_wrow[R3_WAVES][2];
   __shared__ __align__(16) T s_wval[R3_WAVES][2][CW];
 
