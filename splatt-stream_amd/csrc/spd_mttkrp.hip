@@ -650,8 +650,10 @@ walk_range(WsArgs<T> const & a, uint32_t const (*s_gid)[WS_WN], uint32_t const *
   }
 }
 
+// FP32 walks of up to 4 modes run four waves per SIMD (batch of 4, no
+// spills); FP64 and higher orders keep their registers.
 template<typename T, int N, int KIND, int V, int L, int B>
-__global__ void __launch_bounds__(WS_WAVES * 64)
+__global__ void __launch_bounds__(WS_WAVES * 64, (sizeof(T) == 4 && N <= 4) ? 4 : 1)
 k_walks(WsArgs<T> a)
 {
   constexpr int NL = N - 1;                   // internal levels 0..N-2
@@ -1099,7 +1101,7 @@ template<typename T, int N, int KIND, int V, int L>
 static int launch_walks(WsArgs<T> a, hipStream_t st)
 {
   constexpr int CW = L * V;
-  constexpr int B = (N <= 4) ? 8 : 4;
+  constexpr int B = (N <= 4 && sizeof(T) == 8) ? 8 : 4;
   uint32_t const ny = (uint32_t) spd_cdiv(a.R, (uint64_t) CW);
   uint64_t nb = spd_cdiv(a.nchunks, WS_WAVES);
   if(nb > WS_BLOCKS) nb = WS_BLOCKS;
