@@ -93,6 +93,15 @@ splatt_error_type splatt_register_frob(
     splatt_idx_t const * const modes_included,
     splatt_idx_t const num_modes);
 
+/* include/splatt/cpd.h:349-353 (src/cpd/constraints/smooth.c:220-247):
+ * smooth columns (a second-difference penalty with this multiplier), solved
+ * by ADMM whose prox is a banded solve along each column. */
+splatt_error_type splatt_register_smooth(
+    splatt_cpd_opts * opts,
+    splatt_val_t const multiplier,
+    splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes);
+
 /* include/splatt/cpd.h:279-282 (src/cpd/constraints/simplex.cc:167-191):
  * every row of the factor on the probability simplex; row-separable, solved
  * by chunked ADMM (this library needs chunk_sizes > 0 for it). */

@@ -104,7 +104,7 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
  * 2 = non-negative max-col-norm (maxcolnorm-nonneg.c:32-66),
  * 3 = non-negative (nonneg.c), 4 = L1 soft threshold at lam / rho
  * (lasso.c:30-52), 5 = rows onto the probability simplex (simplex.cc:32-104),
- * 0 = none. chunk > 0 runs ADMM in independent chunks of
+ * 6 = column smoothness with multiplier lam (smooth.c:30-184), 0 = none. chunk > 0 runs ADMM in independent chunks of
  * that many rows (admm.c:355-400; the caller passes 0 unless the prox is
  * row-separable). */
 typedef struct { int kind; int prox; double diag_add; uint64_t chunk; double lam; } or_con;
@@ -132,6 +132,11 @@ uint64_t or_admm(int prox, double const * M, double * P, double * D,
 uint64_t or_admm_chunked(int prox, double const * M, double * P, double * D,
     double * G, uint64_t I, uint64_t R, uint64_t chunk, uint64_t max_inner,
     double inner_tol, double lam);
+
+/* The smoothness prox (src/cpd/constraints/smooth.c): each column x of P
+ * becomes rho (lam K + rho I)^{-1} x, K the reference's banded
+ * second-difference matrix. */
+void or_smooth_prox(double * P, uint64_t I, uint64_t R, double lam, double rho);
 
 /* The proximal operators (row-major I x R, in place); thr is the lasso
  * threshold lam / rho (unused by the others). */

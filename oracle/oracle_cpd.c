@@ -203,6 +203,52 @@ void or_prox(int prox, double * P, uint64_t I, uint64_t R, double thr)
   }
 }
 
+/* src/cpd/constraints/smooth.c:30-184: every column x of P (length I) is
+ * replaced by rho A^{-1} x, A = lam K + rho I with K the pentadiagonal
+ * second-difference stencil [1 -4 6 -4 1] whose first and last diagonal
+ * entries are 5 (p_form_banded). A is symmetric positive definite; it is
+ * factored A = L D L^T (L unit lower with two subdiagonals) and solved per
+ * column. */
+void or_smooth_prox(double * P, uint64_t I, uint64_t R, double lam, double rho)
+{
+  if(I == 0) {
+    return;
+  }
+  double * f = malloc(3 * I * sizeof(double));
+  double dm2 = 0., dm1 = 0., l1m1 = 0.;
+  for(uint64_t i=0; i < I; ++i) {
+    double const a = ((i == 0 || i == I - 1) ? 5. * lam : 6. * lam) + rho;
+    double const l2 = (i >= 2) ? lam / dm2 : 0.;
+    double const l1 = (i >= 1) ? (-4. * lam - l2 * dm2 * l1m1) / dm1 : 0.;
+    double const d = a - l2 * l2 * dm2 - l1 * l1 * dm1;
+    f[3 * i] = d;
+    f[3 * i + 1] = l1;
+    f[3 * i + 2] = l2;
+    dm2 = dm1;
+    dm1 = d;
+    l1m1 = l1;
+  }
+  for(uint64_t c=0; c < R; ++c) {
+    double y1 = 0., y2 = 0.;
+    for(uint64_t i=0; i < I; ++i) {
+      double const y = P[i * R + c] - f[3 * i + 1] * y1 - f[3 * i + 2] * y2;
+      P[i * R + c] = y / f[3 * i];
+      y2 = y1;
+      y1 = y;
+    }
+    double x1 = 0., x2 = 0.;
+    for(uint64_t i=I; i-- > 0;) {
+      double const l1n = (i + 1 < I) ? f[3 * (i + 1) + 1] : 0.;
+      double const l2n = (i + 2 < I) ? f[3 * (i + 2) + 2] : 0.;
+      double const x = P[i * R + c] - l1n * x1 - l2n * x2;
+      P[i * R + c] = x * rho;
+      x2 = x1;
+      x1 = x;
+    }
+  }
+  free(f);
+}
+
 /* p_admm_iterate_chunk (src/cpd/admm.c:240-308) on rows of one chunk: L is
  * the Cholesky factor of G + rho I. */
 static uint64_t p_admm_iterate(int prox, double const * M, double * P, double * D,
@@ -221,7 +267,11 @@ static uint64_t p_admm_iterate(int prox, double const * M, double * P, double * 
     for(uint64_t x=0; x < I * R; ++x) {
       P[x] = aux[x] - D[x];
     }
-    or_prox(prox, P, I, R, lam / rho);
+    if(prox == 6) {
+      or_smooth_prox(P, I, R, lam, rho);
+    } else {
+      or_prox(prox, P, I, R, lam / rho);
+    }
     double dual_norm = 0.;
     for(uint64_t x=0; x < I * R; ++x) {
       D[x] += P[x] - aux[x];

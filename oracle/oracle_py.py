@@ -301,6 +301,14 @@ def prox(kind, P, thr=0.0):
     return P
 
 
+def smooth_prox(P, lam, rho):
+    P = np.ascontiguousarray(P, dtype=np.float64).copy()
+    I, R = P.shape
+    lib().or_smooth_prox(_p(P, C.c_double), C.c_uint64(I), C.c_uint64(R), C.c_double(lam),
+                         C.c_double(rho))
+    return P
+
+
 # ---- streaming ---------------------------------------------------------------
 
 class _OrStreamCfg(C.Structure):

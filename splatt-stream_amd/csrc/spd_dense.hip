@@ -1496,10 +1496,79 @@ int dn_apply(void const * M, int mdtype, double const * ginv, uint64_t I,
   return apply_t<double>((double const *) M, ginv, I, R, X, colnorm, st);
 }
 
+// ---- smoothness prox (src/cpd/constraints/smooth.c:30-184) ---------------
+// Each column x of P becomes rho A^{-1} x with A = lam K + rho I, K the
+// reference's banded second-difference matrix (p_form_banded: stencil
+// [1 -4 6 -4 1], first and last diagonal entries 5). A is symmetric
+// positive definite, so it is factored A = L D L^T without pivoting (L unit
+// lower with two subdiagonals) by one thread; then one lane per column runs
+// the forward and backward substitutions down the rows (rows are read
+// coalesced across the lanes). Both passes are sequential in I: this is a
+// latency-bound path for a constraint off the hot path.
+#define SLOT_SMOOTH 8
+
+// f[3i] = D_i, f[3i+1] = L(i, i-1), f[3i+2] = L(i, i-2)
+__global__ void k_smooth_factor(uint64_t I, double lam, double const * __restrict__ rho_p,
+    double * __restrict__ f)
+{
+  if(threadIdx.x != 0 || blockIdx.x != 0) return;
+  double const rho = *rho_p;
+  double dm2 = 0., dm1 = 0., l1m1 = 0.;
+  for(uint64_t i=0; i < I; ++i) {
+    double const a = ((i == 0 || i == I - 1) ? 5. * lam : 6. * lam) + rho;
+    double const l2 = (i >= 2) ? lam / dm2 : 0.;
+    double const l1 = (i >= 1) ? (-4. * lam - l2 * dm2 * l1m1) / dm1 : 0.;
+    double const d = a - l2 * l2 * dm2 - l1 * l1 * dm1;
+    f[3 * i] = d;
+    f[3 * i + 1] = l1;
+    f[3 * i + 2] = l2;
+    dm2 = dm1;
+    dm1 = d;
+    l1m1 = l1;
+  }
+}
+
+__global__ void k_smooth_solve(double * __restrict__ P, uint64_t I, uint32_t R,
+    double const * __restrict__ f, double const * __restrict__ rho_p)
+{
+  uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
+  if(c >= R) return;
+  double const rho = *rho_p;
+  double y1 = 0., y2 = 0.;
+  for(uint64_t i=0; i < I; ++i) {
+    double const y = P[i * R + c] - f[3 * i + 1] * y1 - f[3 * i + 2] * y2;
+    P[i * R + c] = y / f[3 * i];
+    y2 = y1;
+    y1 = y;
+  }
+  double x1 = 0., x2 = 0.;
+  for(uint64_t i=I; i-- > 0;) {
+    double const l1n = (i + 1 < I) ? f[3 * (i + 1) + 1] : 0.;
+    double const l2n = (i + 2 < I) ? f[3 * (i + 2) + 2] : 0.;
+    double const x = P[i * R + c] - l1n * x1 - l2n * x2;
+    P[i * R + c] = x * rho;
+    x2 = x1;
+    x1 = x;
+  }
+}
+
+// Factor the smoothness matrix for rows I and the device rho into scratch.
+static int smooth_factor(uint64_t I, double lam, double const * rho, double ** f,
+    hipStream_t st)
+{
+  void * p = nullptr;
+  SPD_TRY(spd_scratch(SLOT_SMOOTH, 3 * (I ? I : 1) * sizeof(double), &p));
+  hipLaunchKernelGGL(k_smooth_factor, dim3(1), dim3(64), 0, st, I, lam, rho, (double *) p);
+  SPD_LAUNCH_CHECK();
+  *f = (double *) p;
+  return SPD_OK;
+}
+
+// sf: the factors of the smoothness matrix when prox is DN_PROX_SMOOTH
 template<typename TM>
 static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
     uint32_t R, double * P, double * aux, double * D, double * init, int prox, double lam,
-    double * sums, hipStream_t st)
+    double const * sf, double * sums, hipStream_t st)
 {
   if(I == 0) return SPD_OK;
   uint32_t const RP = dn_rpad(R);
@@ -1550,6 +1619,11 @@ static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_
           (double const *) part, nb, R, RP, 0, (double const *) nullptr, norms);
       SPD_LAUNCH_CHECK();
     }
+  }
+  if(prox == DN_PROX_SMOOTH) {
+    hipLaunchKernelGGL(k_smooth_solve, dim3((unsigned) spd_cdiv(R, 64)), dim3(64), 0, st,
+        P, I, R, sf, rho);
+    SPD_LAUNCH_CHECK();
   }
   uint32_t const nb2 = dn_blocks(n, DN_THREADS * 8);
   void * part2 = nullptr;
@@ -1647,7 +1721,9 @@ int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const 
     return SPD_EINPUT;
   }
   if(I == 0 || max_inner == 0) return SPD_OK;
-  if(R <= 64) {
+  double * sf = nullptr;
+  if(prox == DN_PROX_SMOOTH) SPD_TRY(smooth_factor(I, lam, rho, &sf, st));
+  if(R <= 64 && prox != DN_PROX_SMOOTH) {
     if(mdtype == SPD_F32) {
       return admm_solve2_t<float>((float const *) M, ginv, rho, I, R, P, D, aux, init, prox,
           lam, max_inner, tol, sums, ctl, st);
@@ -1655,10 +1731,17 @@ int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const 
     return admm_solve2_t<double>((double const *) M, ginv, rho, I, R, P, D, aux, init, prox,
         lam, max_inner, tol, sums, ctl, st);
   }
-  // ranks above 64: one step at a time, the test on the host
+  // ranks above 64 and the smoothness prox: one step at a time, the test on
+  // the host
   int hctl[2] = {0, 0};
   for(uint64_t k=0; k < max_inner; ++k) {
-    SPD_TRY(dn_admm_step(M, mdtype, ginv, rho, I, R, P, aux, D, init, prox, lam, sums, st));
+    if(mdtype == SPD_F32) {
+      SPD_TRY(admm_t<float>((float const *) M, ginv, rho, I, R, P, aux, D, init, prox, lam, sf,
+          sums, st));
+    } else {
+      SPD_TRY(admm_t<double>((double const *) M, ginv, rho, I, R, P, aux, D, init, prox, lam,
+          sf, sums, st));
+    }
     double s4[4];
     SPD_HIP(hipMemcpyAsync(s4, sums, sizeof(s4), hipMemcpyDeviceToHost, st));
     SPD_HIP(hipStreamSynchronize(st));
@@ -1681,12 +1764,14 @@ int dn_admm_step(void const * M, int mdtype, double const * ginv,
     spd_set_error(SPD_EINPUT, "the row-simplex prox runs in chunked ADMM only");
     return SPD_EINPUT;
   }
+  double * sf = nullptr;
+  if(prox == DN_PROX_SMOOTH) SPD_TRY(smooth_factor(I, lam, rho, &sf, st));
   if(mdtype == SPD_F32) {
-    return admm_t<float>((float const *) M, ginv, rho, I, R, P, aux, D, init, prox, lam, sums,
-        st);
+    return admm_t<float>((float const *) M, ginv, rho, I, R, P, aux, D, init, prox, lam, sf,
+        sums, st);
   }
-  return admm_t<double>((double const *) M, ginv, rho, I, R, P, aux, D, init, prox, lam, sums,
-      st);
+  return admm_t<double>((double const *) M, ginv, rho, I, R, P, aux, D, init, prox, lam, sf,
+      sums, st);
 }
 
 template<typename TM>

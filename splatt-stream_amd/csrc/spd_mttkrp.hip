@@ -225,7 +225,7 @@ combine_runs(uint32_t const * rows, T const * vals, int ld, uint32_t e0, uint32_
 }
 
 template<typename T, int V, int L>
-__global__ void __launch_bounds__(R3_WAVES * 64, 5)
+__global__ void __launch_bounds__(R3_WAVES * 64, sizeof(T) == 4 ? 5 : 1)
 k_root3(Root3Args<T> a)
 {
   typedef typename VecT<T, V>::type VT;
@@ -650,10 +650,12 @@ walk_range(WsArgs<T> const & a, uint32_t const (*s_gid)[WS_WN], uint32_t const *
   }
 }
 
-// FP32 walks of up to 4 modes run four waves per SIMD (batch of 4, no
-// spills); FP64 and higher orders keep their registers.
+// FP32 root and copy walks of up to 4 modes run four waves per SIMD (batch
+// of 4, no spills); FP64, higher orders and the LDS-heavy internal and leaf
+// walks keep their registers.
 template<typename T, int N, int KIND, int V, int L, int B>
-__global__ void __launch_bounds__(WS_WAVES * 64, (sizeof(T) == 4 && N <= 4) ? 4 : 1)
+__global__ void __launch_bounds__(WS_WAVES * 64,
+    (sizeof(T) == 4 && N <= 4 && (KIND == KIND_ROOT || KIND == KIND_COPY)) ? 4 : 1)
 k_walks(WsArgs<T> a)
 {
   constexpr int NL = N - 1;                   // internal levels 0..N-2

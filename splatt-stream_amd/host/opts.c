@@ -271,6 +271,56 @@ void splatt_rowsimp_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
   free(buf);
 }
 
+/* src/cpd/constraints/smooth.c:30-184: every column x of the factor
+ * becomes rho (lambda K + rho I)^{-1} x, K the banded second-difference
+ * matrix of p_form_banded; symmetric positive definite, solved here by a
+ * banded L D L^T factorisation. */
+void splatt_smooth_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
+    idx_t const offset, void * data, val_t const rho, bool const should_parallelize)
+{
+  (void) offset; (void) should_parallelize;
+  val_t const lam = *((val_t const *) data);
+  if(nrows == 0) {
+    return;
+  }
+  val_t * f = (val_t *) malloc(3 * nrows * sizeof(*f));
+  if(f == NULL) {
+    return;
+  }
+  val_t dm2 = 0., dm1 = 0., l1m1 = 0.;
+  for(idx_t i=0; i < nrows; ++i) {
+    val_t const a = ((i == 0 || i == nrows - 1) ? 5. * lam : 6. * lam) + rho;
+    val_t const l2 = (i >= 2) ? lam / dm2 : 0.;
+    val_t const l1 = (i >= 1) ? (-4. * lam - l2 * dm2 * l1m1) / dm1 : 0.;
+    val_t const d = a - l2 * l2 * dm2 - l1 * l1 * dm1;
+    f[3 * i] = d;
+    f[3 * i + 1] = l1;
+    f[3 * i + 2] = l2;
+    dm2 = dm1;
+    dm1 = d;
+    l1m1 = l1;
+  }
+  for(idx_t c=0; c < ncols; ++c) {
+    val_t y1 = 0., y2 = 0.;
+    for(idx_t i=0; i < nrows; ++i) {
+      val_t const y = primal[i * ncols + c] - f[3 * i + 1] * y1 - f[3 * i + 2] * y2;
+      primal[i * ncols + c] = y / f[3 * i];
+      y2 = y1;
+      y1 = y;
+    }
+    val_t x1 = 0., x2 = 0.;
+    for(idx_t i=nrows; i-- > 0;) {
+      val_t const l1n = (i + 1 < nrows) ? f[3 * (i + 1) + 1] : 0.;
+      val_t const l2n = (i + 2 < nrows) ? f[3 * (i + 2) + 2] : 0.;
+      val_t const x = primal[i * ncols + c] - l1n * x1 - l2n * x2;
+      primal[i * ncols + c] = x * rho;
+      x2 = x1;
+      x1 = x;
+    }
+  }
+  free(f);
+}
+
 static void p_lasso_free(void * data)
 {
   free(data);
@@ -346,4 +396,28 @@ splatt_error_type splatt_register_rowsimp(splatt_cpd_opts * opts,
 {
   return p_register_prox(opts, modes_included, num_modes,
       splatt_rowsimp_prox, true, true, "ROW-SIMPLEX");
+}
+
+/* src/cpd/constraints/smooth.c:220-247: one ADMM constraint per mode holding
+ * its multiplier; not row-separable (the prox couples the rows of a
+ * column). */
+splatt_error_type splatt_register_smooth(splatt_cpd_opts * opts,
+    splatt_val_t const multiplier, splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes)
+{
+  for(idx_t m=0; m < num_modes; ++m) {
+    splatt_cpd_constraint * con = splatt_alloc_constraint(SPLATT_CON_ADMM);
+    con->prox_func = splatt_smooth_prox;
+    con->free_func = p_lasso_free;
+    snprintf(con->description, sizeof(con->description), "SMOOTH-COL (%0.1e)", multiplier);
+    val_t * mult = (val_t *) malloc(sizeof(*mult));
+    if(mult == NULL) {
+      splatt_free_constraint(con);
+      return SPLATT_ERROR_NOMEMORY;
+    }
+    *mult = multiplier;
+    con->data = mult;
+    splatt_register_constraint(opts, modes_included[m], con);
+  }
+  return SPLATT_SUCCESS;
 }

@@ -200,6 +200,8 @@ def lib():
     L.splatt_register_frob.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
     L.splatt_register_lasso.restype = C.c_int
     L.splatt_register_lasso.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
+    L.splatt_register_smooth.restype = C.c_int
+    L.splatt_register_smooth.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
     for name in ("splatt_register_maxcolnorm", "splatt_register_maxcolnorm_nonneg",
                  "splatt_register_nonneg", "splatt_register_rowsimp"):
         getattr(L, name).restype = C.c_int

@@ -189,18 +189,18 @@ def test_admm_step_matches_oracle(gpu_available, R, prox):
 
 
 @pytest.mark.parametrize("R,I", [(5, 1), (16, 37), (33, 500), (64, 3000)])
-@pytest.mark.parametrize("prox", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("prox", [0, 1, 2, 3, 4, 6])
 @pytest.mark.parametrize("tol,max_inner", [(1e-2, 50), (7e-2, 25), (0.0, 6)])
 def test_admm_solve_matches_oracle(gpu_available, R, I, prox, tol, max_inner):
     # the whole inner loop: same iterations, P and D at 1e-8 (prox 4: lasso
-    # with multiplier 2)
+    # with multiplier 2; prox 6: column smoothness with multiplier 0.5)
     rng = np.random.default_rng(R * 7 + prox + int(tol * 1000))
     A = rng.standard_normal((3 * R, R))
     G = A.T @ A + 0.5 * np.eye(R)
     M = rng.standard_normal((I, R)) * 4.0
     P0 = rng.standard_normal((I, R))
     D0 = rng.standard_normal((I, R)) * 0.1
-    lam = 2.0 if prox == 4 else 0.0
+    lam = {4: 2.0, 6: 0.5}.get(prox, 0.0)
     wantP, wantD, wits, wrho = orc.admm(prox, M, P0, D0, G, max_inner, tol, lam=lam)
     L = ps.lib()
     RP = _rp(R)
@@ -381,6 +381,22 @@ def test_cpd_rowsimp_matches_oracle(gpu_available, chunk):
     A = res["factors"][0]
     assert np.all(A >= 0)
     assert np.allclose(A.sum(axis=1), 1.0, atol=1e-6)
+    assert abs(res["fit"] - fits[-1]) <= 1e-6 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-5)
+
+
+def test_cpd_smooth_matches_oracle(gpu_available):
+    # splatt_register_smooth on mode 1: ADMM whose prox solves the banded
+    # smoothness system down every column
+    lam_s = 0.3
+
+    def configure(L, co):
+        modes = np.array([1], dtype=np.uint64)
+        assert L.splatt_register_smooth(co, lam_s, modes.ctypes.data_as(ps.u64p), 1) == \
+            ps.SUCCESS
+    res, facs, lam, fits = _run_both(tensor_path("med.tns"), 5, 3,
+                                     cons=[(1, 0, 0.0), (2, 6, 0.0, 0, lam_s), (1, 0, 0.0)],
+                                     unconstrained=False, configure=configure)
     assert abs(res["fit"] - fits[-1]) <= 1e-6 * max(1.0, abs(fits[-1]))
     _model_close(res, facs, lam, 1e-5)
 

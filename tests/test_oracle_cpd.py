@@ -403,3 +403,43 @@ def test_rowsimp_prox_is_the_simplex_projection():
         assert np.allclose(x, want, atol=1e-12)
     assert np.array_equal(orc.prox(5, np.array([[4.0]])), np.array([[1.0]]))
 
+
+def _smooth_matrix(I):
+    # src/cpd/constraints/smooth.c:30-79 (p_form_banded) without lam and rho:
+    # stencil [1 -4 6 -4 1], 5 on the first and last diagonal entries
+    K = np.zeros((I, I))
+    for i in range(I):
+        K[i, i] = 5.0 if i in (0, I - 1) else 6.0
+        if i + 1 < I:
+            K[i, i + 1] = K[i + 1, i] = -4.0
+        if i + 2 < I:
+            K[i, i + 2] = K[i + 2, i] = 1.0
+    return K
+
+
+@pytest.mark.parametrize("I", [1, 2, 3, 4, 7, 40])
+def test_smooth_prox_is_the_banded_solve(I):
+    # every column x becomes rho (lam K + rho I)^{-1} x
+    rng = np.random.default_rng(I)
+    P = rng.standard_normal((I, 3))
+    lam, rho = 0.7, 1.3
+    want = rho * np.linalg.solve(lam * _smooth_matrix(I) + rho * np.eye(I), P)
+    assert np.allclose(orc.smooth_prox(P, lam, rho), want, rtol=1e-12, atol=1e-13)
+
+
+def test_admm_smooth_meets_optimality():
+    # ADMM with the smoothness prox solves
+    # min_P tr(P G P^T) / 2 - tr(M^T P) + lam / 2 sum_c P_c^T K P_c, whose
+    # minimiser satisfies P G + lam K P = M: a linear system solved densely
+    rng = np.random.default_rng(13)
+    I, R, lam = 25, 4, 0.8
+    A = rng.standard_normal((12, R))
+    G = A.T @ A + 0.5 * np.eye(R)
+    M = rng.standard_normal((I, R)) * 3.0
+    Z = np.zeros((I, R))
+    P, D, its, rho = orc.admm(6, M, Z, Z, G, 5000, 1e-20, lam=lam)
+    K = _smooth_matrix(I)
+    big = np.kron(G, np.eye(I)) + np.kron(np.eye(R), lam * K)
+    want = np.linalg.solve(big, M.T.reshape(-1)).reshape(R, I).T
+    assert np.max(np.abs(P - want)) <= 1e-6 * np.max(np.abs(want))
+
