@@ -93,6 +93,14 @@ splatt_error_type splatt_register_frob(
     splatt_idx_t const * const modes_included,
     splatt_idx_t const num_modes);
 
+/* include/splatt/cpd.h:297-300 (src/cpd/constraints/simplex.cc:194-216):
+ * every column of the factor on the probability simplex (ADMM, one step at
+ * a time; the prox couples all rows of a column). */
+splatt_error_type splatt_register_colsimp(
+    splatt_cpd_opts * opts,
+    splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes);
+
 /* include/splatt/cpd.h:349-353 (src/cpd/constraints/smooth.c:220-247):
  * smooth columns (a second-difference penalty with this multiplier), solved
  * by ADMM whose prox is a banded solve along each column. */

@@ -104,7 +104,8 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
  * 2 = non-negative max-col-norm (maxcolnorm-nonneg.c:32-66),
  * 3 = non-negative (nonneg.c), 4 = L1 soft threshold at lam / rho
  * (lasso.c:30-52), 5 = rows onto the probability simplex (simplex.cc:32-104),
- * 6 = column smoothness with multiplier lam (smooth.c:30-184), 0 = none. chunk > 0 runs ADMM in independent chunks of
+ * 6 = column smoothness with multiplier lam (smooth.c:30-184), 7 = columns
+ * onto the probability simplex (simplex.cc:120-158), 0 = none. chunk > 0 runs ADMM in independent chunks of
  * that many rows (admm.c:355-400; the caller passes 0 unless the prox is
  * row-separable). */
 typedef struct { int kind; int prox; double diag_add; uint64_t chunk; double lam; } or_con;

@@ -150,9 +150,27 @@ static void p_simplex_row(double * row, double * buf, uint64_t n)
  * nonneg.c: column norms capped at 1 (after clamping negatives for the
  * non-negative variant); plain clamping for nonneg; lasso.c:30-52: entries
  * above thr lose thr, entries below -thr gain thr, the rest become 0;
- * prox 5 projects every row onto the probability simplex. */
+ * prox 5 projects every row, prox 7 every column, onto the probability
+ * simplex. */
 void or_prox(int prox, double * P, uint64_t I, uint64_t R, double thr)
 {
+  if(prox == 7) {
+    /* simplex.cc:120-158 (splatt_colsimp_prox): every column on its own */
+    double * col = malloc((I ? I : 1) * sizeof(double));
+    double * buf = malloc((I ? I : 1) * sizeof(double));
+    for(uint64_t j=0; j < R; ++j) {
+      for(uint64_t i=0; i < I; ++i) {
+        col[i] = P[j + i * R];
+      }
+      p_simplex_row(col, buf, I);
+      for(uint64_t i=0; i < I; ++i) {
+        P[j + i * R] = col[i];
+      }
+    }
+    free(buf);
+    free(col);
+    return;
+  }
   if(prox == 5) {
     double * buf = malloc((R ? R : 1) * sizeof(double));
     for(uint64_t i=0; i < I; ++i) {

@@ -167,6 +167,7 @@ int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,
 #define SPD_PROX_LASSO 4   /* soft threshold at lam / rho */
 #define SPD_PROX_ROWSIMP 5 /* rows onto the probability simplex (chunked ADMM) */
 #define SPD_PROX_SMOOTH 6  /* columns smoothed: rho (lam K + rho I)^-1 x */
+#define SPD_PROX_COLSIMP 7 /* columns onto the probability simplex */
 
 typedef struct {
   int kind;          /* SPD_CON_* */

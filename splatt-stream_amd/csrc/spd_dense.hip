@@ -1552,6 +1552,68 @@ __global__ void k_smooth_solve(double * __restrict__ P, uint64_t I, uint32_t R,
   }
 }
 
+// ---- column simplex prox (src/cpd/constraints/simplex.cc:120-158) ---------
+// Each column x (I entries) becomes max(x - theta, 0) with theta = (sum of
+// the support - 1) / |support|, the support being the entries above theta.
+// One block per column finds it by Michelot's iteration: start from every
+// entry, then keep the entries above the current theta until the support
+// stops shrinking (it shrinks at each other step, so at most I rounds). The
+// support is the one the reference's sort and running sum find; theta's sum
+// is taken in block order instead of sorted order.
+__global__ void __launch_bounds__(DN_THREADS)
+k_colsimp(double * __restrict__ P, uint64_t I, uint32_t R)
+{
+  __shared__ double s_sum[DN_THREADS / 64];
+  __shared__ unsigned long long s_cnt[DN_THREADS / 64];
+  __shared__ double s_theta;
+  __shared__ int s_stop;
+  uint32_t const c = blockIdx.x;
+  int const t = threadIdx.x;
+  int const lane = t & 63;
+  int const w = t >> 6;
+  double theta = -HUGE_VAL;
+  unsigned long long prev = ~0ull;
+  while(true) {
+    double sum = 0.;
+    unsigned long long cnt = 0;
+    for(uint64_t i=t; i < I; i += DN_THREADS) {
+      double const v = P[i * R + c];
+      if(v > theta) {
+        sum += v;
+        ++cnt;
+      }
+    }
+    for(int off=32; off > 0; off >>= 1) {
+      sum += __shfl_xor(sum, off, 64);
+      cnt += __shfl_xor(cnt, off, 64);
+    }
+    if(lane == 0) {
+      s_sum[w] = sum;
+      s_cnt[w] = cnt;
+    }
+    __syncthreads();
+    if(t == 0) {
+      double a = 0.;
+      unsigned long long n = 0;
+      for(int k=0; k < DN_THREADS / 64; ++k) {
+        a += s_sum[k];
+        n += s_cnt[k];
+      }
+      s_stop = (n == prev || n == 0) ? 1 : 0;
+      if(n > 0) s_theta = (a - 1.) / (double) n;
+      prev = n;
+    }
+    __syncthreads();
+    theta = s_theta;
+    if(s_stop) break;
+    __syncthreads();
+  }
+  for(uint64_t i=t; i < I; i += DN_THREADS) {
+    double const v = P[i * R + c] - theta;
+    P[i * R + c] = (v > 0.) ? v : 0.;
+  }
+}
+
 // Factor the smoothness matrix for rows I and the device rho into scratch.
 static int smooth_factor(uint64_t I, double lam, double const * rho, double ** f,
     hipStream_t st)
@@ -1623,6 +1685,10 @@ static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_
   if(prox == DN_PROX_SMOOTH) {
     hipLaunchKernelGGL(k_smooth_solve, dim3((unsigned) spd_cdiv(R, 64)), dim3(64), 0, st,
         P, I, R, sf, rho);
+    SPD_LAUNCH_CHECK();
+  }
+  if(prox == DN_PROX_COLSIMP) {
+    hipLaunchKernelGGL(k_colsimp, dim3(R), dim3(DN_THREADS), 0, st, P, I, R);
     SPD_LAUNCH_CHECK();
   }
   uint32_t const nb2 = dn_blocks(n, DN_THREADS * 8);
@@ -1723,7 +1789,7 @@ int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const 
   if(I == 0 || max_inner == 0) return SPD_OK;
   double * sf = nullptr;
   if(prox == DN_PROX_SMOOTH) SPD_TRY(smooth_factor(I, lam, rho, &sf, st));
-  if(R <= 64 && prox != DN_PROX_SMOOTH) {
+  if(R <= 64 && prox != DN_PROX_SMOOTH && prox != DN_PROX_COLSIMP) {
     if(mdtype == SPD_F32) {
       return admm_solve2_t<float>((float const *) M, ginv, rho, I, R, P, D, aux, init, prox,
           lam, max_inner, tol, sums, ctl, st);
@@ -1731,8 +1797,8 @@ int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const 
     return admm_solve2_t<double>((double const *) M, ginv, rho, I, R, P, D, aux, init, prox,
         lam, max_inner, tol, sums, ctl, st);
   }
-  // ranks above 64 and the smoothness prox: one step at a time, the test on
-  // the host
+  // ranks above 64 and the column-coupled smoothness and simplex proxes: one
+  // step at a time, the test on the host
   int hctl[2] = {0, 0};
   for(uint64_t k=0; k < max_inner; ++k) {
     if(mdtype == SPD_F32) {

@@ -14,6 +14,7 @@
 #define DN_PROX_LASSO 4
 #define DN_PROX_ROWSIMP 5      // chunked ADMM only
 #define DN_PROX_SMOOTH 6       // one-block ADMM, one step at a time
+#define DN_PROX_COLSIMP 7      // one-block ADMM, one step at a time
 
 // out (R x R, full, symmetric) = X^T X. When div != nullptr the columns of X
 // are first divided in place by div[r] (X[:,r] /= div[r], the reference's

@@ -36,6 +36,8 @@ void splatt_maxcolnorm_prox(val_t * primal, idx_t const nrows, idx_t const ncols
 void splatt_maxcolnorm_nonneg_prox(val_t * primal, idx_t const nrows,
     idx_t const ncols, idx_t const offset, void * data, val_t const rho,
     bool const should_parallelize);
+void splatt_colsimp_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
+    idx_t const offset, void * data, val_t const rho, bool const should_parallelize);
 void splatt_smooth_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
     idx_t const offset, void * data, val_t const rho, bool const should_parallelize);
 void splatt_rowsimp_prox(val_t * primal, idx_t const nrows, idx_t const ncols,

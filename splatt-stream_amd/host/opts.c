@@ -321,6 +321,28 @@ void splatt_smooth_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
   free(f);
 }
 
+/* src/cpd/constraints/simplex.cc:120-158: every column onto the probability
+ * simplex (the row projection applied to a copy of the column). */
+void splatt_colsimp_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
+    idx_t const offset, void * data, val_t const rho, bool const should_parallelize)
+{
+  (void) offset; (void) data; (void) rho; (void) should_parallelize;
+  val_t * col = (val_t *) malloc((nrows ? nrows : 1) * sizeof(*col));
+  if(col == NULL) {
+    return;
+  }
+  for(idx_t j=0; j < ncols; ++j) {
+    for(idx_t i=0; i < nrows; ++i) {
+      col[i] = primal[j + i * ncols];
+    }
+    splatt_rowsimp_prox(col, 1, nrows, 0, NULL, rho, false);
+    for(idx_t i=0; i < nrows; ++i) {
+      primal[j + i * ncols] = col[i];
+    }
+  }
+  free(col);
+}
+
 static void p_lasso_free(void * data)
 {
   free(data);
@@ -420,4 +442,13 @@ splatt_error_type splatt_register_smooth(splatt_cpd_opts * opts,
     splatt_register_constraint(opts, modes_included[m], con);
   }
   return SPLATT_SUCCESS;
+}
+
+/* src/cpd/constraints/simplex.cc:194-216: column simplex; not row-separable
+ * (the prox couples the rows of a column). */
+splatt_error_type splatt_register_colsimp(splatt_cpd_opts * opts,
+    splatt_idx_t const * const modes_included, splatt_idx_t const num_modes)
+{
+  return p_register_prox(opts, modes_included, num_modes,
+      splatt_colsimp_prox, false, true, "COL-SIMPLEX");
 }

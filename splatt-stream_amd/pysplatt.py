@@ -203,7 +203,8 @@ def lib():
     L.splatt_register_smooth.restype = C.c_int
     L.splatt_register_smooth.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
     for name in ("splatt_register_maxcolnorm", "splatt_register_maxcolnorm_nonneg",
-                 "splatt_register_nonneg", "splatt_register_rowsimp"):
+                 "splatt_register_nonneg", "splatt_register_rowsimp",
+                 "splatt_register_colsimp"):
         getattr(L, name).restype = C.c_int
         getattr(L, name).argtypes = [C.POINTER(CpdOpts), u64p, C.c_uint64]
     L.spd_dense_gram.restype = C.c_int

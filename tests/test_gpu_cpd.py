@@ -189,11 +189,12 @@ def test_admm_step_matches_oracle(gpu_available, R, prox):
 
 
 @pytest.mark.parametrize("R,I", [(5, 1), (16, 37), (33, 500), (64, 3000)])
-@pytest.mark.parametrize("prox", [0, 1, 2, 3, 4, 6])
+@pytest.mark.parametrize("prox", [0, 1, 2, 3, 4, 6, 7])
 @pytest.mark.parametrize("tol,max_inner", [(1e-2, 50), (7e-2, 25), (0.0, 6)])
 def test_admm_solve_matches_oracle(gpu_available, R, I, prox, tol, max_inner):
     # the whole inner loop: same iterations, P and D at 1e-8 (prox 4: lasso
-    # with multiplier 2; prox 6: column smoothness with multiplier 0.5)
+    # with multiplier 2; prox 6: column smoothness with multiplier 0.5;
+    # prox 7: column simplex)
     rng = np.random.default_rng(R * 7 + prox + int(tol * 1000))
     A = rng.standard_normal((3 * R, R))
     G = A.T @ A + 0.5 * np.eye(R)
@@ -397,6 +398,22 @@ def test_cpd_smooth_matches_oracle(gpu_available):
     res, facs, lam, fits = _run_both(tensor_path("med.tns"), 5, 3,
                                      cons=[(1, 0, 0.0), (2, 6, 0.0, 0, lam_s), (1, 0, 0.0)],
                                      unconstrained=False, configure=configure)
+    assert abs(res["fit"] - fits[-1]) <= 1e-6 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-5)
+
+
+def test_cpd_colsimp_matches_oracle(gpu_available):
+    # splatt_register_colsimp on mode 2: every column of the factor on the
+    # probability simplex after each ADMM step
+    def configure(L, co):
+        modes = np.array([2], dtype=np.uint64)
+        assert L.splatt_register_colsimp(co, modes.ctypes.data_as(ps.u64p), 1) == ps.SUCCESS
+    res, facs, lam, fits = _run_both(tensor_path("med.tns"), 5, 3,
+                                     cons=[(1, 0, 0.0), (1, 0, 0.0), (2, 7, 0.0)],
+                                     unconstrained=False, configure=configure)
+    C2 = res["factors"][2]
+    assert np.all(C2 >= 0)
+    assert np.allclose(C2.sum(axis=0), 1.0, atol=1e-6)
     assert abs(res["fit"] - fits[-1]) <= 1e-6 * max(1.0, abs(fits[-1]))
     _model_close(res, facs, lam, 1e-5)
 

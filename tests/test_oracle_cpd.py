@@ -443,3 +443,22 @@ def test_admm_smooth_meets_optimality():
     want = np.linalg.solve(big, M.T.reshape(-1)).reshape(R, I).T
     assert np.max(np.abs(P - want)) <= 1e-6 * np.max(np.abs(want))
 
+
+def test_colsimp_prox_projects_every_column():
+    # columns onto the simplex: the row projection of the transpose, and each
+    # column sums to 1 with max(v - theta, 0) for the bisection theta
+    rng = np.random.default_rng(21)
+    V = rng.standard_normal((40, 5)) * 0.3
+    X = orc.prox(7, V)
+    assert np.array_equal(X, orc.prox(5, V.T).T)
+    for v, x in zip(V.T, X.T):
+        lo, hi = v.min() - 1.0, v.max()
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            if np.maximum(v - mid, 0.0).sum() > 1.0:
+                lo = mid
+            else:
+                hi = mid
+        assert abs(x.sum() - 1.0) <= 1e-12
+        assert np.allclose(x, np.maximum(v - 0.5 * (lo + hi), 0.0), atol=1e-12)
+
