@@ -884,6 +884,7 @@ k_admm_pass(Admm2Args<TM> a)
 // final so far: j, or j-1 on convergence, which sets ctl[0]). The code is
 // kept short: a single block runs it and its instruction fetches are cold.
 #define RD_THREADS 1024
+#define ADMM_MAX_BLOCKS 1536       // blocks of an ADMM pass, at most (a multiple of 64)
 template<int E>
 __global__ void __launch_bounds__(RD_THREADS)
 k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
@@ -896,13 +897,38 @@ k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
   __shared__ double s_t[E];
   int const lane = threadIdx.x & 63;
   int const wave = threadIdx.x >> 6;
-  for(int e=wave; e < E; e += NW) {
+  // a lane's partials of an entry (blocks lane, lane + 64, ... < nb <=
+  // ADMM_MAX_BLOCKS) are loaded at once, the next entry's while this one is
+  // summed in block order
+  constexpr int PB = ADMM_MAX_BLOCKS / 64;
+  double xa[PB], xb[PB];
+  auto fetch = [&](int e, double (&x)[PB]) {
     double const * const pe = part + (uint64_t) e * nb;
+    #pragma unroll
+    for(int k=0; k < PB; ++k) {
+      uint32_t const b = lane + 64u * k;
+      x[k] = (b < nb) ? pe[b] : 0.;
+    }
+  };
+  auto total = [&](double const (&x)[PB]) {
     double v = 0.;
-    #pragma unroll 4
-    for(uint32_t b=lane; b < nb; b += 64) v += pe[b];
+    #pragma unroll
+    for(int k=0; k < PB; ++k) {
+      if(lane + 64u * k < nb) v += x[k];
+    }
     for(int off=32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if(lane == 0) s_t[e] = v;
+    return v;
+  };
+  if(wave < E) fetch(wave, xa);
+  for(int e=wave; e < E; e += 2 * NW) {
+    if(e + NW < E) fetch(e + NW, xb);
+    double const va = total(xa);
+    if(lane == 0) s_t[e] = va;
+    if(e + NW < E) {
+      if(e + 2 * NW < E) fetch(e + 2 * NW, xa);
+      double const vb = total(xb);
+      if(lane == 0) s_t[e + NW] = vb;
+    }
   }
   __syncthreads();
   if(threadIdx.x < RP) {
@@ -1732,7 +1758,8 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
 {
   uint32_t const RP = dn_rpad(R);
   // one block per 64 rows, at most six resident blocks per CU's worth
-  uint32_t nb = (uint32_t) (spd_cdiv(I, 64) < 1536 ? spd_cdiv(I, 64) : 1536);
+  uint32_t nb = (uint32_t) (spd_cdiv(I, 64) < ADMM_MAX_BLOCKS ? spd_cdiv(I, 64)
+                                                             : ADMM_MAX_BLOCKS);
   if(nb < 1) nb = 1;
   void * pc = nullptr;
   size_t const part_n = (size_t) nb * (RP + 4);
