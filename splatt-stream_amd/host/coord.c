@@ -13,6 +13,7 @@
 #define _GNU_SOURCE
 #include "host.h"
 #include <math.h>
+#include <omp.h>
 
 sptensor_t * splatt_tt_alloc(splatt_idx_t const nnz, splatt_idx_t const nmodes)
 {
@@ -60,76 +61,192 @@ splatt_idx_t * splatt_coord_get_inds(splatt_coord const * const c,
     splatt_idx_t const mode) { return c->ind[mode]; }
 splatt_val_t * splatt_coord_get_vals(splatt_coord const * const c) { return c->vals; }
 
-static sptensor_t * p_read_text(FILE * fin)
+/* The whole stream into one '\0'-terminated buffer. */
+static char * p_slurp(FILE * fin, size_t * out_len)
 {
-  char * line = NULL;
+  size_t cap = 1 << 20;
   size_t len = 0;
-  ssize_t rd;
-  idx_t nmodes = 0;
-  while((rd = getline(&line, &len, fin)) != -1) {
-    if(rd > 1 && line[0] != '#') {
-      char * save = NULL;
-      for(char * tok = strtok_r(line, " \t\n", &save); tok != NULL;
-          tok = strtok_r(NULL, " \t\n", &save)) {
-        ++nmodes;
+  char * buf = malloc(cap + 1);
+  if(buf == NULL) {
+    return NULL;
+  }
+  while(1) {
+    if(len == cap) {
+      char * nb = realloc(buf, 2 * cap + 1);
+      if(nb == NULL) {
+        free(buf);
+        return NULL;
       }
+      buf = nb;
+      cap *= 2;
+    }
+    size_t const got = fread(buf + len, 1, cap - len, fin);
+    len += got;
+    if(got == 0) {
       break;
     }
   }
+  buf[len] = '\0';
+  *out_len = len;
+  return buf;
+}
+
+/* End of the line starting at s (the '\n' or the buffer's end). */
+static size_t p_line_end(char const * buf, size_t s, size_t len)
+{
+  char const * e = memchr(buf + s, '\n', len - s);
+  return e ? (size_t) (e - buf) : len;
+}
+
+/* A data line as the reference's getline loop sees it (src/io.c:94-104):
+ * its length with the '\n' above 1 and a first character other than '#'. */
+static int p_is_data(char const * buf, size_t s, size_t e, size_t len)
+{
+  size_t const rd = e - s + (e < len ? 1 : 0);
+  return rd > 1 && buf[s] != '#';
+}
+
+/* Text input, the same conversions as the reference's two getline passes
+ * (strtoull per mode, then strtod) over the whole file read at once: the
+ * buffer is cut into one range of whole lines per thread; a first pass
+ * counts each range's data lines, a second converts each line (its '\n'
+ * replaced by '\0', so no conversion runs past it, as with getline's
+ * buffer) into its place and keeps per-thread index extremes. */
+static sptensor_t * p_read_text(FILE * fin)
+{
+  size_t len = 0;
+  char * buf = p_slurp(fin, &len);
+  if(buf == NULL) {
+    return NULL;
+  }
+  /* modes from the first data line */
+  idx_t nmodes = 0;
+  for(size_t s=0; s < len; ) {
+    size_t const e = p_line_end(buf, s, len);
+    if(p_is_data(buf, s, e, len)) {
+      char * line = malloc(e - s + 2);
+      memcpy(line, buf + s, e - s);
+      line[e - s] = '\0';
+      char * save = NULL;
+      /* the line keeps its '\n', which is no separator (src/io.c:288-293) */
+      if(e < len) {
+        line[e - s] = '\n';
+        line[e - s + 1] = '\0';
+      }
+      for(char * tok = strtok_r(line, " \t", &save); tok != NULL;
+          tok = strtok_r(NULL, " \t", &save)) {
+        ++nmodes;
+      }
+      free(line);
+      break;
+    }
+    s = e + 1;
+  }
   if(nmodes < 2) {
-    free(line);
+    free(buf);
     return NULL;
   }
   --nmodes;
   if(nmodes > SPLATT_MAX_NMODES) {
     fprintf(stderr, "SPLATT ERROR: maximum %d modes supported. Found %"
         SPLATT_PF_IDX ".\n", (int) SPLATT_MAX_NMODES, nmodes);
-    free(line);
+    free(buf);
     return NULL;
   }
+  /* one range of whole lines per thread */
+  int const T = omp_get_max_threads() > 0 ? omp_get_max_threads() : 1;
+  size_t * cut = malloc((T + 1) * sizeof(size_t));
+  idx_t * cnt = calloc(T + 1, sizeof(idx_t));
+  idx_t * tmax = calloc((size_t) T * SPLATT_MAX_NMODES, sizeof(idx_t));
+  idx_t * tmin = malloc((size_t) T * SPLATT_MAX_NMODES * sizeof(idx_t));
+  cut[0] = 0;
+  for(int k=1; k < T; ++k) {
+    size_t c = (size_t) ((unsigned __int128) len * k / T);
+    if(c < cut[k - 1]) {
+      c = cut[k - 1];
+    }
+    /* the first line starting at or after c */
+    if(c > 0 && c < len && buf[c - 1] != '\n') {
+      c = p_line_end(buf, c, len);
+      c = (c < len) ? c + 1 : len;
+    }
+    cut[k] = c;
+  }
+  cut[T] = len;
+  #pragma omp parallel for schedule(static, 1) num_threads(T)
+  for(int k=0; k < T; ++k) {
+    idx_t n = 0;
+    for(size_t s=cut[k]; s < cut[k + 1]; ) {
+      size_t const e = p_line_end(buf, s, len);
+      n += p_is_data(buf, s, e, len) ? 1 : 0;
+      s = e + 1;
+    }
+    cnt[k + 1] = n;
+  }
+  for(int k=0; k < T; ++k) {
+    cnt[k + 1] += cnt[k];
+  }
+  idx_t const nnz = cnt[T];
+  sptensor_t * tt = splatt_alloc_coord(nmodes, nnz);
+  #pragma omp parallel for schedule(static, 1) num_threads(T)
+  for(int k=0; k < T; ++k) {
+    idx_t * const mx = tmax + (size_t) k * SPLATT_MAX_NMODES;
+    idx_t * const mn = tmin + (size_t) k * SPLATT_MAX_NMODES;
+    for(idx_t m=0; m < nmodes; ++m) {
+      mx[m] = 0;
+      mn[m] = 1;
+    }
+    idx_t n = cnt[k];
+    for(size_t s=cut[k]; s < cut[k + 1]; ) {
+      size_t const e = p_line_end(buf, s, len);
+      if(p_is_data(buf, s, e, len)) {
+        buf[e] = '\0';
+        char * p = buf + s;
+        for(idx_t m=0; m < nmodes; ++m) {
+          idx_t const v = strtoull(p, &p, 10);
+          tt->ind[m][n] = v;
+          mx[m] = v > mx[m] ? v : mx[m];
+          mn[m] = v < mn[m] ? v : mn[m];
+        }
+        tt->vals[n] = strtod(p, &p);
+        ++n;
+      }
+      s = e + 1;
+    }
+  }
+  free(buf);
   idx_t dims[SPLATT_MAX_NMODES];
   idx_t offs[SPLATT_MAX_NMODES];
   for(idx_t m=0; m < nmodes; ++m) {
     dims[m] = 0;
     offs[m] = 1;
-  }
-  rewind(fin);
-  idx_t nnz = 0;
-  while((rd = getline(&line, &len, fin)) != -1) {
-    if(rd > 1 && line[0] != '#') {
-      char * p = line;
-      for(idx_t m=0; m < nmodes; ++m) {
-        idx_t const v = strtoull(p, &p, 10);
-        dims[m] = v > dims[m] ? v : dims[m];
-        offs[m] = v < offs[m] ? v : offs[m];
-      }
-      ++nnz;
+    for(int k=0; k < T; ++k) {
+      dims[m] = SS_MAX(dims[m], tmax[(size_t) k * SPLATT_MAX_NMODES + m]);
+      offs[m] = SS_MIN(offs[m], tmin[(size_t) k * SPLATT_MAX_NMODES + m]);
     }
   }
+  free(cut);
+  free(cnt);
+  free(tmax);
+  free(tmin);
   for(idx_t m=0; m < nmodes; ++m) {
     if(offs[m] != 0 && offs[m] != 1) {
       fprintf(stderr, "SPLATT: ERROR tensors must be 0 or 1 indexed. Mode %"
           SPLATT_PF_IDX " is %" SPLATT_PF_IDX " indexed.\n", m, offs[m]);
-      free(line);
+      splatt_free_coord(tt);
       return NULL;
     }
   }
-  sptensor_t * tt = splatt_alloc_coord(nmodes, nnz);
   for(idx_t m=0; m < nmodes; ++m) {
     tt->dims[m] = dims[m] + (offs[m] == 0 ? 1 : 0);
-  }
-  rewind(fin);
-  idx_t n = 0;
-  while((rd = getline(&line, &len, fin)) != -1) {
-    if(rd > 1 && line[0] != '#') {
-      char * p = line;
-      for(idx_t m=0; m < nmodes; ++m) {
-        tt->ind[m][n] = strtoull(p, &p, 10) - offs[m];
+    if(offs[m] == 1) {
+      idx_t * const ind = tt->ind[m];
+      #pragma omp parallel for schedule(static)
+      for(idx_t n=0; n < nnz; ++n) {
+        ind[n] -= 1;
       }
-      tt->vals[n++] = strtod(p, &p);
     }
   }
-  free(line);
   return tt;
 }
 

@@ -66,6 +66,66 @@ def test_tt_read_matches_oracle(name):
     L.splatt_tt_free(p)
 
 
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+    return str(path)
+
+
+_EDGE_FILES = {
+    # comments, blank lines, a final line without '\n'
+    "comments": "# a comment\n\n1 2 3 1.5\n#mid\n2 1 1 -2.25e-3\n\n3 3 2 7\n4 1 1 0.1",
+    # a 0-indexed mode and a one-character last line (skipped, as getline
+    # returns 1 for it)
+    "zeroidx": "0 1 1 1.0\n2 1 3 2.0\n1 2 2 3.0\n5",
+    # a trailing blank on the first data line is one more token
+    "trailing": "1 2 3 4.5 \n2 3 4 5.5 \n",
+    # tabs as separators, several spaces, a value in exponent form
+    "tabs": "1\t2\t1\t1e2\n2   1   2   -0.000001\n",
+}
+
+
+@pytest.mark.parametrize("case", sorted(_EDGE_FILES))
+def test_tt_read_edge_cases_match_oracle(tmp_path, case):
+    path = _write(tmp_path / (case + ".tns"), _EDGE_FILES[case])
+    L = ps.lib()
+    p = L.splatt_tt_read(path.encode())
+    assert p
+    tt = p[0]
+    o = orc.Coo.read(path)
+    N, nnz = int(tt.nmodes), int(tt.nnz)
+    assert N == o.nmodes and nnz == o.nnz
+    assert [int(tt.dims[m]) for m in range(N)] == o.dims
+    for m in range(N):
+        assert np.array_equal(np.ctypeslib.as_array(tt.ind[m], shape=(nnz,)), o.inds()[m])
+    assert np.array_equal(np.ctypeslib.as_array(tt.vals, shape=(nnz,)), o.vals())
+    L.splatt_tt_free(p)
+
+
+def test_tt_read_many_lines_matches_oracle(tmp_path):
+    # enough lines that every reader thread gets a range, with comments and
+    # blank lines at range edges; values printed with 17 significant digits
+    rng = np.random.default_rng(3)
+    n = 200_000
+    ind = rng.integers(1, 5000, size=(n, 3))
+    val = rng.standard_normal(n)
+    lines = ["%d %d %d %.17g" % (a, b, c, v) for (a, b, c), v in zip(ind, val)]
+    for k in range(0, n, 997):
+        lines[k] = "# skipped\n\n" + lines[k]
+    path = _write(tmp_path / "many.tns", "\n".join(lines) + "\n")
+    L = ps.lib()
+    p = L.splatt_tt_read(path.encode())
+    tt = p[0]
+    o = orc.Coo.read(path)
+    nnz = int(tt.nnz)
+    assert nnz == o.nnz == n
+    for m in range(3):
+        assert np.array_equal(np.ctypeslib.as_array(tt.ind[m], shape=(nnz,)), o.inds()[m])
+    assert np.array_equal(np.ctypeslib.as_array(tt.vals, shape=(nnz,)), o.vals())
+    assert np.array_equal(np.ctypeslib.as_array(tt.vals, shape=(nnz,)), val)
+    L.splatt_tt_free(p)
+
+
 def test_coord_load_and_accessors():
     L = ps.lib()
     c = L.splatt_alloc_coord(3, 1)
