@@ -357,7 +357,9 @@ splatt_error_type splatt_coord_load(char const * const fname,
 }
 
 /* src/sptensor.c:190-252: per mode, relabel the present indices to
- * 0..k-1 keeping their order, record local->global in indmap. */
+ * 0..k-1 keeping their order, record local->global in indmap. Marking
+ * (atomic stores of 1) and relabelling run on all host threads; the
+ * numbering of the present indices is a serial pass over the dimension. */
 splatt_idx_t splatt_tt_remove_empty(sptensor_t * const tt)
 {
   idx_t removed = 0;
@@ -366,11 +368,15 @@ splatt_idx_t splatt_tt_remove_empty(sptensor_t * const tt)
     maxdim = SS_MAX(maxdim, tt->dims[m]);
   }
   idx_t * map = splatt_malloc((maxdim ? maxdim : 1) * sizeof(idx_t));
+  idx_t const nnz = tt->nnz;
   for(idx_t m=0; m < tt->nmodes; ++m) {
     idx_t const dim = tt->dims[m];
+    idx_t * const ind = tt->ind[m];
     memset(map, 0, dim * sizeof(idx_t));
-    for(idx_t n=0; n < tt->nnz; ++n) {
-      map[tt->ind[m][n]] = 1;
+    #pragma omp parallel for schedule(static)
+    for(idx_t n=0; n < nnz; ++n) {
+      #pragma omp atomic write
+      map[ind[n]] = 1;
     }
     idx_t k = 0;
     for(idx_t i=0; i < dim; ++i) {
@@ -391,8 +397,9 @@ splatt_idx_t splatt_tt_remove_empty(sptensor_t * const tt)
         map[i] = next++;
       }
     }
-    for(idx_t n=0; n < tt->nnz; ++n) {
-      tt->ind[m][n] = map[tt->ind[m][n]];
+    #pragma omp parallel for schedule(static)
+    for(idx_t n=0; n < nnz; ++n) {
+      ind[n] = map[ind[n]];
     }
     tt->dims[m] = k;
   }
