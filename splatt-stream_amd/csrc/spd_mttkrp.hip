@@ -1467,6 +1467,8 @@ static int get_copy(spd_csf * c, int level, int rows_pos, RowCopy ** out, hipStr
     for(int q=0; q < SPD_MAX_MODES; ++q) spd_free(r->cg[q]);
     spd_free(r->crow); spd_free(r->v64); spd_free(r->v32);
     delete r;
+    // short of device memory after all: walk the CSF instead
+    if(rc == SPD_ENOMEM) return SPD_OK;
     if(rc == SPD_EHIP) spd_set_error(SPD_EHIP, "row-sorted copy build failed");
     return rc;
   }
