@@ -1489,6 +1489,10 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
     spd_set_error(SPD_EINPUT, "mode %d not in tensor", mode);
     return SPD_EINPUT;
   }
+  if(c->nnz == 0) {
+    // no nonzeros: the output is zero and no kernel has work
+    return spd_zero_async(out, c->dims[mode] * R * sizeof(T), st);
+  }
   // the 3-mode root kernel addresses factor rows with 32-bit byte offsets
   uint64_t const lim = 0xFFFFFFFFull;
   if(N == 3 && depth == 0 && c->dims[c->perm[1]] * R * sizeof(T) <= lim &&
