@@ -126,6 +126,13 @@ def test_tt_read_many_lines_matches_oracle(tmp_path):
     L.splatt_tt_free(p)
 
 
+def test_layout_switch_rejects_unknown_values():
+    L = ps.lib()
+    assert L.splatt_amd_set_mttkrp_layout(ps.LAYOUT_CSF) == ps.SUCCESS
+    assert L.splatt_amd_set_mttkrp_layout(ps.LAYOUT_AUTO) == ps.SUCCESS
+    assert L.splatt_amd_set_mttkrp_layout(7) == ps.ERROR_BADINPUT
+
+
 def test_coord_load_and_accessors():
     L = ps.lib()
     c = L.splatt_alloc_coord(3, 1)
