@@ -23,7 +23,7 @@ extern "C" {
 
 #define SPD_OK          0
 #define SPD_EHIP       -1   /* HIP runtime error */
-#define SPD_ENOMEM     -2   /* device allocation failed */
+#define SPD_ENOMEM     -2   /* device (or host bookkeeping) allocation failed */
 #define SPD_EINPUT     -3   /* bad input (index out of range, sizes) */
 #define SPD_ENODEV     -4   /* no usable GPU */
 #define SPD_ENUMERIC   -5   /* non-SPD Gram matrix, NaN, ... */

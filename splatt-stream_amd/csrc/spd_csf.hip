@@ -58,6 +58,10 @@ extern "C" int spd_coo_upload(int nmodes, uint64_t nnz, uint64_t const * dims,
     }
   }
   spd_coo * c = (spd_coo *) calloc(1, sizeof(spd_coo));
+  if(c == nullptr) {
+    spd_set_error(SPD_ENOMEM, "host allocation of the COO handle failed");
+    return SPD_ENOMEM;
+  }
   c->nmodes = nmodes;
   c->nnz = nnz;
   for(int m=0; m < nmodes; ++m) {
@@ -452,6 +456,10 @@ extern "C" int spd_csf_build(spd_coo const * coo, int const * perm,
     return SPD_EINPUT;
   }
   spd_csf * c = (spd_csf *) calloc(1, sizeof(spd_csf));
+  if(c == nullptr) {
+    spd_set_error(SPD_ENOMEM, "host allocation of the CSF handle failed");
+    return SPD_ENOMEM;
+  }
   c->nmodes = N;
   c->nnz = n;
   for(int m=0; m < N; ++m) {
@@ -544,8 +552,12 @@ extern "C" int spd_csf_build(spd_coo const * coo, int const * perm,
       std::vector<uint64_t> hkey(n);
       rc = spd_d2h(hkey.data(), tkey, n * 8);
       c->tile_nnz = (uint64_t *) calloc(c->ntiles + 1, sizeof(uint64_t));
+      if(c->tile_nnz == nullptr) {
+        spd_set_error(SPD_ENOMEM, "host allocation of the tile counts failed");
+        rc = SPD_ENOMEM;
+      }
       for(uint64_t i=0; i < n && rc == SPD_OK; ++i) c->tile_nnz[hkey[i] + 1]++;
-      for(uint64_t t=0; t < c->ntiles; ++t) c->tile_nnz[t+1] += c->tile_nnz[t];
+      for(uint64_t t=0; t < c->ntiles && rc == SPD_OK; ++t) c->tile_nnz[t+1] += c->tile_nnz[t];
     }
     for(int l=0; l < N; ++l) spd_free(tl.p[l]);
     spd_free(tv);

@@ -1065,6 +1065,10 @@ static int get_plan(spd_csf * c, uint32_t S, MttkrpPlan ** out, hipStream_t st)
     if(p->S == S) { *out = p; return SPD_OK; }
   }
   MttkrpPlan * p = (MttkrpPlan *) calloc(1, sizeof(MttkrpPlan));
+  if(p == nullptr) {
+    spd_set_error(SPD_ENOMEM, "host allocation of an MTTKRP plan failed");
+    return SPD_ENOMEM;
+  }
   p->S = S;
   p->nitems = (uint32_t) spd_cdiv(c->nnz, S);
   int const NL = c->nmodes - 1;
