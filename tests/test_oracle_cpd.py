@@ -209,6 +209,35 @@ def test_prox_closed_forms():
     assert np.allclose(mcn, np.where(qn > 1, Q / np.where(qn > 1, qn, 1.0), Q))
 
 
+def test_maxcolnorm_threshold_is_one():
+    # src/cpd/constraints/maxcolnorm.c:32-61: a column is divided by its norm
+    # only when that norm exceeds 1; columns of norm in (0, 1] are untouched
+    # whatever their size (0.3, 0.55, 0.7, 0.95), those above land on norm 1
+    rng = np.random.default_rng(14)
+    P = rng.standard_normal((40, 6))
+    want_norms = np.array([0.3, 0.55, 0.7, 0.95, 1.2, 4.0])
+    P = P / np.linalg.norm(P, axis=0) * want_norms
+    for kind in (1, 2):
+        Q = np.where(P < 0, 0.0, P) if kind == 2 else P
+        qn = np.linalg.norm(Q, axis=0)
+        got = orc.prox(kind, P)
+        below = qn <= 1.0
+        assert np.array_equal(got[:, below], Q[:, below]), kind
+        assert np.allclose(np.linalg.norm(got[:, ~below], axis=0), 1.0, rtol=1e-14)
+        assert np.allclose(got[:, ~below], Q[:, ~below] / qn[~below], rtol=1e-15, atol=0)
+    # a max-column-norm ADMM whose least-squares solution has column norms
+    # between 0.5 and 1 converges to that solution (the constraint is inactive)
+    R, I = 4, 30
+    G = _spd(R, 15, scale=2.0)
+    M = rng.standard_normal((I, R))
+    ls = np.linalg.solve(G, M.T).T
+    M = M / np.linalg.norm(ls, axis=0) * np.array([0.6, 0.75, 0.9, 0.98])
+    ls = np.linalg.solve(G, M.T).T
+    assert np.all((np.linalg.norm(ls, axis=0) > 0.5) & (np.linalg.norm(ls, axis=0) < 1))
+    P1, _, _, _ = orc.admm(1, M, np.zeros((I, R)), np.zeros((I, R)), G, 3000, 0.0)
+    assert np.allclose(P1, ls, rtol=1e-8, atol=1e-11)
+
+
 def test_admm_identity_prox_converges_to_solve():
     R, I = 5, 40
     G = _spd(R, 1)

@@ -22,7 +22,18 @@
                      that model); (f) the refit equals CPD-ALS with the
                      default options from rand() values drawn right after
                      the slice, and consumes exactly one draw pair per
-                     factor entry of the window.
+                     factor entry of the window; (g) with closed-form
+                     (frob) modes, the time row and each mode's update at a
+                     later slice make the gradient of the forgetting-weighted
+                     streaming objective vanish, the objective built from
+                     dense reconstructions of the slices (pins the
+                     forgetting factor and the orientation of the history
+                     cross-Gram); (h) with two outer iterations, the history
+                     holds every outer iteration's time row (pins the time
+                     Gram update inside the outer loop); (i) max-column-norm
+                     mode updates equal the ADMM solve, from a zero dual, of
+                     the normal equations read off that objective's
+                     gradient (pins the dual reset before each mode).
 """
 import ctypes
 
@@ -252,3 +263,208 @@ def test_stream_refit_window_options_and_draws():
         for _ in range(2 * R * sum(kd)):
             libc.rand()
     assert libc.rand() == after_full
+
+
+# ---- the per-slice update against the streaming objective -----------------
+#
+# StreamCPD.cc:406-492 updates, per slice t, the time row and then every
+# other mode. Each update minimises a quadratic whose terms are fixed by the
+# streaming model (Smith et al., SDM'18): the slice's own squared residual,
+# the history term sum_g ||[[A_old, B_old, g]] - [[A, B, g]]||^2 over
+# weighted time rows g (the forgetting factor f^(age) on every earlier
+# slice, and -- the time Gram being updated inside the outer loop -- the
+# current slice's earlier outer-iteration rows with weight 1), and the frob
+# penalty. The checks below build that objective from dense reconstructions
+# of the slices (numpy, no Gram algebra), take its gradient, and require it
+# to vanish at the oracle's output (closed-form modes), or derive its normal
+# equations from the gradient and require the oracle's mode update to be the
+# ADMM solve of them from a zero dual (max-column-norm modes).
+
+def _stream_full_first(T, d0, d1, seed, big=1.0):
+    """Time mode 2 with T slices; slice 0 touches every row of modes 0 and 1,
+    so the factors never grow after it."""
+    rng = np.random.default_rng(seed)
+    ii, jj, kk, vv = [], [], [], []
+    first = set()
+    for i in range(d0):
+        first.add((i, int(rng.integers(d1))))
+    for j in range(d1):
+        first.add((int(rng.integers(d0)), j))
+    for t in range(T):
+        cells = set(first) if t == 0 else set()
+        n = int(rng.integers(d0 * d1 // 4, d0 * d1 // 2))
+        for lin in rng.choice(d0 * d1, size=n, replace=False):
+            cells.add((int(lin) // d1, int(lin) % d1))
+        for (i, j) in sorted(cells):
+            ii.append(i)
+            jj.append(j)
+            kk.append(t)
+            vv.append((rng.random() + 0.1) * big)
+    inds = [np.array(ii, dtype=np.uint64), np.array(jj, dtype=np.uint64),
+            np.array(kk, dtype=np.uint64)]
+    return inds, np.array(vv), [d0, d1, T]
+
+
+def _slice_dense(sub, sv, sd, s):
+    w = sub[2] == s
+    X = np.zeros((sd[0], sd[1]))
+    np.add.at(X, (sub[0][w].astype(int), sub[1][w].astype(int)), sv[w])
+    return X
+
+
+def _recon(A, B, c):
+    return (A * c) @ B.T
+
+
+def _grad(which, A, B, c, X, hist, Aold, Bold, lam):
+    """Gradient in A (which=0) or B (which=1) of
+    1/2 ||X - [[A,B,c]]||^2 + 1/2 sum_g ||[[Aold,Bold,g]] - [[A,B,g]]||^2
+    + lam/2 ||A or B||^2."""
+    terms = [(_recon(A, B, c) - X, c)]
+    terms += [(_recon(A, B, g) - _recon(Aold, Bold, g), g) for g in hist]
+    if which == 0:
+        return sum(E @ (B * g) for E, g in terms) + lam * A
+    return sum(E.T @ (A * g) for E, g in terms) + lam * B
+
+
+def _normal_equations(which, A, B, c, X, hist, Aold, Bold, lam):
+    """(G, M) with gradient(U) = U G - M, read off the gradient of the
+    objective at zero and at unit rows (the Hessian is the same for every
+    row)."""
+    R = A.shape[1]
+    Z = np.zeros_like(A if which == 0 else B)
+    args = lambda U: (U, B) if which == 0 else (A, U)
+    M = -_grad(which, *args(Z), c, X, hist, Aold, Bold, lam)
+    G = np.zeros((R, R))
+    for r in range(R):
+        E = Z.copy()
+        E[0, r] = 1.0
+        G[r] = _grad(which, *args(E), c, X, hist, Aold, Bold, lam)[0] + M[0]
+    return G, M
+
+
+def _runs(coo, T, seed, max_its, cons, forget=0.95):
+    """The oracle run on the first k slices, for k = 1 .. T."""
+    out = {}
+    for k in range(1, T + 1):
+        sub_coo, sub, sv, sd = _truncate(coo, k)
+        libc.srand(seed)
+        out[k] = orc.stream_cpd(sub_coo, 2, 3, forget, max_its, cons)
+    return out, sub, sv, sd
+
+
+def _time_row(A, B, X, lam_t, gram_from=None):
+    """argmin_c 1/2 ||X - [[A,B,c]]||^2 + lam_t/2 ||c||^2 by least squares
+    on the unfolded slice. gram_from replaces the design's Gram (the first
+    slice's quirk: its Gram comes from the unnormalised start factors)."""
+    K = np.einsum("ir,jr->ijr", A, B).reshape(-1, A.shape[1])
+    KtK = K.T @ K if gram_from is None else gram_from
+    return np.linalg.solve(KtK + lam_t * np.eye(A.shape[1]), K.T @ X.reshape(-1))
+
+
+FROB = [(1, 0, 1e-4), (1, 0, 2e-4), (1, 0, 1e-2)]
+
+
+def test_stream_update_is_stationary_for_the_forgetting_objective():
+    """One outer iteration per slice, frob (closed form) on every mode: at
+    the last slice the time row, then A, then B are exact minimisers of the
+    slice + forgetting-weighted history objective. A dropped forgetting
+    factor or a transposed history cross-Gram breaks it."""
+    T, f = 4, 0.95
+    inds, vals, dims = _stream_full_first(T, 7, 6, 31)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    orc.stream_prepare(coo, 2)
+    runs, sub, sv, sd = _runs(coo, T, 9, 1, FROB, f)
+    prev, last = runs[T - 1], runs[T]
+    assert np.all(last["outer_its"] == 1)
+    Ap, Bp = prev["factors"][:2]
+    A, B = last["factors"][:2]
+    assert Ap.shape == A.shape and Bp.shape == B.shape    # no growth
+    c = last["time"]
+    X = _slice_dense(sub, sv, sd, T - 1)
+    hist = [np.sqrt(f ** (T - 1 - s)) * c[s] for s in range(T - 1)]
+    # time row from the factors at the slice's start
+    ct = _time_row(Ap, Bp, X, FROB[2][2])
+    assert np.allclose(c[T - 1], ct, rtol=1e-10, atol=1e-12)
+    gA = _grad(0, A, Bp, c[T - 1], X, hist, Ap, Bp, FROB[0][2])
+    gB = _grad(1, A, B, c[T - 1], X, hist, Ap, Bp, FROB[1][2])
+    scale = np.max(np.abs(X)) * np.max(np.abs(c))
+    assert np.max(np.abs(gA)) <= 1e-10 * scale
+    assert np.max(np.abs(gB)) <= 1e-10 * scale
+    # the same objective without forgetting, or with the history cross-Gram
+    # transposed, is not stationary at the output
+    flat = [c[s] for s in range(T - 1)]
+    assert np.max(np.abs(_grad(1, A, B, c[T - 1], X, flat, Ap, Bp, FROB[1][2]))) > 1e-4 * scale
+    Gt, Mt = _normal_equations(1, A, B, c[T - 1], X, hist, Ap, Bp, FROB[1][2])
+    assert np.allclose(B @ Gt, Mt, rtol=1e-9, atol=1e-10 * scale)
+
+
+def test_stream_time_gram_takes_every_outer_iteration():
+    """Two outer iterations per slice: the history of slice t holds both
+    time rows of every earlier slice (each with its forgetting weight) and
+    the first-iteration row of slice t itself (StreamCPD.cc:420-435 adds
+    each outer iteration's row to the time Gram). The first-iteration rows
+    are not returned; they are recomputed as least-squares time rows from
+    the factors at each slice's start."""
+    T, f, R = 3, 0.95, 3
+    inds, vals, dims = _stream_full_first(T, 7, 6, 37)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    orc.stream_prepare(coo, 2)
+    runs, sub, sv, sd = _runs(coo, T, 4, 2, FROB, f)
+    assert np.all(runs[T]["outer_its"] == 2)
+    lam_t = FROB[2][2]
+    # start factors of slice 0: rand() rows, normalised, Gram unnormalised
+    libc.srand(4)
+    U0 = orc.fill_rand(sd[0] * R).reshape(sd[0], R)
+    U1 = orc.fill_rand(sd[1] * R).reshape(sd[1], R)
+    G0 = (U0.T @ U0) * (U1.T @ U1)
+    N0 = U0 / np.linalg.norm(U0, axis=0)
+    N1 = U1 / np.linalg.norm(U1, axis=0)
+    first = [_time_row(N0, N1, _slice_dense(sub, sv, sd, 0), lam_t, gram_from=G0)]
+    for s in range(1, T):
+        As, Bs = runs[s]["factors"][:2]
+        first.append(_time_row(As, Bs, _slice_dense(sub, sv, sd, s), lam_t))
+    c = runs[T]["time"]
+    hist = []
+    for s in range(T - 1):
+        w = np.sqrt(f ** (T - 1 - s))
+        hist += [w * first[s], w * c[s]]
+    hist.append(first[T - 1])
+    Ap, Bp = runs[T - 1]["factors"][:2]
+    A, B = runs[T]["factors"][:2]
+    X = _slice_dense(sub, sv, sd, T - 1)
+    scale = np.max(np.abs(X)) * np.max(np.abs(c))
+    gB = _grad(1, A, B, c[T - 1], X, hist, Ap, Bp, FROB[1][2])
+    assert np.max(np.abs(gB)) <= 1e-9 * scale
+    # one row per slice (the final one) does not give a stationary point
+    once = [np.sqrt(f ** (T - 1 - s)) * c[s] for s in range(T - 1)]
+    assert np.max(np.abs(_grad(1, A, B, c[T - 1], X, once, Ap, Bp, FROB[1][2]))) > 1e-4 * scale
+
+
+def test_stream_mode_update_is_admm_from_a_zero_dual():
+    """Max-column-norm modes (the streaming default): each mode update at a
+    slice is the ADMM solve (25 iterations at most, tolerance 7e-2, the
+    stream's settings) of the objective's normal equations, started from
+    the mode's factor and a zero dual (StreamCPD.cc:466-469 clears the dual
+    before every mode's ADMM). The previous slice's ADMM ended with an active
+    column constraint, so a dual carried over would change the result."""
+    T, f = 4, 0.95
+    inds, vals, dims = _stream_full_first(T, 7, 6, 41, big=3.0)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    orc.stream_prepare(coo, 2)
+    runs, sub, sv, sd = _runs(coo, T, 12, 1, CONS, f)
+    Ap, Bp = runs[T - 1]["factors"][:2]
+    A, B = runs[T]["factors"][:2]
+    c = runs[T]["time"]
+    # the previous slice left a column on the constraint (norm 1): its
+    # ADMM's final projection was active and its dual non-zero
+    assert np.any(np.abs(np.linalg.norm(Ap, axis=0) - 1.0) < 1e-12)
+    assert np.any(np.abs(np.linalg.norm(Bp, axis=0) - 1.0) < 1e-12)
+    X = _slice_dense(sub, sv, sd, T - 1)
+    hist = [np.sqrt(f ** (T - 1 - s)) * c[s] for s in range(T - 1)]
+    GA, MA = _normal_equations(0, Ap, Bp, c[T - 1], X, hist, Ap, Bp, 0.0)
+    wantA, _, _, _ = orc.admm(1, MA, Ap, np.zeros_like(Ap), GA, 25, 7e-2)
+    assert np.allclose(A, wantA, rtol=1e-10, atol=1e-12)
+    GB, MB = _normal_equations(1, A, Bp, c[T - 1], X, hist, Ap, Bp, 0.0)
+    wantB, _, _, _ = orc.admm(1, MB, Bp, np.zeros_like(Bp), GB, 25, 7e-2)
+    assert np.allclose(B, wantB, rtol=1e-10, atol=1e-12)
