@@ -229,10 +229,9 @@ def test_maxcolnorm_threshold_is_one():
     # between 0.5 and 1 converges to that solution (the constraint is inactive)
     R, I = 4, 30
     G = _spd(R, 15, scale=2.0)
-    M = rng.standard_normal((I, R))
-    ls = np.linalg.solve(G, M.T).T
-    M = M / np.linalg.norm(ls, axis=0) * np.array([0.6, 0.75, 0.9, 0.98])
-    ls = np.linalg.solve(G, M.T).T
+    ls = rng.standard_normal((I, R))
+    ls = ls / np.linalg.norm(ls, axis=0) * np.array([0.6, 0.75, 0.9, 0.98])
+    M = ls @ G
     assert np.all((np.linalg.norm(ls, axis=0) > 0.5) & (np.linalg.norm(ls, axis=0) < 1))
     P1, _, _, _ = orc.admm(1, M, np.zeros((I, R)), np.zeros((I, R)), G, 3000, 0.0)
     assert np.allclose(P1, ls, rtol=1e-8, atol=1e-11)
