@@ -11,6 +11,7 @@
 #include "splatt/global_options.h"
 #include "splatt/coord.h"
 #include "splatt/csf.h"
+#include "splatt/mpi.h"
 #include "splatt/cpd.h"
 #include "splatt/api_kernels.h"
 #include "splatt/api_options.h"

@@ -127,6 +127,23 @@ splatt_error_type splatt_register_lasso(
     splatt_idx_t const * const modes_included,
     splatt_idx_t const num_modes);
 
+/* Non-negative factor with a Frobenius penalty (src/cpd/constraints/
+ * nonneg_frob.c:74-108) and with an L1 penalty (nonneg_lasso.c:67-99):
+ * ADMM, entry-wise proxes max(x rho/(multiplier+rho), 0) and
+ * max(x - multiplier/rho, 0); row-separable. The reference registers them
+ * from its command line (src/cmds/cmd_cpd_constraint.h:89-90, "ntf-frob"
+ * and "ntf-l1") without a declaration in its public header. */
+splatt_error_type splatt_register_ntf_frob(
+    splatt_cpd_opts * opts,
+    splatt_val_t const multiplier,
+    splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes);
+splatt_error_type splatt_register_ntf_lasso(
+    splatt_cpd_opts * opts,
+    splatt_val_t const multiplier,
+    splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes);
+
 /* Options and constraint objects: include/splatt/cpd.h:372-418. */
 splatt_cpd_opts * splatt_alloc_cpd_opts(void);
 void splatt_free_cpd_opts(splatt_cpd_opts * const opts);

@@ -70,6 +70,13 @@ struct _splatt_kruskal
   double fit;
 };
 
+/* Communication info of the non-MPI build, src/splatt_mpi.h:36-47. */
+struct _splatt_comm_info
+{
+  int world_rank;
+  int world_npes;
+};
+
 /* Dense matrix, src/matrix.h:10-16. */
 typedef struct
 {

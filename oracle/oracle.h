@@ -105,7 +105,10 @@ int or_cpd_als(or_csf * const * csfs, uint64_t const * mode_csf,
  * 3 = non-negative (nonneg.c), 4 = L1 soft threshold at lam / rho
  * (lasso.c:30-52), 5 = rows onto the probability simplex (simplex.cc:32-104),
  * 6 = column smoothness with multiplier lam (smooth.c:30-184), 7 = columns
- * onto the probability simplex (simplex.cc:120-158), 0 = none. chunk > 0 runs ADMM in independent chunks of
+ * onto the probability simplex (simplex.cc:120-158), 8 = non-negative with
+ * a Frobenius penalty lam (nonneg_frob.c:32-55), 9 = non-negative with an
+ * L1 penalty lam (nonneg_lasso.c:25-45), 0 = none. chunk > 0 runs ADMM in
+ * independent chunks of
  * that many rows (admm.c:355-400; the caller passes 0 unless the prox is
  * row-separable). */
 typedef struct { int kind; int prox; double diag_add; uint64_t chunk; double lam; } or_con;
@@ -140,7 +143,8 @@ uint64_t or_admm_chunked(int prox, double const * M, double * P, double * D,
 void or_smooth_prox(double * P, uint64_t I, uint64_t R, double lam, double rho);
 
 /* The proximal operators (row-major I x R, in place); thr is the lasso
- * threshold lam / rho (unused by the others). */
+ * threshold lam / rho for prox 4 (lasso.c) and 9 (nonneg_lasso.c), the
+ * scale rho / (lam + rho) for prox 8 (nonneg_frob.c), unused by the others. */
 void or_prox(int prox, double * P, uint64_t I, uint64_t R, double thr);
 
 /* ---- streaming (the files under src/cpd/stream) --------------------- */

@@ -179,6 +179,22 @@ void or_prox(int prox, double * P, uint64_t I, uint64_t R, double thr)
     free(buf);
     return;
   }
+  if(prox == 8) {
+    /* nonneg_frob.c:32-55: scaled by thr = rho / (lambda + rho), clamped */
+    for(uint64_t x=0; x < I * R; ++x) {
+      double const v = P[x] * thr;
+      P[x] = (v > 0.) ? v : 0.;
+    }
+    return;
+  }
+  if(prox == 9) {
+    /* nonneg_lasso.c:25-45: one-sided soft threshold at thr = lambda / rho */
+    for(uint64_t x=0; x < I * R; ++x) {
+      double const v = P[x];
+      P[x] = (v > thr) ? v - thr : 0.;
+    }
+    return;
+  }
   if(prox == 4) {
     for(uint64_t x=0; x < I * R; ++x) {
       double const v = P[x];
@@ -288,7 +304,7 @@ static uint64_t p_admm_iterate(int prox, double const * M, double * P, double * 
     if(prox == 6) {
       or_smooth_prox(P, I, R, lam, rho);
     } else {
-      or_prox(prox, P, I, R, lam / rho);
+      or_prox(prox, P, I, R, (prox == 8) ? rho / (lam + rho) : lam / rho);
     }
     double dual_norm = 0.;
     for(uint64_t x=0; x < I * R; ++x) {

@@ -168,6 +168,8 @@ int spd_dense_admm_chunked(void const * M, int mdtype, double const * ginv,
 #define SPD_PROX_ROWSIMP 5 /* rows onto the probability simplex (chunked ADMM) */
 #define SPD_PROX_SMOOTH 6  /* columns smoothed: rho (lam K + rho I)^-1 x */
 #define SPD_PROX_COLSIMP 7 /* columns onto the probability simplex */
+#define SPD_PROX_NTF_FROB 8  /* max(x rho / (lam + rho), 0) */
+#define SPD_PROX_NTF_LASSO 9 /* max(x - lam / rho, 0) */
 
 typedef struct {
   int kind;          /* SPD_CON_* */

@@ -690,14 +690,13 @@ struct Admm2Args {
   int const * ctl;          // ctl[0] != 0: converged, nothing to do
 };
 
-// nc: the column's norm (max-col-norm variants) or the lasso threshold
-// lam / rho (src/cpd/constraints/lasso.c:30-52)
+// nc: the column's norm (max-col-norm variants) or the entry-wise scalar of
+// dn_prox_scalar (src/cpd/constraints/lasso.c:30-52, nonneg_frob.c:32-55,
+// nonneg_lasso.c:25-45)
 __device__ __forceinline__ double dn_prox_apply(double p, int prox, double nc)
 {
-  if(prox == DN_PROX_NONNEG) {
-    p = (p > 0.) ? p : 0.;
-  } else if(prox == DN_PROX_LASSO) {
-    p = (p > nc) ? p - nc : ((p < -nc) ? p + nc : 0.);
+  if(prox == DN_PROX_NONNEG || dn_prox_scalar_kind(prox)) {
+    p = dn_prox_entry(p, prox, nc);
   } else if(prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG) {
     if(prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) p = 0.;
     if(!(nc <= 1.)) p /= nc;
@@ -742,9 +741,10 @@ k_admm_pass(Admm2Args<TM> a)
   int const lr = lane / RP;
   int const lc = lane - lr * RP;
   bool const lok = (lr < RPI) && ((uint32_t) lc < R);
-  double const thr = a.lam / rho;
-  double const ncur = (prox == DN_PROX_LASSO) ? thr : ((MODE >= 1 && lok) ? a.cur_n[lc] : 1.);
-  double const nprev = (prox == DN_PROX_LASSO) ? thr : ((MODE == 2 && lok) ? a.prev_n[lc] : 1.);
+  double const thr = dn_prox_scalar(prox, a.lam, rho);
+  bool const sk = dn_prox_scalar_kind(prox);
+  double const ncur = sk ? thr : ((MODE >= 1 && lok) ? a.cur_n[lc] : 1.);
+  double const nprev = sk ? thr : ((MODE == 2 && lok) ? a.prev_n[lc] : 1.);
   uint64_t const ngroups = (I + 15) / 16;
   for(uint64_t gi = (uint64_t) blockIdx.x * 4 + wave; gi < ngroups; gi += (uint64_t) gridDim.x * 4) {
     uint64_t const row0 = gi * 16;
@@ -956,7 +956,7 @@ __global__ void k_admm_final(Bufs3 bf, double * __restrict__ P, double * __restr
     int const * __restrict__ ctl)
 {
   int const k = ctl[1];
-  double const thr = lam / *rho;
+  double const thr = dn_prox_scalar(prox, lam, *rho);
   double const * const src = bf.b[k % 3];
   double const * const nrm = bf.n[k % 3];
   uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -965,7 +965,7 @@ __global__ void k_admm_final(Bufs3 bf, double * __restrict__ P, double * __restr
   uint32_t const sc = (uint32_t) (stride % R);
   for(; e < n; e += stride) {
     double const p1 = src[e];
-    double const p = dn_prox_apply(p1, prox, (prox == DN_PROX_LASSO) ? thr : nrm[c]);
+    double const p = dn_prox_apply(p1, prox, dn_prox_scalar_kind(prox) ? thr : nrm[c]);
     P[e] = p;
     D[e] = p - p1;
     c += sc;
@@ -1016,7 +1016,7 @@ k_admm_chunk(TM const * __restrict__ M, double const * __restrict__ ginv, uint32
   }
   uint32_t const TR = CH_TILE / R;
   double const rho = *rho_p;
-  double const thr = lam / rho;
+  double const thr = dn_prox_scalar(prox, lam, rho);
   uint32_t it = 0;
   while(it < max_inner) {
     double pn = 0., pr = 0., dr = 0., dn = 0.;
@@ -1093,10 +1093,8 @@ k_admm_chunk(TM const * __restrict__ M, double const * __restrict__ ginv, uint32
         if constexpr (SX) {
           p = s_p[e] - s_th[i];
           p = (p > 0.) ? p : 0.;
-        } else if(prox == DN_PROX_NONNEG) {
-          p = (p > 0.) ? p : 0.;
-        } else if(prox == DN_PROX_LASSO) {
-          p = (p > thr) ? p - thr : ((p < -thr) ? p + thr : 0.);
+        } else {
+          p = dn_prox_entry(p, prox, thr);
         }
         double const d = d0 + (p - x);
         P[base + e] = p;
@@ -1205,7 +1203,7 @@ k_admm_post(double * __restrict__ P, double const * __restrict__ aux,
     double const * __restrict__ norms, int prox, double lam, double const * __restrict__ rho,
     uint64_t n, uint32_t R, double * __restrict__ part)
 {
-  double const thr = lam / *rho;
+  double const thr = dn_prox_scalar(prox, lam, *rho);
   __shared__ double s[4][DN_THREADS];
   uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
@@ -1214,10 +1212,8 @@ k_admm_post(double * __restrict__ P, double const * __restrict__ aux,
   double pn = 0., pr = 0., dr = 0., dn = 0.;
   for(; e < n; e += stride) {
     double p = P[e];
-    if(prox == DN_PROX_NONNEG) {
-      p = (p > 0.) ? p : 0.;
-    } else if(prox == DN_PROX_LASSO) {
-      p = (p > thr) ? p - thr : ((p < -thr) ? p + thr : 0.);
+    if(prox == DN_PROX_NONNEG || dn_prox_scalar_kind(prox)) {
+      p = dn_prox_entry(p, prox, thr);
     } else if(prox == DN_PROX_MAXCOLNORM || prox == DN_PROX_MAXCOLNORM_NONNEG) {
       if(prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) p = 0.;
       double const nc = norms[c];
@@ -1902,7 +1898,7 @@ int dn_admm_chunked(void const * M, int mdtype, double const * ginv, double cons
 {
   if(I == 0 || max_inner == 0) return SPD_OK;
   if(chunk == 0 || R == 0 || R > CH_TILE || spd_cdiv(I, chunk) > 0x7FFFFFFFull ||
-      (prox != DN_PROX_NONNEG && prox != DN_PROX_LASSO && prox != DN_PROX_ROWSIMP &&
+      (prox != DN_PROX_NONNEG && !dn_prox_scalar_kind(prox) && prox != DN_PROX_ROWSIMP &&
        prox != DN_PROX_NONE)) {
     spd_set_error(SPD_EINPUT, "chunked ADMM: unsupported shape (rows %llu, rank %u, "
         "chunk %llu, prox %d)", (unsigned long long) I, R, (unsigned long long) chunk, prox);

@@ -15,6 +15,34 @@
 #define DN_PROX_ROWSIMP 5      // chunked ADMM only
 #define DN_PROX_SMOOTH 6       // one-block ADMM, one step at a time
 #define DN_PROX_COLSIMP 7      // one-block ADMM, one step at a time
+#define DN_PROX_NTF_FROB 8     // max(p * rho / (lam + rho), 0)
+#define DN_PROX_NTF_LASSO 9    // max(p - lam / rho, 0)
+
+// Entry-wise operators that take one scalar of lam and rho: the lasso
+// thresholds (lam / rho) and the non-negative frob scale (rho / (lam + rho)).
+__host__ __device__ __forceinline__ bool dn_prox_scalar_kind(int prox)
+{
+  return prox == DN_PROX_LASSO || prox == DN_PROX_NTF_LASSO || prox == DN_PROX_NTF_FROB;
+}
+__host__ __device__ __forceinline__ double dn_prox_scalar(int prox, double lam, double rho)
+{
+  return (prox == DN_PROX_NTF_FROB) ? rho / (lam + rho) : lam / rho;
+}
+// Entry-wise operators that need nothing but the entry and that scalar.
+__host__ __device__ __forceinline__ double dn_prox_entry(double p, int prox, double thr)
+{
+  if(prox == DN_PROX_NONNEG) {
+    p = (p > 0.) ? p : 0.;
+  } else if(prox == DN_PROX_LASSO) {
+    p = (p > thr) ? p - thr : ((p < -thr) ? p + thr : 0.);
+  } else if(prox == DN_PROX_NTF_LASSO) {
+    p = (p > thr) ? p - thr : 0.;
+  } else if(prox == DN_PROX_NTF_FROB) {
+    p = p * thr;
+    p = (p > 0.) ? p : 0.;
+  }
+  return p;
+}
 
 // out (R x R, full, symmetric) = X^T X. When div != nullptr the columns of X
 // are first divided in place by div[r] (X[:,r] /= div[r], the reference's

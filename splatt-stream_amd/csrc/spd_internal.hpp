@@ -64,6 +64,38 @@ int spd_scratch(int slot, size_t bytes, void ** ptr);
 void spd_scratch_bind(SpdScratchCtx * ctx);
 // Free a context's buffers (stream-ordered on its stream).
 void spd_scratch_release(SpdScratchCtx * ctx);
+// The context bound to this thread (nullptr: the library's own).
+SpdScratchCtx * spd_scratch_bound(void);
+
+// Scratch for one call on a caller's stream. Entry points that accept any
+// hipStream_t and return without synchronising must not share the library's
+// scratch: two calls on two streams would write the same buffers at once.
+// When the thread has no context bound and st is not the library stream,
+// this binds a context of its own for the call; its buffers are allocated
+// and freed in stream order on st (hipMallocAsync / hipFreeAsync), so they
+// stay alive for the kernels the call queued and no call waits for another.
+struct SpdCallScratch {
+  SpdScratchCtx ctx;
+  SpdScratchCtx * prev;
+  bool active;
+  explicit SpdCallScratch(hipStream_t st) : ctx(), prev(spd_scratch_bound()), active(false)
+  {
+    if(prev == nullptr && st != (hipStream_t) spd_stream()) {
+      ctx.st = st;
+      spd_scratch_bind(&ctx);
+      active = true;
+    }
+  }
+  ~SpdCallScratch()
+  {
+    if(active) {
+      spd_scratch_release(&ctx);
+      spd_scratch_bind(prev);
+    }
+  }
+  SpdCallScratch(SpdCallScratch const &) = delete;
+  SpdCallScratch & operator=(SpdCallScratch const &) = delete;
+};
 
 // Stream-ordered device allocation from the device's memory pool: no
 // device-wide synchronisation (hipFree waits for the whole device), the

@@ -1307,6 +1307,8 @@ static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
   if(st != spd_hs(nullptr)) {
     SPD_HIP(hipStreamSynchronize(spd_hs(nullptr)));
   }
+  // the carry buffers belong to this call when it runs on a caller's stream
+  SpdCallScratch scratch(st);
   uint64_t const nwaves = spd_cdiv(c->nnz, R3_WN);
   uint32_t const nblocks = (uint32_t) spd_cdiv(nwaves, R3_WAVES);
   void * cv = nullptr;
@@ -1531,6 +1533,11 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
     a.crow = cp->crow;
     a.vals = (sizeof(T) == 4) ? (T const *) cp->v32 : (T const *) cp->v64;
   }
+  HotPlan * hp = nullptr;
+  if(depth > 0 && cp == nullptr) SPD_TRY(get_hot(c, depth, &hp, spd_hs(nullptr)));
+  // the carry buffers belong to this call when it runs on a caller's stream
+  // (everything above is built on the library stream and kept with the CSF)
+  SpdCallScratch scratch(st);
   if(depth == 0 || cp != nullptr) {
     // two partial rows per range of 256 / (64 / L) positions
     int V = 1, L = 16;
@@ -1543,8 +1550,6 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
     a.carry = (T *) cv;
     a.carry_row = (uint32_t *) cr;
   } else {
-    HotPlan * hp = nullptr;
-    SPD_TRY(get_hot(c, depth, &hp, spd_hs(nullptr)));
     // privatise rows that collect at least two contributions
     uint32_t nhot = 0;
     while(nhot < hp->counts.size() && hp->counts[nhot] >= 2) ++nhot;

@@ -283,7 +283,7 @@ static void slice_order(Slice const * sl, int N, int t, uint64_t const * cap,
 
 // Host part of a slice: order by the modes after the streamed one
 // (cyclically), relabel by first appearance, upload. Device part: the step.
-int run_slice(Stream * s, Slice * sl, WorkerRes & w)
+static int run_slice_body(Stream * s, Slice * sl, WorkerRes & w)
 {
   int const N = s->N;
   int const t = s->cfg.stream_mode;
@@ -328,16 +328,24 @@ int run_slice(Stream * s, Slice * sl, WorkerRes & w)
     hv[x] = sl->vals[order[x]];
     xn += hv[x] * hv[x];
   }
+  // every index is checked before any is labelled, so a rejected slice
+  // leaves the stream's labels as they were
   for(int m=0; m < N; ++m) {
-    s->state->batch_max[m] = 0;
     if(m == t) continue;
     for(uint64_t x=0; x < n; ++x) {
-      uint64_t const o = sl->ind[m][order[x]];
+      uint64_t const o = sl->ind[m][x];
       if(o >= s->cap[m]) {
         spd_set_error(SPD_EINPUT, "serve: index %llu of mode %d beyond its capacity %llu",
             (unsigned long long) o, m, (unsigned long long) s->cap[m]);
         return SPD_EINPUT;
       }
+    }
+  }
+  for(int m=0; m < N; ++m) {
+    s->state->batch_max[m] = 0;
+    if(m == t) continue;
+    for(uint64_t x=0; x < n; ++x) {
+      uint64_t const o = sl->ind[m][order[x]];
       if(s->lab[m][o] == UINT32_MAX) s->lab[m][o] = (uint32_t) s->next_label[m]++;
       uint32_t const l = s->lab[m][o];
       hind[m][x] = l;
@@ -372,6 +380,15 @@ int run_slice(Stream * s, Slice * sl, WorkerRes & w)
       hipMemcpyDeviceToHost, s->hs));
   SPD_HIP(hipStreamSynchronize(s->hs));
   return SPD_OK;
+}
+
+// Every exit leaves the slice's work finished: the stream's next slice may
+// run on another worker's stream, which waits only for its own work.
+int run_slice(Stream * s, Slice * sl, WorkerRes & w)
+{
+  int const rc = run_slice_body(s, sl, w);
+  if(rc != SPD_OK && s->hs != nullptr) (void) hipStreamSynchronize(s->hs);
+  return rc;
 }
 
 void worker_main(spd_server * sv, int wi)

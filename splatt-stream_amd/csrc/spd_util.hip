@@ -230,6 +230,11 @@ void spd_scratch_bind(SpdScratchCtx * ctx)
   t_ctx = ctx;
 }
 
+SpdScratchCtx * spd_scratch_bound(void)
+{
+  return t_ctx;
+}
+
 void spd_scratch_release(SpdScratchCtx * ctx)
 {
   if(ctx == nullptr) return;

@@ -23,6 +23,26 @@ void splatt_free(void * ptr)
   free(ptr);
 }
 
+/* src/mpi_dummy.c:15-34: the non-MPI communicator is rank 0 of 1. */
+splatt_comm_info * splatt_alloc_comm_info(void)
+{
+  splatt_comm_info * ci = (splatt_comm_info *) splatt_malloc(sizeof(*ci));
+  if(ci == NULL) {
+    return NULL;
+  }
+  ci->world_rank = 0;
+  ci->world_npes = 1;
+  return ci;
+}
+
+void splatt_free_comm_info(splatt_comm_info * comm_info)
+{
+  if(comm_info == NULL) {
+    return;
+  }
+  splatt_free(comm_info);
+}
+
 int splatt_version_major(void) { return SPLATT_VER_MAJOR; }
 int splatt_version_minor(void) { return SPLATT_VER_MINOR; }
 int splatt_version_subminor(void) { return SPLATT_VER_SUBMINOR; }

@@ -78,6 +78,12 @@ int spl_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
   } else if(con->prox_func == splatt_lasso_prox) {
     out->prox = SPD_PROX_LASSO;
     out->lam = *((val_t const *) con->data);
+  } else if(con->prox_func == splatt_ntf_frob_prox) {
+    out->prox = SPD_PROX_NTF_FROB;
+    out->lam = *((val_t const *) con->data);
+  } else if(con->prox_func == splatt_ntf_lasso_prox) {
+    out->prox = SPD_PROX_NTF_LASSO;
+    out->lam = *((val_t const *) con->data);
   } else if(con->prox_func == splatt_colsimp_prox) {
     out->prox = SPD_PROX_COLSIMP;
   } else if(con->prox_func == splatt_smooth_prox) {

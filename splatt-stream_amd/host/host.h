@@ -46,6 +46,10 @@ void splatt_lasso_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
     idx_t const offset, void * data, val_t const rho, bool const should_parallelize);
 void splatt_nonneg_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
     idx_t const offset, void * data, val_t const rho, bool const should_parallelize);
+void splatt_ntf_frob_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
+    idx_t const offset, void * data, val_t const rho, bool const should_parallelize);
+void splatt_ntf_lasso_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
+    idx_t const offset, void * data, val_t const rho, bool const should_parallelize);
 
 /* A registered constraint as the device drivers run it (0, or -1 when the
  * GPU path does not provide it; the reason is printed). */

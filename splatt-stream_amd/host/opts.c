@@ -348,6 +348,77 @@ static void p_lasso_free(void * data)
   free(data);
 }
 
+/* src/cpd/constraints/nonneg_frob.c:32-55: every entry scaled by
+ * rho / (lambda + rho), then clamped at zero. */
+void splatt_ntf_frob_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
+    idx_t const offset, void * data, val_t const rho, bool const should_parallelize)
+{
+  (void) offset;
+  (void) should_parallelize;
+  val_t const mult = rho / (*((val_t const *) data) + rho);
+  for(idx_t x=0; x < nrows * ncols; ++x) {
+    val_t const v = primal[x] * mult;
+    primal[x] = (v > 0.) ? v : 0.;
+  }
+}
+
+/* src/cpd/constraints/nonneg_lasso.c:25-45: one-sided soft threshold at
+ * lambda / rho (entries below it become zero). */
+void splatt_ntf_lasso_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
+    idx_t const offset, void * data, val_t const rho, bool const should_parallelize)
+{
+  (void) offset;
+  (void) should_parallelize;
+  val_t const thr = *((val_t const *) data) / rho;
+  for(idx_t x=0; x < nrows * ncols; ++x) {
+    val_t const v = primal[x];
+    primal[x] = (v > thr) ? v - thr : 0.;
+  }
+}
+
+/* One row-separable, sparsity-inducing ADMM constraint per mode holding its
+ * own copy of the multiplier (the shape of nonneg_frob.c:74-108 and
+ * nonneg_lasso.c:67-99, which both describe themselves as NTF-L1-REG). */
+static splatt_error_type p_register_mult(splatt_cpd_opts * opts, splatt_val_t multiplier,
+    splatt_idx_t const * modes, splatt_idx_t num,
+    void (*prox)(val_t *, idx_t const, idx_t const, idx_t const, void *,
+        val_t const, bool const), char const * fmt)
+{
+  for(idx_t m=0; m < num; ++m) {
+    splatt_cpd_constraint * con = splatt_alloc_constraint(SPLATT_CON_ADMM);
+    con->prox_func = prox;
+    con->free_func = p_lasso_free;
+    con->hints.row_separable = true;
+    con->hints.sparsity_inducing = true;
+    snprintf(con->description, sizeof(con->description), fmt, multiplier);
+    val_t * mult = (val_t *) malloc(sizeof(*mult));
+    if(mult == NULL) {
+      splatt_free_constraint(con);
+      return SPLATT_ERROR_NOMEMORY;
+    }
+    *mult = multiplier;
+    con->data = mult;
+    splatt_register_constraint(opts, modes[m], con);
+  }
+  return SPLATT_SUCCESS;
+}
+
+splatt_error_type splatt_register_ntf_frob(splatt_cpd_opts * opts,
+    splatt_val_t const multiplier, splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes)
+{
+  return p_register_mult(opts, multiplier, modes_included, num_modes,
+      splatt_ntf_frob_prox, "NTF-L1-REG (%0.1e)");
+}
+
+splatt_error_type splatt_register_ntf_lasso(splatt_cpd_opts * opts,
+    splatt_val_t const multiplier, splatt_idx_t const * const modes_included,
+    splatt_idx_t const num_modes)
+{
+  return p_register_mult(opts, multiplier, modes_included, num_modes,
+      splatt_ntf_lasso_prox, "NTF-L1-REG (%0.1e)");
+}
+
 static splatt_error_type p_register_prox(splatt_cpd_opts * opts,
     splatt_idx_t const * modes, splatt_idx_t num,
     void (*prox)(val_t *, idx_t const, idx_t const, idx_t const, void *,

@@ -84,6 +84,10 @@ class CpdOpts(C.Structure):
                 ("chunk_sizes", C.c_uint64 * MAX_NMODES)]
 
 
+class CommInfo(C.Structure):
+    _fields_ = [("world_rank", C.c_int), ("world_npes", C.c_int)]
+
+
 class GlobalOpts(C.Structure):
     _fields_ = [("num_threads", C.c_int), ("verbosity", C.c_int),
                 ("random_seed", C.c_int)]
@@ -202,6 +206,12 @@ def lib():
     L.splatt_register_lasso.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
     L.splatt_register_smooth.restype = C.c_int
     L.splatt_register_smooth.argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
+    for name in ("splatt_register_ntf_frob", "splatt_register_ntf_lasso"):
+        getattr(L, name).restype = C.c_int
+        getattr(L, name).argtypes = [C.POINTER(CpdOpts), C.c_double, u64p, C.c_uint64]
+    L.splatt_alloc_comm_info.restype = C.POINTER(CommInfo)
+    L.splatt_alloc_comm_info.argtypes = []
+    L.splatt_free_comm_info.argtypes = [C.POINTER(CommInfo)]
     for name in ("splatt_register_maxcolnorm", "splatt_register_maxcolnorm_nonneg",
                  "splatt_register_nonneg", "splatt_register_rowsimp",
                  "splatt_register_colsimp"):

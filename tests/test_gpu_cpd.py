@@ -188,20 +188,23 @@ def test_admm_step_matches_oracle(gpu_available, R, prox):
                                np.sum(D * D)], rtol=1e-12)
 
 
-@pytest.mark.parametrize("R,I", [(5, 1), (16, 37), (33, 500), (64, 3000)])
-@pytest.mark.parametrize("prox", [0, 1, 2, 3, 4, 6, 7])
+@pytest.mark.parametrize("R,I", [(5, 1), (16, 37), (24, 700), (32, 2000), (33, 500),
+                                 (64, 3000)])
+@pytest.mark.parametrize("prox", [0, 1, 2, 3, 4, 6, 7, 8, 9])
 @pytest.mark.parametrize("tol,max_inner", [(1e-2, 50), (7e-2, 25), (0.0, 6)])
 def test_admm_solve_matches_oracle(gpu_available, R, I, prox, tol, max_inner):
     # the whole inner loop: same iterations, P and D at 1e-8 (prox 4: lasso
     # with multiplier 2; prox 6: column smoothness with multiplier 0.5;
-    # prox 7: column simplex)
+    # prox 7: column simplex; 8: non-negative frob 0.5; 9: non-negative L1
+    # 2). Ranks 17-32 (24, 32: the streaming workload's rank) run the
+    # two-tile pass.
     rng = np.random.default_rng(R * 7 + prox + int(tol * 1000))
     A = rng.standard_normal((3 * R, R))
     G = A.T @ A + 0.5 * np.eye(R)
     M = rng.standard_normal((I, R)) * 4.0
     P0 = rng.standard_normal((I, R))
     D0 = rng.standard_normal((I, R)) * 0.1
-    lam = {4: 2.0, 6: 0.5}.get(prox, 0.0)
+    lam = {4: 2.0, 6: 0.5, 8: 0.5, 9: 2.0}.get(prox, 0.0)
     wantP, wantD, wits, wrho = orc.admm(prox, M, P0, D0, G, max_inner, tol, lam=lam)
     L = ps.lib()
     RP = _rp(R)
@@ -367,6 +370,29 @@ def test_cpd_lasso_matches_oracle(gpu_available, chunk):
     _model_close(res, facs, lam, 1e-5)
 
 
+@pytest.mark.parametrize("chunk", [50, 0])
+def test_cpd_ntf_matches_oracle(gpu_available, chunk):
+    # splatt_register_ntf_frob on mode 0 and splatt_register_ntf_lasso on
+    # mode 2 (nonneg_frob.c, nonneg_lasso.c), chunked or one block
+    lf, ll = 0.5, 1e-4
+
+    def configure(L, co):
+        m0 = np.array([0], dtype=np.uint64)
+        m2 = np.array([2], dtype=np.uint64)
+        assert L.splatt_register_ntf_frob(co, lf, m0.ctypes.data_as(ps.u64p), 1) == ps.SUCCESS
+        assert L.splatt_register_ntf_lasso(co, ll, m2.ctypes.data_as(ps.u64p), 1) == ps.SUCCESS
+        for m in range(3):
+            co[0].chunk_sizes[m] = chunk
+    res, facs, lam, fits = _run_both(tensor_path("med.tns"), 5, 3,
+                                     cons=[(2, 8, 0.0, chunk, lf), (1, 0, 0.0),
+                                           (2, 9, 0.0, chunk, ll)],
+                                     unconstrained=False, configure=configure)
+    for m in (0, 2):
+        assert np.all(res["factors"][m] >= 0)
+    assert abs(res["fit"] - fits[-1]) <= 1e-6 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-5)
+
+
 @pytest.mark.parametrize("chunk", [50, 7])
 def test_cpd_rowsimp_matches_oracle(gpu_available, chunk):
     # splatt_register_rowsimp on mode 0: chunked ADMM whose prox projects
@@ -429,10 +455,11 @@ def test_cpd_rowsimp_needs_chunks(gpu_available):
     assert res["rc"] == ps.ERROR_BADINPUT
 
 
-@pytest.mark.parametrize("prox,lam", [(3, 0.0), (4, 2.0), (5, 0.0)])
+@pytest.mark.parametrize("prox,lam", [(3, 0.0), (4, 2.0), (5, 0.0), (8, 0.5), (9, 2.0)])
 def test_admm_chunked_matches_oracle(gpu_available, prox, lam):
     # the chunk kernel alone: rank 72 keeps the inverse in global memory,
-    # FP32 MTTKRP input, 33-row chunks; nonneg, lasso and row simplex
+    # FP32 MTTKRP input, 33-row chunks; nonneg, lasso, row simplex and the
+    # non-negative frob and L1 forms
     I, R, chunk = 300, 72, 33
     rng = np.random.default_rng(31)
     A = rng.standard_normal((3 * R, R))
@@ -463,3 +490,84 @@ def test_admm_chunked_matches_oracle(gpu_available, prox, lam):
     tol = 1e-8
     assert np.allclose(P, wantP, rtol=tol, atol=tol * np.max(np.abs(wantP)))
     assert np.allclose(D, wantD, rtol=tol, atol=tol * np.max(np.abs(wantP)))
+
+
+# ---- the benchmark workloads at their own kernel instantiations -----------
+
+def test_cpd_uniform_1m_rank16_20_iterations(gpu_available):
+    # CPD-ALS, rank 16, 20 iterations, FP64, 1000x800x600 with 1,000,000
+    # uniform nonzeros (values U(0,1)), two-CSF allocation: the fit of every
+    # iteration and the final model against the oracle from the same start
+    print("case cpd 1000x800x600 1M R16", flush=True)
+    inds, vals, dims = ps.synth_coord([1000, 800, 600], 1_000_000, vals="uniform",
+                                      seed=20261203)
+    cs = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=ps.CSF_TWOMODE))
+    res = ps.run_cpd(cs, 16, max_its=20, tol=0.0, seed=1)
+    assert res["rc"] == ps.SUCCESS, ps.last_error()
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    perm0 = cs.arrays(0)["perm"]
+    ocsfs = [orc.Csf(coo, perm0), orc.Csf(coo, cs.arrays(1)["perm"])]
+    mode_csf = [1 if perm0[-1] == m else 0 for m in range(3)]
+    facs, lam, fits = orc.cpd_als_con(ocsfs, mode_csf, res["init"], 20, 0.0,
+                                      [(0, 0, 0.0)] * 3, True, nthreads=8)
+    assert len(fits) == 20
+    assert abs(res["fit"] - fits[-1]) <= 1e-8 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-6)
+    cs.free()
+
+
+def _c3_tensor(nnz, seed=20261208):
+    # the 4-mode workload's shape at a reduced nonzero count: Zipf(1.1) on
+    # the first three modes, uniform on the fourth (2000), values U(-1,1)
+    return ps.synth_coord([50_000, 30_000, 10_000, 2_000], nnz, zipf=[1.1, 1.1, 1.1, 0.0],
+                          vals="symmetric", compact=False, seed=seed)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_mttkrp_4mode_rank64_twomode(gpu_available, prec):
+    # every mode of a two-CSF 4-mode tensor at rank 64 (the root, internal
+    # and leaf walks at the rank-64 lane layout), FP64 and FP32, against the
+    # long-double COO oracle
+    print("case 4-mode rank 64 prec", prec, flush=True)
+    inds, vals, dims = _c3_tensor(600_000)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    cs = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=ps.CSF_TWOMODE))
+    R = 64
+    mats = [ps.synth_matrix(d, R, 500 + m) for m, d in enumerate(dims)]
+    L = ps.lib()
+    assert L.splatt_amd_set_mttkrp_precision(prec) == ps.SUCCESS
+    try:
+        for mode in range(4):
+            got = cs.mttkrp(mode, mats)
+            if prec == 0:
+                want = orc.mttkrp_coo(coo, mats, mode)
+                tol = 1e-12
+            else:
+                m32 = [m.astype(np.float32) for m in mats]
+                want = orc.mttkrp_coo(coo, m32, mode, f32=True)
+                tol = 1e-5
+            err = np.linalg.norm(got - want) / np.linalg.norm(want)
+            assert err <= tol, (mode, err)
+    finally:
+        L.splatt_amd_set_mttkrp_precision(0)
+    cs.free()
+
+
+def test_cpd_4mode_rank64_fp64(gpu_available):
+    # three FP64 CPD-ALS iterations of the 4-mode shape at rank 64 against
+    # the oracle from the same start (the factors stay well posed over so
+    # few iterations)
+    print("case cpd 4-mode rank 64", flush=True)
+    inds, vals, dims = _c3_tensor(300_000, seed=7)
+    cs = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=ps.CSF_TWOMODE))
+    res = ps.run_cpd(cs, 64, max_its=3, tol=0.0, seed=3)
+    assert res["rc"] == ps.SUCCESS, ps.last_error()
+    coo = orc.Coo.from_arrays(inds, vals, res["dims"])
+    perm0 = cs.arrays(0)["perm"]
+    ocsfs = [orc.Csf(coo, perm0), orc.Csf(coo, cs.arrays(1)["perm"])]
+    mode_csf = [1 if perm0[-1] == m else 0 for m in range(4)]
+    facs, lam, fits = orc.cpd_als_con(ocsfs, mode_csf, res["init"], 3, 0.0,
+                                      [(0, 0, 0.0)] * 4, True, nthreads=8)
+    assert abs(res["fit"] - fits[-1]) <= 1e-8 * max(1.0, abs(fits[-1]))
+    _model_close(res, facs, lam, 1e-6)
+    cs.free()

@@ -171,3 +171,52 @@ def test_inner_product_is_mode_independent(gpu_available):
     # linearity in the values
     s2 = ps.CsfSet.convert(inds, 2.0 * vals, ps.Opts(csf_alloc=ps.CSF_ONEMODE))
     assert rel(s2.mttkrp(0, mats), 2.0 * s.mttkrp(0, mats)) <= 1e-14
+
+
+def _hip():
+    h = ctypes.CDLL("libamdhip64.so")
+    h.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    h.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    return h
+
+
+def test_device_entry_point_two_caller_streams(gpu_available):
+    # splatt_amd_mttkrp_dev returns without synchronising and accepts any
+    # stream: back-to-back calls on two caller streams (no wait between
+    # them) must not share partial-row buffers. Both results against the
+    # oracle, for every mode of an all-mode CSF (each a root output with
+    # carried partial rows), FP32 and FP64.
+    inds, vals, dims = ps.synth_coord([3000, 2500, 2000], 400000, zipf=[1.2] * 3, seed=91)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    opts = ps.Opts(csf_alloc=ps.CSF_ALLMODE)
+    s = ps.CsfSet.convert(inds, vals, opts)
+    R = 32
+    sets = [[ps.synth_matrix(d, R, 300 + 10 * k + m) for m, d in enumerate(dims)]
+            for k in range(2)]
+    L = ps.lib()
+    h = _hip()
+    streams = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for st in streams:
+        assert h.hipStreamCreate(ctypes.byref(st)) == 0
+    try:
+        for prec, dt in ((ps.F64, np.float64), (ps.F32, np.float32)):
+            dms = [[ps.DevBuf.from_host(m.astype(dt)) for m in mats] for mats in sets]
+            arrs = [(ctypes.c_void_p * 3)(*[b.ptr for b in dm]) for dm in dms]
+            for mode in range(3):
+                outs = [ps.DevBuf(dims[mode] * R * np.dtype(dt).itemsize) for _ in range(2)]
+                for k in range(2):
+                    assert L.splatt_amd_mttkrp_dev(mode, R, s.ptr, arrs[k], outs[k].ptr, prec,
+                                                   opts.ptr, streams[k]) == ps.SUCCESS
+                for st in streams:
+                    assert h.hipStreamSynchronize(st) == 0
+                for k in range(2):
+                    got = outs[k].to_host((dims[mode], R), dt)
+                    if prec == ps.F64:
+                        assert rel(got, orc.mttkrp_coo(coo, sets[k], mode)) <= TOL64
+                    else:
+                        m32 = [m.astype(np.float32) for m in sets[k]]
+                        assert rel(got, orc.mttkrp_coo(coo, m32, mode, f32=True)) <= TOL32
+    finally:
+        for st in streams:
+            h.hipStreamDestroy(st)

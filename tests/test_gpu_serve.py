@@ -71,7 +71,8 @@ def _serve(streams, max_inflight, interleave=True, budget=0):
 def _streams():
     specs = [([300, 200], 8, (200, 600)), ([1000, 50], 16, (300, 900)),
              ([80, 700], 4, (100, 300)), ([500, 500], 32, (400, 800)),
-             ([2000, 1500], 8, (500, 1500)), ([120, 90], 24, (50, 200))]
+             ([2000, 1500], 8, (500, 1500)), ([120, 90], 24, (50, 200)),
+             ([900, 700], 64, (600, 1200))]
     out = []
     for k, (cap, R, per) in enumerate(specs):
         out.append(dict(cap=cap, R=R, seed=100 + k,
@@ -127,3 +128,28 @@ def test_admission_by_budget(gpu_available):
         for m in range(3):
             assert np.allclose(res[k][m], alone[k][m], rtol=1e-9,
                                atol=1e-12 * max(1.0, np.max(np.abs(alone[k][m]))))
+
+
+def test_served_rank32_stream_matches_oracle(gpu_available):
+    # a rank-32 Zipf(1.2) stream served alone against the oracle's streaming
+    # CPD: the server draws new rows from the stream's own rand() sequence,
+    # which is glibc's after srand(seed)
+    import ctypes
+    import oracle_py as orc
+    libc = ctypes.CDLL("libc.so.6")
+    cap = [3000, 2000]
+    s = dict(cap=cap, R=32, seed=77, slices=_slices(6, cap, (1500, 3000), seed=77, zipf=1.2))
+    res, info, _ = _serve([s], 1)
+    inds = [np.concatenate([sl[0][m] for sl in s["slices"]]) for m in range(3)]
+    vals = np.concatenate([sl[1] for sl in s["slices"]])
+    dims = [int(inds[0].max()) + 1, int(inds[1].max()) + 1, len(s["slices"])]
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    orc.stream_prepare(coo, 2)
+    libc.srand(77)
+    ref = orc.stream_cpd(coo, 2, 32, 0.95, 5, [(2, 1, 0.0), (2, 1, 0.0), (1, 0, 1e-2)])
+    assert [i["outer"] for i in info] == [int(x) for x in ref["outer_its"]]
+    assert np.allclose([i["local_err"] for i in info], ref["local_err"], rtol=1e-8, atol=1e-12)
+    for m, want in ((0, ref["factors"][0]), (1, ref["factors"][1]), (2, ref["time"])):
+        got = res[0][m]
+        assert got.shape == want.shape, m
+        assert np.allclose(got, want, rtol=1e-8, atol=1e-8 * np.max(np.abs(want))), m

@@ -212,3 +212,13 @@ def test_stream_first_device_call_keeps_rand(gpu_available):
                        timeout=300, env=env)
     assert r.returncode == 0, r.stderr[-2000:]
     assert float(r.stdout.strip().splitlines()[-1]) <= 1e-10
+
+
+def test_stream_rank32_zipf_frob_matches_oracle(gpu_available):
+    # the streaming workload's settings: rank 32, Zipf(1.2) indices, frob
+    # 1e-2 on the time mode, 5 outer iterations, forgetting 0.95
+    print("case stream rank 32 zipf", flush=True)
+    inds, vals, dims = _stream(10, [20000, 8000], (8000, 20000), seed=32, zipf=1.2)
+    dev = ps.run_stream(inds, vals, dims, 32, 2, forget=0.95, max_its=5, seed=4, frob=1e-2)
+    ref = _oracle(inds, vals, dims, 2, 32, 0.95, 5, 4, 1e-2)
+    _compare(dev, ref, 2)

@@ -204,3 +204,46 @@ def test_mttkrp_ws_map():
         # privatise iff dims*T <= 0.02*nnz: only mode 1 (7*8 = 56 <= 200)
         assert [bool(ws[0].is_privatized[m]) for m in range(3)] == [False, True, False]
         L.splatt_mttkrp_free_ws(ws)
+
+
+def _integration_program():
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    m = re.search(r"<!-- program: cpd_example.c -->\s*```c\n(.*?)```", text, re.S)
+    assert m, "INTEGRATION.md lost its example program"
+    return m.group(1)
+
+
+def build_integration_program(tmpdir):
+    """INTEGRATION.md's C program, compiled against include/ and linked
+    with libsplatt.so exactly as the document says (C99 and C++)."""
+    import shutil
+    import subprocess
+    src = os.path.join(str(tmpdir), "cpd_example.c")
+    with open(src, "w") as f:
+        f.write(_integration_program())
+    exe = os.path.join(str(tmpdir), "cpd_example")
+    pkg = os.path.join(ROOT, "splatt-stream_amd")
+    inc = os.path.join(ROOT, "include")
+    for cc, std in (("gcc", "-std=c99"), ("g++", "-std=c++14")):
+        if shutil.which(cc) is None:
+            pytest.skip("needs the host " + cc)
+        lang = ["-x", "c++"] if cc == "g++" else []
+        subprocess.run([cc, std, "-Wall", "-Werror"] + lang + [src, "-I" + inc, "-L" + pkg,
+                        "-lsplatt", "-Wl,-rpath," + pkg, "-o", exe],
+                       check=True, capture_output=True, timeout=120)
+    return exe
+
+
+def test_integration_program_compiles_and_links(tmp_path):
+    exe = build_integration_program(tmp_path)
+    assert os.path.exists(exe)
+
+
+def test_comm_info_stubs():
+    # tests/api_test.c:95-107 of the reference: the non-MPI communicator
+    L = ps.lib()
+    ci = L.splatt_alloc_comm_info()
+    assert ci
+    assert ci[0].world_rank == 0 and ci[0].world_npes == 1
+    L.splatt_free_comm_info(ci)
+    L.splatt_free_comm_info(None)

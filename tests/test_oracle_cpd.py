@@ -409,6 +409,49 @@ def test_admm_lasso_meets_l1_optimality():
     assert its0 == its4 and np.array_equal(P0, P4) and np.array_equal(D0, D4)
 
 
+def test_ntf_proxes_are_nonneg_proximal_points():
+    # nonneg_frob.c / nonneg_lasso.c: with step 1/rho, x = prox(v) minimises
+    # rho/2 (x - v)^2 + lam/2 x^2 (frob) or rho/2 (x - v)^2 + lam x (lasso)
+    # over x >= 0; the KKT conditions of that scalar problem are checked on
+    # every entry, and a brute-force grid minimum agrees
+    rng = np.random.default_rng(17)
+    v = rng.standard_normal((30, 6)) * 2.0
+    lam, rho = 0.7, 1.9
+    xf = orc.prox(8, v, rho / (lam + rho))
+    xl = orc.prox(9, v, lam / rho)
+    for x, g in ((xf, rho * (xf - v) + lam * xf), (xl, rho * (xl - v) + lam)):
+        assert np.all(x >= 0)
+        pos = x > 0
+        assert np.max(np.abs(g[pos])) <= 1e-13 * np.abs(v).max() * rho
+        assert np.all(g[~pos] >= -1e-13)
+        assert 0 < pos.sum() < pos.size
+    grid = np.linspace(0.0, 8.0, 800001)
+    for (i, j) in ((0, 0), (3, 2), (7, 5)):
+        f = rho / 2 * (grid - v[i, j]) ** 2 + lam / 2 * grid ** 2
+        assert abs(grid[np.argmin(f)] - xf[i, j]) <= 2e-5
+        f = rho / 2 * (grid - v[i, j]) ** 2 + lam * grid
+        assert abs(grid[np.argmin(f)] - xl[i, j]) <= 2e-5
+
+
+@pytest.mark.parametrize("prox", [8, 9])
+def test_admm_ntf_meets_kkt(prox):
+    # ADMM with these proxes solves min_{P >= 0} tr(P G P^T)/2 - tr(M^T P)
+    # + lam/2 |P|^2 (8) or + lam sum(P) (9): per entry the gradient
+    # g = P G - M + lam P (8) or + lam (9) is 0 where P > 0, >= 0 where P = 0
+    rng = np.random.default_rng(19 + prox)
+    I, R, lam = 40, 5, 1.5
+    A = rng.standard_normal((15, R))
+    G = A.T @ A + 0.5 * np.eye(R)
+    M = rng.standard_normal((I, R)) * 4.0
+    Z = np.zeros((I, R))
+    P, D, its, rho = orc.admm(prox, M, Z, Z, G, 5000, 1e-15, lam=lam)
+    g = P @ G - M + (lam * P if prox == 8 else lam)
+    pos = P > 0
+    assert np.all(P >= 0) and 0 < pos.sum() < pos.size
+    assert np.max(np.abs(g[pos])) <= 1e-5 * np.abs(M).max()
+    assert np.all(g[~pos] >= -1e-5 * np.abs(M).max())
+
+
 def test_rowsimp_prox_is_the_simplex_projection():
     # the Euclidean projection of v onto {x >= 0, sum x = 1} is
     # max(v - theta, 0) with theta the root of sum max(v - theta, 0) = 1;
