@@ -17,6 +17,7 @@
 //     give bit-identical results.
 // Ranks up to 64 take the MFMA kernels; larger ranks take plain kernels.
 #include "spd_dense.hpp"
+#include "spd_batch.hpp"
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -78,10 +79,11 @@ __device__ __forceinline__ double block_partsum(double const * __restrict__ part
 // ---- Gram on MFMA (R <= 64) -------------------------------------------------
 
 template<int TT, bool DIV, bool CROSS>
-__global__ void __launch_bounds__(DN_THREADS)
-k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
+__device__ __forceinline__ void
+b_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
     double const * __restrict__ div, float * __restrict__ X32,
-    double * __restrict__ part, double const * __restrict__ Y)
+    double * __restrict__ part, double const * __restrict__ Y,
+    uint32_t blk, uint32_t nblk)
 {
   constexpr int RP = TT * 16;
   __shared__ double s_acc[RP * RP];
@@ -106,8 +108,8 @@ k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
     dv[t] = (DIV && cok[t]) ? div[col] : 1.;
   }
   uint64_t const nsteps = (I + 3) / 4;
-  uint64_t const stride = (uint64_t) gridDim.x * 4 * GR_U;
-  for(uint64_t s0 = ((uint64_t) blockIdx.x * 4 + wave) * GR_U; s0 < nsteps; s0 += stride) {
+  uint64_t const stride = (uint64_t) nblk * 4 * GR_U;
+  for(uint64_t s0 = ((uint64_t) blk * 4 + wave) * GR_U; s0 < nsteps; s0 += stride) {
     double a[GR_U][TT];
     double b[CROSS ? GR_U : 1][TT];
     #pragma unroll
@@ -167,20 +169,30 @@ k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
     }
     __syncthreads();
   }
-  double * const out = part + (uint64_t) blockIdx.x * RP * RP;
+  double * const out = part + (uint64_t) blk * RP * RP;
   for(int e=threadIdx.x; e < RP * RP; e += DN_THREADS) out[e] = s_acc[e];
+}
+
+template<int TT, bool DIV, bool CROSS>
+__global__ void __launch_bounds__(DN_THREADS)
+k_gram_mfma(double * __restrict__ X, uint64_t I, uint32_t R,
+    double const * __restrict__ div, float * __restrict__ X32,
+    double * __restrict__ part, double const * __restrict__ Y)
+{
+  b_gram_mfma<TT, DIV, CROSS>(X, I, R, div, X32, part, Y, blockIdx.x, gridDim.x);
 }
 
 // Sum of the block partials into the full R x R: the upper triangle
 // mirrored (sym) or every entry as computed (cross products). 32 entries
 // per block.
 #define GR_RED_THREADS 1024
-__global__ void __launch_bounds__(GR_RED_THREADS)
-k_gram_reduce(double const * __restrict__ part, uint32_t nb,
-    uint32_t R, uint32_t ld, double * __restrict__ out, int sym)
+__device__ __forceinline__ void
+b_gram_reduce(double const * __restrict__ part, uint32_t nb,
+    uint32_t R, uint32_t ld, double * __restrict__ out, int sym,
+    uint32_t blk, uint32_t nblk)
 {
   __shared__ double s[GR_RED_THREADS / 32][32];
-  uint32_t const e = blockIdx.x * 32 + threadIdx.x % 32;
+  uint32_t const e = blk * 32 + threadIdx.x % 32;
   bool const valid = e < R * R;
   uint32_t const i = valid ? e / R : 0;
   uint32_t const j = valid ? e - i * R : 0;
@@ -188,6 +200,13 @@ k_gram_reduce(double const * __restrict__ part, uint32_t nb,
   uint32_t const b = (sym && j < i) ? i : j;
   double const tot = block_partsum<32, GR_RED_THREADS>(part, nb, ld * ld, valid, a * ld + b, s);
   if(threadIdx.x < 32 && valid) out[e] = tot;
+}
+
+__global__ void __launch_bounds__(GR_RED_THREADS)
+k_gram_reduce(double const * __restrict__ part, uint32_t nb,
+    uint32_t R, uint32_t ld, double * __restrict__ out, int sym)
+{
+  b_gram_reduce(part, nb, R, ld, out, sym, blockIdx.x, gridDim.x);
 }
 
 // ---- plain kernels for any rank -----------------------------------------------
@@ -213,11 +232,13 @@ k_gram_plain(double const * __restrict__ X, double const * __restrict__ Y,
 }
 
 // X[:,r] /= div[r], FP32 copy.
-__global__ void k_div_cols(double * __restrict__ X, uint64_t n, uint32_t R,
-    double const * __restrict__ div, float * __restrict__ X32)
+__device__ __forceinline__ void
+b_div_cols(double * __restrict__ X, uint64_t n, uint32_t R,
+    double const * __restrict__ div, float * __restrict__ X32,
+    uint32_t blk, uint32_t nblk)
 {
-  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  uint64_t e = (uint64_t) blk * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) nblk * blockDim.x;
   uint32_t c = (uint32_t) (e % R);
   uint32_t const sc = (uint32_t) (stride % R);
   for(; e < n; e += stride) {
@@ -232,16 +253,24 @@ __global__ void k_div_cols(double * __restrict__ X, uint64_t n, uint32_t R,
   }
 }
 
+__global__ void
+k_div_cols(double * __restrict__ X, uint64_t n, uint32_t R,
+    double const * __restrict__ div, float * __restrict__ X32)
+{
+  b_div_cols(X, n, R, div, X32, blockIdx.x, gridDim.x);
+}
+
 // Per block and column: sum over the block's rows of f(X[i,c]) * Y[i,c]
 // (Y == nullptr: f(X)^2, f = max(x,0) when CLAMP). Block partials in
 // part[b * R + c].
 template<typename TY, bool CLAMP>
-__global__ void __launch_bounds__(DN_THREADS)
-k_coldot(double const * __restrict__ X, TY const * __restrict__ Y, uint64_t I,
-    uint32_t R, uint64_t rows_per_block, double * __restrict__ part)
+__device__ __forceinline__ void
+b_coldot(double const * __restrict__ X, TY const * __restrict__ Y, uint64_t I,
+    uint32_t R, uint64_t rows_per_block, double * __restrict__ part,
+    uint32_t blk, uint32_t nblk)
 {
   __shared__ double s[DN_THREADS];
-  uint64_t const r0 = (uint64_t) blockIdx.x * rows_per_block;
+  uint64_t const r0 = (uint64_t) blk * rows_per_block;
   uint64_t const r1 = (r0 + rows_per_block < I) ? r0 + rows_per_block : I;
   int const t = threadIdx.x;
   if(R <= DN_THREADS) {
@@ -262,7 +291,7 @@ k_coldot(double const * __restrict__ X, TY const * __restrict__ Y, uint64_t I,
     if(t < (int) R) {
       double tot = 0.;
       for(uint32_t k=0; k < nq; ++k) tot += s[t + k * R];
-      part[(uint64_t) blockIdx.x * R + t] = tot;
+      part[(uint64_t) blk * R + t] = tot;
     }
   } else {
     for(uint32_t c=t; c < R; c += DN_THREADS) {
@@ -273,16 +302,24 @@ k_coldot(double const * __restrict__ X, TY const * __restrict__ Y, uint64_t I,
         double const y = (Y == nullptr) ? x : (double) Y[i * R + c];
         acc += x * y;
       }
-      part[(uint64_t) blockIdx.x * R + c] = acc;
+      part[(uint64_t) blk * R + c] = acc;
     }
   }
+}
+
+template<typename TY, bool CLAMP>
+__global__ void __launch_bounds__(DN_THREADS)
+k_coldot(double const * __restrict__ X, TY const * __restrict__ Y, uint64_t I,
+    uint32_t R, uint64_t rows_per_block, double * __restrict__ part)
+{
+  b_coldot<TY, CLAMP>(X, Y, I, R, rows_per_block, part, blockIdx.x, gridDim.x);
 }
 
 // Column totals of block partials (row stride ld): mode 0 writes
 // out[c] = sqrt(total_c); mode 1 writes out[0] = sum_c total_c * lambda[c]
 // (columns taken in order). One block.
-__global__ void __launch_bounds__(DN_THREADS)
-k_colreduce(double const * __restrict__ part, uint32_t nb, uint32_t R, uint32_t ld,
+__device__ __forceinline__ void
+b_colreduce(double const * __restrict__ part, uint32_t nb, uint32_t R, uint32_t ld,
     int mode, double const * __restrict__ lambda, double * __restrict__ out)
 {
   __shared__ double s[DN_THREADS / 32][32];
@@ -306,6 +343,13 @@ k_colreduce(double const * __restrict__ part, uint32_t nb, uint32_t R, uint32_t 
       out[0] = tot;
     }
   }
+}
+
+__global__ void __launch_bounds__(DN_THREADS)
+k_colreduce(double const * __restrict__ part, uint32_t nb, uint32_t R, uint32_t ld,
+    int mode, double const * __restrict__ lambda, double * __restrict__ out)
+{
+  b_colreduce(part, nb, R, ld, mode, lambda, out);
 }
 
 // ---- normal equations: form, Cholesky, inverse -----------------------------------
@@ -400,8 +444,8 @@ k_chol_inv(GramSrc g, int nmodes, int mode, uint32_t R, uint32_t RP,
 // than a chain of dependent ones. The result is written from its upper
 // triangle, so it is exactly symmetric.
 template<int RP, int NT>
-__global__ void __launch_bounds__(NT)
-k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
+__device__ __forceinline__ void
+b_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
     double * __restrict__ ginv, double * __restrict__ rho_out, int * __restrict__ info)
 {
   __shared__ double A[RP][RP + 1];
@@ -533,6 +577,14 @@ k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
   }
 }
 
+template<int RP, int NT>
+__global__ void __launch_bounds__(NT)
+k_gj_inv(GramSrc g, int nmodes, int mode, uint32_t R, double diag_add, int admm,
+    double * __restrict__ ginv, double * __restrict__ rho_out, int * __restrict__ info)
+{
+  b_gj_inv<RP, NT>(g, nmodes, mode, R, diag_add, admm, ginv, rho_out, info);
+}
+
 // ---- apply the inverse on MFMA (R <= 64) ---------------------------------------
 
 template<typename TM>
@@ -551,8 +603,9 @@ struct ApplyArgs {
 };
 
 template<int TT, typename TM, int KIND>
-__global__ void __launch_bounds__(DN_THREADS)
-k_apply_mfma(ApplyArgs<TM> a)
+__device__ __forceinline__ void
+b_apply_mfma(ApplyArgs<TM> a,
+    uint32_t blk, uint32_t nblk)
 {
   constexpr int RP = TT * 16;
   constexpr int LD = RP + 1;
@@ -577,7 +630,7 @@ k_apply_mfma(ApplyArgs<TM> a)
   int const lr = lane / RP;
   int const lc = lane - lr * RP;
   uint64_t const ngroups = (I + 15) / 16;
-  for(uint64_t gi = (uint64_t) blockIdx.x * 4 + wave; gi < ngroups; gi += (uint64_t) gridDim.x * 4) {
+  for(uint64_t gi = (uint64_t) blk * 4 + wave; gi < ngroups; gi += (uint64_t) nblk * 4) {
     uint64_t const row0 = gi * 16;
     // stage 16 rows of the right-hand side (coalesced: the rows are contiguous)
     #pragma unroll
@@ -652,9 +705,16 @@ k_apply_mfma(ApplyArgs<TM> a)
       __syncthreads();
     }
     for(int e=threadIdx.x; e < RP; e += DN_THREADS) {
-      a.colsq[(uint64_t) blockIdx.x * RP + e] = s_cs[e];
+      a.colsq[(uint64_t) blk * RP + e] = s_cs[e];
     }
   }
+}
+
+template<int TT, typename TM, int KIND>
+__global__ void __launch_bounds__(DN_THREADS)
+k_apply_mfma(ApplyArgs<TM> a)
+{
+  b_apply_mfma<TT, TM, KIND>(a, blockIdx.x, gridDim.x);
 }
 
 // ---- ADMM with the iterate held as the pre-prox matrix (R <= 64) -----------
@@ -709,8 +769,9 @@ __device__ __forceinline__ double dn_prox_apply(double p, int prox, double nc)
 // P'_{j-1}, test against P'_{j-2}). Block partials: column sums of squares
 // of P'_j and the four test terms.
 template<int TT, typename TM, int MODE>
-__global__ void __launch_bounds__(DN_THREADS)
-k_admm_pass(Admm2Args<TM> a)
+__device__ __forceinline__ void
+b_admm_pass(Admm2Args<TM> a,
+    uint32_t blk, uint32_t nblk)
 {
   if(a.ctl[0] != 0) return;
   constexpr int RP = TT * 16;
@@ -746,7 +807,7 @@ k_admm_pass(Admm2Args<TM> a)
   double const ncur = sk ? thr : ((MODE >= 1 && lok) ? a.cur_n[lc] : 1.);
   double const nprev = sk ? thr : ((MODE == 2 && lok) ? a.prev_n[lc] : 1.);
   uint64_t const ngroups = (I + 15) / 16;
-  for(uint64_t gi = (uint64_t) blockIdx.x * 4 + wave; gi < ngroups; gi += (uint64_t) gridDim.x * 4) {
+  for(uint64_t gi = (uint64_t) blk * 4 + wave; gi < ngroups; gi += (uint64_t) nblk * 4) {
     uint64_t const row0 = gi * 16;
     // stage M + rho (P + D) of the state j-1; D kept for the output; the
     // test terms of iteration j-1. Every load of the group is issued first.
@@ -845,8 +906,8 @@ k_admm_pass(Admm2Args<TM> a)
   // block partials in a fixed order: RP column sums, then the four terms
   __syncthreads();
   // entry-major: part[e * gridDim.x + block], so the reduction reads rows
-  double * const part = a.part + blockIdx.x;
-  uint64_t const pstride = gridDim.x;
+  double * const part = a.part + blk;
+  uint64_t const pstride = nblk;
   #pragma unroll
   for(int t=0; t < TT; ++t) {
     cs[t] += __shfl_xor(cs[t], 16, 64);
@@ -877,6 +938,13 @@ k_admm_pass(Admm2Args<TM> a)
   if(threadIdx.x < 4) part[(uint64_t) (RP + threadIdx.x) * pstride] = s_w[threadIdx.x * DN_THREADS];
 }
 
+template<int TT, typename TM, int MODE>
+__global__ void __launch_bounds__(DN_THREADS)
+k_admm_pass(Admm2Args<TM> a)
+{
+  b_admm_pass<TT, TM, MODE>(a, blockIdx.x, gridDim.x);
+}
+
 // After pass j (one block of RD_THREADS): the block partials (entry-major)
 // are added by wave w for entries w, w + 16, ...: lane l sums blocks l,
 // l + 64, ... in order, then a butterfly. Norms of P'_j for the next prox,
@@ -886,8 +954,8 @@ k_admm_pass(Admm2Args<TM> a)
 #define RD_THREADS 1024
 #define ADMM_MAX_BLOCKS 1536       // blocks of an ADMM pass, at most (a multiple of 64)
 template<int E>
-__global__ void __launch_bounds__(RD_THREADS)
-k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
+__device__ __forceinline__ void
+b_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
     double * __restrict__ norms, double * __restrict__ sums, int * __restrict__ ctl,
     double tol, int check, int j)
 {
@@ -948,19 +1016,30 @@ k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
   }
 }
 
+template<int E>
+__global__ void __launch_bounds__(RD_THREADS)
+k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
+    double * __restrict__ norms, double * __restrict__ sums, int * __restrict__ ctl,
+    double tol, int check, int j)
+{
+  b_admm_pass_reduce<E>(part, nb, R, norms, sums, ctl, tol, check, j);
+}
+
 struct Bufs3 { double * b[3]; double * n[3]; };
 
 // P = prox(P'_k), D = P - P'_k for the final state k = ctl[1] (k >= 1).
-__global__ void k_admm_final(Bufs3 bf, double * __restrict__ P, double * __restrict__ D,
+__device__ __forceinline__ void
+b_admm_final(Bufs3 bf, double * __restrict__ P, double * __restrict__ D,
     uint64_t n, uint32_t R, int prox, double lam, double const * __restrict__ rho,
-    int const * __restrict__ ctl)
+    int const * __restrict__ ctl,
+    uint32_t blk, uint32_t nblk)
 {
   int const k = ctl[1];
   double const thr = dn_prox_scalar(prox, lam, *rho);
   double const * const src = bf.b[k % 3];
   double const * const nrm = bf.n[k % 3];
-  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  uint64_t e = (uint64_t) blk * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) nblk * blockDim.x;
   uint32_t c = (uint32_t) (e % R);
   uint32_t const sc = (uint32_t) (stride % R);
   for(; e < n; e += stride) {
@@ -971,6 +1050,14 @@ __global__ void k_admm_final(Bufs3 bf, double * __restrict__ P, double * __restr
     c += sc;
     if(c >= R) c -= R;
   }
+}
+
+__global__ void
+k_admm_final(Bufs3 bf, double * __restrict__ P, double * __restrict__ D,
+    uint64_t n, uint32_t R, int prox, double lam, double const * __restrict__ rho,
+    int const * __restrict__ ctl)
+{
+  b_admm_final(bf, P, D, n, R, prox, lam, rho, ctl, blockIdx.x, gridDim.x);
 }
 
 __global__ void k_ctl_reset(int * ctl)
@@ -1253,7 +1340,8 @@ k_sum4(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
   if(threadIdx.x < 4) out[threadIdx.x] = v;
 }
 
-__global__ void k_cpd_norm(GramSrc g, int nmodes, uint32_t R,
+__device__ __forceinline__ void
+b_cpd_norm(GramSrc g, int nmodes, uint32_t R,
     double const * __restrict__ lambda, double * __restrict__ out)
 {
   __shared__ double s[64];
@@ -1275,6 +1363,13 @@ __global__ void k_cpd_norm(GramSrc g, int nmodes, uint32_t R,
     for(int k=0; k < 64; ++k) tot += s[k];
     out[0] = fabs(tot);
   }
+}
+
+__global__ void
+k_cpd_norm(GramSrc g, int nmodes, uint32_t R,
+    double const * __restrict__ lambda, double * __restrict__ out)
+{
+  b_cpd_norm(g, nmodes, R, lambda, out);
 }
 
 // ---- host wrappers -----------------------------------------------------------------
@@ -1976,4 +2071,489 @@ extern "C" int spd_dense_admm_step(void const * M, int mdtype, double const * gi
   SPD_TRY(spd_init());
   return dn_admm_step(M, mdtype, ginv, rho, I, (uint32_t) R, P, aux, D, init, prox, 0.,
       sums, spd_hs(nullptr));
+}
+
+// ---- batched launches: one kernel covers every slice of a batch ----------
+//
+// Each kernel below runs the device body of its per-slice counterpart for
+// work item (slice s, block b), with b < nb(s) computed from the slice's
+// sizes by the same formula the per-slice launcher uses, so every slice
+// gets exactly its per-slice arithmetic (spd_batch.hpp). Items of slices
+// that a gate excludes (converged, other constraint kind, ...) are not
+// counted.
+
+#define BG_ALL 0          // every slice
+#define BG_LIVE 1         // slices whose outer loop runs
+#define BG_FIRST 2        // first slice of its stream
+#define BG_ADMM 3         // live, mode solved by ADMM, ADMM not converged
+#define BG_CLOSED 4       // live, mode solved in closed form
+#define BG_SMALL 5        // slices whose local error uses this mode
+
+__host__ __device__ __forceinline__ uint32_t bn_clamp(uint64_t nb, uint64_t hi)
+{
+  if(nb < 1) nb = 1;
+  if(nb > hi) nb = hi;
+  return (uint32_t) nb;
+}
+// the block counts of the per-slice launchers
+__host__ __device__ __forceinline__ uint32_t bn_gram(uint64_t I)
+{
+  return bn_clamp(((I + 3) / 4 + 4 * GR_U * 4 - 1) / (4 * GR_U * 4), DN_MAX_BLOCKS);
+}
+__host__ __device__ __forceinline__ uint32_t bn_rows64(uint64_t I)
+{
+  return bn_clamp((I + 63) / 64, DN_MAX_BLOCKS);
+}
+__host__ __device__ __forceinline__ uint32_t bn_elem(uint64_t n)
+{
+  return bn_clamp((n + DN_THREADS * 8 - 1) / (DN_THREADS * 8), DN_MAX_BLOCKS);
+}
+__host__ __device__ __forceinline__ uint32_t bn_apply(uint64_t I)
+{
+  return bn_clamp(((I + 15) / 16 + 15) / 16, DN_MAX_BLOCKS);
+}
+__host__ __device__ __forceinline__ uint32_t bn_admm(uint64_t I)
+{
+  return bn_clamp((I + 63) / 64, ADMM_MAX_BLOCKS);
+}
+
+__device__ __forceinline__ bool bt_gate(BSlot const & s, int gate, int mode)
+{
+  switch(gate) {
+  case BG_LIVE: return s.c->done == 0;
+  case BG_FIRST: return s.first != 0;
+  case BG_ADMM: return s.c->done == 0 && s.kind[mode] == SPD_CON_ADMM && s.c->admm[0] == 0;
+  case BG_CLOSED: return s.c->done == 0 && s.kind[mode] != SPD_CON_ADMM;
+  case BG_SMALL: return s.small == mode;
+  default: return true;
+  }
+}
+
+// Gram operands: what 0: Unew[mode] -> aTa[mode]; 1: Uold[mode]^T Unew[mode]
+// -> cross + k R^2; 2: the time row (one row) -> tgram.
+__device__ __forceinline__ double * bt_gram_x(BSlot const & s, int what, int mode)
+{
+  return (what == 0) ? s.Unew[mode] : ((what == 1) ? s.Uold[mode] : s.time_row);
+}
+__device__ __forceinline__ uint64_t bt_gram_rows(BSlot const & s, int what, int mode)
+{
+  return (what == 2) ? 1 : s.rows[mode];
+}
+
+template<int TT, bool CROSS>
+__global__ void __launch_bounds__(DN_THREADS)
+kb_gram(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode, int what,
+    int gate)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(bt_gate(s, gate, mode)) nb = bn_gram(bt_gram_rows(s, what, mode));
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    b_gram_mfma<TT, false, CROSS>(bt_gram_x(s, what, mode), bt_gram_rows(s, what, mode), s.R,
+        nullptr, nullptr, s.part, CROSS ? s.Unew[mode] : nullptr, w - s_it.pref[q],
+        s_it.pref[q + 1] - s_it.pref[q]);
+    __syncthreads();
+  }
+}
+
+// the reduction: out = aTa[mode] (what 0), cross + k R^2 (1), tgram (2)
+__global__ void __launch_bounds__(GR_RED_THREADS)
+kb_gram_reduce(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode,
+    int what, int k, int gate)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(bt_gate(s, gate, mode)) nb = (s.R * s.R + 31) / 32;
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    uint32_t const RP = dn_rpad(s.R);
+    double * const out = (what == 0) ? s.aTa[mode] :
+        ((what == 1) ? s.cross + (uint64_t) k * s.R * s.R : s.tgram);
+    b_gram_reduce(s.part, bn_gram(bt_gram_rows(s, what, mode)), s.R, RP, out,
+        what == 1 ? 0 : 1, w - s_it.pref[q], 0);
+    __syncthreads();
+  }
+}
+
+// column sums of squares (colnorm) or of X o M (innerprod) of X = what 0:
+// Unew[mode]; 1: the local-error factor of mode `mode` against Mbuf
+__global__ void __launch_bounds__(DN_THREADS)
+kb_coldot(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode, int what,
+    int gate)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  uint32_t const t = (uint32_t) hd->t;
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(bt_gate(s, gate, mode)) nb = bn_rows64(s.rows[mode]);
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    uint64_t const I = s.rows[mode];
+    uint32_t const nb = s_it.pref[q + 1] - s_it.pref[q];
+    uint64_t const rpb = spd_cdiv(I > 0 ? I : 1, nb);
+    double const * X = ((uint32_t) mode == t) ? s.time_row : s.Unew[mode];
+    b_coldot<double, false>(X, what == 0 ? (double const *) nullptr : (double const *) s.Mbuf,
+        I, s.R, rpb, s.part, w - s_it.pref[q], nb);
+    __syncthreads();
+  }
+}
+
+// per slice: colnorm -> tmpv (what 0) or the inner product with lambda =
+// ones -> c->inner[0] (what 1)
+__global__ void __launch_bounds__(DN_THREADS)
+kb_colreduce(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode, int what,
+    int gate)
+{
+  uint32_t const q = blockIdx.x;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  if(!bt_gate(s, gate, mode)) return;
+  b_colreduce(s.part, bn_rows64(s.rows[mode]), s.R, s.R, what,
+      what == 1 ? s.lam1 : (double const *) nullptr, what == 1 ? &s.c->inner[0] : s.tmpv);
+}
+
+__global__ void __launch_bounds__(DN_THREADS)
+kb_div_cols(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode, int gate)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(bt_gate(s, gate, mode)) nb = bn_elem(s.rows[mode] * s.R);
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    b_div_cols(s.Unew[mode], s.rows[mode] * s.R, s.R, s.tmpv, nullptr, w - s_it.pref[q],
+        s_it.pref[q + 1] - s_it.pref[q]);
+    __syncthreads();
+  }
+}
+
+// normal-equation inverse of `mode`, one block per slice: ADMM modes add
+// rho (to c->rho), the streamed mode never
+template<int RP, int NT>
+__global__ void __launch_bounds__(NT)
+kb_gj_inv(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode, int gate)
+{
+  uint32_t const q = blockIdx.x;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  if(!bt_gate(s, gate, mode)) return;
+  GramSrc g = {};
+  for(int m=0; m < hd->N; ++m) g.aTa[m] = s.aTa[m];
+  int const admm = (mode != hd->t && s.kind[mode] == SPD_CON_ADMM) ? 1 : 0;
+  b_gj_inv<RP, NT>(g, hd->N, mode, s.R, s.diag_add[mode], admm, s.ginv, &s.c->rho,
+      &s.c->info);
+}
+
+// what 0: time row = Mbuf[0] ginv; 1: Unew[mode] = Mbuf ginv (closed form);
+// 2: Mbuf += Uold[mode] H (history)
+template<int TT, int KIND>
+__global__ void __launch_bounds__(DN_THREADS)
+kb_apply(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode, int what,
+    int gate)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(bt_gate(s, gate, mode)) nb = bn_apply(what == 0 ? 1 : s.rows[mode]);
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    ApplyArgs<double> a = {};
+    a.R = s.R;
+    if(what == 0) {
+      a.M = s.Mbuf; a.ginv = s.ginv; a.I = 1; a.X = s.time_row;
+    } else if(what == 1) {
+      a.M = s.Mbuf; a.ginv = s.ginv; a.I = s.rows[mode]; a.X = s.Unew[mode];
+    } else {
+      a.M = s.Uold[mode]; a.ginv = s.H; a.I = s.rows[mode]; a.X = s.Mbuf;
+    }
+    a.colsq = nullptr;
+    b_apply_mfma<TT, double, KIND>(a, w - s_it.pref[q], s_it.pref[q + 1] - s_it.pref[q]);
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ Bufs3 bt_bufs(BSlot const & s, int mode)
+{
+  Bufs3 bf;
+  bf.b[0] = s.Unew[mode];
+  bf.b[1] = s.aux;
+  bf.b[2] = s.init;
+  for(int k=0; k < 3; ++k) bf.n[k] = s.c->nrm[k];
+  return bf;
+}
+
+// ADMM pass j of `mode` (the iterate in the three rotating buffers); the
+// dual starts at zero (the stream clears it before each mode's ADMM)
+template<int TT, int MODE>
+__global__ void __launch_bounds__(DN_THREADS)
+kb_admm_pass(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode, int j)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(bt_gate(s, BG_ADMM, mode) && (uint32_t) j <= s.max_inner) nb = bn_admm(s.rows[mode]);
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    Bufs3 const bf = bt_bufs(s, mode);
+    Admm2Args<double> a = {};
+    a.M = s.Mbuf;
+    a.ginv = s.ginv;
+    a.I = s.rows[mode];
+    a.R = s.R;
+    a.rho = &s.c->rho;
+    a.prox = s.prox[mode];
+    a.lam = s.lam[mode];
+    a.P0 = s.Unew[mode];
+    a.D0 = s.Dual[mode];
+    a.part = s.part;
+    a.ctl = s.c->admm;
+    a.cur = (j >= 2) ? bf.b[(j - 1) % 3] : nullptr;
+    a.cur_n = (j >= 2) ? bf.n[(j - 1) % 3] : nullptr;
+    a.prev = (j >= 3) ? bf.b[(j - 2) % 3] : nullptr;
+    a.prev_n = (j >= 3) ? bf.n[(j - 2) % 3] : nullptr;
+    a.out = bf.b[j % 3];
+    b_admm_pass<TT, double, MODE>(a, w - s_it.pref[q], s_it.pref[q + 1] - s_it.pref[q]);
+    __syncthreads();
+  }
+}
+
+template<int E>
+__global__ void __launch_bounds__(RD_THREADS)
+kb_admm_reduce(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode, int j)
+{
+  uint32_t const q = blockIdx.x;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  if(!bt_gate(s, BG_ADMM, mode) || (uint32_t) j > s.max_inner) return;
+  b_admm_pass_reduce<E>(s.part, bn_admm(s.rows[mode]), s.R, s.c->nrm[j % 3], s.c->sums,
+      s.c->admm, s.inner_tol, j >= 2 ? 1 : 0, j);
+}
+
+__global__ void __launch_bounds__(DN_THREADS)
+kb_admm_final(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(s.c->done == 0 && s.kind[mode] == SPD_CON_ADMM) nb = bn_elem(s.rows[mode] * s.R);
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    b_admm_final(bt_bufs(s, mode), s.Unew[mode], s.Dual[mode], s.rows[mode] * s.R, s.R,
+        s.prox[mode], s.lam[mode], &s.c->rho, s.c->admm, w - s_it.pref[q],
+        s_it.pref[q + 1] - s_it.pref[q]);
+    __syncthreads();
+  }
+}
+
+// ||Z||^2 of the slice's model (time Gram of its one row) -> c->inner[1]
+__global__ void __launch_bounds__(64)
+kb_cpd_norm(BSlot const * __restrict__ S, BHead const * __restrict__ hd)
+{
+  uint32_t const q = blockIdx.x;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  GramSrc g = {};
+  for(int m=0; m < hd->N; ++m) g.aTa[m] = (m == hd->t) ? s.tgram : s.aTa[m];
+  b_cpd_norm(g, hd->N, s.R, s.lam1, &s.c->inner[1]);
+}
+
+namespace {
+uint32_t bt_grid(BSlot const * hS, uint32_t K, uint32_t (*nbf)(uint64_t), int mode, int what)
+{
+  uint64_t tot = 0;
+  for(uint32_t q=0; q < K; ++q) {
+    uint64_t const I = (what == 2) ? 1 : hS[q].rows[mode];
+    tot += nbf(I);
+  }
+  if(tot < 1) tot = 1;
+  return (uint32_t) (tot < BT_GRID ? tot : BT_GRID);
+}
+uint32_t bn_gram_h(uint64_t I) { return bn_gram(I); }
+uint32_t bn_rows64_h(uint64_t I) { return bn_rows64(I); }
+uint32_t bn_apply_h(uint64_t I) { return bn_apply(I); }
+uint32_t bn_admm_h(uint64_t I) { return bn_admm(I); }
+}
+
+int dnb_gram(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K, uint32_t TT,
+    int mode, int what, int k, int gate, hipStream_t st)
+{
+  uint32_t const grid = bt_grid(hS, K, bn_gram_h, mode, what);
+  bool const cross = (what == 1);
+  switch(TT * 2 + (cross ? 1 : 0)) {
+  case 2: hipLaunchKernelGGL((kb_gram<1, false>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  case 3: hipLaunchKernelGGL((kb_gram<1, true>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  case 4: hipLaunchKernelGGL((kb_gram<2, false>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  case 5: hipLaunchKernelGGL((kb_gram<2, true>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  case 6: hipLaunchKernelGGL((kb_gram<3, false>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  case 7: hipLaunchKernelGGL((kb_gram<3, true>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  case 8: hipLaunchKernelGGL((kb_gram<4, false>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  default: hipLaunchKernelGGL((kb_gram<4, true>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  }
+  SPD_LAUNCH_CHECK();
+  uint32_t const rg = K * ((TT * 16 * TT * 16 + 31) / 32);
+  hipLaunchKernelGGL(kb_gram_reduce, dim3(rg < BT_GRID ? rg : BT_GRID), dim3(GR_RED_THREADS), 0,
+      st, dS, dH, mode, what, k, gate);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int dnb_colnorm_div(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K, int mode,
+    hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_coldot, dim3(bt_grid(hS, K, bn_rows64_h, mode, 0)), dim3(DN_THREADS), 0,
+      st, dS, dH, mode, 0, BG_FIRST);
+  SPD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(kb_colreduce, dim3(K), dim3(DN_THREADS), 0, st, dS, dH, mode, 0, BG_FIRST);
+  SPD_LAUNCH_CHECK();
+  uint64_t tot = 0;
+  for(uint32_t q=0; q < K; ++q) tot += bn_elem(hS[q].rows[mode] * hS[q].R);
+  uint32_t const grid = (uint32_t) (tot < BT_GRID ? (tot ? tot : 1) : BT_GRID);
+  hipLaunchKernelGGL(kb_div_cols, dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, BG_FIRST);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int dnb_innerprod(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K, int mode,
+    hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_coldot, dim3(bt_grid(hS, K, bn_rows64_h, mode, 0)), dim3(DN_THREADS), 0,
+      st, dS, dH, mode, 1, BG_SMALL);
+  SPD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(kb_colreduce, dim3(K), dim3(DN_THREADS), 0, st, dS, dH, mode, 1, BG_SMALL);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int dnb_gj_inv(uint32_t K, BSlot const * dS, BHead const * dH, uint32_t TT, int mode, int gate,
+    hipStream_t st)
+{
+  switch(TT) {
+  case 1: hipLaunchKernelGGL((kb_gj_inv<16, 64>), dim3(K), dim3(64), 0, st, dS, dH, mode, gate); break;
+  case 2: hipLaunchKernelGGL((kb_gj_inv<32, 128>), dim3(K), dim3(128), 0, st, dS, dH, mode, gate); break;
+  case 3: hipLaunchKernelGGL((kb_gj_inv<48, 128>), dim3(K), dim3(128), 0, st, dS, dH, mode, gate); break;
+  default: hipLaunchKernelGGL((kb_gj_inv<64, 256>), dim3(K), dim3(256), 0, st, dS, dH, mode, gate); break;
+  }
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+template<int KIND>
+static int dnb_apply_k(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K,
+    uint32_t TT, int mode, int what, int gate, hipStream_t st)
+{
+  uint32_t const grid = bt_grid(hS, K, bn_apply_h, mode, what == 0 ? 2 : 0);
+  switch(TT) {
+  case 1: hipLaunchKernelGGL((kb_apply<1, KIND>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  case 2: hipLaunchKernelGGL((kb_apply<2, KIND>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  case 3: hipLaunchKernelGGL((kb_apply<3, KIND>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  default: hipLaunchKernelGGL((kb_apply<4, KIND>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, what, gate); break;
+  }
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int dnb_apply(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K, uint32_t TT,
+    int mode, int what, int gate, hipStream_t st)
+{
+  if(what == 2) return dnb_apply_k<2>(hS, dS, dH, K, TT, mode, what, gate, st);
+  return dnb_apply_k<0>(hS, dS, dH, K, TT, mode, what, gate, st);
+}
+
+template<int TT>
+static int dnb_admm_t(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K,
+    int mode, uint32_t max_inner, hipStream_t st)
+{
+  constexpr int E = TT * 16 + 4;
+  uint32_t const grid = bt_grid(hS, K, bn_admm_h, mode, 0);
+  for(uint32_t j=1; j <= max_inner; ++j) {
+    if(j == 1) {
+      hipLaunchKernelGGL((kb_admm_pass<TT, 0>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, (int) j);
+    } else if(j == 2) {
+      hipLaunchKernelGGL((kb_admm_pass<TT, 1>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, (int) j);
+    } else {
+      hipLaunchKernelGGL((kb_admm_pass<TT, 2>), dim3(grid), dim3(DN_THREADS), 0, st, dS, dH, mode, (int) j);
+    }
+    SPD_LAUNCH_CHECK();
+    hipLaunchKernelGGL((kb_admm_reduce<E>), dim3(K), dim3(RD_THREADS), 0, st, dS, dH, mode, (int) j);
+    SPD_LAUNCH_CHECK();
+  }
+  uint64_t tot = 0;
+  for(uint32_t q=0; q < K; ++q) tot += bn_elem(hS[q].rows[mode] * hS[q].R);
+  uint32_t const g2 = (uint32_t) (tot < BT_GRID ? (tot ? tot : 1) : BT_GRID);
+  hipLaunchKernelGGL(kb_admm_final, dim3(g2), dim3(DN_THREADS), 0, st, dS, dH, mode);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+// the ADMM inner loop of `mode` for every live slice whose mode is solved
+// by ADMM (its ctl was reset and its inverse with rho formed before)
+int dnb_admm(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K, uint32_t TT,
+    int mode, uint32_t max_inner, hipStream_t st)
+{
+  if(max_inner == 0) return SPD_OK;
+  switch(TT) {
+  case 1: return dnb_admm_t<1>(hS, dS, dH, K, mode, max_inner, st);
+  case 2: return dnb_admm_t<2>(hS, dS, dH, K, mode, max_inner, st);
+  case 3: return dnb_admm_t<3>(hS, dS, dH, K, mode, max_inner, st);
+  default: return dnb_admm_t<4>(hS, dS, dH, K, mode, max_inner, st);
+  }
+}
+
+int dnb_cpd_norm(uint32_t K, BSlot const * dS, BHead const * dH, hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_cpd_norm, dim3(K), dim3(64), 0, st, dS, dH);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
 }

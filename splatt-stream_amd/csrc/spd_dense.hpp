@@ -63,7 +63,7 @@ int dn_chol_inv(double const * const * aTa, int nmodes, int mode, uint32_t R,
     hipStream_t st);
 
 // Row-padded size used for ginv.
-static inline uint32_t dn_rpad(uint32_t R) { return (R + 15) / 16 * 16; }
+__host__ __device__ static inline uint32_t dn_rpad(uint32_t R) { return (R + 15) / 16 * 16; }
 
 // X = M ginv (rows of M times the inverse, i.e. the Cholesky solve of the
 // reference's potrs). M is FP64 (mdtype SPD_F64) or FP32. When colnorm is

@@ -42,7 +42,7 @@ static inline hipStream_t spd_hs(void * s)
   return s ? (hipStream_t) s : (hipStream_t) spd_stream();
 }
 
-static inline uint64_t spd_cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+__host__ __device__ static inline uint64_t spd_cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Exclusive prefix sum of n uint32 values (in place allowed: in == out).
 // Returns the total in *total (host) if total != NULL (synchronises).

@@ -13,6 +13,8 @@
 // applied slice by slice) runs in the worker before its device part.
 #include "spd_internal.hpp"
 #include "spd_stream_state.hpp"
+#include "spd_batch.hpp"
+#include "spd_batch_host.hpp"
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
@@ -69,6 +71,34 @@ struct Stream {
   std::vector<double> kept[SPD_MAX_MODES];
   uint64_t kept_rows[SPD_MAX_MODES] = {};
   bool has_kept = false;
+  // batched mode: slices prepared on the host ahead of the device step
+  bool batchable = false;
+  bool prepping = false;
+  std::deque<struct PSlice *> prepared;
+  uint64_t prep_rows[SPD_MAX_MODES] = {};   // active rows after the prepared slices
+  uint64_t prep_count = 0;
+  uint8_t * d_slice = nullptr;              // the slice's device arrays
+  size_t d_slice_cap = 0;
+  double * d_part = nullptr;                // partial sums of the batched kernels
+};
+
+// A slice prepared for the batched step: ordered, relabelled, its copies
+// ordered by each other mode, the new rows' values, all in one pinned buffer
+// laid out as it is copied to the device.
+struct PSlice {
+  Slice * sl = nullptr;
+  uint64_t nnz = 0;
+  uint64_t rows[SPD_MAX_MODES] = {};
+  uint64_t grow0[SPD_MAX_MODES] = {};
+  int first = 0;
+  int small = 0;
+  double xnorm = 0.;
+  uint8_t * buf = nullptr;                  // pinned
+  size_t cap = 0;
+  size_t bytes = 0;
+  size_t off_ind[SPD_MAX_MODES][SPD_MAX_MODES] = {};
+  size_t off_vals[SPD_MAX_MODES] = {};
+  size_t off_grow[SPD_MAX_MODES] = {};
 };
 
 // n factor values from a glibc generator state (src/util.c:15-40: the
@@ -150,6 +180,13 @@ struct spd_server {
   int64_t next_stream = 0;
   uint64_t pending = 0;
   bool stop = false;
+  // batched mode (max_inflight > 1): host preparation threads and engines
+  bool batched = false;
+  std::vector<std::thread> preppers;
+  std::vector<std::thread> engines;
+  std::vector<BtEngine *> eng;
+  std::vector<PSlice *> pool;       // prepared-slice buffers to reuse
+  std::condition_variable cv_prep;
 };
 
 namespace {
@@ -163,7 +200,11 @@ void admit_waiting(spd_server * sv)
     sv->waiting.pop_front();
     s->admitted = true;
     sv->used += s->footprint;
-    if(!s->queue.empty() && !s->busy) sv->ready.push_back(s->id);
+    if(sv->batched) {
+      sv->cv_work.notify_all();
+    } else if(!s->queue.empty() && !s->busy) {
+      sv->ready.push_back(s->id);
+    }
   }
 }
 
@@ -199,6 +240,11 @@ void release_device(Stream * s)
   (void) spd_free_on(s->d_vals, s->hs);
   s->d_vals = nullptr;
   s->slice_cap = 0;
+  (void) spd_free_on(s->d_slice, s->hs);
+  s->d_slice = nullptr;
+  s->d_slice_cap = 0;
+  (void) spd_free_on(s->d_part, s->hs);
+  s->d_part = nullptr;
   s->allocated = false;
 }
 
@@ -435,6 +481,432 @@ void worker_main(spd_server * sv, int wi)
   }
 }
 
+// ---- batched mode -----------------------------------------------------------
+
+// Streams whose every mode the batched step handles: rank <= 64, ADMM with
+// an entry-wise or column-norm prox, or closed forms.
+bool stream_batchable(Stream const * s)
+{
+  if(s->cfg.rank > 64) return false;
+  for(int m=0; m < s->N; ++m) {
+    spd_con const & c = s->cfg.con[m];
+    if(c.kind != SPD_CON_ADMM) continue;
+    if(m == s->cfg.stream_mode) return false;
+    switch(c.prox) {
+    case SPD_PROX_MAXCOLNORM: case SPD_PROX_MAXCOLNORM_NONNEG: case SPD_PROX_NONNEG:
+    case SPD_PROX_LASSO: case SPD_PROX_NTF_FROB: case SPD_PROX_NTF_LASSO:
+      break;
+    default:
+      return false;
+    }
+  }
+  return true;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t) 255; }
+
+// Host part of a slice for the batched step (the per-slice path does the
+// same in run_slice_body and StreamState::slice): the parser's order and
+// first-appearance labels, the running dims, the new rows' rand() values
+// (drawn in mode order, as grow_mats does), a copy of the nonzeros ordered
+// by each mode other than the streamed one and the one after it (a stable
+// counting sort by label, the order the device radix sort gives), and the
+// slice's squared norm.
+int prepare_slice(Stream * s, Slice * sl, PSlice * ps)
+{
+  int const N = s->N;
+  int const t = s->cfg.stream_mode;
+  int const lead = (t + 1) % N;
+  uint32_t const R = (uint32_t) s->cfg.rank;
+  uint64_t const n = sl->nnz;
+  for(int m=0; m < N; ++m) {
+    if(m == t) continue;
+    for(uint64_t x=0; x < n; ++x) {
+      uint64_t const o = sl->ind[m][x];
+      if(o >= s->cap[m]) {
+        spd_set_error(SPD_EINPUT, "serve: index %llu of mode %d beyond its capacity %llu",
+            (unsigned long long) o, m, (unsigned long long) s->cap[m]);
+        return SPD_EINPUT;
+      }
+    }
+  }
+  std::vector<uint32_t> order;
+  slice_order(sl, N, t, s->cap, order);
+  // layout: copy lead, the other copies, then the new rows
+  size_t off = 0;
+  int copies[SPD_MAX_MODES];
+  int nc = 0;
+  copies[nc++] = lead;
+  for(int c=0; c < N; ++c) if(c != t && c != lead) copies[nc++] = c;
+  for(int q=0; q < nc; ++q) {
+    int const c = copies[q];
+    for(int m=0; m < N; ++m) {
+      if(m == t) continue;
+      ps->off_ind[c][m] = off;
+      off = align256(off + n * sizeof(uint32_t));
+    }
+    ps->off_vals[c] = off;
+    off = align256(off + n * sizeof(double));
+  }
+  // labels and running dims
+  uint64_t bmax[SPD_MAX_MODES] = {};
+  std::vector<uint32_t> lab_tmp[SPD_MAX_MODES];
+  for(int m=0; m < N; ++m) {
+    if(m == t) continue;
+    lab_tmp[m].resize(n);
+    uint32_t * const labs = s->lab[m].data();
+    for(uint64_t x=0; x < n; ++x) {
+      uint64_t const o = sl->ind[m][order[x]];
+      if(labs[o] == UINT32_MAX) labs[o] = (uint32_t) s->next_label[m]++;
+      uint32_t const l = labs[o];
+      lab_tmp[m][x] = l;
+      if(l > bmax[m]) bmax[m] = l;
+    }
+  }
+  uint64_t bd[SPD_MAX_MODES];
+  for(int m=0; m < N; ++m) {
+    if(m == t) { bd[m] = 1; continue; }
+    bd[m] = (bmax[m] + 1 > s->prep_rows[m]) ? bmax[m] + 1 : s->prep_rows[m];
+    ps->grow0[m] = s->prep_rows[m];
+    ps->rows[m] = bd[m];
+    if(bd[m] > ps->grow0[m]) {
+      ps->off_grow[m] = off;
+      off = align256(off + (bd[m] - ps->grow0[m]) * R * sizeof(double));
+    }
+  }
+  ps->rows[t] = 1;
+  ps->bytes = off;
+  if(off > ps->cap) {
+    if(ps->buf) (void) hipHostFree(ps->buf);
+    ps->buf = nullptr;
+    ps->cap = 0;
+    size_t nc2 = 1 << 20;
+    while(nc2 < off) nc2 *= 2;
+    SPD_HIP(hipHostMalloc((void **) &ps->buf, nc2, hipHostMallocDefault));
+    ps->cap = nc2;
+  }
+  uint8_t * const B = ps->buf;
+  // copy lead: the slice's own order
+  double xn = 0.;
+  {
+    double * const v = (double *) (B + ps->off_vals[lead]);
+    for(uint64_t x=0; x < n; ++x) {
+      v[x] = sl->vals[order[x]];
+      xn += v[x] * v[x];
+    }
+    for(int m=0; m < N; ++m) {
+      if(m == t) continue;
+      memcpy(B + ps->off_ind[lead][m], lab_tmp[m].data(), n * sizeof(uint32_t));
+    }
+  }
+  // the other copies: stable counting sort by that mode's label
+  std::vector<uint64_t> cnt;
+  std::vector<uint32_t> pos(n);
+  for(int q=1; q < nc; ++q) {
+    int const c = copies[q];
+    cnt.assign(bd[c] + 1, 0);
+    for(uint64_t x=0; x < n; ++x) ++cnt[lab_tmp[c][x] + 1];
+    for(uint64_t k=0; k < bd[c]; ++k) cnt[k + 1] += cnt[k];
+    for(uint64_t x=0; x < n; ++x) pos[cnt[lab_tmp[c][x]]++] = (uint32_t) x;
+    double const * const v0 = (double const *) (B + ps->off_vals[lead]);
+    double * const v = (double *) (B + ps->off_vals[c]);
+    for(uint64_t x=0; x < n; ++x) v[x] = v0[pos[x]];
+    for(int m=0; m < N; ++m) {
+      if(m == t) continue;
+      uint32_t * const d = (uint32_t *) (B + ps->off_ind[c][m]);
+      uint32_t const * const src = lab_tmp[m].data();
+      for(uint64_t x=0; x < n; ++x) d[x] = src[pos[x]];
+    }
+  }
+  // new rows, modes in order (grow_mats)
+  for(int m=0; m < N; ++m) {
+    if(m == t || bd[m] <= ps->grow0[m]) continue;
+    fill_rand_state(&s->rd, (double *) (B + ps->off_grow[m]), (bd[m] - ps->grow0[m]) * R);
+    s->prep_rows[m] = bd[m];
+  }
+  // the local error's mode: the smallest dim, first on ties (the streamed
+  // mode's dim is 1)
+  int small = 0;
+  for(int m=1; m < N; ++m) if(bd[m] < bd[small]) small = m;
+  ps->small = small;
+  ps->first = (s->prep_count == 0) ? 1 : 0;
+  ++s->prep_count;
+  ps->xnorm = xn;
+  ps->nnz = n;
+  ps->sl = sl;
+  return SPD_OK;
+}
+
+PSlice * pool_get(spd_server * sv)
+{
+  // caller holds sv->mu
+  if(!sv->pool.empty()) {
+    PSlice * p = sv->pool.back();
+    sv->pool.pop_back();
+    return p;
+  }
+  return new PSlice();
+}
+
+void pool_put(spd_server * sv, PSlice * p)
+{
+  // caller holds sv->mu
+  p->sl = nullptr;
+  sv->pool.push_back(p);
+}
+
+// slices prepared ahead of the device step, per stream
+#define PREP_AHEAD 2
+
+void finish_stream_locked(spd_server * sv, Stream * s, std::unique_lock<std::mutex> & lk);
+
+void prep_main(spd_server * sv)
+{
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while(true) {
+    Stream * pick = nullptr;
+    sv->cv_prep.wait(lk, [&] {
+      if(sv->stop) return true;
+      double best = 0.;
+      for(auto & kv : sv->streams) {
+        Stream * s = kv.second;
+        if(!s->batchable || s->prepping || s->queue.empty() ||
+            s->prepared.size() >= PREP_AHEAD) continue;
+        double const ts = s->queue.front()->t_submit;
+        if(pick == nullptr || ts < best) { pick = s; best = ts; }
+      }
+      return pick != nullptr;
+    });
+    if(sv->stop) break;
+    Stream * s = pick;
+    s->prepping = true;
+    Slice * sl = s->queue.front();
+    s->queue.pop_front();
+    PSlice * ps = pool_get(sv);
+    lk.unlock();
+    int const rc = prepare_slice(s, sl, ps);
+    lk.lock();
+    s->prepping = false;
+    if(rc == SPD_OK) {
+      s->prepared.push_back(ps);
+    } else {
+      pool_put(sv, ps);
+      sl->t_done = sst_wall_s();
+      sl->status = rc;
+      snprintf(sl->err, sizeof(sl->err), "%s", spd_last_error());
+      --sv->pending;
+      if(s->closing && s->queue.empty() && s->prepared.empty() && !s->busy) {
+        s->busy = true;
+        finish_stream_locked(sv, s, lk);
+      }
+      sv->cv_done.notify_all();
+    }
+    sv->cv_work.notify_all();
+    sv->cv_prep.notify_all();
+  }
+}
+
+// the shape a batch shares: modes, streamed mode, padded rank
+uint64_t stream_class(Stream const * s)
+{
+  return ((uint64_t) s->N << 16) | ((uint64_t) s->cfg.stream_mode << 8) |
+      ((s->cfg.rank + 15) / 16);
+}
+
+bool stream_ready(Stream const * s)
+{
+  if(!s->admitted || s->busy) return false;
+  return s->batchable ? !s->prepared.empty() : !s->queue.empty();
+}
+
+// Device part of a batch of prepared slices (the streams are busy: nobody
+// else touches them).
+int run_batch(spd_server * sv, BtEngine * e, WorkerRes & w, std::vector<Stream *> & ss,
+    std::vector<PSlice *> & pp)
+{
+  hipStream_t const est = bt_engine_stream(e);
+  uint32_t const K = (uint32_t) ss.size();
+  std::vector<BtSlice> bs(K);
+  std::vector<BtSlice *> ptrs(K);
+  for(uint32_t q=0; q < K; ++q) {
+    Stream * s = ss[q];
+    PSlice * ps = pp[q];
+    int const N = s->N;
+    int const t = s->cfg.stream_mode;
+    uint32_t const R = (uint32_t) s->cfg.rank;
+    s->hs = est;
+    if(!s->allocated) {
+      s->allocated = true;
+      s->state = new StreamState();
+      SPD_TRY(s->state->init(N, s->cfg, s->cap, est, 1));
+      uint64_t maxcap = 1;
+      for(int m=0; m < N; ++m) if(m != t && s->cap[m] > maxcap) maxcap = s->cap[m];
+      SPD_TRY(spd_malloc_on((void **) &s->d_part, bt_part_doubles(R, maxcap) * sizeof(double),
+          est));
+    } else {
+      s->state->set_stream(est);
+    }
+    if(ps->bytes > s->d_slice_cap) {
+      size_t nc = s->d_slice_cap ? s->d_slice_cap : (1 << 20);
+      while(nc < ps->bytes) nc *= 2;
+      (void) spd_free_on(s->d_slice, est);
+      s->d_slice = nullptr;
+      s->d_slice_cap = 0;
+      SPD_TRY(spd_malloc_on((void **) &s->d_slice, nc, est));
+      s->d_slice_cap = nc;
+    }
+    SPD_HIP(hipMemcpyAsync(s->d_slice, ps->buf, ps->bytes, hipMemcpyHostToDevice, est));
+    BtSlice & b = bs[q];
+    b.S = s->state;
+    b.nnz = ps->nnz;
+    int const lead = (t + 1) % N;
+    for(int m=0; m < N; ++m) {
+      b.rows[m] = ps->rows[m];
+      b.grow0[m] = ps->grow0[m];
+      b.d_grow[m] = (m != t && ps->rows[m] > ps->grow0[m]) ?
+          (double const *) (s->d_slice + ps->off_grow[m]) : nullptr;
+      for(int c=0; c < N; ++c) {
+        if(c == t || m == t) continue;
+        b.d_ind[c][m] = (uint32_t const *) (s->d_slice + ps->off_ind[c][m]);
+      }
+    }
+    for(int c=0; c < N; ++c) {
+      if(c != t) b.d_vals[c] = (double const *) (s->d_slice + ps->off_vals[c]);
+    }
+    (void) lead;
+    b.d_part = s->d_part;
+    b.first = ps->first;
+    b.small = ps->small;
+    b.xnorm = ps->xnorm;
+    ptrs[q] = &b;
+  }
+  double const t0 = sst_wall_s();
+  SPD_TRY(bt_engine_run(e, ptrs.data(), K));
+  double const secs = sst_wall_s() - t0;
+  for(uint32_t q=0; q < K; ++q) {
+    Stream * s = ss[q];
+    PSlice * ps = pp[q];
+    BtSlice const & b = bs[q];
+    for(int m=0; m < s->N; ++m) {
+      if(m == s->cfg.stream_mode) continue;
+      s->state->rows[m] = ps->rows[m];
+      s->state->prev_dim[m] = ps->rows[m];
+    }
+    ++s->state->nslices;
+    s->time_rows.insert(s->time_rows.end(), b.time_row.begin(), b.time_row.end());
+    ps->sl->local_err = b.local_err;
+    ps->sl->outer = b.outer;
+    ps->sl->gpu_s = secs;
+  }
+  (void) w;
+  (void) sv;
+  return SPD_OK;
+}
+
+void finish_stream_locked(spd_server * sv, Stream * s, std::unique_lock<std::mutex> & lk)
+{
+  // caller holds lk; the stream is busy
+  bool const more = s->batchable ? (!s->prepared.empty() || !s->queue.empty() || s->prepping)
+                                 : !s->queue.empty();
+  if(!more && s->closing) {
+    lk.unlock();
+    release_device(s);
+    lk.lock();
+    s->busy = false;
+    release_budget(sv, s);
+  } else {
+    s->busy = false;
+  }
+}
+
+void engine_main(spd_server * sv, int ei)
+{
+  (void) hipSetDevice(0);
+  BtEngine * const e = sv->eng[ei];
+  WorkerRes & w = sv->wres[ei];
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while(true) {
+    std::vector<Stream *> ss;
+    sv->cv_work.wait(lk, [&] {
+      if(sv->stop) return true;
+      // the longest-waiting ready stream decides the batch's class
+      Stream * first = nullptr;
+      double best = 0.;
+      for(auto & kv : sv->streams) {
+        Stream * s = kv.second;
+        if(!stream_ready(s)) continue;
+        double const ts = s->batchable ? s->prepared.front()->sl->t_submit
+                                       : s->queue.front()->t_submit;
+        if(first == nullptr || ts < best) { first = s; best = ts; }
+      }
+      if(first == nullptr) return false;
+      ss.push_back(first);
+      if(first->batchable) {
+        uint64_t const cls = stream_class(first);
+        std::vector<std::pair<double, Stream *>> cand;
+        for(auto & kv : sv->streams) {
+          Stream * s = kv.second;
+          if(s == first || !s->batchable || !stream_ready(s) || stream_class(s) != cls) continue;
+          cand.emplace_back(s->prepared.front()->sl->t_submit, s);
+        }
+        std::sort(cand.begin(), cand.end(), [](auto const & a, auto const & b) {
+          return a.first < b.first; });
+        for(auto & c : cand) {
+          if((int) ss.size() >= sv->max_inflight || ss.size() >= BT_MAXK) break;
+          ss.push_back(c.second);
+        }
+      }
+      return true;
+    });
+    if(sv->stop) break;
+    for(Stream * s : ss) s->busy = true;
+    if(!ss[0]->batchable) {
+      // the per-slice path, one slice
+      Stream * s = ss[0];
+      Slice * sl = s->queue.front();
+      s->queue.pop_front();
+      lk.unlock();
+      int const rc = run_slice(s, sl, w);
+      spd_scratch_bind(nullptr);
+      double const now = sst_wall_s();
+      lk.lock();
+      sl->t_done = now;
+      sl->status = rc;
+      if(rc != SPD_OK) snprintf(sl->err, sizeof(sl->err), "%s", spd_last_error());
+      --sv->pending;
+      finish_stream_locked(sv, s, lk);
+      sv->cv_work.notify_all();
+      sv->cv_done.notify_all();
+      continue;
+    }
+    std::vector<PSlice *> pp;
+    for(Stream * s : ss) {
+      pp.push_back(s->prepared.front());
+      s->prepared.pop_front();
+    }
+    sv->cv_prep.notify_all();
+    lk.unlock();
+    int const rc = run_batch(sv, e, w, ss, pp);
+    char msg[160] = "";
+    if(rc != SPD_OK) snprintf(msg, sizeof(msg), "%s", spd_last_error());
+    if(rc != SPD_OK) (void) hipStreamSynchronize(bt_engine_stream(e));
+    double const now = sst_wall_s();
+    lk.lock();
+    for(size_t q=0; q < ss.size(); ++q) {
+      Slice * sl = pp[q]->sl;
+      sl->t_done = now;
+      sl->status = rc;
+      if(rc != SPD_OK) snprintf(sl->err, sizeof(sl->err), "%s", msg);
+      --sv->pending;
+      pool_put(sv, pp[q]);
+    }
+    for(Stream * s : ss) finish_stream_locked(sv, s, lk);
+    sv->cv_work.notify_all();
+    sv->cv_prep.notify_all();
+    sv->cv_done.notify_all();
+  }
+}
+
 }  // namespace
 
 extern "C" int spd_server_create(size_t budget, int max_inflight, spd_server ** out)
@@ -453,6 +925,28 @@ extern "C" int spd_server_create(size_t budget, int max_inflight, spd_server ** 
   spd_server * sv = new spd_server();
   sv->budget = budget;
   sv->max_inflight = max_inflight;
+  if(max_inflight > 1) {
+    // batched: engines (each its own HIP stream) run batches of up to
+    // max_inflight slices of one shape class; host threads prepare slices
+    sv->batched = true;
+    int const ne = BT_ENGINES;
+    sv->wres.resize(ne);
+    for(int k=0; k < ne; ++k) {
+      BtEngine * e = nullptr;
+      int const rc = bt_engine_create(&e);
+      if(rc != SPD_OK) {
+        for(BtEngine * x : sv->eng) bt_engine_free(x);
+        delete sv;
+        return rc;
+      }
+      sv->eng.push_back(e);
+      sv->wres[k].hs = bt_engine_stream(e);
+    }
+    for(int k=0; k < ne; ++k) sv->engines.emplace_back(engine_main, sv, k);
+    for(int k=0; k < BT_PREP_THREADS; ++k) sv->preppers.emplace_back(prep_main, sv);
+    *out = sv;
+    return SPD_OK;
+  }
   sv->wres.resize(max_inflight);
   for(int w=0; w < max_inflight; ++w) {
     // default priority, non-blocking w.r.t. the legacy null stream
@@ -488,6 +982,14 @@ extern "C" int spd_server_open(spd_server * sv, int N, uint64_t const * cap,
   }
   initstate_r(seed, s->rstate, sizeof(s->rstate), &s->rd);
   s->footprint = StreamState::footprint(N, cfg->stream_mode, cfg->rank, s->cap);
+  s->batchable = sv->batched && stream_batchable(s);
+  if(s->batchable) {
+    uint64_t maxcap = 1;
+    for(int m=0; m < N; ++m) {
+      if(m != cfg->stream_mode && s->cap[m] > maxcap) maxcap = s->cap[m];
+    }
+    s->footprint += bt_part_doubles((uint32_t) cfg->rank, maxcap) * sizeof(double);
+  }
   std::lock_guard<std::mutex> lk(sv->mu);
   if(s->footprint > sv->budget) {
     delete s;
@@ -527,7 +1029,10 @@ extern "C" int spd_server_submit(spd_server * sv, int64_t id, uint64_t nnz,
   sv->slices.push_back(sl);
   s->queue.push_back(sl);
   ++sv->pending;
-  if(s->admitted && !s->busy && s->queue.size() == 1) {
+  if(sv->batched) {
+    if(s->batchable) sv->cv_prep.notify_one();
+    else sv->cv_work.notify_all();
+  } else if(s->admitted && !s->busy && s->queue.size() == 1) {
     sv->ready.push_back(id);
     sv->cv_work.notify_one();
   }
@@ -546,7 +1051,8 @@ extern "C" int spd_server_drain(spd_server * sv)
     bool running = !sv->ready.empty();
     for(auto & kv : sv->streams) {
       Stream const * s = kv.second;
-      if(s->busy || (s->admitted && !s->queue.empty())) running = true;
+      if(s->busy || s->prepping || (s->admitted && (!s->queue.empty() || !s->prepared.empty())))
+        running = true;
     }
     stuck = !running;
     return stuck;
@@ -589,7 +1095,9 @@ extern "C" int spd_server_factors(spd_server * sv, int64_t id, int mode, double 
     return SPD_EINPUT;
   }
   Stream * s = it->second;
-  sv->cv_done.wait(lk, [&] { return s->queue.empty() && !s->busy; });
+  sv->cv_done.wait(lk, [&] {
+    return s->queue.empty() && s->prepared.empty() && !s->prepping && !s->busy;
+  });
   if(mode < 0 || mode >= s->N) {
     spd_set_error(SPD_EINPUT, "serve: bad mode %d", mode);
     return SPD_EINPUT;
@@ -630,7 +1138,7 @@ extern "C" int spd_server_close(spd_server * sv, int64_t id)
   Stream * s = it->second;
   if(s->closing) return SPD_OK;
   s->closing = true;
-  if(s->queue.empty() && !s->busy) {
+  if(s->queue.empty() && s->prepared.empty() && !s->prepping && !s->busy) {
     for(auto w = sv->waiting.begin(); w != sv->waiting.end(); ++w) {
       if(*w == id) { sv->waiting.erase(w); break; }
     }
@@ -670,7 +1178,10 @@ extern "C" void spd_server_free(spd_server * sv)
     sv->stop = true;
   }
   sv->cv_work.notify_all();
+  sv->cv_prep.notify_all();
   for(auto & w : sv->workers) w.join();
+  for(auto & w : sv->engines) w.join();
+  for(auto & w : sv->preppers) w.join();
   for(auto & kv : sv->streams) {
     kv.second->closing = true;
     kv.second->busy = false;
@@ -679,14 +1190,20 @@ extern "C" void spd_server_free(spd_server * sv)
     delete kv.second;
   }
   for(Slice * sl : sv->slices) delete sl;
+  for(PSlice * p : sv->pool) {
+    if(p->buf) (void) hipHostFree(p->buf);
+    delete p;
+  }
+  for(auto & kv : sv->streams) (void) kv;
   for(WorkerRes & w : sv->wres) {
     (void) hipStreamSynchronize(w.hs);
     for(int m=0; m < SPD_MAX_MODES; ++m) {
       if(w.hind[m]) (void) hipHostFree(w.hind[m]);
     }
     if(w.hv) (void) hipHostFree(w.hv);
-    (void) hipStreamDestroy(w.hs);
+    if(!sv->batched) (void) hipStreamDestroy(w.hs);
   }
+  for(BtEngine * e : sv->eng) bt_engine_free(e);
   delete sv;
 }
 

@@ -16,6 +16,7 @@
 // reads the two scalars of the outer change measure per mode.
 #include "spd_internal.hpp"
 #include "spd_dense.hpp"
+#include "spd_batch.hpp"
 #include "spd_stream_state.hpp"
 #include <math.h>
 #include <stdlib.h>
@@ -49,8 +50,9 @@ struct CooArgs {
 #define CO_CHUNK 16
 #define CO_BATCH 8
 template<int L, int NC>
-__global__ void __launch_bounds__(CO_THREADS)
-k_coo_mttkrp(CooArgs a)
+__device__ __forceinline__ void
+b_coo_mttkrp(CooArgs a,
+    uint32_t blk, uint32_t nblk)
 {
   constexpr int G = 64 / L;
   constexpr int W = CO_THREADS / 64;
@@ -63,8 +65,8 @@ k_coo_mttkrp(CooArgs a)
   uint32_t const R = a.R;
   int const N = a.N;
   int const mode = a.mode;
-  uint64_t const group = ((uint64_t) blockIdx.x * W + wave) * G + g;
-  uint64_t const ngroups = (uint64_t) gridDim.x * W * G;
+  uint64_t const group = ((uint64_t) blk * W + wave) * G + g;
+  uint64_t const ngroups = (uint64_t) nblk * W * G;
   double acc[NC];
   #pragma unroll
   for(int k=0; k < NC; ++k) acc[k] = 0.;
@@ -149,20 +151,28 @@ k_coo_mttkrp(CooArgs a)
     for(uint32_t col=threadIdx.x; col < R; col += CO_THREADS) {
       double sum = 0.;
       for(int w=0; w < W * G; ++w) sum += s_red[w][col];
-      a.part[(uint64_t) blockIdx.x * R + col] = sum;
+      a.part[(uint64_t) blk * R + col] = sum;
     }
   }
 }
 
+template<int L, int NC>
+__global__ void __launch_bounds__(CO_THREADS)
+k_coo_mttkrp(CooArgs a)
+{
+  b_coo_mttkrp<L, NC>(a, blockIdx.x, gridDim.x);
+}
+
 // out[c] = sum_b part[b * R + c]: one block per 32 columns, the partials
 // split over 32 thread groups and added in a fixed order.
-__global__ void __launch_bounds__(1024)
-k_rowsum(double const * __restrict__ part, uint32_t nb, uint32_t R,
-    double * __restrict__ out)
+__device__ __forceinline__ void
+b_rowsum(double const * __restrict__ part, uint32_t nb, uint32_t R,
+    double * __restrict__ out,
+    uint32_t blk, uint32_t nblk)
 {
   __shared__ double s[32][33];
   int const t = threadIdx.x;
-  uint32_t const c = blockIdx.x * 32 + t % 32;
+  uint32_t const c = blk * 32 + t % 32;
   int const q = t / 32;
   double v = 0.;
   if(c < R) {
@@ -177,9 +187,17 @@ k_rowsum(double const * __restrict__ part, uint32_t nb, uint32_t R,
   }
 }
 
+__global__ void __launch_bounds__(1024)
+k_rowsum(double const * __restrict__ part, uint32_t nb, uint32_t R,
+    double * __restrict__ out)
+{
+  b_rowsum(part, nb, R, out, blockIdx.x, gridDim.x);
+}
+
 // Time Gram: symmetric copy of the upper triangle into old, then
 // A = old + t t^T (StreamCPD.cc:420-435). One block.
-__global__ void k_time_gram(double * __restrict__ A, double const * __restrict__ t,
+__device__ __forceinline__ void
+b_time_gram(double * __restrict__ A, double const * __restrict__ t,
     double * __restrict__ old, uint32_t R)
 {
   for(uint32_t e=threadIdx.x; e < R * R; e += blockDim.x) {
@@ -195,10 +213,18 @@ __global__ void k_time_gram(double * __restrict__ A, double const * __restrict__
   }
 }
 
+__global__ void
+k_time_gram(double * __restrict__ A, double const * __restrict__ t,
+    double * __restrict__ old, uint32_t R)
+{
+  b_time_gram(A, t, old, R);
+}
+
 struct HistSrc { double const * m[SPD_MAX_MODES]; };
 
 // H (RP x RP, zero padded) = old_gram o cross_0 o cross_1 ...
-__global__ void k_hist(double const * __restrict__ old, HistSrc cr, int ncross,
+__device__ __forceinline__ void
+b_hist(double const * __restrict__ old, HistSrc cr, int ncross,
     uint32_t R, uint32_t RP, double * __restrict__ H)
 {
   for(uint32_t e=threadIdx.x; e < RP * RP; e += blockDim.x) {
@@ -213,6 +239,13 @@ __global__ void k_hist(double const * __restrict__ old, HistSrc cr, int ncross,
   }
 }
 
+__global__ void
+k_hist(double const * __restrict__ old, HistSrc cr, int ncross,
+    uint32_t R, uint32_t RP, double * __restrict__ H)
+{
+  b_hist(old, cr, ncross, R, RP, H);
+}
+
 __global__ void k_scale(double * __restrict__ x, uint64_t n, double f)
 {
   uint64_t const e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,13 +253,14 @@ __global__ void k_scale(double * __restrict__ x, uint64_t n, double f)
 }
 
 // Block partials of sum (a-b)^2 and sum b^2.
-__global__ void __launch_bounds__(256)
-k_diff2(double const * __restrict__ a, double const * __restrict__ b, uint64_t n,
-    double * __restrict__ part)
+__device__ __forceinline__ void
+b_diff2(double const * __restrict__ a, double const * __restrict__ b, uint64_t n,
+    double * __restrict__ part,
+    uint32_t blk, uint32_t nblk)
 {
   __shared__ double s[2][256];
-  uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t const stride = (uint64_t) gridDim.x * blockDim.x;
+  uint64_t e = (uint64_t) blk * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t) nblk * blockDim.x;
   double d2 = 0., b2 = 0.;
   for(; e < n; e += stride) {
     double const x = a[e] - b[e];
@@ -244,11 +278,18 @@ k_diff2(double const * __restrict__ a, double const * __restrict__ b, uint64_t n
     }
     __syncthreads();
   }
-  if(t < 2) part[(uint64_t) blockIdx.x * 2 + t] = s[t][0];
+  if(t < 2) part[(uint64_t) blk * 2 + t] = s[t][0];
 }
 
 __global__ void __launch_bounds__(256)
-k_sum2(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
+k_diff2(double const * __restrict__ a, double const * __restrict__ b, uint64_t n,
+    double * __restrict__ part)
+{
+  b_diff2(a, b, n, part, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void
+b_sum2(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
 {
   __shared__ double s[128][2];
   int const t = threadIdx.x;
@@ -263,6 +304,12 @@ k_sum2(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
     for(int j=0; j < 128; ++j) tot += s[j][t];
     out[t] = tot;
   }
+}
+
+__global__ void __launch_bounds__(256)
+k_sum2(double const * __restrict__ part, uint32_t nb, double * __restrict__ out)
+{
+  b_sum2(part, nb, out);
 }
 
 __global__ void k_fill_f64(double * __restrict__ x, uint64_t n, double v)
@@ -909,5 +956,389 @@ int sst_check_cfg(int N, spd_stream_cfg const & cfg)
         "constraints only");
     return SPD_EINPUT;
   }
+  return SPD_OK;
+}
+
+// ---- batched launches (spd_batch.hpp): every slice of a batch at once ----
+
+#define SB_ALL 0
+#define SB_LIVE 1
+#define SB_SMALL 2        // slices whose local error uses this mode
+
+__device__ __forceinline__ bool sb_gate(BSlot const & s, int gate, int mode)
+{
+  if(gate == SB_SMALL) return s.small == mode;
+  return gate == SB_ALL || s.c->done == 0;
+}
+
+__host__ __device__ __forceinline__ uint32_t sb_coo_nb(uint64_t nnz, uint32_t L, int reduce)
+{
+  uint64_t const per_block = (uint64_t) (CO_THREADS / L) * CO_CHUNK;
+  uint64_t const nb = ((nnz > 0 ? nnz : 1) + per_block - 1) / per_block;
+  uint64_t const cap = reduce ? 2048u : 8192u;
+  return (uint32_t) (nb < cap ? nb : cap);
+}
+
+// COO MTTKRP of output mode `mode` over copy `copy` of each slice's
+// nonzeros: the streamed mode reduces to one row (block partials in part),
+// other modes add into Mbuf (cleared before)
+template<int L, int NC>
+__global__ void __launch_bounds__(CO_THREADS)
+kb_coo(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode, int copy,
+    int gate)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  int const red = (mode == hd->t);
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(sb_gate(s, gate, mode)) nb = sb_coo_nb(s.nnz, L, red);
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    CooArgs a = {};
+    for(int m=0; m < hd->N; ++m) {
+      a.ind[m] = s.ind[copy][m];
+      a.mats[m] = (m == hd->t) ? s.time_row : s.Unew[m];
+    }
+    a.vals = s.vals[copy];
+    a.nnz = s.nnz;
+    a.R = s.R;
+    a.N = hd->N;
+    a.mode = mode;
+    a.reduce = red;
+    a.out = s.Mbuf;
+    a.part = s.part;
+    b_coo_mttkrp<L, NC>(a, w - s_it.pref[q], s_it.pref[q + 1] - s_it.pref[q]);
+    __syncthreads();
+  }
+}
+
+// the streamed mode's row (the only mode reduced to one row): Mbuf[0, :] =
+// sum of the block partials
+__global__ void __launch_bounds__(1024)
+kb_rowsum(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int L, int gate)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(sb_gate(s, gate, hd->t)) nb = (s.R + 31) / 32;
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    b_rowsum(s.part, sb_coo_nb(s.nnz, (uint32_t) L, 1), s.R, s.Mbuf, w - s_it.pref[q], 0);
+    __syncthreads();
+  }
+}
+
+// what 0: Mbuf rows of `mode`; 1: Dual rows of `mode`
+__global__ void kb_zero(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode,
+    int what, int gate)
+{
+  uint32_t const q = blockIdx.y;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  if(!sb_gate(s, gate, mode)) return;
+  double * const x = (what == 0) ? s.Mbuf : s.Dual[mode];
+  uint64_t const n = s.rows[mode] * s.R;
+  for(uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; e < n;
+      e += (uint64_t) gridDim.x * blockDim.x) {
+    x[e] = 0.;
+  }
+}
+
+__global__ void kb_time_gram(BSlot const * __restrict__ S, BHead const * __restrict__ hd)
+{
+  uint32_t const q = blockIdx.x;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  if(s.c->done) return;
+  b_time_gram(s.aTa[hd->t], s.time_row, s.old_gram, s.R);
+}
+
+// H of `mode` from the cross Grams of the other non-streamed modes
+__global__ void kb_hist(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int ncross)
+{
+  uint32_t const q = blockIdx.x;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  if(s.c->done) return;
+  HistSrc hs = {};
+  for(int k=0; k < ncross; ++k) hs.m[k] = s.cross + (uint64_t) k * s.R * s.R;
+  b_hist(s.old_gram, hs, ncross, s.R, dn_rpad(s.R), s.H);
+}
+
+__host__ __device__ __forceinline__ uint32_t sb_diff_nb(uint64_t n)
+{
+  uint64_t const nb = ((n > 0 ? n : 1) + 256 * 8 - 1) / (256 * 8);
+  return (uint32_t) (nb < 1024 ? nb : 1024);
+}
+
+__global__ void __launch_bounds__(256)
+kb_diff2(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode)
+{
+  __shared__ BItems s_it;
+  uint32_t const K = hd->K;
+  {
+    uint32_t nb = 0;
+    if(threadIdx.x < K) {
+      BSlot const & s = S[threadIdx.x];
+      if(s.c->done == 0) nb = sb_diff_nb(s.rows[mode] * s.R);
+    }
+    bt_items(s_it, K, nb);
+  }
+  for(uint32_t w=blockIdx.x; w < s_it.total; w += gridDim.x) {
+    uint32_t const q = bt_find(s_it, K, w);
+    BSlot const & s = S[q];
+    b_diff2(s.Uold[mode], s.Unew[mode], s.rows[mode] * s.R, s.part, w - s_it.pref[q],
+        s_it.pref[q + 1] - s_it.pref[q]);
+    __syncthreads();
+  }
+}
+
+// the mode's change measure added to the outer iteration's delta, in the
+// per-slice order: delta += sqrt(sum (old-new)^2) / sqrt(sum new^2)
+__global__ void __launch_bounds__(256)
+kb_sum2_delta(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int mode)
+{
+  uint32_t const q = blockIdx.x;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  if(s.c->done) return;
+  b_sum2(s.part, sb_diff_nb(s.rows[mode] * s.R), s.c->diff);
+  __syncthreads();
+  if(threadIdx.x == 0) s.c->delta += sqrt(s.c->diff[0]) / sqrt(s.c->diff[1]);
+}
+
+__global__ void kb_ctl_reset(BSlot const * __restrict__ S, BHead const * __restrict__ hd)
+{
+  uint32_t const q = threadIdx.x;
+  if(q >= hd->K) return;
+  BCtl * const c = S[q].c;
+  c->admm[0] = 0;
+  c->admm[1] = 0;
+}
+
+// start of a slice's step: the outer loop runs, nothing measured yet
+__global__ void kb_begin(BSlot const * __restrict__ S, BHead const * __restrict__ hd)
+{
+  uint32_t const q = threadIdx.x;
+  if(q >= hd->K) return;
+  BCtl * const c = S[q].c;
+  c->done = 0;
+  c->outer = 0;
+  c->delta = 0.;
+  c->prev_delta = 0.;
+  c->admm[0] = 0;
+  c->admm[1] = 0;
+  c->info = 0;
+}
+
+// end of outer iteration o (StreamCPD.cc:476-486): stop when o > 0 and the
+// change measure moved less than the tolerance
+__global__ void kb_outer_check(BSlot const * __restrict__ S, BHead const * __restrict__ hd, int o)
+{
+  uint32_t const q = threadIdx.x;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  BCtl * const c = s.c;
+  if(c->done) return;
+  double const delta = c->delta;
+  c->outer = o + 1;
+  if(o > 0 && fabs(delta - c->prev_delta) < s.tol) {
+    c->done = 1;
+  } else {
+    c->prev_delta = delta;
+  }
+  if((uint32_t) (o + 1) >= s.max_its) c->done = 1;
+  c->delta = 0.;
+}
+
+// after the outer loop: the time Gram forgets, old <- new
+__global__ void kb_finish(BSlot const * __restrict__ S, BHead const * __restrict__ hd)
+{
+  uint32_t const q = blockIdx.y;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  uint32_t const R = s.R;
+  if(blockIdx.x == 0) {
+    double * const g = s.aTa[hd->t];
+    for(uint32_t e=threadIdx.x; e < R * R; e += blockDim.x) g[e] *= s.forget;
+  }
+  for(int m=0; m < hd->N; ++m) {
+    if(m == hd->t) continue;
+    uint64_t const n = s.rows[m] * R;
+    for(uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; e < n;
+        e += (uint64_t) gridDim.x * blockDim.x) {
+      s.Uold[m][e] = s.Unew[m][e];
+    }
+  }
+}
+
+// new factor rows (values drawn on the host) into Unew
+__global__ void kb_grow(BSlot const * __restrict__ S, BHead const * __restrict__ hd)
+{
+  uint32_t const q = blockIdx.y;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  for(int m=0; m < hd->N; ++m) {
+    if(m == hd->t || s.grow[m] == nullptr) continue;
+    uint64_t const n = (s.rows[m] - s.grow0[m]) * s.R;
+    double * const dst = s.Unew[m] + s.grow0[m] * s.R;
+    for(uint64_t e = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; e < n;
+        e += (uint64_t) gridDim.x * blockDim.x) {
+      dst[e] = s.grow[m][e];
+    }
+  }
+}
+
+// per slice results for the host: time row (R), outer its, <X,Z>, ||Z||^2,
+// Cholesky status
+__global__ void kb_collect(BSlot const * __restrict__ S, BHead const * __restrict__ hd,
+    double * __restrict__ out, uint32_t ld)
+{
+  uint32_t const q = blockIdx.x;
+  if(q >= hd->K) return;
+  BSlot const & s = S[q];
+  double * const o = out + (uint64_t) q * ld;
+  for(uint32_t r=threadIdx.x; r < s.R; r += blockDim.x) o[r] = s.time_row[r];
+  if(threadIdx.x == 0) {
+    o[s.R] = (double) s.c->outer;
+    o[s.R + 1] = s.c->inner[0];
+    o[s.R + 2] = s.c->inner[1];
+    o[s.R + 3] = (double) s.c->info;
+  }
+}
+
+namespace {
+uint32_t sb_grid(uint64_t tot)
+{
+  if(tot < 1) tot = 1;
+  return (uint32_t) (tot < BT_GRID ? tot : BT_GRID);
+}
+}
+
+int sstb_coo(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K, uint32_t TT,
+    int mode, int copy, int gate, int t, hipStream_t st)
+{
+  int const L = (TT == 1) ? 16 : ((TT == 2) ? 32 : 64);
+  int const red = (mode == t);
+  uint64_t tot = 0;
+  uint64_t maxn = 1;
+  for(uint32_t q=0; q < K; ++q) {
+    tot += sb_coo_nb(hS[q].nnz, (uint32_t) L, red);
+    uint64_t const n = hS[q].rows[mode] * hS[q].R;
+    if(n > maxn) maxn = n;
+  }
+  if(!red) {
+    uint64_t gx = (maxn + 255) / 256;
+    if(gx > 512) gx = 512;
+    hipLaunchKernelGGL(kb_zero, dim3((unsigned) gx, K), dim3(256), 0, st, dS, dH, mode, 0, gate);
+    SPD_LAUNCH_CHECK();
+  }
+  if(L == 16) hipLaunchKernelGGL((kb_coo<16, 1>), dim3(sb_grid(tot)), dim3(CO_THREADS), 0, st, dS, dH, mode, copy, gate);
+  else if(L == 32) hipLaunchKernelGGL((kb_coo<32, 1>), dim3(sb_grid(tot)), dim3(CO_THREADS), 0, st, dS, dH, mode, copy, gate);
+  else hipLaunchKernelGGL((kb_coo<64, 1>), dim3(sb_grid(tot)), dim3(CO_THREADS), 0, st, dS, dH, mode, copy, gate);
+  SPD_LAUNCH_CHECK();
+  if(red) {
+    hipLaunchKernelGGL(kb_rowsum, dim3(sb_grid((uint64_t) K * 2)), dim3(1024), 0, st, dS, dH, L,
+        gate);
+    SPD_LAUNCH_CHECK();
+  }
+  return SPD_OK;
+}
+
+int sstb_zero_dual(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K, int mode,
+    hipStream_t st)
+{
+  uint64_t maxn = 1;
+  for(uint32_t q=0; q < K; ++q) {
+    uint64_t const n = hS[q].rows[mode] * hS[q].R;
+    if(n > maxn) maxn = n;
+  }
+  uint64_t gx = (maxn + 255) / 256;
+  if(gx > 512) gx = 512;
+  hipLaunchKernelGGL(kb_zero, dim3((unsigned) gx, K), dim3(256), 0, st, dS, dH, mode, 1, SB_LIVE);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int sstb_time_gram(uint32_t K, BSlot const * dS, BHead const * dH, hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_time_gram, dim3(K), dim3(256), 0, st, dS, dH);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int sstb_hist(uint32_t K, BSlot const * dS, BHead const * dH, int ncross, hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_hist, dim3(K), dim3(256), 0, st, dS, dH, ncross);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int sstb_delta(BSlot const * hS, BSlot const * dS, BHead const * dH, uint32_t K, int mode,
+    hipStream_t st)
+{
+  uint64_t tot = 0;
+  for(uint32_t q=0; q < K; ++q) tot += sb_diff_nb(hS[q].rows[mode] * hS[q].R);
+  hipLaunchKernelGGL(kb_diff2, dim3(sb_grid(tot)), dim3(256), 0, st, dS, dH, mode);
+  SPD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(kb_sum2_delta, dim3(K), dim3(256), 0, st, dS, dH, mode);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int sstb_ctl_reset(BSlot const * dS, BHead const * dH, hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_ctl_reset, dim3(1), dim3(64), 0, st, dS, dH);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int sstb_begin(BSlot const * dS, BHead const * dH, hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_begin, dim3(1), dim3(64), 0, st, dS, dH);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int sstb_outer_check(BSlot const * dS, BHead const * dH, int o, hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_outer_check, dim3(1), dim3(64), 0, st, dS, dH, o);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int sstb_finish(uint32_t K, BSlot const * dS, BHead const * dH, hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_finish, dim3(256, K), dim3(256), 0, st, dS, dH);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int sstb_grow(uint32_t K, BSlot const * dS, BHead const * dH, hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_grow, dim3(64, K), dim3(256), 0, st, dS, dH);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+int sstb_collect(uint32_t K, BSlot const * dS, BHead const * dH, double * out, uint32_t ld,
+    hipStream_t st)
+{
+  hipLaunchKernelGGL(kb_collect, dim3(K), dim3(64), 0, st, dS, dH, out, ld);
+  SPD_LAUNCH_CHECK();
   return SPD_OK;
 }

@@ -529,7 +529,8 @@ def test_mttkrp_4mode_rank64_twomode(gpu_available, prec):
     # and leaf walks at the rank-64 lane layout), FP64 and FP32, against the
     # long-double COO oracle
     print("case 4-mode rank 64 prec", prec, flush=True)
-    inds, vals, dims = _c3_tensor(600_000)
+    inds, vals, _ = _c3_tensor(600_000)
+    dims = [int(i.max()) + 1 for i in inds]       # the CSF's dims (not compacted)
     coo = orc.Coo.from_arrays(inds, vals, dims)
     cs = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=ps.CSF_TWOMODE))
     R = 64
