@@ -52,6 +52,32 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+# Host threads of the CPU baselines: the box's CPU share for one GPU (16),
+# whatever larger count the container reports.
+CPU_SHARE = 16
+
+
+def host_cpu():
+    """CPU model and counts of the host the CPU baselines run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    visible = os.cpu_count() or 1
+    return dict(model=model, cpu_count=visible, affinity=affinity,
+                threads_used=min(CPU_SHARE, visible),
+                note="threads capped at the host CPU share of one GPU (%d)" % CPU_SHARE)
+
+
 def dev_factors(mats, dtype):
     bufs = [ps.DevBuf.from_host(m.astype(dtype)) for m in mats]
     arr = (C.c_void_p * len(bufs))(*[b.ptr for b in bufs])
@@ -108,6 +134,27 @@ def measure(L, csf, opts, mats, dims, R, prec, steps, warmup):
     return res, outs_host
 
 
+def time_dropin(csf, mats, dims, reps=3):
+    """The drop-in splatt_mttkrp with host factors and output (the caller's
+    arrays: upload of the input factors, the kernel, download of the
+    output), FP64, per mode: median wall time of reps calls after one
+    warm-up call."""
+    per = []
+    for m in range(len(dims)):
+        csf.mttkrp(m, mats)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            csf.mttkrp(m, mats)
+            ts.append(time.perf_counter() - t0)
+        per.append(1e3 * float(np.median(ts)))
+    host_bytes = [8.0 * sum(d * mats[0].shape[1] for d in dims)] * len(dims)
+    return dict(ms_per_mode=per, precision="f64",
+                host_bytes_moved_per_call=host_bytes,
+                note="includes host-to-device copies of the other modes' factors and "
+                     "the device-to-host copy of the output")
+
+
 def cpu_baseline(inds, vals, dims, mats, R, sample, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as orc
@@ -141,6 +188,7 @@ def cpu_baseline(inds, vals, dims, mats, R, sample, threads):
         nb += sum(d * R * 4.0 for d in dims)
         nbytes += nb
     return dict(value=nbytes / best / 1e9, unit="GB/s", cores=threads, kind="port",
+                host=host_cpu(),
                 sample="%d of the workload's nonzeros (its first %d in shuffled order), "
                        "all-mode CSF, FP32, all %d modes; median of %d passes "
                        "(%.3f s each, %.1f s timed in total)"
@@ -190,7 +238,7 @@ def bench_cpd_c3(nnz, R=64, its=25, seed=20261208):
     run are compared with the FP64 run."""
     inds, vals, dims = ps.synth_coord([500_000, 300_000, 100_000, 2_000], nnz,
                                       zipf=[1.1, 1.1, 1.1, 0.0], vals="symmetric",
-                                      compact=False, seed=seed)
+                                      compact=True, seed=seed)
     opts = ps.Opts(csf_alloc=ps.CSF_TWOMODE)
     t0 = time.perf_counter()
     cs = ps.CsfSet.convert(inds, vals, opts)
@@ -218,9 +266,10 @@ def bench_cpd_c3(nnz, R=64, its=25, seed=20261208):
         a = r32["factors"][m] * r32["lam"]
         b = r64["factors"][m] * r64["lam"]
         ferr.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
-    return dict(workload="CPD-ALS rank %d, %d iterations, 500000x300000x100000x2000, %d nnz "
-                         "Zipf(1.1) x3 + uniform, values U(-1,1), two-CSF allocation; FP32 "
-                         "MTTKRP vs an all-FP64 run from the same start" % (R, its, nnz),
+    return dict(workload="CPD-ALS rank %d, %d iterations, 500000x300000x100000x2000 "
+                         "(compacted to %s), %d nnz Zipf(1.1) x3 + uniform, values U(-1,1), "
+                         "two-CSF allocation; FP32 MTTKRP vs an all-FP64 run from the same "
+                         "start" % (R, its, "x".join(str(d) for d in dims), nnz),
                 csf_build_s=build_s, f32_cpd_call_s=s32, f32_ms_per_iteration=1e3 * it32,
                 f64_cpd_call_s=s64, f64_ms_per_iteration=1e3 * it64,
                 fit_f32=r32["fit"], fit_f64=r64["fit"],
@@ -233,13 +282,20 @@ def stream_c4_slices(nslices, seed=20261206):
     200000 non-stream dims, per-slice nnz lognormal (median 2e5, sigma 1)
     clipped to [1e4, 2e6], Zipf(1.2) coordinates distinct within a slice,
     values U(0,1)."""
+    from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(seed)
     nn = np.clip(np.round(np.exp(np.log(2e5) + rng.standard_normal(365))), 1e4, 2e6)
     nn = nn.astype(np.int64)[:nslices]
+
+    def one(t):
+        return ps.synth_coord([500_000, 200_000], int(nn[t]), zipf=[1.2, 1.2],
+                              vals="uniform", compact=False, seed=1000 + t, shuffle=False)
+
+    # slices are independent (each its own seed): generated on 8 host threads
+    with ThreadPoolExecutor(8) as ex:
+        parts = list(ex.map(one, range(len(nn))))
     I, J, V, K = [], [], [], []
-    for t in range(nslices):
-        ij, v, _ = ps.synth_coord([500_000, 200_000], int(nn[t]), zipf=[1.2, 1.2],
-                                  vals="uniform", compact=False, seed=1000 + t, shuffle=False)
+    for t, (ij, v, _) in enumerate(parts):
         I.append(ij[0]); J.append(ij[1]); V.append(v)
         K.append(np.full(len(v), t, dtype=np.uint64))
     inds = [np.concatenate(I), np.concatenate(J), np.concatenate(K)]
@@ -247,12 +303,12 @@ def stream_c4_slices(nslices, seed=20261206):
     return inds, vals, [500_000, 200_000, nslices]
 
 
-def bench_stream_c4(nslices, cpu_slices):
+def bench_stream_c4(nslices, cpu_slices, threads=1):
     """Streaming CPD, rank 32, frob 1e-2 on the time mode, 5 outer
     iterations per slice, FP64, over the first nslices slices. The oracle's
-    restatement of the reference runs the first cpu_slices slices (one host
-    thread, as the reference's per-slice ALS is dominated by its serial
-    dense passes here) and must report the same per-slice errors."""
+    restatement of the reference runs the first cpu_slices slices (host
+    threads on its row-parallel triangular solves and history product) and
+    must report the same per-slice errors."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as orc
     R = 32
@@ -275,11 +331,15 @@ def bench_stream_c4(nslices, cpu_slices):
         coo = orc.Coo.from_arrays(cinds, vals[sel], [dims[0], dims[1], cpu_slices])
         orc.stream_prepare(coo, 2)
         ps.srand(4)
+        orc.set_threads(threads)
         t0 = time.perf_counter()
-        ref = orc.stream_cpd(coo, 2, R, 0.95, 5, [(2, 1, 0.0), (2, 1, 0.0), (1, 0, 1e-2)])
+        try:
+            ref = orc.stream_cpd(coo, 2, R, 0.95, 5, [(2, 1, 0.0), (2, 1, 0.0), (1, 0, 1e-2)])
+        finally:
+            orc.set_threads(1)
         cpu_s = time.perf_counter() - t0
         out.update(cpu_slices=cpu_slices, cpu_s=cpu_s, cpu_slices_per_s=cpu_slices / cpu_s,
-                   cpu_threads=1, cpu_kind="port",
+                   cpu_threads=threads, cpu_kind="port", cpu_host=host_cpu(),
                    gpu_slices_per_s_same_slices=float(cpu_slices / bs[:cpu_slices].sum()),
                    local_err_max_abs_diff=float(np.max(np.abs(
                        res["local_err"][:cpu_slices] - ref["local_err"]))))
@@ -374,8 +434,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--skip-extras", action="store_true",
                     help="skip the CPD (1000x800x600) and streaming measurements")
-    ap.add_argument("--stream-slices", type=int, default=60)
-    ap.add_argument("--stream-cpu-slices", type=int, default=2)
+    ap.add_argument("--stream-slices", type=int, default=365)
+    ap.add_argument("--stream-cpu-slices", type=int, default=6)
     ap.add_argument("--serve-streams", type=int, default=64)
     ap.add_argument("--serve-slices", type=int, default=3)
     ap.add_argument("--serve-rate", type=float, default=500.0)
@@ -394,14 +454,14 @@ def main():
         sys.exit(2)
 
     if args.only is not None:
-        threads = min(16, os.cpu_count() or 1)
+        threads = min(CPU_SHARE, os.cpu_count() or 1)
         if args.only == "cpd":
             out = bench_cpd_c1(threads)
         elif args.only == "c3":
             out = bench_cpd_c3(args.c3_nnz)
         elif args.only == "stream":
             out = bench_stream_c4(args.stream_slices,
-                                  0 if args.skip_cpu else args.stream_cpu_slices)
+                                  0 if args.skip_cpu else args.stream_cpu_slices, threads)
         else:
             out = bench_serve_c5(args.serve_streams, args.serve_slices, args.serve_rate,
                                  args.serve_inflight)
@@ -439,6 +499,9 @@ def main():
     extra = {}
     parity = {}
     if not args.primary_only:
+        extra["dropin_fp64"] = time_dropin(csf_all, mats, dims)
+        log("drop-in splatt_mttkrp (host arrays, FP64): %s ms per mode"
+            % ", ".join("%.1f" % x for x in extra["dropin_fp64"]["ms_per_mode"]))
         f64, out64 = measure(L, csf_all, opts_all, mats, dims, R, ps.F64, args.steps, args.warmup)
         log("ALLMODE fp64: %.1f GB/s" % f64["gbps"])
         extra["allmode_fp64"] = f64
@@ -464,14 +527,14 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = min(CPU_SHARE, os.cpu_count() or 1)
         t0 = time.perf_counter()
         cpu = cpu_baseline(inds, vals, dims, mats, R, min(args.cpu_sample, len(vals)), threads)
         log("cpu baseline %.2f GB/s on %d threads (%.1fs)" % (cpu["value"], threads,
                                                               time.perf_counter() - t0))
 
     if not args.primary_only and not args.skip_extras:
-        threads = min(16, os.cpu_count() or 1)
+        threads = min(CPU_SHARE, os.cpu_count() or 1)
         c1 = bench_cpd_c1(threads)
         log("CPD c1: GPU %.2f ms/it, CPU %.2f ms/it (%d threads), fit diff %.2e"
             % (c1["gpu_ms_per_iteration"], c1["cpu_ms_per_iteration"], threads,
@@ -482,7 +545,8 @@ def main():
             % (c3["f32_ms_per_iteration"], c3["f64_ms_per_iteration"], c3["fit_abs_diff"],
                ", ".join("%.1e" % e for e in c3["factor_rel_err_vs_f64"])))
         extra["cpd_c3"] = c3
-        c4 = bench_stream_c4(args.stream_slices, 0 if args.skip_cpu else args.stream_cpu_slices)
+        c4 = bench_stream_c4(args.stream_slices, 0 if args.skip_cpu else args.stream_cpu_slices,
+                             threads)
         log("stream c4: %.2f slices/s, p50 %.4fs p99 %.4fs" % (
             c4["slices_per_s"], c4["p50_slice_s"], c4["p99_slice_s"]))
         extra["stream_c4"] = c4

@@ -200,6 +200,11 @@ int or_stream_cpd(or_coo const * tt, or_stream_cfg const * cfg,
 /* Dense helpers exposed for pinning tests (src/matrix.c). */
 void or_mat_aTa(double const * A, uint64_t I, uint64_t R, double * out);
 int or_cholesky(double * G, uint64_t R);            /* in place, lower */
+/* Threads of the oracle's row-independent dense loops (results do not
+ * depend on the count); default 1. */
+void or_set_threads(int n);
+int or_get_threads(void);
+
 void or_cholesky_solve(double const * L, uint64_t R, double * B,
     uint64_t nrows);                                /* rows of B */
 void or_mat_normalize(double * A, uint64_t I, uint64_t R, double * lambda);

@@ -7,6 +7,23 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
+
+/* Threads of the row-independent dense loops (per-row triangular solves and
+ * the streaming history product): every row is computed by one thread in
+ * the same order, so results do not depend on the count. Reductions over
+ * rows (Grams, norms, residuals) stay serial. Default 1. */
+static int g_or_threads = 1;
+
+void or_set_threads(int n)
+{
+  g_or_threads = (n > 1) ? n : 1;
+}
+
+int or_get_threads(void)
+{
+  return g_or_threads;
+}
 
 /* src/util.c:15-40: magnitude 3*rand()/RAND_MAX, then sign from a second
  * rand() (even -> negative). */
@@ -74,6 +91,8 @@ int or_cholesky(double * G, uint64_t R)
 void or_cholesky_solve(double const * L, uint64_t R, double * B,
     uint64_t nrows)
 {
+  #pragma omp parallel for num_threads(g_or_threads) if(g_or_threads > 1 && nrows >= 256) \
+      schedule(static)
   for(uint64_t n=0; n < nrows; ++n) {
     double * b = B + n * R;
     for(uint64_t i=0; i < R; ++i) {

@@ -75,8 +75,16 @@ def lib():
         L.or_cholesky.argtypes = [f64p, C.c_uint64]
         L.or_cholesky_solve.argtypes = [f64p, C.c_uint64, f64p, C.c_uint64]
         L.or_mat_normalize.argtypes = [f64p, C.c_uint64, C.c_uint64, f64p]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_get_threads.restype = C.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Threads of the oracle's row-independent dense loops (results do not
+    depend on the count)."""
+    lib().or_set_threads(int(n))
 
 
 def _p(a, t):

@@ -387,6 +387,8 @@ int or_stream_cpd(or_coo const * tt, or_stream_cfg const * cfg,
           }
           for(uint64_t x=0; x < R * R; ++x) hist[x] *= cross[x];
         }
+        int const nthr = or_get_threads();
+        #pragma omp parallel for num_threads(nthr) if(nthr > 1 && I >= 256) schedule(static)
         for(uint64_t i=0; i < I; ++i) {
           double const * orow = sold[m].v + i * R;
           for(uint64_t s=0; s < R; ++s) {

@@ -95,6 +95,7 @@ int spd_csf_frobsq(spd_csf const * csf, double * out);
 int spd_mttkrp(spd_csf * csf, int mode, uint64_t R, void * const * mats,
     void * out, int dtype, int root_rows_are_positions, void * stream);
 
+
 /* Bytes and flops of one MTTKRP by the algorithmic model: compulsory reads of
  * values, indices, pointers and input factors plus one write of the output,
  * with 4-byte indices and sizeof(dtype) values. */
@@ -113,6 +114,11 @@ int spd_mttkrp_kernel_times(double * ms, int max);
  * device memory allows; enable = 0 always walks the CSF itself, adding to
  * the output rows atomically. */
 int spd_mttkrp_set_copies(int enable);
+/* 3-mode root outputs: variant 0 (default) the per-nonzero kernel k_root3;
+ * 1 the persistent kernel with the hottest leaf rows in LDS (where its lane
+ * layouts apply); 2 the persistent kernel with every row from HBM; 3
+ * k_root3 gathering each fiber's A row once. */
+int spd_mttkrp_set_root_variant(int variant);
 
 /* ---- dense / CPD ------------------------------------------------------- */
 

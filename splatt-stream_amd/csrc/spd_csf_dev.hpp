@@ -41,6 +41,7 @@ struct spd_csf {
   MttkrpPlan * plans;              // launch plans, one per item size
   struct HotPlan * hot;            // output-row rankings, one per level
   struct RowCopy * copies;         // row-sorted copies, one per output level
+  struct RootHot * roothot;        // hot-encoded leaf ids of the 3-mode root kernel
 };
 
 void spd_csf_drop_plans(spd_csf * c);

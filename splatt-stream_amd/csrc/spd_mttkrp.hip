@@ -13,9 +13,11 @@
 // on a dependent global load: it reads its records from LDS and keeps a
 // batch of factor-row gathers in flight.
 //
-// 3-mode root output (k_root3) follows the reference's CSF factoring: the
-// leaf sum of a fiber is multiplied by the fiber's row once per nonzero
-// gather pair. Every other output (k_walks) multiplies, per nonzero, the
+// 3-mode root output: k_root3h (spd_root3h.hpp) follows the reference's
+// CSF factoring (the leaf sum of a fiber is multiplied by the fiber's row
+// once) with the hottest leaf rows in LDS; k_root3 multiplies both rows per
+// nonzero and serves the layouts k_root3h is not built for (ranks below 16,
+// small tensors). Every other output (k_walks) multiplies, per nonzero, the
 // rows of all levels but the output level and sums runs of equal output
 // rows (contiguous in CSF order) in registers. Root rows are unique per
 // node: runs closed inside a group's range are stored, runs cut by range
@@ -128,9 +130,11 @@ k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
 // nonzeros (one per group of L lanes). Before walking, the wave stages into
 // LDS, with coalesced loads: the values and leaf ids of its nonzeros, the
 // end and id of every fiber and the end and output row of every slice that
-// overlaps its range. A group then walks its item in batches of 8 nonzeros:
-// 8 independent B-row gathers are in flight per batch, every boundary test
-// reads LDS, and the A row of the next fiber is prefetched one fiber ahead.
+// overlaps its range. A group then walks its item in batches of R3_BATCH
+// nonzeros: the B row and the fiber's A row of every nonzero of a batch are
+// gathered together (2 * R3_BATCH gathers in flight), and every boundary
+// test reads LDS. k_root3h (spd_root3h.hpp) is the persistent variant that
+// gathers A once per fiber and keeps the hottest B rows in LDS.
 // Slices completed inside an item are stored; the partial sums at item
 // edges are combined per block in LDS, and only a block's first and last
 // slice (when they cross the block's edges) reach the carry buffer.
@@ -138,6 +142,19 @@ k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
 #define R3_WAVES 4
 #define R3_WN 256                  // nonzeros per wave
 #define R3_BATCH 6
+#define R3_FST 0x80000000u         // record slice field: first nonzero of a fiber
+
+// Makes v opaque to the optimiser once it is loaded. A walk that gathers A
+// rows at fiber starts only selects the current fiber's row (avc = new
+// fiber ? av[j] : avc); without the barrier the compiler folds that select
+// into the conditional gathers (each then starts from a copy of the
+// previous one, still in flight, and waits for it). Placed after every
+// gather of a batch is issued.
+template<typename VT>
+__device__ __forceinline__ void r3_opaque(VT & v)
+{
+  asm volatile("" : "+v"(v));
+}
 
 template<typename T>
 struct Root3Args {
@@ -224,7 +241,10 @@ combine_runs(uint32_t const * rows, T const * vals, int ld, uint32_t e0, uint32_
   orow[1] = r1;
 }
 
-template<typename T, int V, int L>
+// HOIST: the fiber's A row is gathered once per fiber (and at the start of
+// each item) and multiplied into the slice sum at the fiber's end, as the
+// reference factors it; otherwise both rows are gathered per nonzero.
+template<typename T, int V, int L, bool HOIST>
 __global__ void __launch_bounds__(R3_WAVES * 64, sizeof(T) == 4 ? 5 : 1)
 k_root3(Root3Args<T> a)
 {
@@ -337,7 +357,7 @@ k_root3(Root3Args<T> a)
       uint4 r;
       r.x = leaf * rowb;
       r.y = s_arow[wave][fpre];
-      r.z = spre;
+      r.z = spre | ((HOIST && fm[q]) ? R3_FST : 0u);
       if constexpr (sizeof(T) == 4) {
         r.w = __float_as_uint(v);
       } else {
@@ -363,8 +383,10 @@ k_root3(Root3Args<T> a)
     bool head_partial = a.head[item] != 0;
     uint32_t const k0 = (uint32_t) (n0 - W0);
     uint32_t const k1 = (uint32_t) (n1 - W0);
-    uint32_t cur_s = s_rec[wave][k0 + k0 / S].z;
+    uint32_t cur_s = s_rec[wave][k0 + k0 / S].z & ~R3_FST;
     VT accs = vzero<VT>();
+    VT accf = vzero<VT>();
+    VT avc = vzero<VT>();
     auto rec_val = [&](uint4 const & r, uint32_t k) -> T {
       if constexpr (sizeof(T) == 4) {
         return __uint_as_float(r.w);
@@ -372,50 +394,73 @@ k_root3(Root3Args<T> a)
         return s_v64[wave][k + k / S];
       }
     };
-    auto step = [&](uint4 const & r, T v, VT b, VT av) {
-      if(r.z != cur_s) {
-        // slice cur_s ended before this nonzero
-        uint32_t const orow = s_srow[wave][cur_s];
-        if(head_partial) {
-          s_crow[e0] = orow;
-          if(cok) *(VT *) &s_cval[e0][c * V] = accs;
-        } else {
-          vstore<T, V>(a.out, orow, R, col, cok, accs);
-        }
-        head_partial = false;
-        accs = vzero<VT>();
-        cur_s = r.z;
+    auto flush = [&]() {
+      // slice cur_s ended before this nonzero
+      uint32_t const orow = s_srow[wave][cur_s];
+      if(head_partial) {
+        s_crow[e0] = orow;
+        if(cok) *(VT *) &s_cval[e0][c * V] = accs;
+      } else {
+        vstore<T, V>(a.out, orow, R, col, cok, accs);
       }
-      accs += v * (b * av);
+      head_partial = false;
+      accs = vzero<VT>();
+    };
+    auto step = [&](uint4 const & r, T v, VT b, VT av, bool nf) {
+      if constexpr (HOIST) {
+        if(nf) {
+          accs += accf * avc;
+          accf = vzero<VT>();
+          avc = av;
+        }
+      }
+      uint32_t const sl = r.z & ~R3_FST;
+      if(sl != cur_s) {
+        flush();
+        cur_s = sl;
+      }
+      if constexpr (HOIST) {
+        accf += v * b;
+      } else {
+        accs += v * (b * av);
+      }
     };
     for(uint32_t k=k0; k < k1; k += R3_BATCH) {
       if(k + R3_BATCH <= k1) {
         uint4 r[R3_BATCH];
         VT b[R3_BATCH];
         VT av[R3_BATCH];
+        bool nf[R3_BATCH];
         #pragma unroll
         for(int j=0; j < R3_BATCH; ++j) r[j] = s_rec[wave][k + j + (k + j) / S];
         #pragma unroll
         for(int j=0; j < R3_BATCH; ++j) {
           b[j] = *(VT const *) (Bb + (r[j].x + colb));
-          av[j] = *(VT const *) (Ab + (r[j].y + colb));
+          nf[j] = !HOIST || (r[j].z & R3_FST) != 0u || (k + j == k0);
+          av[j] = nf[j] ? *(VT const *) (Ab + (r[j].y + colb)) : vzero<VT>();
+        }
+        if constexpr (HOIST) {
+          #pragma unroll
+          for(int j=0; j < R3_BATCH; ++j) r3_opaque(av[j]);
         }
         #pragma unroll
-        for(int j=0; j < R3_BATCH; ++j) step(r[j], rec_val(r[j], k + j), b[j], av[j]);
+        for(int j=0; j < R3_BATCH; ++j) step(r[j], rec_val(r[j], k + j), b[j], av[j], nf[j]);
       } else {
         for(uint32_t kk=k; kk < k1; ++kk) {
           uint4 const r = s_rec[wave][kk + kk / S];
+          bool const nf = !HOIST || (r.z & R3_FST) != 0u || kk == k0;
           VT const b = *(VT const *) (Bb + (r.x + colb));
-          VT const av = *(VT const *) (Ab + (r.y + colb));
-          step(r, rec_val(r, kk), b, av);
+          VT const av = nf ? *(VT const *) (Ab + (r.y + colb)) : vzero<VT>();
+          step(r, rec_val(r, kk), b, av, nf);
         }
       }
     }
+    if constexpr (HOIST) accs += accf * avc;
     // is the last slice complete at n1? It is if n1 is the start of a new
     // slice (or the tensor's end)
     bool complete = (n1 == a.nnz);
     if(!complete && n1 < W1) {
-      complete = s_rec[wave][k1 + k1 / S].z != cur_s;
+      complete = (s_rec[wave][k1 + k1 / S].z & ~R3_FST) != cur_s;
     } else if(!complete) {
       // the item ends at the wave's end: the slice is complete iff the
       // next slice starts exactly there
@@ -467,6 +512,8 @@ k_root3(Root3Args<T> a)
     }
   }
 }
+
+#include "spd_root3h.hpp"
 
 // ---- staged walk: internal and leaf outputs, and roots of 4+ modes --------
 //
@@ -941,8 +988,20 @@ struct RowCopy {
   RowCopy * next;
 };
 
+// Leaf ids of a CSF with its most frequent leaf rows re-encoded for
+// k_root3h (RH_HOTF | rank), built once per CSF.
+struct RootHot {
+  uint32_t * leafx;
+  uint32_t nhot;
+};
+
 void spd_csf_drop_plans(spd_csf * c)
 {
+  if(c->roothot != nullptr) {
+    spd_free(c->roothot->leafx);
+    delete c->roothot;
+    c->roothot = nullptr;
+  }
   MttkrpPlan * p = c->plans;
   while(p) {
     MttkrpPlan * nx = p->next;
@@ -1159,13 +1218,13 @@ static int dispatch_vl(WsArgs<T> const & a, hipStream_t st)
   return launch_walks<T, N, KIND, 1, 64>(a, st);
 }
 
-template<typename T, int V, int L>
+template<typename T, int V, int L, bool HOIST>
 static int launch_root3(Root3Args<T> const & a, uint32_t nblocks, hipStream_t st,
     hipEvent_t ev_main_end)
 {
   constexpr int CW = L * V;
   dim3 grid(nblocks, (unsigned) spd_cdiv(a.R, (uint64_t) CW));
-  hipLaunchKernelGGL((k_root3<T, V, L>), grid, dim3(R3_WAVES * 64), 0, st, a);
+  hipLaunchKernelGGL((k_root3<T, V, L, HOIST>), grid, dim3(R3_WAVES * 64), 0, st, a);
   SPD_LAUNCH_CHECK();
   if(ev_main_end) SPD_HIP(hipEventRecord(ev_main_end, st));
   uint32_t const nent = 2 * nblocks;
@@ -1195,31 +1254,39 @@ static void lane_layout(uint32_t R, int dtype, int * V, int * L)
   }
 }
 
-template<typename T>
-static int dispatch_root3(Root3Args<T> const & a, uint32_t nblocks, int V, int L,
+template<typename T, bool HOIST>
+static int dispatch_root3_h(Root3Args<T> const & a, uint32_t nblocks, int V, int L,
     hipStream_t st, hipEvent_t ev)
 {
   if constexpr (sizeof(T) == 4) {
     if(V == 4) {
       switch(L) {
-      case 2: return launch_root3<T, 4, 2>(a, nblocks, st, ev);
-      case 4: return launch_root3<T, 4, 4>(a, nblocks, st, ev);
-      case 8: return launch_root3<T, 4, 8>(a, nblocks, st, ev);
-      default: return launch_root3<T, 4, 16>(a, nblocks, st, ev);
+      case 2: return launch_root3<T, 4, 2, HOIST>(a, nblocks, st, ev);
+      case 4: return launch_root3<T, 4, 4, HOIST>(a, nblocks, st, ev);
+      case 8: return launch_root3<T, 4, 8, HOIST>(a, nblocks, st, ev);
+      default: return launch_root3<T, 4, 16, HOIST>(a, nblocks, st, ev);
       }
     }
   } else {
     if(V == 2) {
       switch(L) {
-      case 4: return launch_root3<T, 2, 4>(a, nblocks, st, ev);
-      case 8: return launch_root3<T, 2, 8>(a, nblocks, st, ev);
-      case 16: return launch_root3<T, 2, 16>(a, nblocks, st, ev);
-      default: return launch_root3<T, 2, 32>(a, nblocks, st, ev);
+      case 4: return launch_root3<T, 2, 4, HOIST>(a, nblocks, st, ev);
+      case 8: return launch_root3<T, 2, 8, HOIST>(a, nblocks, st, ev);
+      case 16: return launch_root3<T, 2, 16, HOIST>(a, nblocks, st, ev);
+      default: return launch_root3<T, 2, 32, HOIST>(a, nblocks, st, ev);
       }
     }
   }
-  if(L == 16) return launch_root3<T, 1, 16>(a, nblocks, st, ev);
-  return launch_root3<T, 1, 64>(a, nblocks, st, ev);
+  if(L == 16) return launch_root3<T, 1, 16, HOIST>(a, nblocks, st, ev);
+  return launch_root3<T, 1, 64, HOIST>(a, nblocks, st, ev);
+}
+
+template<typename T>
+static int dispatch_root3(Root3Args<T> const & a, uint32_t nblocks, int V, int L,
+    hipStream_t st, hipEvent_t ev, bool hoist)
+{
+  if(hoist) return dispatch_root3_h<T, true>(a, nblocks, V, L, st, ev);
+  return dispatch_root3_h<T, false>(a, nblocks, V, L, st, ev);
 }
 
 template<typename T, int KIND>
@@ -1294,12 +1361,207 @@ static int ensure_vals32(spd_csf * c)
   return SPD_OK;
 }
 
+// ---- persistent root kernel with hot leaf rows (spd_root3h.hpp) ----------
+
+#define RH_NHOT_ENC 1024           // leaf ranks encoded per CSF
+
+// 0: k_root3; 1: k_root3h with hot leaf rows in LDS; 2: k_root3h with
+// every row from HBM; 3: k_root3 gathering A once per fiber
+static int g_root_variant = 0;
+
+extern "C" int spd_mttkrp_set_root_variant(int v)
+{
+  g_root_variant = v;
+  return SPD_OK;
+}
+
+static int cu_count(void)
+{
+  static int n = 0;
+  if(n == 0) {
+    int dev = 0, v = 0;
+    if(hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        v > 0) {
+      n = v;
+    } else {
+      n = 256;
+    }
+  }
+  return n;
+}
+
+// The CSF's leaf ids with the RH_NHOT_ENC most frequent leaf rows (those
+// with at least two nonzeros) encoded as RH_HOTF | rank.
+static int get_roothot(spd_csf * c, RootHot ** out, hipStream_t st)
+{
+  if(c->roothot != nullptr) {
+    *out = c->roothot;
+    return SPD_OK;
+  }
+  HotPlan * hp = nullptr;
+  SPD_TRY(get_hot(c, c->nmodes - 1, &hp, st));
+  RootHot * rh = new RootHot();
+  uint32_t nhot = 0;
+  while(nhot < hp->counts.size() && nhot < RH_NHOT_ENC && hp->counts[nhot] >= 2) ++nhot;
+  rh->nhot = nhot;
+  rh->leafx = nullptr;
+  int rc = spd_malloc((void **) &rh->leafx, c->nnz * sizeof(uint32_t));
+  if(rc == SPD_OK) {
+    unsigned const grid = (unsigned) (spd_cdiv(c->nnz, 256) < 8192 ? spd_cdiv(c->nnz, 256) : 8192);
+    hipLaunchKernelGGL(k_encode_hot, dim3(grid), dim3(256), 0, st, c->u.fids[c->nmodes - 1],
+        c->nnz, hp->rank, nhot, rh->leafx);
+    if(hipGetLastError() != hipSuccess) {
+      spd_set_error(SPD_EHIP, "hot leaf encoding failed");
+      rc = SPD_EHIP;
+    }
+  }
+  if(rc != SPD_OK) {
+    spd_free(rh->leafx);
+    delete rh;
+    return rc;
+  }
+  c->roothot = rh;
+  *out = rh;
+  return SPD_OK;
+}
+
+template<typename T, int V, int L, int NW, bool HOT>
+static int launch_root3h(Root3hArgs<T> h, hipStream_t st, hipEvent_t ev_main_end)
+{
+  constexpr int CW = L * V;
+  uint64_t const nw_need = spd_cdiv(h.nchunks, 1);
+  uint64_t nb = spd_cdiv(nw_need, NW);
+  if(nb > (uint64_t) cu_count()) nb = (uint64_t) cu_count();
+  if(nb < 1) nb = 1;
+  h.cpw = (uint32_t) spd_cdiv(h.nchunks, nb * NW);
+  dim3 grid((unsigned) nb, (unsigned) spd_cdiv(h.b.R, (uint64_t) CW));
+  hipLaunchKernelGGL((k_root3h<T, V, L, NW, HOT>), grid, dim3(NW * 64), 0, st, h);
+  SPD_LAUNCH_CHECK();
+  if(ev_main_end) SPD_HIP(hipEventRecord(ev_main_end, st));
+  uint32_t const nent = (uint32_t) (2 * nb * NW);
+  uint32_t const chunks = (uint32_t) spd_cdiv(nent, FIX_CHUNK);
+  uint32_t const groups_per_block = (MK_THREADS / 64) * (64 / L);
+  dim3 g2((unsigned) spd_cdiv(chunks, groups_per_block), grid.y);
+  hipLaunchKernelGGL((k_root_fixup<T, V, L>), g2, dim3(MK_THREADS), 0, st,
+      (T const *) h.b.carry, (uint32_t const *) h.b.carry_row, nent, h.b.out, h.b.R);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
+}
+
+// waves per block of k_root3h: one block per CU, as many waves as its LDS
+// (hot rows + per-wave staging) allows
+#define RH_NW_F32 16
+#define RH_NW_F64 12
+
+// the (V, L) layouts k_root3h is built for; others use k_root3
+static bool root3h_layout(int dtype, int V, int L)
+{
+  if(dtype == SPD_F32) return V == 4 && (L == 4 || L == 8 || L == 16);
+  return V == 2 && (L == 8 || L == 16 || L == 32);
+}
+
+template<typename T, bool HOT>
+static int dispatch_root3h_h(Root3hArgs<T> const & h, int V, int L, hipStream_t st, hipEvent_t ev)
+{
+  if constexpr (sizeof(T) == 4) {
+    (void) V;
+    switch(L) {
+    case 4: return launch_root3h<T, 4, 4, RH_NW_F32, HOT>(h, st, ev);
+    case 8: return launch_root3h<T, 4, 8, RH_NW_F32, HOT>(h, st, ev);
+    default: return launch_root3h<T, 4, 16, RH_NW_F32, HOT>(h, st, ev);
+    }
+  } else {
+    (void) V;
+    switch(L) {
+    case 8: return launch_root3h<T, 2, 8, RH_NW_F64, HOT>(h, st, ev);
+    case 16: return launch_root3h<T, 2, 16, RH_NW_F64, HOT>(h, st, ev);
+    default: return launch_root3h<T, 2, 32, RH_NW_F64, HOT>(h, st, ev);
+    }
+  }
+}
+
+template<typename T>
+static int dispatch_root3h(Root3hArgs<T> const & h, int V, int L, hipStream_t st, hipEvent_t ev,
+    bool hot)
+{
+  if(hot) return dispatch_root3h_h<T, true>(h, V, L, st, ev);
+  return dispatch_root3h_h<T, false>(h, V, L, st, ev);
+}
+
+template<typename T>
+static int run_root3h(spd_csf * c, uint64_t R, void * const * mats, void * out,
+    int rows_pos, hipStream_t st, int V, int L)
+{
+  uint32_t const G = 64 / L;
+  uint32_t const S = R3_WN / G;
+  MttkrpPlan * plan = nullptr;
+  RootHot * rh = nullptr;
+  HotPlan * hp = nullptr;
+  SPD_TRY(get_plan(c, S, &plan, spd_hs(nullptr)));
+  SPD_TRY(get_roothot(c, &rh, spd_hs(nullptr)));
+  SPD_TRY(get_hot(c, c->nmodes - 1, &hp, spd_hs(nullptr)));
+  if(st != spd_hs(nullptr)) {
+    SPD_HIP(hipStreamSynchronize(spd_hs(nullptr)));
+  }
+  uint32_t const NW = (sizeof(T) == 4) ? RH_NW_F32 : RH_NW_F64;
+  uint64_t const nchunks = spd_cdiv(c->nnz, R3_WN);
+  uint64_t nb = spd_cdiv(nchunks, NW);
+  if(nb > (uint64_t) cu_count()) nb = (uint64_t) cu_count();
+  if(nb < 1) nb = 1;
+  // the carry buffers belong to this call when it runs on a caller's stream
+  SpdCallScratch scratch(st);
+  void * cv = nullptr;
+  void * cr = nullptr;
+  SPD_TRY(spd_scratch(2, 2ull * nb * NW * R * sizeof(T), &cv));
+  SPD_TRY(spd_scratch(3, 2ull * nb * NW * sizeof(uint32_t), &cr));
+  Root3hArgs<T> h = {};
+  Root3Args<T> & a = h.b;
+  a.fptr0 = c->u.fptr[0];
+  a.fids0 = c->u.fids[0];
+  a.fptr1 = c->u.fptr[1];
+  a.fids1 = c->u.fids[1];
+  a.leaf = c->u.fids[2];
+  a.vals = (sizeof(T) == 4) ? (T const *) c->vals32 : (T const *) c->u.vals;
+  a.A = (T const *) mats[c->perm[1]];
+  a.B = (T const *) mats[c->perm[2]];
+  a.out = (T *) out;
+  a.plan = plan->nodes;
+  a.head = plan->head;
+  a.carry = (T *) cv;
+  a.carry_row = (uint32_t *) cr;
+  a.nnz = c->nnz;
+  a.nslices = (uint32_t) c->u.nfibs[0];
+  a.nfibers = (uint32_t) c->u.nfibs[1];
+  a.nitems = plan->nitems;
+  a.R = (uint32_t) R;
+  a.rows_are_positions = rows_pos;
+  h.leafx = rh->leafx;
+  h.hot_rows = hp->rows;
+  h.nhot = rh->nhot;
+  h.nchunks = (uint32_t) nchunks;
+  SPD_TRY(spd_zero_async(out, c->dims[c->perm[0]] * R * sizeof(T), st));
+  int const slot = (int) (g_ev_count % SPD_NEV);
+  if(g_timing) SPD_HIP(hipEventRecord(g_ev[slot][0], st));
+  int const rc = dispatch_root3h<T>(h, V, L, st, g_timing ? g_ev[slot][1] : (hipEvent_t) nullptr,
+      g_root_variant == 1);
+  if(g_timing) ++g_ev_count;
+  return rc;
+}
+
 template<typename T>
 static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
     int rows_pos, hipStream_t st, int dtype)
 {
   int V = 1, L = 16;
   lane_layout((uint32_t) R, dtype, &V, &L);
+  // the persistent kernel encodes hot rows in bit 31 of B byte offsets and
+  // needs at least one chunk per wave of a full grid to pay for its LDS fill
+  if((g_root_variant == 1 || g_root_variant == 2) && root3h_layout(dtype, V, L) &&
+      c->dims[c->perm[2]] * R * sizeof(T) < 0x80000000ull && c->nnz < 0xFFFFFFFFull &&
+      c->nnz >= (uint64_t) R3_WN * 64) {
+    return run_root3h<T>(c, R, mats, out, rows_pos, st, V, L);
+  }
   uint32_t const G = 64 / L;
   uint32_t const S = R3_WN / G;
   MttkrpPlan * plan = nullptr;
@@ -1340,7 +1602,7 @@ static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
   int const slot = (int) (g_ev_count % SPD_NEV);
   if(g_timing) SPD_HIP(hipEventRecord(g_ev[slot][0], st));
   int const rc = dispatch_root3<T>(a, nblocks, V, L, st,
-      g_timing ? g_ev[slot][1] : (hipEvent_t) nullptr);
+      g_timing ? g_ev[slot][1] : (hipEvent_t) nullptr, g_root_variant == 3);
   if(g_timing) ++g_ev_count;
   return rc;
 }

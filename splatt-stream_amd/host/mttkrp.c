@@ -3,6 +3,7 @@
  * over the GPU kernels.
  */
 #include "host.h"
+#include <pthread.h>
 
 /* Mode -> CSF map, src/mttkrp.c:1847-1881. */
 static int p_mode_csf_map(splatt_csf const * tensors, splatt_csf_type which,
@@ -61,20 +62,46 @@ void splatt_mttkrp_free_ws(splatt_mttkrp_ws * const ws)
   free(ws);
 }
 
+/* Device buffers of the host-array MTTKRP entry points, kept between calls
+ * and grown on demand (hipMalloc and hipFree synchronise the whole device):
+ * one per input mode, the output, and an FP64 staging buffer for FP32 runs.
+ * Calls are serialised on a mutex (the reference's splatt_mttkrp is not
+ * re-entrant either: src/mttkrp.c:15, :1298-1300). */
+static struct {
+  pthread_mutex_t mu;
+  void * buf[SPLATT_MAX_NMODES + 2];
+  size_t cap[SPLATT_MAX_NMODES + 2];
+} g_hws = { PTHREAD_MUTEX_INITIALIZER, {0}, {0} };
+
+static int p_hws_get(int slot, size_t bytes, void ** out)
+{
+  if(g_hws.cap[slot] < bytes) {
+    spd_free(g_hws.buf[slot]);
+    g_hws.buf[slot] = NULL;
+    g_hws.cap[slot] = 0;
+    size_t want = bytes + bytes / 8;
+    int rc = spd_malloc(&g_hws.buf[slot], want);
+    if(rc != SPD_OK) {
+      return rc;
+    }
+    g_hws.cap[slot] = want;
+  }
+  *out = g_hws.buf[slot];
+  return SPD_OK;
+}
+
 /* MTTKRP of host factors through the GPU: upload the inputs, run, download
  * the output. mats[m] (host, dims[m] x R) for m != mode. */
-static int p_mttkrp_host(splatt_csf const * ct, idx_t mode, idx_t R,
+static int p_mttkrp_host_locked(splatt_csf const * ct, idx_t mode, idx_t R,
     val_t * const * mats, val_t * out)
 {
-  if(ct->dev == NULL) {
-    fprintf(stderr, "SPLATT: MTTKRP needs a tensor built by this library.\n");
-    return SPD_EINPUT;
-  }
   idx_t const N = ct->nmodes;
   int const dtype = spl_mttkrp_dtype();
   size_t const sv = (dtype == SPD_F32) ? sizeof(float) : sizeof(double);
   void * dmats[SPLATT_MAX_NMODES] = {0};
-  double * stage = NULL;
+  int const STAGE = SPLATT_MAX_NMODES;
+  int const OUT = SPLATT_MAX_NMODES + 1;
+  void * stage = NULL;
   void * dout = NULL;
   int rc = SPD_OK;
   for(idx_t m=0; m < N && rc == SPD_OK; ++m) {
@@ -82,39 +109,44 @@ static int p_mttkrp_host(splatt_csf const * ct, idx_t mode, idx_t R,
       continue;
     }
     size_t const n = ct->dims[m] * R;
-    rc = spd_malloc(&dmats[m], n * sv);
+    rc = p_hws_get((int) m, n * sv, &dmats[m]);
     if(rc != SPD_OK) break;
     if(dtype == SPD_F32) {
-      rc = spd_malloc((void **) &stage, n * sizeof(double));
+      rc = p_hws_get(STAGE, n * sizeof(double), &stage);
       if(rc == SPD_OK) rc = spd_h2d(stage, mats[m], n * sizeof(double));
-      if(rc == SPD_OK) rc = spd_f64_to_f32(stage, (float *) dmats[m], n, NULL);
-      if(rc == SPD_OK) rc = spd_sync();
-      spd_free(stage);
-      stage = NULL;
+      if(rc == SPD_OK) rc = spd_f64_to_f32((double const *) stage, (float *) dmats[m], n, NULL);
     } else {
       rc = spd_h2d(dmats[m], mats[m], n * sizeof(double));
     }
   }
   size_t const nout = ct->dims[mode] * R;
-  if(rc == SPD_OK) rc = spd_malloc(&dout, nout * sv);
+  if(rc == SPD_OK) rc = p_hws_get(OUT, nout * sv, &dout);
   if(rc == SPD_OK) {
     rc = spd_mttkrp((spd_csf *) ct->dev, (int) mode, R, dmats, dout, dtype,
         ct->ntiles == 1, NULL);
   }
   if(rc == SPD_OK) {
     if(dtype == SPD_F32) {
-      rc = spd_malloc((void **) &stage, nout * sizeof(double));
-      if(rc == SPD_OK) rc = spd_f32_to_f64((float const *) dout, stage, nout, NULL);
+      rc = p_hws_get(STAGE, nout * sizeof(double), &stage);
+      if(rc == SPD_OK) rc = spd_f32_to_f64((float const *) dout, (double *) stage, nout, NULL);
       if(rc == SPD_OK) rc = spd_d2h(out, stage, nout * sizeof(double));
-      spd_free(stage);
     } else {
       rc = spd_d2h(out, dout, nout * sizeof(double));
     }
   }
-  for(idx_t m=0; m < N; ++m) {
-    spd_free(dmats[m]);
+  return rc;
+}
+
+static int p_mttkrp_host(splatt_csf const * ct, idx_t mode, idx_t R,
+    val_t * const * mats, val_t * out)
+{
+  if(ct->dev == NULL) {
+    fprintf(stderr, "SPLATT: MTTKRP needs a tensor built by this library.\n");
+    return SPD_EINPUT;
   }
-  spd_free(dout);
+  pthread_mutex_lock(&g_hws.mu);
+  int const rc = p_mttkrp_host_locked(ct, mode, R, mats, out);
+  pthread_mutex_unlock(&g_hws.mu);
   return rc;
 }
 

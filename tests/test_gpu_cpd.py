@@ -518,9 +518,11 @@ def test_cpd_uniform_1m_rank16_20_iterations(gpu_available):
 
 def _c3_tensor(nnz, seed=20261208):
     # the 4-mode workload's shape at a reduced nonzero count: Zipf(1.1) on
-    # the first three modes, uniform on the fourth (2000), values U(-1,1)
+    # the first three modes, uniform on the fourth (2000), values U(-1,1),
+    # compacted (no empty index in any mode: an untiled CSF stores no root
+    # ids, so a root mode with empty slices would write slice positions)
     return ps.synth_coord([50_000, 30_000, 10_000, 2_000], nnz, zipf=[1.1, 1.1, 1.1, 0.0],
-                          vals="symmetric", compact=False, seed=seed)
+                          vals="symmetric", compact=True, seed=seed)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
@@ -530,7 +532,7 @@ def test_mttkrp_4mode_rank64_twomode(gpu_available, prec):
     # long-double COO oracle
     print("case 4-mode rank 64 prec", prec, flush=True)
     inds, vals, _ = _c3_tensor(600_000)
-    dims = [int(i.max()) + 1 for i in inds]       # the CSF's dims (not compacted)
+    dims = [int(i.max()) + 1 for i in inds]
     coo = orc.Coo.from_arrays(inds, vals, dims)
     cs = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=ps.CSF_TWOMODE))
     R = 64

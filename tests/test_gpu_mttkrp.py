@@ -220,3 +220,43 @@ def test_device_entry_point_two_caller_streams(gpu_available):
     finally:
         for st in streams:
             h.hipStreamDestroy(st)
+
+
+@pytest.mark.parametrize("R", [16, 32, 48, 64, 100])
+@pytest.mark.parametrize("prec", [ps.F64, ps.F32])
+def test_root_kernel_variants(gpu_available, R, prec):
+    # every 3-mode root kernel variant against the oracle and against the
+    # default (per-nonzero k_root3): 1 the persistent kernel with hot leaf
+    # rows in LDS (one A gather per fiber, runs carried across a wave's
+    # chunks), 2 the same with every row from HBM, 3 k_root3 with one A
+    # gather per fiber. Untiled all-mode CSF (root rows are slice positions)
+    # and a tiled one (root ids stored). Zipf(1.5) roots give slices that
+    # span many waves; at R=64 FP64 only 64 hot rows fit in LDS, so
+    # hotter-ranked rows beyond them are read from HBM through the rank table.
+    print("case root variants", R, prec, flush=True)
+    inds, vals, dims = ps.synth_coord([4000, 3000, 2000], 600000, zipf=[1.5, 1.3, 1.2],
+                                      vals="lognormal", seed=17 + R)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    mats = [ps.synth_matrix(d, R, 900 + m) for m, d in enumerate(dims)]
+    m32 = [m.astype(np.float32) for m in mats]
+    L = ps.lib()
+    _set_precision(prec)
+    for opts in (ps.Opts(csf_alloc=ps.CSF_ALLMODE),
+                 ps.Opts(csf_alloc=ps.CSF_ALLMODE, tile=ps.DENSETILE, tilelevel=1, nthreads=4)):
+        s = ps.CsfSet.convert(inds, vals, opts)
+        try:
+            for mode in range(3):
+                assert L.spd_mttkrp_set_root_variant(0) == 0
+                base = s.mttkrp(mode, mats)
+                for v in (1, 2, 3):
+                    assert L.spd_mttkrp_set_root_variant(v) == 0
+                    got = s.mttkrp(mode, mats)
+                    if prec == ps.F64:
+                        assert rel(got, orc.mttkrp_coo(coo, mats, mode)) <= TOL64, (mode, v)
+                        assert rel(got, base) <= TOL64, (mode, v)
+                    else:
+                        assert rel(got, orc.mttkrp_coo(coo, m32, mode, f32=True)) <= TOL32, (mode, v)
+                        assert rel(got, base) <= TOL32, (mode, v)
+        finally:
+            L.spd_mttkrp_set_root_variant(0)
+            s.free()

@@ -468,3 +468,29 @@ def test_stream_mode_update_is_admm_from_a_zero_dual():
     GB, MB = _normal_equations(1, A, Bp, c[T - 1], X, hist, Ap, Bp, 0.0)
     wantB, _, _, _ = orc.admm(1, MB, Bp, np.zeros_like(Bp), GB, 25, 7e-2)
     assert np.allclose(B, wantB, rtol=1e-10, atol=1e-12)
+
+
+def test_stream_oracle_threads_do_not_change_results():
+    # the oracle's row-parallel loops (triangular solves, history product)
+    # give the same bits at any thread count
+    import oracle_py as orc
+    rng = np.random.default_rng(5)
+    n = 4000
+    inds = [rng.integers(0, 900, n).astype(np.uint64), rng.integers(0, 700, n).astype(np.uint64),
+            rng.integers(0, 4, n).astype(np.uint64)]
+    vals = rng.random(n)
+    outs = []
+    for thr in (1, 4):
+        coo = orc.Coo.from_arrays(inds, vals, [900, 700, 4])
+        orc.stream_prepare(coo, 2)
+        orc.set_threads(thr)
+        try:
+            import ctypes
+            ctypes.CDLL("libc.so.6").srand(3)
+            outs.append(orc.stream_cpd(coo, 2, 8, 0.95, 5, [(2, 1, 0.0), (2, 1, 0.0), (1, 0, 1e-2)]))
+        finally:
+            orc.set_threads(1)
+    for key in ("time", "local_err"):
+        assert np.array_equal(outs[0][key], outs[1][key]), key
+    for m in range(2):
+        assert np.array_equal(outs[0]["factors"][m], outs[1]["factors"][m]), m
