@@ -55,6 +55,33 @@ splatt_error_type splatt_amd_mttkrp_dev(
     double const * const options,
     void * stream);
 
+/* splatt_amd_mttkrp_dev that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and result as splatt_amd_mttkrp_dev. Its partial-row
+ * buffers are one fixed device array of splatt_amd_mttkrp_noalloc_bytes()
+ * (16 MiB) per GPU, held until the process ends and shared by every call on
+ * that GPU: calls that overlap on one GPU (other host threads, or streams
+ * without an order between them) race and must not be issued. The CSF must
+ * already carry what splatt_amd_mttkrp_dev builds on its first call for the
+ * same mode, rank and precision (launch plans, hot-row tables, row-sorted
+ * copies, the FP32 values): make that call once beforehand, outside the
+ * allocation-free region. Returns SPLATT_ERROR_BADINPUT (with the reason in
+ * splatt_amd_last_error) when something is missing or when the output's
+ * partial rows exceed the array; every 3-mode root output on the
+ * persistent kernel fits. */
+splatt_error_type splatt_amd_mttkrp_dev_noalloc(
+    splatt_idx_t const mode,
+    splatt_idx_t const ncolumns,
+    splatt_csf const * const tensors,
+    void ** d_mats,
+    void * d_out,
+    int precision,
+    double const * const options,
+    void * stream);
+
+size_t splatt_amd_mttkrp_noalloc_bytes(void);
+
 /* Algorithmic bytes and flops of one MTTKRP of `mode` on the tensor the
  * options' CSF map selects (4-byte indices, value size of `precision`). */
 splatt_error_type splatt_amd_mttkrp_cost(

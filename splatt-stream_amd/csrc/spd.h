@@ -95,6 +95,15 @@ int spd_csf_frobsq(spd_csf const * csf, double * out);
 int spd_mttkrp(spd_csf * csf, int mode, uint64_t R, void * const * mats,
     void * out, int dtype, int root_rows_are_positions, void * stream);
 
+/* spd_mttkrp without any allocation: partial rows in a fixed device array of
+ * spd_mttkrp_noalloc_bytes() bytes per GPU, shared by every call on that GPU
+ * (overlapping calls race). The CSF must already hold what spd_mttkrp builds
+ * on first use for this mode, rank and precision (plans, hot-row tables,
+ * row-sorted copies, the FP32 values); outputs whose partial rows exceed the
+ * array are rejected (SPD_EINPUT). */
+int spd_mttkrp_noalloc(spd_csf * csf, int mode, uint64_t R, void * const * mats,
+    void * out, int dtype, int rows_pos, void * stream);
+size_t spd_mttkrp_noalloc_bytes(void);
 
 /* Bytes and flops of one MTTKRP by the algorithmic model: compulsory reads of
  * values, indices, pointers and input factors plus one write of the output,

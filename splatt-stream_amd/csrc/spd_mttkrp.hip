@@ -30,6 +30,7 @@
 #include "spd_csf_dev.hpp"
 #include <stdlib.h>
 #include <vector>
+#include <mutex>
 
 #define MK_THREADS 256
 #define KIND_ROOT 0
@@ -1361,6 +1362,59 @@ static int ensure_vals32(spd_csf * c)
   return SPD_OK;
 }
 
+// ---- fixed scratch (the allocation-free entry point) ----------------------
+
+// A caller-side scratch region: when a run_* function gets one, it builds
+// nothing (plans, hot-row tables and row-sorted copies must exist) and
+// carves its partial-row buffers from the region instead of allocating.
+struct MkFixed {
+  char * p;
+  size_t cap;
+};
+
+static MttkrpPlan * find_plan(spd_csf * c, uint32_t S)
+{
+  for(MttkrpPlan * p = c->plans; p; p = p->next) if(p->S == S) return p;
+  return nullptr;
+}
+
+static HotPlan * find_hot(spd_csf * c, int level)
+{
+  for(HotPlan * h = c->hot; h; h = h->next) if(h->level == level) return h;
+  return nullptr;
+}
+
+static RowCopy * find_copy(spd_csf * c, int level, int rows_pos)
+{
+  for(RowCopy * r = c->copies; r; r = r->next) {
+    if(r->level == level && r->rows_pos == rows_pos) return r;
+  }
+  return nullptr;
+}
+
+static int not_prepared(char const * what)
+{
+  spd_set_error(SPD_EINPUT, "allocation-free MTTKRP: the CSF has no %s for this mode, rank "
+      "and precision yet (one splatt_amd_mttkrp_dev call with the same arguments builds it)",
+      what);
+  return SPD_EINPUT;
+}
+
+// partial rows (cv_bytes) and their row ids (cr_bytes) from the region
+static int fixed_carve(MkFixed const * fx, size_t cv_bytes, size_t cr_bytes, void ** cv,
+    void ** cr)
+{
+  size_t const a = (cv_bytes + 255) & ~(size_t) 255;
+  if(a + cr_bytes > fx->cap) {
+    spd_set_error(SPD_EINPUT, "allocation-free MTTKRP: its partial rows need %zu bytes, "
+        "above the %zu bytes of its scratch", a + cr_bytes, fx->cap);
+    return SPD_EINPUT;
+  }
+  *cv = fx->p;
+  *cr = fx->p + a;
+  return SPD_OK;
+}
+
 // ---- persistent root kernel with hot leaf rows (spd_root3h.hpp) ----------
 
 #define RH_NHOT_ENC 1024           // leaf ranks encoded per CSF
@@ -1491,16 +1545,25 @@ static int dispatch_root3h(Root3hArgs<T> const & h, int V, int L, hipStream_t st
 
 template<typename T>
 static int run_root3h(spd_csf * c, uint64_t R, void * const * mats, void * out,
-    int rows_pos, hipStream_t st, int V, int L)
+    int rows_pos, hipStream_t st, int V, int L, MkFixed const * fx)
 {
   uint32_t const G = 64 / L;
   uint32_t const S = R3_WN / G;
   MttkrpPlan * plan = nullptr;
   RootHot * rh = nullptr;
   HotPlan * hp = nullptr;
-  SPD_TRY(get_plan(c, S, &plan, spd_hs(nullptr)));
-  SPD_TRY(get_roothot(c, &rh, spd_hs(nullptr)));
-  SPD_TRY(get_hot(c, c->nmodes - 1, &hp, spd_hs(nullptr)));
+  if(fx == nullptr) {
+    SPD_TRY(get_plan(c, S, &plan, spd_hs(nullptr)));
+    SPD_TRY(get_roothot(c, &rh, spd_hs(nullptr)));
+    SPD_TRY(get_hot(c, c->nmodes - 1, &hp, spd_hs(nullptr)));
+  } else {
+    plan = find_plan(c, S);
+    rh = c->roothot;
+    hp = find_hot(c, c->nmodes - 1);
+    if(plan == nullptr || rh == nullptr || hp == nullptr) {
+      return not_prepared("launch plan or hot leaf-row table");
+    }
+  }
   if(st != spd_hs(nullptr)) {
     SPD_HIP(hipStreamSynchronize(spd_hs(nullptr)));
   }
@@ -1510,11 +1573,16 @@ static int run_root3h(spd_csf * c, uint64_t R, void * const * mats, void * out,
   if(nb > (uint64_t) cu_count()) nb = (uint64_t) cu_count();
   if(nb < 1) nb = 1;
   // the carry buffers belong to this call when it runs on a caller's stream
-  SpdCallScratch scratch(st);
+  SpdCallScratch scratch(fx ? spd_hs(nullptr) : st);
   void * cv = nullptr;
   void * cr = nullptr;
-  SPD_TRY(spd_scratch(2, 2ull * nb * NW * R * sizeof(T), &cv));
-  SPD_TRY(spd_scratch(3, 2ull * nb * NW * sizeof(uint32_t), &cr));
+  if(fx != nullptr) {
+    SPD_TRY(fixed_carve(fx, 2ull * nb * NW * R * sizeof(T), 2ull * nb * NW * sizeof(uint32_t),
+        &cv, &cr));
+  } else {
+    SPD_TRY(spd_scratch(2, 2ull * nb * NW * R * sizeof(T), &cv));
+    SPD_TRY(spd_scratch(3, 2ull * nb * NW * sizeof(uint32_t), &cr));
+  }
   Root3hArgs<T> h = {};
   Root3Args<T> & a = h.b;
   a.fptr0 = c->u.fptr[0];
@@ -1551,7 +1619,7 @@ static int run_root3h(spd_csf * c, uint64_t R, void * const * mats, void * out,
 
 template<typename T>
 static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
-    int rows_pos, hipStream_t st, int dtype)
+    int rows_pos, hipStream_t st, int dtype, MkFixed const * fx)
 {
   int V = 1, L = 16;
   lane_layout((uint32_t) R, dtype, &V, &L);
@@ -1560,23 +1628,32 @@ static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
   if((g_root_variant == 1 || g_root_variant == 2) && root3h_layout(dtype, V, L) &&
       c->dims[c->perm[2]] * R * sizeof(T) < 0x80000000ull && c->nnz < 0xFFFFFFFFull &&
       c->nnz >= (uint64_t) R3_WN * 64) {
-    return run_root3h<T>(c, R, mats, out, rows_pos, st, V, L);
+    return run_root3h<T>(c, R, mats, out, rows_pos, st, V, L, fx);
   }
   uint32_t const G = 64 / L;
   uint32_t const S = R3_WN / G;
   MttkrpPlan * plan = nullptr;
-  SPD_TRY(get_plan(c, S, &plan, spd_hs(nullptr)));
+  if(fx == nullptr) {
+    SPD_TRY(get_plan(c, S, &plan, spd_hs(nullptr)));
+  } else if((plan = find_plan(c, S)) == nullptr) {
+    return not_prepared("launch plan");
+  }
   if(st != spd_hs(nullptr)) {
     SPD_HIP(hipStreamSynchronize(spd_hs(nullptr)));
   }
   // the carry buffers belong to this call when it runs on a caller's stream
-  SpdCallScratch scratch(st);
+  SpdCallScratch scratch(fx ? spd_hs(nullptr) : st);
   uint64_t const nwaves = spd_cdiv(c->nnz, R3_WN);
   uint32_t const nblocks = (uint32_t) spd_cdiv(nwaves, R3_WAVES);
   void * cv = nullptr;
   void * cr = nullptr;
-  SPD_TRY(spd_scratch(2, 2ull * nblocks * R * sizeof(T), &cv));
-  SPD_TRY(spd_scratch(3, 2ull * nblocks * sizeof(uint32_t), &cr));
+  if(fx != nullptr) {
+    SPD_TRY(fixed_carve(fx, 2ull * nblocks * R * sizeof(T), 2ull * nblocks * sizeof(uint32_t),
+        &cv, &cr));
+  } else {
+    SPD_TRY(spd_scratch(2, 2ull * nblocks * R * sizeof(T), &cv));
+    SPD_TRY(spd_scratch(3, 2ull * nblocks * sizeof(uint32_t), &cr));
+  }
   Root3Args<T> a = {};
   a.fptr0 = c->u.fptr[0];
   a.fids0 = c->u.fids[0];
@@ -1748,7 +1825,7 @@ static int get_copy(spd_csf * c, int level, int rows_pos, RowCopy ** out, hipStr
 
 template<typename T>
 static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
-    void * out, int rows_pos, hipStream_t st, int dtype)
+    void * out, int rows_pos, hipStream_t st, int dtype, MkFixed const * fx)
 {
   int const N = c->nmodes;
   int depth = -1;
@@ -1765,11 +1842,15 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   uint64_t const lim = 0xFFFFFFFFull;
   if(N == 3 && depth == 0 && c->dims[c->perm[1]] * R * sizeof(T) <= lim &&
       c->dims[c->perm[2]] * R * sizeof(T) <= lim) {
-    return run_root3<T>(c, R, mats, out, rows_pos, st, dtype);
+    return run_root3<T>(c, R, mats, out, rows_pos, st, dtype, fx);
   }
   MttkrpPlan * plan = nullptr;
   // plans are built on the library stream; make the caller's stream wait
-  SPD_TRY(get_plan(c, WS_WN, &plan, spd_hs(nullptr)));
+  if(fx == nullptr) {
+    SPD_TRY(get_plan(c, WS_WN, &plan, spd_hs(nullptr)));
+  } else if((plan = find_plan(c, WS_WN)) == nullptr) {
+    return not_prepared("launch plan");
+  }
   WsArgs<T> a = {};
   for(int l=0; l < N - 1; ++l) {
     a.fptr[l] = c->u.fptr[l];
@@ -1789,17 +1870,23 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   a.depth = depth;
   a.rows_are_positions = rows_pos;
   RowCopy * cp = nullptr;
-  if(depth > 0 && g_rowcopy) SPD_TRY(get_copy(c, depth, rows_pos, &cp, spd_hs(nullptr)));
+  if(depth > 0 && g_rowcopy) {
+    if(fx == nullptr) SPD_TRY(get_copy(c, depth, rows_pos, &cp, spd_hs(nullptr)));
+    else cp = find_copy(c, depth, rows_pos);
+  }
   if(cp != nullptr) {
     for(int q=0; q < N - 1; ++q) a.cg[q] = cp->cg[q];
     a.crow = cp->crow;
     a.vals = (sizeof(T) == 4) ? (T const *) cp->v32 : (T const *) cp->v64;
   }
   HotPlan * hp = nullptr;
-  if(depth > 0 && cp == nullptr) SPD_TRY(get_hot(c, depth, &hp, spd_hs(nullptr)));
+  if(depth > 0 && cp == nullptr) {
+    if(fx == nullptr) SPD_TRY(get_hot(c, depth, &hp, spd_hs(nullptr)));
+    else if((hp = find_hot(c, depth)) == nullptr) return not_prepared("hot-row table");
+  }
   // the carry buffers belong to this call when it runs on a caller's stream
   // (everything above is built on the library stream and kept with the CSF)
-  SpdCallScratch scratch(st);
+  SpdCallScratch scratch(fx ? spd_hs(nullptr) : st);
   if(depth == 0 || cp != nullptr) {
     // two partial rows per range of 256 / (64 / L) positions
     int V = 1, L = 16;
@@ -1807,8 +1894,12 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
     uint64_t const nent = 2ull * plan->nitems * (64 / L);
     void * cv = nullptr;
     void * cr = nullptr;
-    SPD_TRY(spd_scratch(2, nent * R * sizeof(T), &cv));
-    SPD_TRY(spd_scratch(3, nent * sizeof(uint32_t), &cr));
+    if(fx != nullptr) {
+      SPD_TRY(fixed_carve(fx, nent * R * sizeof(T), nent * sizeof(uint32_t), &cv, &cr));
+    } else {
+      SPD_TRY(spd_scratch(2, nent * R * sizeof(T), &cv));
+      SPD_TRY(spd_scratch(3, nent * sizeof(uint32_t), &cr));
+    }
     a.carry = (T *) cv;
     a.carry_row = (uint32_t *) cr;
   } else {
@@ -1847,9 +1938,74 @@ extern "C" int spd_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats
   }
   if(dtype == SPD_F32) {
     SPD_TRY(ensure_vals32(c));
-    return run_mttkrp<float>(c, mode, R, mats, out, rows_pos, st, dtype);
+    return run_mttkrp<float>(c, mode, R, mats, out, rows_pos, st, dtype, nullptr);
   }
-  return run_mttkrp<double>(c, mode, R, mats, out, rows_pos, st, dtype);
+  return run_mttkrp<double>(c, mode, R, mats, out, rows_pos, st, dtype, nullptr);
+}
+
+// ---- MTTKRP without allocation ---------------------------------------------
+//
+// The partial rows of the root and row-copy walks live in one file-scope
+// device array per GPU instead of per-call allocations. Host code never
+// passes the array's name to a kernel (in host code it is a stand-in, not a
+// device address): its address on the current device comes from
+// hipGetSymbolAddress, taken once per device on that device's first call.
+// Every call on a GPU uses the same array, so calls that overlap on one GPU
+// (two host threads, or two streams without an ordering between them) race
+// on it. The kernels write every partial row they read in the same launch
+// pair (the root walks write both carry rows of every wave or block), so no
+// clearing is needed between calls.
+
+#define SPD_NOALLOC_BYTES (16u << 20)
+#define SPD_NOALLOC_DEVICES 64
+
+static __device__ __align__(256) unsigned char scratch[SPD_NOALLOC_BYTES];
+static void * g_scratch_ptr[SPD_NOALLOC_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int noalloc_scratch(void ** out)
+{
+  int dev = 0;
+  SPD_HIP(hipGetDevice(&dev));
+  if(dev < 0 || dev >= SPD_NOALLOC_DEVICES) {
+    spd_set_error(SPD_EINPUT, "device %d out of range", dev);
+    return SPD_EINPUT;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if(g_scratch_ptr[dev] == nullptr) {
+    void * p = nullptr;
+    hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if(e != hipSuccess || p == nullptr) {
+      (void) hipGetLastError();
+      spd_set_error(SPD_EHIP, "hipGetSymbolAddress of the MTTKRP scratch failed: %s",
+          hipGetErrorString(e));
+      return SPD_EHIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SPD_OK;
+}
+
+extern "C" size_t spd_mttkrp_noalloc_bytes(void)
+{
+  return SPD_NOALLOC_BYTES;
+}
+
+extern "C" int spd_mttkrp_noalloc(spd_csf * c, int mode, uint64_t R, void * const * mats,
+    void * out, int dtype, int rows_pos, void * stream)
+{
+  hipStream_t st = spd_hs(stream);
+  if(R == 0 || R >= (1ull << 31)) {
+    spd_set_error(SPD_EINPUT, "bad rank %llu", (unsigned long long) R);
+    return SPD_EINPUT;
+  }
+  if(dtype == SPD_F32 && c->vals32 == nullptr) return not_prepared("FP32 copy of the values");
+  void * p = nullptr;
+  SPD_TRY(noalloc_scratch(&p));
+  MkFixed const fx = {(char *) p, (size_t) SPD_NOALLOC_BYTES};
+  if(dtype == SPD_F32) return run_mttkrp<float>(c, mode, R, mats, out, rows_pos, st, dtype, &fx);
+  return run_mttkrp<double>(c, mode, R, mats, out, rows_pos, st, dtype, &fx);
 }
 
 extern "C" int spd_mttkrp_model(spd_csf const * c, int mode, uint64_t R, int dtype,

@@ -202,6 +202,32 @@ splatt_error_type splatt_amd_mttkrp_dev(splatt_idx_t const mode,
   return spl_report(rc, "splatt_amd_mttkrp_dev");
 }
 
+splatt_error_type splatt_amd_mttkrp_dev_noalloc(splatt_idx_t const mode,
+    splatt_idx_t const ncolumns, splatt_csf const * const tensors,
+    void ** d_mats, void * d_out, int precision, double const * const options,
+    void * stream)
+{
+  idx_t map[SPLATT_MAX_NMODES];
+  idx_t ncsf = 0;
+  if(mode >= tensors->nmodes || p_mode_csf_map(tensors,
+        (splatt_csf_type) options[SPLATT_OPTION_CSF_ALLOC], map, &ncsf) != 0) {
+    return SPLATT_ERROR_BADINPUT;
+  }
+  splatt_csf const * ct = tensors + map[mode];
+  if(ct->dev == NULL) {
+    return SPLATT_ERROR_BADINPUT;
+  }
+  int const dtype = (precision == SPLATT_AMD_F32) ? SPD_F32 : SPD_F64;
+  int rc = spd_mttkrp_noalloc((spd_csf *) ct->dev, (int) mode, ncolumns, d_mats, d_out,
+      dtype, ct->ntiles == 1, stream);
+  return spl_report(rc, "splatt_amd_mttkrp_dev_noalloc");
+}
+
+size_t splatt_amd_mttkrp_noalloc_bytes(void)
+{
+  return spd_mttkrp_noalloc_bytes();
+}
+
 splatt_error_type splatt_amd_mttkrp_cost(splatt_idx_t const mode,
     splatt_idx_t const ncolumns, splatt_csf const * const tensors,
     int precision, double const * const options, double * bytes, double * flops)

@@ -173,6 +173,10 @@ def lib():
     L.splatt_amd_mttkrp_dev.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(SplattCsf),
                                         C.POINTER(C.c_void_p), C.c_void_p, C.c_int,
                                         C.POINTER(C.c_double), C.c_void_p]
+    L.splatt_amd_mttkrp_dev_noalloc.restype = C.c_int
+    L.splatt_amd_mttkrp_dev_noalloc.argtypes = L.splatt_amd_mttkrp_dev.argtypes
+    L.splatt_amd_mttkrp_noalloc_bytes.restype = C.c_size_t
+    L.splatt_amd_mttkrp_noalloc_bytes.argtypes = []
     L.splatt_amd_mttkrp_cost.restype = C.c_int
     L.splatt_amd_mttkrp_cost.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(SplattCsf),
                                          C.c_int, C.POINTER(C.c_double),
