@@ -443,6 +443,10 @@ def main():
     ap.add_argument("--c3-nnz", type=int, default=100_000_000)
     ap.add_argument("--only", choices=["cpd", "c3", "stream", "serve"], default=None,
                     help="run one secondary workload alone and print its JSON (profiling)")
+    ap.add_argument("--root-variant", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="3-mode root MTTKRP kernel: 0 per-nonzero k_root3 (default), "
+                         "1 persistent with hot leaf rows in LDS, 2 persistent, "
+                         "3 k_root3 gathering A once per fiber")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a streaming kernel of known traffic (FP64->FP32 "
                          "conversion of 2^26 values: 512 MiB read, 256 MiB written) "
@@ -470,6 +474,7 @@ def main():
 
     L = ps.lib()
     R = args.rank
+    L.spd_mttkrp_set_root_variant(args.root_variant)
     if args.calibrate:
         n = 1 << 26
         src = ps.DevBuf(n * 8)

@@ -117,6 +117,70 @@ def test_chol_reports_bad_pivot(gpu_available):
     assert info.to_host((1,), np.int32)[0] == 1
 
 
+def _ld_cholesky_solve(G, M):
+    """X = M G^{-1} in long double (64-bit mantissa) by a textbook Cholesky:
+    the near-exact solution that the FP64 paths are measured against."""
+    G = np.asarray(G, dtype=np.longdouble)
+    R = G.shape[0]
+    Lc = np.zeros((R, R), dtype=np.longdouble)
+    for j in range(R):
+        d = G[j, j] - np.dot(Lc[j, :j], Lc[j, :j])
+        Lc[j, j] = np.sqrt(d)
+        for i in range(j + 1, R):
+            Lc[i, j] = (G[i, j] - np.dot(Lc[i, :j], Lc[j, :j])) / Lc[j, j]
+    B = np.asarray(M, dtype=np.longdouble).T.copy()
+    for i in range(R):
+        B[i] = (B[i] - Lc[i, :i] @ B[:i]) / Lc[i, i]
+    for i in range(R - 1, -1, -1):
+        B[i] = (B[i] - Lc[i + 1:, i] @ B[i + 1:]) / Lc[i, i]
+    return B.T
+
+
+@pytest.mark.parametrize("R", [16, 32, 64])
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6, 1e8, 1e10])
+def test_gj_inverse_matches_cholesky_when_ill_conditioned(gpu_available, R, kappa):
+    # The normal equations are solved as M G^{-1} with an explicit
+    # Gauss-Jordan inverse instead of the reference's potrf/potrs
+    # (src/matrix.c:104-151). On SPD Grams with condition numbers 1e2..1e10
+    # its forward error is held to that of LAPACK's Cholesky solve (scipy's
+    # dpotrf/dpotrs, the reference's routines), both measured against a
+    # long-double solution: within 20x of it (plus 1e-14), and within the
+    # backward-stable bound R * kappa * eps * 50. After a non-positive pivot
+    # both the reference (which prints dpotrf's info and continues) and this
+    # library (info = the 1-based column, as dpotrf) leave the solution
+    # undefined; test_chol_reports_bad_pivot checks the status.
+    import scipy.linalg as sla
+    L = ps.lib()
+    rng = np.random.default_rng(int(R * 1000 + np.log10(kappa)))
+    Q, _ = np.linalg.qr(rng.standard_normal((R, R)))
+    lam = np.geomspace(1.0, 1.0 / kappa, R)
+    G = (Q * lam) @ Q.T
+    G = 0.5 * (G + G.T)
+    I = 2000
+    M = rng.standard_normal((I, R))
+    bufs = [_dev(G), _dev(np.ones((R, R)))]
+    arr = (C.c_void_p * 2)(*[b.ptr for b in bufs])
+    RP = _rp(R)
+    ginv = ps.DevBuf(RP * RP * 8)
+    rho = ps.DevBuf(8)
+    info = ps.DevBuf(8)
+    assert L.spd_dense_chol_inv(arr, 2, 1, R, 0.0, 0, ginv.ptr, rho.ptr, info.ptr) == 0
+    assert info.to_host((1,), np.int32)[0] == 0
+    dM = _dev(M)
+    X = ps.DevBuf(I * R * 8)
+    cn = ps.DevBuf(R * 8)
+    assert L.spd_dense_apply(dM.ptr, 0, ginv.ptr, I, R, X.ptr, cn.ptr) == 0
+    got = X.to_host((I, R), np.float64)
+    chol = sla.cho_solve(sla.cho_factor(G, lower=True), M.T).T
+    exact = _ld_cholesky_solve(G, M)
+    nrm = float(np.linalg.norm(exact.astype(np.float64)))
+    err_gj = float(np.linalg.norm((got - exact).astype(np.float64))) / nrm
+    err_ch = float(np.linalg.norm((chol - exact).astype(np.float64))) / nrm
+    print("R %d kappa %.0e: GJ %.2e Cholesky %.2e" % (R, kappa, err_gj, err_ch))
+    assert err_gj <= 20.0 * err_ch + 1e-14, (err_gj, err_ch)
+    assert err_gj <= 50.0 * R * kappa * 2.2e-16, err_gj
+
+
 @pytest.mark.parametrize("R", [2, 16, 24, 48, 64, 70])
 @pytest.mark.parametrize("f32", [False, True])
 def test_apply(gpu_available, R, f32):
