@@ -21,7 +21,9 @@ import argparse
 import ctypes as C
 import json
 import os
+import shutil
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -420,6 +422,104 @@ def bench_serve_c5(nstreams, per_stream, rate, inflight):
                 throughput_ratio=batched["slices_per_s"] / serial["slices_per_s"])
 
 
+def _coord_equal(p, q):
+    a, b = p[0], q[0]
+    if a.nmodes != b.nmodes or a.nnz != b.nnz:
+        return False
+    n = int(a.nnz)
+    for m in range(int(a.nmodes)):
+        if a.dims[m] != b.dims[m]:
+            return False
+        if not np.array_equal(np.ctypeslib.as_array(a.ind[m], shape=(n,)),
+                              np.ctypeslib.as_array(b.ind[m], shape=(n,))):
+            return False
+    return np.array_equal(np.ctypeslib.as_array(a.vals, shape=(n,)).view(np.uint64),
+                          np.ctypeslib.as_array(b.vals, shape=(n,)).view(np.uint64))
+
+
+def bench_ingest(nnz, threads, seed=20261209):
+    """Text input (src/io.c:273-348): a .tns file of the primary workload's
+    shape read by the host reader (all host threads) and by the GPU reader,
+    and splatt_csf_load (file to all-mode CSFs) against the same steps on
+    the host reader's tensor. The file is read once beforehand, so both
+    readers take it from the page cache."""
+    inds, vals, dims = ps.synth_coord([200_000, 150_000, 50_000], nnz, zipf=[1.2] * 3,
+                                      vals="lognormal", seed=seed)
+    tmp = tempfile.mkdtemp(prefix="splatt_ingest_", dir="/tmp")
+    try:
+        path = os.path.join(tmp, "ingest.tns")
+        t0 = time.perf_counter()
+        ps.write_tns(path, inds, vals)
+        size = os.path.getsize(path)
+        log("ingest: wrote %.2f GB of text (%d lines) in %.1fs"
+            % (size / 1e9, nnz, time.perf_counter() - t0))
+        with open(path, "rb") as f:
+            text = f.read()
+        L = ps.lib()
+        L.spd_tns_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_int, C.c_void_p,
+                                    C.POINTER(C.c_void_p), ps.u64p]
+        L.spd_tns_parse.restype = C.c_int
+        # the GPU part alone: text in host memory to a device COO
+        parse_s = []
+        for _ in range(3):
+            h = C.c_void_p()
+            nh = C.c_uint64(0)
+            L.splatt_amd_sync()
+            t0 = time.perf_counter()
+            rc = L.spd_tns_parse(text, len(text), 3, None, C.byref(h), C.byref(nh))
+            parse_s.append(time.perf_counter() - t0)
+            if rc != 0:
+                raise RuntimeError("spd_tns_parse: " + ps.last_error())
+            L.spd_coo_free(h)
+        del text
+        t0 = time.perf_counter()
+        hp = L.splatt_tt_read(path.encode())
+        host_s = time.perf_counter() - t0
+        dp = L.splatt_alloc_coord(1, 1)
+        nh = C.c_uint64(0)
+        t0 = time.perf_counter()
+        rc = L.splatt_amd_coord_load_dev(path.encode(), dp, C.byref(nh))
+        dev_s = time.perf_counter() - t0
+        if rc != ps.SUCCESS:
+            raise RuntimeError("splatt_amd_coord_load_dev: " + ps.last_error())
+        same = _coord_equal(hp, dp)
+        L.splatt_free_coord(dp)
+        L.splatt_tt_free(hp)
+        # splatt_csf_load against read + compact + build on the host path
+        opts = ps.Opts(csf_alloc=ps.CSF_ALLMODE)
+        out = C.POINTER(ps.SplattCsf)()
+        nm = C.c_uint64(0)
+        t0 = time.perf_counter()
+        rc = L.splatt_csf_load(path.encode(), C.byref(nm), C.byref(out), opts.ptr)
+        load_s = time.perf_counter() - t0
+        if rc != ps.SUCCESS:
+            raise RuntimeError("splatt_csf_load: " + ps.last_error())
+        L.splatt_free_csf(out, opts.ptr)
+        t0 = time.perf_counter()
+        hp = L.splatt_tt_read(path.encode())
+        L.splatt_tt_remove_empty(hp)
+        cs = L.splatt_csf_alloc(hp, opts.ptr)
+        hostpath_s = time.perf_counter() - t0
+        L.splatt_free_csf(cs, opts.ptr)
+        L.splatt_tt_free(hp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    gpu = min(parse_s)
+    return {
+        "workload": "read a %.2f GB .tns text (%d lines, 3 modes, %%.17g values) into a COO"
+                    % (size / 1e9, nnz),
+        "text_bytes": size,
+        "host_reader_s": host_s, "host_reader_threads": threads,
+        "host_reader_gbps": size / host_s / 1e9,
+        "gpu_reader_s": dev_s, "gpu_reader_gbps": size / dev_s / 1e9,
+        "gpu_parse_s": gpu, "gpu_parse_gbps": size / gpu / 1e9,
+        "gpu_reader_host_lines": int(nh.value),
+        "gpu_equals_host_reader": bool(same),
+        "csf_load_allmode_s": load_s,
+        "host_read_compact_build_allmode_s": hostpath_s,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,7 +541,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--serve-inflight", type=int, default=16)
     ap.add_argument("--c3-nnz", type=int, default=100_000_000)
-    ap.add_argument("--only", choices=["cpd", "c3", "stream", "serve"], default=None,
+    ap.add_argument("--ingest-nnz", type=int, default=20_000_000)
+    ap.add_argument("--only", choices=["cpd", "c3", "stream", "serve", "ingest"], default=None,
                     help="run one secondary workload alone and print its JSON (profiling)")
     ap.add_argument("--root-variant", type=int, default=0, choices=[0, 1, 2, 3],
                     help="3-mode root MTTKRP kernel: 0 per-nonzero k_root3 (default), "
@@ -463,6 +564,8 @@ def main():
             out = bench_cpd_c1(threads)
         elif args.only == "c3":
             out = bench_cpd_c3(args.c3_nnz)
+        elif args.only == "ingest":
+            out = bench_ingest(args.ingest_nnz, threads)
         elif args.only == "stream":
             out = bench_stream_c4(args.stream_slices,
                                   0 if args.skip_cpu else args.stream_cpu_slices, threads)
@@ -561,6 +664,13 @@ def main():
             % (c5["batched"]["slices_per_s"], c5["batched"]["p50_latency_s"],
                c5["batched"]["p99_latency_s"], c5["one_at_a_time"]["slices_per_s"]))
         extra["serve_c5"] = c5
+        ing = bench_ingest(args.ingest_nnz, threads)
+        log("ingest: host reader %.2f GB/s (%d threads), GPU reader %.2f GB/s (parse alone "
+            "%.2f GB/s), same result %s; csf_load %.2fs vs host path %.2fs"
+            % (ing["host_reader_gbps"], threads, ing["gpu_reader_gbps"], ing["gpu_parse_gbps"],
+               ing["gpu_equals_host_reader"], ing["csf_load_allmode_s"],
+               ing["host_read_compact_build_allmode_s"]))
+        extra["ingest"] = ing
 
     line = {
         "metric": "MTTKRP HBM GB/s",

@@ -82,6 +82,18 @@ splatt_error_type splatt_amd_mttkrp_dev_noalloc(
 
 size_t splatt_amd_mttkrp_noalloc_bytes(void);
 
+/* splatt_coord_load (src/io.c:62-108, :273-348 through tt_read) with the
+ * text parsed on the GPU: the same coordinates (0-based), values and dims
+ * as the host reader, bit for bit. Lines whose fields the GPU conversion
+ * does not take (hexadecimal or special values, more than 19 significant
+ * digits, results outside the normal doubles, malformed fields) are
+ * converted on the host with strtoull/strtod; *nhost (may be NULL) receives
+ * their number. splatt_csf_load reads its input the same way. */
+splatt_error_type splatt_amd_coord_load_dev(
+    char const * const fname,
+    splatt_coord * const coord,
+    uint64_t * const nhost);
+
 /* Algorithmic bytes and flops of one MTTKRP of `mode` on the tensor the
  * options' CSF map selects (4-byte indices, value size of `precision`). */
 splatt_error_type splatt_amd_mttkrp_cost(

@@ -27,6 +27,7 @@ extern "C" {
 #define SPD_EINPUT     -3   /* bad input (index out of range, sizes) */
 #define SPD_ENODEV     -4   /* no usable GPU */
 #define SPD_ENUMERIC   -5   /* non-SPD Gram matrix, NaN, ... */
+#define SPD_EHOST      -6   /* input the device path leaves to the host path */
 
 #define SPD_MAX_MODES 8
 
@@ -52,6 +53,39 @@ typedef struct spd_coo spd_coo;
 int spd_coo_upload(int nmodes, uint64_t nnz, uint64_t const * dims,
     uint64_t const * const * inds, double const * vals, spd_coo ** out);
 void spd_coo_free(spd_coo * coo);
+void spd_coo_dims(spd_coo const * coo, int * nmodes, uint64_t * nnz, uint64_t * dims);
+/* Copy the coordinates (inds[m]: nnz entries each, host) and values back. */
+int spd_coo_download(spd_coo const * coo, uint64_t * const * inds, double * vals);
+
+/* ---- text tensor input (src/io.c:62-108, :273-348) --------------------- */
+
+/* Host conversion of one data line [line, line + len) (no '\n'): nmodes
+ * strtoull fields then one strtod field, exactly as the host reader. 0 on
+ * success. */
+typedef int (*spd_line_fn)(char const * line, size_t len, int nmodes,
+    uint64_t * inds, double * val);
+
+/* Parse a whole .tns text of len bytes with nmodes index fields per line
+ * on the device into a 0-based device COO (dims from the largest index; a
+ * mode whose smallest index is 0 is 0-indexed, else 1-indexed). Lines whose
+ * fields the device conversion does not take are converted by line_fn.
+ * *nhost = their number. SPD_EHOST (and *out NULL) when more than
+ * SPD_TNS_HOST_MAX lines would need line_fn: the host reader is then the
+ * faster path. */
+#define SPD_TNS_HOST_MAX (1u << 20)
+int spd_tns_parse(char const * text, size_t len, int nmodes, spd_line_fn line_fn,
+    spd_coo ** out, uint64_t * nhost);
+
+/* Renumber each mode's occurring indices 0..k-1 in order and set dims to
+ * k (src/sptensor.c:190-252). *removed = total indices removed. */
+int spd_coo_remove_empty(spd_coo * coo, uint64_t * removed);
+
+/* The device field conversions run on the host, for checking against the
+ * C library. Field i is [buf + off[i], buf + off[i + 1]); kind 0 converts
+ * it as a value (vals[i]), kind 1 as an index (idx[i]); ok[i] = 0 where the
+ * conversion leaves the field to the C library. */
+int spd_decimal_convert(char const * buf, uint64_t const * off, uint64_t n, int kind,
+    double * vals, uint64_t * idx, int * ok);
 
 /* Lexicographic order of the coordinates with perm[0] most significant:
  * order_host[k] (n entries, host) = position of the k-th smallest. Equal

@@ -15,6 +15,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <omp.h>
+#include <stdio.h>
 
 #define GEN_MAX_MODES 8
 #define GEN_CHUNK (1u << 20)
@@ -284,4 +285,56 @@ void splatt_gen_matrix(uint64_t nrows, uint64_t ncols, uint64_t seed, double * o
 {
   uint64_t s = seed ^ 0x2468ACE013579BDFull;
   for(uint64_t i=0; i < nrows * ncols; ++i) out[i] = 2.0 * u01(&s) - 1.0;
+}
+
+/* Write a tensor as .tns text (1-indexed, "i_1 ... i_N value\n", values
+ * with 17 significant digits, %.17g, so they read back exactly). The lines
+ * are formatted by all threads into per-thread chunks, then written in
+ * order. 0 on success. */
+int splatt_gen_write_tns(char const * path, int nmodes, uint64_t nnz,
+    uint64_t const * const * inds, double const * vals)
+{
+  FILE * f = fopen(path, "wb");
+  if(f == NULL) {
+    return -1;
+  }
+  uint64_t const chunk = 1 << 16;
+  uint64_t const nchunks = (nnz + chunk - 1) / chunk;
+  int const T = omp_get_max_threads();
+  /* rounds of T chunks: format in parallel, write in order */
+  size_t const cap = (size_t) chunk * (size_t) (21 * nmodes + 32);
+  char * bufs = malloc((size_t) T * cap);
+  size_t * lens = malloc((size_t) T * sizeof(size_t));
+  int rc = (bufs && lens) ? 0 : -1;
+  for(uint64_t c0=0; c0 < nchunks && rc == 0; c0 += (uint64_t) T) {
+    #pragma omp parallel for schedule(static, 1) num_threads(T)
+    for(int t=0; t < T; ++t) {
+      uint64_t const c = c0 + (uint64_t) t;
+      lens[t] = 0;
+      if(c >= nchunks) {
+        continue;
+      }
+      char * p = bufs + (size_t) t * cap;
+      uint64_t const lo = c * chunk;
+      uint64_t const hi = (lo + chunk < nnz) ? lo + chunk : nnz;
+      for(uint64_t n=lo; n < hi; ++n) {
+        for(int m=0; m < nmodes; ++m) {
+          p += sprintf(p, "%llu ", (unsigned long long) (inds[m][n] + 1));
+        }
+        p += sprintf(p, "%.17g\n", vals[n]);
+      }
+      lens[t] = (size_t) (p - (bufs + (size_t) t * cap));
+    }
+    for(int t=0; t < T && rc == 0; ++t) {
+      if(lens[t] && fwrite(bufs + (size_t) t * cap, 1, lens[t], f) != lens[t]) {
+        rc = -1;
+      }
+    }
+  }
+  free(bufs);
+  free(lens);
+  if(fclose(f) != 0) {
+    rc = -1;
+  }
+  return rc;
 }

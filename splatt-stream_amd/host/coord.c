@@ -14,6 +14,8 @@
 #include "host.h"
 #include <math.h>
 #include <omp.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 sptensor_t * splatt_tt_alloc(splatt_idx_t const nnz, splatt_idx_t const nmodes)
 {
@@ -62,7 +64,7 @@ splatt_idx_t * splatt_coord_get_inds(splatt_coord const * const c,
 splatt_val_t * splatt_coord_get_vals(splatt_coord const * const c) { return c->vals; }
 
 /* The whole stream into one '\0'-terminated buffer. */
-static char * p_slurp(FILE * fin, size_t * out_len)
+char * spl_slurp(FILE * fin, size_t * out_len)
 {
   size_t cap = 1 << 20;
   size_t len = 0;
@@ -91,6 +93,49 @@ static char * p_slurp(FILE * fin, size_t * out_len)
   return buf;
 }
 
+/* A regular file read whole into one '\0'-terminated buffer: its size from
+ * fstat, its bytes by all host threads (pread of one range each, so the
+ * page-cache copy runs in parallel and the buffer is touched first by the
+ * thread that fills it). NULL when fin is not a regular file or a read
+ * comes up short (the caller then reads the stream with spl_slurp). */
+char * spl_read_whole(FILE * fin, size_t * out_len)
+{
+  struct stat st;
+  int const fd = fileno(fin);
+  if(fd < 0 || fstat(fd, &st) != 0 || !S_ISREG(st.st_mode)) {
+    return NULL;
+  }
+  size_t const len = (size_t) st.st_size;
+  char * buf = malloc(len + 1);
+  if(buf == NULL) {
+    return NULL;
+  }
+  size_t const chunk = (size_t) 8 << 20;
+  size_t const nch = (len + chunk - 1) / chunk;
+  int bad = 0;
+  #pragma omp parallel for schedule(dynamic, 1) reduction(|:bad)
+  for(size_t c=0; c < nch; ++c) {
+    size_t const lo = c * chunk;
+    size_t const n = (lo + chunk < len) ? chunk : len - lo;
+    size_t got = 0;
+    while(got < n) {
+      ssize_t const r = pread(fd, buf + lo + got, n - got, (off_t) (lo + got));
+      if(r <= 0) {
+        bad = 1;
+        break;
+      }
+      got += (size_t) r;
+    }
+  }
+  if(bad) {
+    free(buf);
+    return NULL;
+  }
+  buf[len] = '\0';
+  *out_len = len;
+  return buf;
+}
+
 /* End of the line starting at s (the '\n' or the buffer's end). */
 static size_t p_line_end(char const * buf, size_t s, size_t len)
 {
@@ -106,29 +151,23 @@ static int p_is_data(char const * buf, size_t s, size_t e, size_t len)
   return rd > 1 && buf[s] != '#';
 }
 
-/* Text input, the same conversions as the reference's two getline passes
- * (strtoull per mode, then strtod) over the whole file read at once: the
- * buffer is cut into one range of whole lines per thread; a first pass
- * counts each range's data lines, a second converts each line (its '\n'
- * replaced by '\0', so no conversion runs past it, as with getline's
- * buffer) into its place and keeps per-thread index extremes. */
-static sptensor_t * p_read_text(FILE * fin)
+/* Modes of a text tensor (src/io.c:273-300): the fields of its first data
+ * line, split at ' ' and '\t' with the line's '\n' kept (a trailing blank
+ * before it makes the '\n' a field of its own, as in the reference), less
+ * the value. 0 when there is no data line or fewer than two fields. */
+idx_t spl_text_nmodes(char const * buf, size_t len)
 {
-  size_t len = 0;
-  char * buf = p_slurp(fin, &len);
-  if(buf == NULL) {
-    return NULL;
-  }
-  /* modes from the first data line */
   idx_t nmodes = 0;
   for(size_t s=0; s < len; ) {
     size_t const e = p_line_end(buf, s, len);
     if(p_is_data(buf, s, e, len)) {
       char * line = malloc(e - s + 2);
+      if(line == NULL) {
+        return 0;
+      }
       memcpy(line, buf + s, e - s);
       line[e - s] = '\0';
       char * save = NULL;
-      /* the line keeps its '\n', which is no separator (src/io.c:288-293) */
       if(e < len) {
         line[e - s] = '\n';
         line[e - s + 1] = '\0';
@@ -142,15 +181,48 @@ static sptensor_t * p_read_text(FILE * fin)
     }
     s = e + 1;
   }
-  if(nmodes < 2) {
-    free(buf);
+  return nmodes < 2 ? 0 : nmodes - 1;
+}
+
+/* One data line [line, line + len) converted as the host reader converts
+ * it: strtoull per mode, then strtod, on a '\0'-terminated copy. */
+int spl_line_convert(char const * line, size_t len, int nmodes,
+    uint64_t * inds, double * val)
+{
+  char small[256];
+  char * tmp = (len < sizeof(small)) ? small : malloc(len + 1);
+  if(tmp == NULL) {
+    return -1;
+  }
+  memcpy(tmp, line, len);
+  tmp[len] = '\0';
+  char * p = tmp;
+  for(int m=0; m < nmodes; ++m) {
+    inds[m] = strtoull(p, &p, 10);
+  }
+  *val = strtod(p, &p);
+  if(tmp != small) {
+    free(tmp);
+  }
+  return 0;
+}
+
+/* Text input, the same conversions as the reference's two getline passes
+ * (strtoull per mode, then strtod) over the whole file read at once: the
+ * buffer is cut into one range of whole lines per thread; a first pass
+ * counts each range's data lines, a second converts each line (its '\n'
+ * replaced by '\0', so no conversion runs past it, as with getline's
+ * buffer) into its place and keeps per-thread index extremes. buf holds
+ * len bytes and a '\0'; its newlines are overwritten. */
+sptensor_t * spl_parse_text(char * buf, size_t len)
+{
+  idx_t const nmodes = spl_text_nmodes(buf, len);
+  if(nmodes == 0) {
     return NULL;
   }
-  --nmodes;
   if(nmodes > SPLATT_MAX_NMODES) {
     fprintf(stderr, "SPLATT ERROR: maximum %d modes supported. Found %"
         SPLATT_PF_IDX ".\n", (int) SPLATT_MAX_NMODES, nmodes);
-    free(buf);
     return NULL;
   }
   /* one range of whole lines per thread */
@@ -214,7 +286,6 @@ static sptensor_t * p_read_text(FILE * fin)
       s = e + 1;
     }
   }
-  free(buf);
   idx_t dims[SPLATT_MAX_NMODES];
   idx_t offs[SPLATT_MAX_NMODES];
   for(idx_t m=0; m < nmodes; ++m) {
@@ -250,6 +321,21 @@ static sptensor_t * p_read_text(FILE * fin)
   return tt;
 }
 
+static sptensor_t * p_read_text(FILE * fin)
+{
+  size_t len = 0;
+  char * buf = spl_read_whole(fin, &len);
+  if(buf == NULL) {
+    buf = spl_slurp(fin, &len);
+  }
+  if(buf == NULL) {
+    return NULL;
+  }
+  sptensor_t * tt = spl_parse_text(buf, len);
+  free(buf);
+  return tt;
+}
+
 static int p_read_idx(FILE * fin, uint64_t width, idx_t * out, idx_t count)
 {
   if(width == sizeof(uint64_t)) {
@@ -272,7 +358,7 @@ static int p_read_idx(FILE * fin, uint64_t width, idx_t * out, idx_t count)
   return 0;
 }
 
-static sptensor_t * p_read_binary(FILE * fin)
+sptensor_t * spl_read_binary(FILE * fin)
 {
   int32_t magic = 0;
   uint64_t iw = 0, vw = 0;
@@ -330,7 +416,7 @@ sptensor_t * splatt_tt_read(char const * const ifname)
   char const * suffix = strrchr(ifname, '.');
   sptensor_t * tt = NULL;
   if(suffix != NULL && strcmp(suffix, ".bin") == 0) {
-    tt = p_read_binary(fin);
+    tt = spl_read_binary(fin);
   } else {
     tt = p_read_text(fin);
   }

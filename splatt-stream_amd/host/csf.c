@@ -385,17 +385,30 @@ splatt_error_type splatt_csf_convert(splatt_idx_t const nmodes,
   return (*tensors == NULL) ? SPLATT_ERROR_BADINPUT : SPLATT_SUCCESS;
 }
 
+/* src/csf.c:18-37: read, drop empty slices, build. The tensor goes from the
+ * file to device memory (the text parsed there), loses its empty slices
+ * there and the CSFs are built from it; no host copy of the coordinates. */
 splatt_error_type splatt_csf_load(char const * const fname,
     splatt_idx_t * nmodes, splatt_csf ** tensors, double const * const options)
 {
-  sptensor_t * tt = splatt_tt_read(fname);
-  if(tt == NULL) {
-    return SPLATT_ERROR_BADINPUT;
+  *tensors = NULL;
+  spd_coo * dcoo = NULL;
+  int rc = spl_coo_read_dev(fname, &dcoo, NULL);
+  if(rc != SPD_OK) {
+    return (rc == SPD_EINPUT) ? SPLATT_ERROR_BADINPUT : spl_report(rc, "tensor input");
   }
-  splatt_tt_remove_empty(tt);
-  *tensors = splatt_csf_alloc(tt, options);
-  *nmodes = tt->nmodes;
-  splatt_tt_free(tt);
+  rc = spd_coo_remove_empty(dcoo, NULL);
+  if(rc != SPD_OK) {
+    spd_coo_free(dcoo);
+    return spl_report(rc, "empty-slice removal");
+  }
+  int nm = 0;
+  uint64_t nnz = 0;
+  idx_t dims[SPLATT_MAX_NMODES];
+  spd_coo_dims(dcoo, &nm, &nnz, (uint64_t *) dims);
+  *tensors = p_alloc_all(dcoo, (idx_t) nm, nnz, dims, options);
+  *nmodes = (idx_t) nm;
+  spd_coo_free(dcoo);
   return (*tensors == NULL) ? SPLATT_ERROR_BADINPUT : SPLATT_SUCCESS;
 }
 

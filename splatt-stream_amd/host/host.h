@@ -56,6 +56,25 @@ void splatt_ntf_lasso_prox(val_t * primal, idx_t const nrows, idx_t const ncols,
 int spl_map_constraint(splatt_cpd_constraint const * con, idx_t mode,
     idx_t chunk, spd_con * out);
 
+/* Pieces of the host text reader (coord.c). spl_slurp: the whole stream
+ * in one '\0'-terminated buffer; spl_read_whole: the same for a regular
+ * file, read by all host threads (NULL otherwise). spl_text_nmodes: modes
+ * from the first data
+ * line (0 if none). spl_line_convert: one data line converted with
+ * strtoull/strtod (an spd_line_fn). spl_parse_text: the whole buffer (its
+ * newlines are overwritten). spl_read_binary: the .bin format. */
+char * spl_slurp(FILE * fin, size_t * out_len);
+char * spl_read_whole(FILE * fin, size_t * out_len);
+idx_t spl_text_nmodes(char const * buf, size_t len);
+int spl_line_convert(char const * line, size_t len, int nmodes, uint64_t * inds,
+    double * val);
+sptensor_t * spl_parse_text(char * buf, size_t len);
+sptensor_t * spl_read_binary(FILE * fin);
+
+/* A tensor file into device memory (coordinates 0-based, dims as the host
+ * reader's); *nhost = lines converted on the host. SPD_* status. */
+int spl_coo_read_dev(char const * fname, spd_coo ** out, uint64_t * nhost);
+
 /* glibc rand() initialisation (src/util.c:15-40). */
 val_t spl_rand_val(void);
 void spl_fill_rand(val_t * vals, idx_t nelems);

@@ -254,6 +254,22 @@ def lib():
     L.splatt_cpd_stream.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_double,
                                     C.POINTER(CpdOpts), C.POINTER(GlobalOpts)]
     L.splatt_amd_server_create.restype = C.c_void_p
+    L.splatt_amd_coord_load_dev.argtypes = [C.c_char_p, C.POINTER(SplattCoord), u64p]
+    L.splatt_amd_coord_load_dev.restype = C.c_int
+    L.spd_decimal_convert.argtypes = [C.c_char_p, u64p, C.c_uint64, C.c_int, f64p, u64p,
+                                      C.POINTER(C.c_int)]
+    L.spd_decimal_convert.restype = C.c_int
+    L.spd_coo_upload.argtypes = [C.c_int, C.c_uint64, u64p, C.POINTER(u64p), f64p,
+                                 C.POINTER(C.c_void_p)]
+    L.spd_coo_upload.restype = C.c_int
+    L.spd_coo_free.argtypes = [C.c_void_p]
+    L.spd_coo_free.restype = None
+    L.spd_coo_remove_empty.argtypes = [C.c_void_p, u64p]
+    L.spd_coo_remove_empty.restype = C.c_int
+    L.spd_coo_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int), u64p, u64p]
+    L.spd_coo_dims.restype = None
+    L.spd_coo_download.argtypes = [C.c_void_p, C.POINTER(u64p), f64p]
+    L.spd_coo_download.restype = C.c_int
     L.splatt_amd_server_create.argtypes = [C.c_size_t, C.c_int]
     L.splatt_amd_server_open.restype = C.c_int64
     L.splatt_amd_server_open.argtypes = [C.c_void_p, C.c_uint64, u64p, C.c_uint64, C.c_uint64,
@@ -290,6 +306,8 @@ def gen():
     G.splatt_gen_shuffle.argtypes = [C.c_int, C.c_uint64, C.POINTER(u64p), f64p,
                                      C.c_uint64]
     G.splatt_gen_matrix.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, f64p]
+    G.splatt_gen_write_tns.argtypes = [C.c_char_p, C.c_int, C.c_uint64, C.POINTER(u64p), f64p]
+    G.splatt_gen_write_tns.restype = C.c_int
     _gen = G
     return G
 
@@ -346,6 +364,16 @@ def synth_coord(dims, nnz, zipf=None, vals="uniform", compact=True, seed=1,
     if shuffle:
         gen().splatt_gen_shuffle(n, nnz, ptrs, _ptr(v, C.c_double), seed)
     return inds, v, [int(d) for d in dims_out]
+
+
+def write_tns(path, inds, vals):
+    """Write a COO as 1-indexed .tns text (values as %.17g)."""
+    n = len(inds)
+    keep = [_u64(a) for a in inds]
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    ptrs = (u64p * n)(*[_ptr(a, C.c_uint64) for a in keep])
+    if gen().splatt_gen_write_tns(path.encode(), n, len(v), ptrs, _ptr(v, C.c_double)) != 0:
+        raise OSError("writing " + path)
 
 
 def synth_matrix(nrows, ncols, seed):
