@@ -634,26 +634,41 @@ extern "C" int spd_csf_export_tile(spd_csf const * c, uint64_t t, uint64_t ** fp
   SPD_TRY(tile_offsets(c, off));
   uint64_t const * o0 = &off[t * N];
   uint64_t const * o1 = &off[(t + 1) * N];
-  std::vector<uint32_t> buf;
+  // 4-byte device entries widened to the mirror's 8-byte ones (less the
+  // tile's first child for fptr) chunk by chunk, on the host threads, while
+  // the next chunk copies
+  auto widen = [](uint32_t const * dev, uint64_t cnt, uint64_t * dst, uint64_t base) -> int {
+    return spd_d2h_chunks(dev, cnt * sizeof(uint32_t),
+        [&](void const * h, size_t off, size_t nbytes) {
+          uint32_t const * const s = (uint32_t const *) h;
+          uint64_t * const d = dst + off / sizeof(uint32_t);
+          spd_par_range(nbytes / sizeof(uint32_t), [&](uint64_t lo, uint64_t hi) {
+            for(uint64_t f=lo; f < hi; ++f) d[f] = (uint64_t) s[f] - base;
+          });
+        });
+  };
   for(int l=0; l < N; ++l) {
     uint64_t const cnt = o1[l] - o0[l];
     if(cnt == 0) continue;
     if(l < N - 1 && fptr != nullptr && fptr[l] != nullptr) {
-      buf.resize(cnt);
-      SPD_TRY(spd_d2h(buf.data(), a->fptr[l] + o0[l], cnt * sizeof(uint32_t)));
       uint64_t const base = o0[l + 1];
-      for(uint64_t f=0; f < cnt; ++f) fptr[l][f] = (uint64_t) buf[f] - base;
+      SPD_TRY(widen(a->fptr[l] + o0[l], cnt, fptr[l], base));
       fptr[l][cnt] = o1[l + 1] - base;
     }
     if(fids != nullptr && fids[l] != nullptr) {
-      buf.resize(cnt);
-      SPD_TRY(spd_d2h(buf.data(), a->fids[l] + o0[l], cnt * sizeof(uint32_t)));
-      for(uint64_t f=0; f < cnt; ++f) fids[l][f] = buf[f];
+      SPD_TRY(widen(a->fids[l] + o0[l], cnt, fids[l], 0));
     }
   }
   uint64_t const nn = o1[N - 1] - o0[N - 1];
   if(vals != nullptr && nn > 0) {
-    SPD_TRY(spd_d2h(vals, a->vals + o0[N - 1], nn * sizeof(double)));
+    SPD_TRY(spd_d2h_chunks(a->vals + o0[N - 1], nn * sizeof(double),
+        [&](void const * h, size_t off, size_t nbytes) {
+          double * const d = vals + off / sizeof(double);
+          double const * const s = (double const *) h;
+          spd_par_range(nbytes / sizeof(double), [&](uint64_t lo, uint64_t hi) {
+            if(hi > lo) memcpy(d + lo, s + lo, (hi - lo) * sizeof(double));
+          });
+        }));
   }
   return SPD_OK;
 }

@@ -5,6 +5,9 @@
 #include <stdint.h>
 #include <stdio.h>
 #include "spd.h"
+#include <functional>
+#include <thread>
+#include <vector>
 
 // Records an error message for spd_last_error().
 void spd_set_error(int code, const char * fmt, ...);
@@ -103,6 +106,32 @@ struct SpdCallScratch {
 // after the work queued before the free.
 int spd_malloc_on(void ** ptr, size_t bytes, hipStream_t st);
 int spd_free_on(void * ptr, hipStream_t st);
+
+// Device-to-host copy of `bytes` in chunks through pinned staging on the
+// library stream: fn(host_chunk, byte_offset, chunk_bytes) is called for
+// each chunk in order (on the calling thread) while the next one copies.
+int spd_d2h_chunks(void const * src, size_t bytes,
+    std::function<void(void const *, size_t, size_t)> const & fn);
+
+// Run f(lo, hi) over [0, n) split across up to 16 host threads (one range
+// each; small n runs on the calling thread).
+template<typename F>
+void spd_par_range(uint64_t n, F const & f)
+{
+  unsigned hc = std::thread::hardware_concurrency();
+  unsigned const T = (hc == 0) ? 1u : (hc < 16u ? hc : 16u);
+  if(T == 1 || n < ((uint64_t) 1 << 18)) {
+    f((uint64_t) 0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for(unsigned k=0; k < T; ++k) {
+    uint64_t const lo = n * k / T, hi = n * (k + 1) / T;
+    th.emplace_back([&f, lo, hi] { f(lo, hi); });
+  }
+  for(std::thread & x : th) x.join();
+}
 
 // Zero fill with a kernel on stream st.
 int spd_zero_async(void * ptr, size_t bytes, hipStream_t st);

@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <mutex>
+#include <functional>
 
 static char g_err[512] = "";
 static hipStream_t g_stream = nullptr;
@@ -132,6 +133,48 @@ extern "C" int spd_d2h(void * dst, void const * src, size_t bytes)
   if(bytes == 0) return SPD_OK;
   SPD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
   SPD_HIP(hipStreamSynchronize(g_stream));
+  return SPD_OK;
+}
+
+// Chunked device-to-host copy through two pinned staging buffers: chunk
+// i + 1 is copied while the caller's function consumes chunk i (a DMA from
+// pinned memory instead of the runtime's pageable path, and the consumer's
+// stores into its own memory run beside the copy). One copy at a time per
+// process (the buffers are shared).
+#define SPD_STAGE_BYTES ((size_t) 64 << 20)
+static std::mutex g_stage_mu;
+static void * g_stage[2] = {nullptr, nullptr};
+static hipEvent_t g_stage_ev[2];
+
+int spd_d2h_chunks(void const * src, size_t bytes,
+    std::function<void(void const *, size_t, size_t)> const & fn)
+{
+  if(bytes == 0) return SPD_OK;
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  if(g_stage[0] == nullptr) {
+    for(int k=0; k < 2; ++k) {
+      SPD_HIP(hipHostMalloc(&g_stage[k], SPD_STAGE_BYTES, hipHostMallocDefault));
+      SPD_HIP(hipEventCreateWithFlags(&g_stage_ev[k], hipEventDisableTiming));
+    }
+  }
+  hipStream_t const st = g_stream;
+  size_t const nch = (bytes + SPD_STAGE_BYTES - 1) / SPD_STAGE_BYTES;
+  auto issue = [&](size_t i) -> int {
+    size_t const off = i * SPD_STAGE_BYTES;
+    size_t const n = (off + SPD_STAGE_BYTES < bytes) ? SPD_STAGE_BYTES : bytes - off;
+    SPD_HIP(hipMemcpyAsync(g_stage[i % 2], (char const *) src + off, n, hipMemcpyDeviceToHost,
+        st));
+    SPD_HIP(hipEventRecord(g_stage_ev[i % 2], st));
+    return SPD_OK;
+  };
+  SPD_TRY(issue(0));
+  for(size_t i=0; i < nch; ++i) {
+    if(i + 1 < nch) SPD_TRY(issue(i + 1));
+    SPD_HIP(hipEventSynchronize(g_stage_ev[i % 2]));
+    size_t const off = i * SPD_STAGE_BYTES;
+    size_t const n = (off + SPD_STAGE_BYTES < bytes) ? SPD_STAGE_BYTES : bytes - off;
+    fn(g_stage[i % 2], off, n);
+  }
   return SPD_OK;
 }
 
