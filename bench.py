@@ -395,7 +395,7 @@ def run_server(streams, arrivals, max_inflight):
                 mean_gpu_s=float(np.mean([i["gpu"] for i in info])))
 
 
-def bench_serve_c5(nstreams, per_stream, rate, inflight):
+def bench_serve_c5(nstreams, per_stream, rate, inflight, steady_slices=0):
     """Serving: nstreams streams, Poisson arrivals at `rate` slices/s in
     aggregate, served with `inflight` slices in flight against the
     one-slice-at-a-time loop (the same server with one slice in flight)."""
@@ -415,11 +415,37 @@ def bench_serve_c5(nstreams, per_stream, rate, inflight):
     batched = run_server(streams, arrivals, inflight)
     serial = run_server(streams, arrivals, 1)
     nnz = sum(len(v) for s in streams for _, v in s["slices"])
-    return dict(workload="serving: %d streams x %d slices (%d nnz), Poisson arrivals %.0f "
-                         "slices/s, dims U[10k,1M], rank in {8,16,32,64}, frob 1e-2, 5 its, "
-                         "FP64" % (nstreams, per_stream, nnz, rate),
-                inflight=inflight, batched=batched, one_at_a_time=serial,
-                throughput_ratio=batched["slices_per_s"] / serial["slices_per_s"])
+    out = dict(workload="serving: %d streams x %d slices (%d nnz), Poisson arrivals %.0f "
+                        "slices/s, dims U[10k,1M], rank in {8,16,32,64}, frob 1e-2, 5 its, "
+                        "FP64" % (nstreams, per_stream, nnz, rate),
+               inflight=inflight, batched=batched, one_at_a_time=serial,
+               throughput_ratio=batched["slices_per_s"] / serial["slices_per_s"])
+    if steady_slices > 0:
+        out["steady"] = bench_serve_steady(nstreams, steady_slices,
+                                           0.55 * batched["slices_per_s"], inflight)
+    return out
+
+
+def bench_serve_steady(nstreams, per_stream, rate, inflight):
+    """Steady-state serving: the same kind of streams with per_stream slices
+    each, Poisson arrivals at `rate` (below the concurrent service rate, so
+    latencies are service times, not queue growth)."""
+    streams = serve_c5_streams(nstreams, per_stream, seed=20261210)
+    rng = np.random.default_rng(11)
+    arrivals = []
+    t = 0.0
+    nxt = [0] * nstreams
+    while len(arrivals) < nstreams * per_stream:
+        t += rng.exponential(1.0 / rate)
+        k = int(rng.integers(0, nstreams))
+        while nxt[k] >= per_stream:
+            k = (k + 1) % nstreams
+        arrivals.append((t, k, nxt[k]))
+        nxt[k] += 1
+    r = run_server(streams, arrivals, inflight)
+    r.update(streams=nstreams, slices_per_stream=per_stream, arrival_rate=rate,
+             inflight=inflight)
+    return r
 
 
 def _coord_equal(p, q):
@@ -540,6 +566,8 @@ def main():
     ap.add_argument("--serve-slices", type=int, default=3)
     ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--serve-inflight", type=int, default=16)
+    ap.add_argument("--serve-steady-slices", type=int, default=20,
+                    help="slices per stream of the steady-state serving run (0: skip)")
     ap.add_argument("--c3-nnz", type=int, default=100_000_000)
     ap.add_argument("--ingest-nnz", type=int, default=20_000_000)
     ap.add_argument("--only", choices=["cpd", "c3", "stream", "serve", "ingest"], default=None,
@@ -571,7 +599,7 @@ def main():
                                   0 if args.skip_cpu else args.stream_cpu_slices, threads)
         else:
             out = bench_serve_c5(args.serve_streams, args.serve_slices, args.serve_rate,
-                                 args.serve_inflight)
+                                 args.serve_inflight, args.serve_steady_slices)
         print(json.dumps(out))
         return
 
@@ -659,10 +687,14 @@ def main():
             c4["slices_per_s"], c4["p50_slice_s"], c4["p99_slice_s"]))
         extra["stream_c4"] = c4
         c5 = bench_serve_c5(args.serve_streams, args.serve_slices, args.serve_rate,
-                            args.serve_inflight)
+                            args.serve_inflight, args.serve_steady_slices)
         log("serve c5: %.2f slices/s (p50 %.3fs p99 %.3fs) vs one-at-a-time %.2f slices/s"
             % (c5["batched"]["slices_per_s"], c5["batched"]["p50_latency_s"],
                c5["batched"]["p99_latency_s"], c5["one_at_a_time"]["slices_per_s"]))
+        if "steady" in c5:
+            log("serve c5 steady state: %.2f slices/s at arrivals %.1f/s, p50 %.3fs p99 %.3fs"
+                % (c5["steady"]["slices_per_s"], c5["steady"]["arrival_rate"],
+                   c5["steady"]["p50_latency_s"], c5["steady"]["p99_latency_s"]))
         extra["serve_c5"] = c5
         ing = bench_ingest(args.ingest_nnz, threads)
         log("ingest: host reader %.2f GB/s (%d threads), GPU reader %.2f GB/s (parse alone "
