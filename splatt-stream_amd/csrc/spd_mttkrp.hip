@@ -305,6 +305,39 @@ k_root3(Root3Args<T> a)
   if(wn > 0) {
     S0 = a.plan[2ull * item0 + 0];
     F0 = a.plan[2ull * item0 + 1];
+    // this lane's PER consecutive leaf ids and values, issued first: one
+    // 16-byte load each (two for FP64 values) when all PER lie in the range
+    static_assert(PER == 4, "the vector staging loads assume 4 positions per lane");
+    uint32_t lf[PER];
+    T lv[PER];
+    if(lane * PER + PER <= wn) {
+      uint4 const l4 = *(uint4 const *) (a.leaf + W0 + lane * PER);
+      lf[0] = l4.x;
+      lf[1] = l4.y;
+      lf[2] = l4.z;
+      lf[3] = l4.w;
+      if constexpr (sizeof(T) == 4) {
+        float4 const v4 = *(float4 const *) (a.vals + W0 + lane * PER);
+        lv[0] = v4.x;
+        lv[1] = v4.y;
+        lv[2] = v4.z;
+        lv[3] = v4.w;
+      } else {
+        double2 const v0 = *(double2 const *) (a.vals + W0 + lane * PER);
+        double2 const v1 = *(double2 const *) (a.vals + W0 + lane * PER + 2);
+        lv[0] = v0.x;
+        lv[1] = v0.y;
+        lv[2] = v1.x;
+        lv[3] = v1.y;
+      }
+    } else {
+      #pragma unroll
+      for(int q=0; q < PER; ++q) {
+        uint32_t const p = lane * PER + q;
+        lf[q] = (p < wn) ? a.leaf[W0 + p] : 0u;
+        lv[q] = (p < wn) ? a.vals[W0 + p] : (T) 0;
+      }
+    }
     uint32_t const nfw = (a.nfibers - F0 < (uint32_t) (R3_WN + 1)) ? a.nfibers - F0 : (uint32_t) (R3_WN + 1);
     uint32_t const nsw = (a.nslices - S0 < (uint32_t) (R3_WN + 1)) ? a.nslices - S0 : (uint32_t) (R3_WN + 1);
     for(uint32_t k=lane; k < R3_WN; k += 64) {
@@ -349,12 +382,8 @@ k_root3(Root3Args<T> a)
       fpre += fm[q];
       spre += sm[q];
       uint32_t const p = lane * PER + q;
-      uint32_t leaf = 0;
-      T v = (T) 0;
-      if(p < wn) {
-        leaf = a.leaf[W0 + p];
-        v = a.vals[W0 + p];
-      }
+      uint32_t const leaf = lf[q];
+      T const v = lv[q];
       uint4 r;
       r.x = leaf * rowb;
       r.y = s_arow[wave][fpre];
