@@ -422,7 +422,7 @@ def bench_serve_c5(nstreams, per_stream, rate, inflight, steady_slices=0):
                throughput_ratio=batched["slices_per_s"] / serial["slices_per_s"])
     if steady_slices > 0:
         out["steady"] = bench_serve_steady(nstreams, steady_slices,
-                                           0.55 * batched["slices_per_s"], inflight)
+                                           0.4 * batched["slices_per_s"], inflight)
     return out
 
 
