@@ -768,7 +768,10 @@ __device__ __forceinline__ double dn_prox_apply(double p, int prox, double nc)
 // test of iteration 1 against P_0, D_0). MODE 2: pass j >= 3 (state
 // P'_{j-1}, test against P'_{j-2}). Block partials: column sums of squares
 // of P'_j and the four test terms.
-template<int TT, typename TM, int MODE>
+// O32: every matrix of the pass is below 2 GiB, so element offsets are
+// formed as 32-bit byte offsets from the (uniform) base pointers: one
+// offset register per load instead of a 64-bit address per matrix.
+template<int TT, typename TM, int MODE, bool O32 = false>
 __device__ __forceinline__ void
 b_admm_pass(Admm2Args<TM> a,
     uint32_t blk, uint32_t nblk)
@@ -818,17 +821,38 @@ b_admm_pass(Admm2Args<TM> a,
       uint64_t const row = row0 + r;
       bool const ok = lok && r < 16 && row < I;
       uint64_t const idx = ok ? row * R + lc : 0;
-      mv[q] = ok ? (double) a.M[idx] : 0.;
-      if(MODE == 0) {
-        cv[q] = ok ? a.P0[idx] : 0.;
-        xv[q] = ok ? a.D0[idx] : 0.;
-      } else if(MODE == 1) {
-        cv[q] = ok ? a.cur[idx] : 0.;
-        xv[q] = ok ? a.P0[idx] : 0.;
-        yv[q] = ok ? a.D0[idx] : 0.;
+      if constexpr (O32) {
+        uint32_t const i32 = (uint32_t) idx;
+        uint32_t const bm = i32 * (uint32_t) sizeof(TM);
+        uint32_t const bd = i32 * (uint32_t) sizeof(double);
+        auto ld = [&](double const * base) -> double {
+          return *(double const *) ((char const *) base + bd);
+        };
+        mv[q] = ok ? (double) *(TM const *) ((char const *) a.M + bm) : 0.;
+        if(MODE == 0) {
+          cv[q] = ok ? ld(a.P0) : 0.;
+          xv[q] = ok ? ld(a.D0) : 0.;
+        } else if(MODE == 1) {
+          cv[q] = ok ? ld(a.cur) : 0.;
+          xv[q] = ok ? ld(a.P0) : 0.;
+          yv[q] = ok ? ld(a.D0) : 0.;
+        } else {
+          cv[q] = ok ? ld(a.cur) : 0.;
+          xv[q] = ok ? ld(a.prev) : 0.;
+        }
       } else {
-        cv[q] = ok ? a.cur[idx] : 0.;
-        xv[q] = ok ? a.prev[idx] : 0.;
+        mv[q] = ok ? (double) a.M[idx] : 0.;
+        if(MODE == 0) {
+          cv[q] = ok ? a.P0[idx] : 0.;
+          xv[q] = ok ? a.D0[idx] : 0.;
+        } else if(MODE == 1) {
+          cv[q] = ok ? a.cur[idx] : 0.;
+          xv[q] = ok ? a.P0[idx] : 0.;
+          yv[q] = ok ? a.D0[idx] : 0.;
+        } else {
+          cv[q] = ok ? a.cur[idx] : 0.;
+          xv[q] = ok ? a.prev[idx] : 0.;
+        }
       }
     }
     double dv[NQ];
@@ -895,7 +919,11 @@ b_admm_pass(Admm2Args<TM> a,
         uint64_t const row = row0 + rl;
         if(row < I && col < R) {
           double const p = acc[t][r] - s_w[wave * 16 * LD + rl * LD + col];
-          a.out[row * R + col] = p;
+          if constexpr (O32) {
+            *(double *) ((char *) a.out + (uint32_t) (row * R + col) * (uint32_t) sizeof(double)) = p;
+          } else {
+            a.out[row * R + col] = p;
+          }
           double const q = (prox == DN_PROX_MAXCOLNORM_NONNEG && p < 0.) ? 0. : p;
           cs[t] += q * q;
         }
@@ -938,11 +966,11 @@ b_admm_pass(Admm2Args<TM> a,
   if(threadIdx.x < 4) part[(uint64_t) (RP + threadIdx.x) * pstride] = s_w[threadIdx.x * DN_THREADS];
 }
 
-template<int TT, typename TM, int MODE>
+template<int TT, typename TM, int MODE, bool O32>
 __global__ void __launch_bounds__(DN_THREADS)
 k_admm_pass(Admm2Args<TM> a)
 {
-  b_admm_pass<TT, TM, MODE>(a, blockIdx.x, gridDim.x);
+  b_admm_pass<TT, TM, MODE, O32>(a, blockIdx.x, gridDim.x);
 }
 
 // After pass j (one block of RD_THREADS): the block partials (entry-major)
@@ -1825,12 +1853,24 @@ static int launch_admm2(Admm2Args<TM> const & a, int j, uint32_t nb, double * no
     double * sums, int * ctl, double tol, hipStream_t st)
 {
   constexpr int E = TT * 16 + 4;
-  if(j == 1) {
-    hipLaunchKernelGGL((k_admm_pass<TT, TM, 0>), dim3(nb), dim3(DN_THREADS), 0, st, a);
-  } else if(j == 2) {
-    hipLaunchKernelGGL((k_admm_pass<TT, TM, 1>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+  // every matrix of the pass (I x R, 8 bytes at most) below 2 GiB
+  bool const o32 = (uint64_t) a.I * a.R * sizeof(double) < (1ull << 31);
+  if(o32) {
+    if(j == 1) {
+      hipLaunchKernelGGL((k_admm_pass<TT, TM, 0, true>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+    } else if(j == 2) {
+      hipLaunchKernelGGL((k_admm_pass<TT, TM, 1, true>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+    } else {
+      hipLaunchKernelGGL((k_admm_pass<TT, TM, 2, true>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+    }
   } else {
-    hipLaunchKernelGGL((k_admm_pass<TT, TM, 2>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+    if(j == 1) {
+      hipLaunchKernelGGL((k_admm_pass<TT, TM, 0, false>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+    } else if(j == 2) {
+      hipLaunchKernelGGL((k_admm_pass<TT, TM, 1, false>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+    } else {
+      hipLaunchKernelGGL((k_admm_pass<TT, TM, 2, false>), dim3(nb), dim3(DN_THREADS), 0, st, a);
+    }
   }
   SPD_LAUNCH_CHECK();
   hipLaunchKernelGGL((k_admm_pass_reduce<E>), dim3(1), dim3(RD_THREADS), 0, st,
