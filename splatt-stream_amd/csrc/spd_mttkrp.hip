@@ -748,12 +748,12 @@ k_walks(WsArgs<T> a)
   constexpr int HS = PRIV ? WS_WN : 1;
   __shared__ uint32_t s_row[WS_WAVES][NLS][WS_WN];  // row id of local node k
   // child pointers, then node start positions, then gathered row ids
-  __shared__ uint32_t s_tmp[WS_WAVES][NL][WS_WN];
+  __shared__ __align__(16) uint32_t s_tmp[WS_WAVES][NL][WS_WN];
   __shared__ uint8_t s_mk[WS_WAVES][NLS][WS_WN];    // node-start marks
-  __shared__ uint32_t s_orow[WS_WAVES][WS_WN];      // output row per position
+  __shared__ __align__(16) uint32_t s_orow[WS_WAVES][WS_WN];   // output row per position
   __shared__ int32_t s_osl[WS_WAVES][HS];           // its LDS slot
   __shared__ int32_t s_nsl[WS_WAVES][HS];           // slot of output-level node k
-  __shared__ T s_val[WS_WAVES][KIND == KIND_EXPAND ? 1 : WS_WN];
+  __shared__ __align__(16) T s_val[WS_WAVES][KIND == KIND_EXPAND ? 1 : WS_WN];
   extern __shared__ __align__(16) unsigned char ws_dyn[];
   T * hot = (T *) ws_dyn;
 
@@ -777,14 +777,30 @@ k_walks(WsArgs<T> a)
     uint64_t const W1 = (W0 + WS_WN < a.nnz) ? W0 + WS_WN : a.nnz;
     uint32_t const wn = (uint32_t) (W1 - W0);
     if constexpr (!STRUCT) {
-      // a row-sorted copy: records are stored as they are walked
-      #pragma unroll
-      for(int q=0; q < NG; ++q) {
-        for(uint32_t p=lane; p < wn; p += 64) s_tmp[wave][q][p] = a.cg[q][W0 + p];
-      }
-      for(uint32_t p=lane; p < wn; p += 64) {
-        s_orow[wave][p] = a.crow[W0 + p];
-        s_val[wave][p] = a.vals[W0 + p];
+      // a row-sorted copy: records are stored as they are walked; each lane
+      // copies PER consecutive positions with 16-byte loads and stores when
+      // all of them lie in the chunk
+      static_assert(PER == 4, "the vector copy assumes 4 positions per lane");
+      uint32_t const p0 = (uint32_t) lane * PER;
+      if(p0 + PER <= wn) {
+        #pragma unroll
+        for(int q=0; q < NG; ++q) {
+          *(uint4 *) &s_tmp[wave][q][p0] = *(uint4 const *) (a.cg[q] + W0 + p0);
+        }
+        *(uint4 *) &s_orow[wave][p0] = *(uint4 const *) (a.crow + W0 + p0);
+        if constexpr (sizeof(T) == 4) {
+          *(float4 *) &s_val[wave][p0] = *(float4 const *) (a.vals + W0 + p0);
+        } else {
+          *(double2 *) &s_val[wave][p0] = *(double2 const *) (a.vals + W0 + p0);
+          *(double2 *) &s_val[wave][p0 + 2] = *(double2 const *) (a.vals + W0 + p0 + 2);
+        }
+      } else {
+        for(uint32_t p=p0; p < wn && p < p0 + PER; ++p) {
+          #pragma unroll
+          for(int q=0; q < NG; ++q) s_tmp[wave][q][p] = a.cg[q][W0 + p];
+          s_orow[wave][p] = a.crow[W0 + p];
+          s_val[wave][p] = a.vals[W0 + p];
+        }
       }
     } else {
       // nodes overlapping the chunk: from the node holding W0 to the node
@@ -854,13 +870,49 @@ k_walks(WsArgs<T> a)
         }
       }
       __builtin_amdgcn_wave_barrier();
-      // records
+      // records; each lane's PER leaf ids (and values) with one 16-byte
+      // load each when all of them lie in the chunk
       bool const leaf_out = (KIND == KIND_LEAF) || (KIND == KIND_EXPAND && d == NL);
+      uint32_t lf[PER];
+      T lv[PER];
+      {
+        uint32_t const p0 = (uint32_t) lane * PER;
+        if(p0 + PER <= wn) {
+          uint4 const l4 = *(uint4 const *) (a.fids[N - 1] + W0 + p0);
+          lf[0] = l4.x;
+          lf[1] = l4.y;
+          lf[2] = l4.z;
+          lf[3] = l4.w;
+          if constexpr (KIND != KIND_EXPAND) {
+            if constexpr (sizeof(T) == 4) {
+              float4 const v4 = *(float4 const *) (a.vals + W0 + p0);
+              lv[0] = v4.x;
+              lv[1] = v4.y;
+              lv[2] = v4.z;
+              lv[3] = v4.w;
+            } else {
+              double2 const v0 = *(double2 const *) (a.vals + W0 + p0);
+              double2 const v1 = *(double2 const *) (a.vals + W0 + p0 + 2);
+              lv[0] = v0.x;
+              lv[1] = v0.y;
+              lv[2] = v1.x;
+              lv[3] = v1.y;
+            }
+          }
+        } else {
+          #pragma unroll
+          for(int q=0; q < PER; ++q) {
+            uint32_t const p = p0 + q;
+            lf[q] = (p < wn) ? a.fids[N - 1][W0 + p] : 0u;
+            if constexpr (KIND != KIND_EXPAND) lv[q] = (p < wn) ? a.vals[W0 + p] : (T) 0;
+          }
+        }
+      }
       #pragma unroll
       for(int q=0; q < PER; ++q) {
         uint32_t const p = lane * PER + q;
         if(p < wn) {
-          uint32_t const leaf = a.fids[N - 1][W0 + p];
+          uint32_t const leaf = lf[q];
           uint32_t orow = leaf;
           int32_t osl = -1;
           #pragma unroll
@@ -888,7 +940,7 @@ k_walks(WsArgs<T> a)
           } else {
             s_orow[wave][p] = orow;
             if constexpr (PRIV) s_osl[wave][p] = osl;
-            s_val[wave][p] = a.vals[W0 + p];
+            s_val[wave][p] = lv[q];
           }
         }
       }
