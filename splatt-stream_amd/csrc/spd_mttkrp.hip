@@ -242,6 +242,25 @@ combine_runs(uint32_t const * rows, T const * vals, int ld, uint32_t e0, uint32_
   orow[1] = r1;
 }
 
+// Sum of v over the G = 64/L groups of a wave, returned to every lane (lanes
+// of one column position c in different groups are L apart). Addition is
+// commutative, so every group gets the same bits.
+template<typename T, int V, int L>
+__device__ __forceinline__ typename VecT<T, V>::type
+group_allsum(typename VecT<T, V>::type v)
+{
+  #pragma unroll
+  for(int o=L; o < 64; o <<= 1) {
+    if constexpr (V == 1) {
+      v += __shfl_xor(v, o, 64);
+    } else {
+      #pragma unroll
+      for(int k=0; k < V; ++k) v[k] += __shfl_xor(v[k], o, 64);
+    }
+  }
+  return v;
+}
+
 // HOIST: the fiber's A row is gathered once per fiber (and at the start of
 // each item) and multiplied into the slice sum at the fiber's end, as the
 // reference factors it; otherwise both rows are gathered per nonzero.
@@ -397,7 +416,6 @@ k_root3(Root3Args<T> a)
       s_rec[wave][p + p / S] = r;
     }
   }
-  for(uint32_t e=threadIdx.x; e < NE; e += R3_WAVES * 64) s_crow[e] = CARRY_NONE;
   __syncthreads();
 
   // ---- walk this group's item ----
@@ -408,6 +426,11 @@ k_root3(Root3Args<T> a)
   uint32_t const e0 = (wave * G + g) * 2;
   char const * const Bb = (char const *) a.B;
   char const * const Ab = (char const *) a.A;
+  // the item's partial slices: H, the first slice when it began before the
+  // item and ends inside it or at its end (its sum goes to LDS entry e0 when
+  // it ends); T, the last slice when it continues past the item's end
+  uint32_t hrow = CARRY_NONE, trow = CARRY_NONE;
+  VT tval = vzero<VT>();
   if(n0 < a.nnz) {
     uint64_t const n1 = (n0 + S < a.nnz) ? n0 + S : a.nnz;
     bool head_partial = a.head[item] != 0;
@@ -428,7 +451,7 @@ k_root3(Root3Args<T> a)
       // slice cur_s ended before this nonzero
       uint32_t const orow = s_srow[wave][cur_s];
       if(head_partial) {
-        s_crow[e0] = orow;
+        hrow = orow;
         if(cok) *(VT *) &s_cval[e0][c * V] = accs;
       } else {
         vstore<T, V>(a.out, orow, R, col, cok, accs);
@@ -501,42 +524,126 @@ k_root3(Root3Args<T> a)
     if(complete && !head_partial) {
       vstore<T, V>(a.out, orow, R, col, cok, accs);
     } else if(complete) {
-      s_crow[e0] = orow;
+      hrow = orow;
       if(cok) *(VT *) &s_cval[e0][c * V] = accs;
     } else {
-      s_crow[e0 + 1] = orow;
-      if(cok) *(VT *) &s_cval[e0 + 1][c * V] = accs;
+      trow = orow;
+      tval = accs;
     }
   }
   // ---- combine partial slices: per wave, then per block ----
-  // Entries are in nonzero order, so one slice's entries are adjacent. A run
-  // (entries of one row) that may continue past an edge is kept open; a run
-  // closed on both sides is complete and stored. Open runs of a wave go to
-  // its two wave slots (slot 1 if the run reaches the wave's end), and the
-  // block combines the wave slots the same way into its two carry rows.
-  if(g == 0) {
-    uint32_t const e_begin = wave * G * 2;
+  // Entries (H and T of every item) are in nonzero order, so one slice's
+  // entries are adjacent. A run (entries of one row) that may continue past
+  // an edge is kept open; a run closed on both sides is complete and stored.
+  // Open runs of a wave go to its two wave slots (slot 1 if the run reaches
+  // the wave's end, slot 0 if it began before the wave), and the block
+  // combines the wave slots the same way into its two carry rows. When all
+  // of a level's entries belong to one row (a wave inside one long slice, the
+  // common case) they form a single run, summed across groups in registers;
+  // otherwise the entries go through LDS to a serial walk over the runs.
+  {
+    uint32_t const x = (hrow != CARRY_NONE) ? hrow : trow;
+    uint64_t const m = __ballot(x != CARRY_NONE);
+    uint32_t ref = CARRY_NONE;
+    bool same = true;
+    if(m != 0) {
+      ref = __shfl(x, __ffsll((unsigned long long) m) - 1, 64);
+      same = __all((hrow == CARRY_NONE || hrow == ref) && (trow == CARRY_NONE || trow == ref));
+    }
     bool const wopen_first = (item0 < a.nitems) && (wn > 0) && a.head[item0] != 0;
     uint32_t last_item = item0 + G - 1;
     if(last_item >= a.nitems) last_item = a.nitems - 1;
-    uint32_t const e_lastp1 = e_begin + (last_item - item0) * 2 + 1;
-    combine_runs<T, V>(s_crow, &s_cval[0][0], CW, e_begin, e_begin + 2 * G,
-        wn > 0 ? wopen_first : false, wn > 0 ? e_lastp1 : 0xFFFFFFFFu,
-        &s_wrow[wave][0], &s_wval[wave][0][0], CW, a.out, R, col, c, cok);
+    if(same) {
+      VT v = vzero<VT>();
+      if(hrow != CARRY_NONE && cok) v += *(VT const *) &s_cval[e0][c * V];
+      if(trow != CARRY_NONE) v += tval;
+      v = group_allsum<T, V, L>(v);
+      // the run reaches the wave's end if the last item's T entry is in it
+      uint32_t const tlast = (wn > 0) ? __shfl(trow, (int) ((last_item - item0) * L), 64) : CARRY_NONE;
+      uint32_t r0 = CARRY_NONE, r1 = CARRY_NONE;
+      if(m != 0) {
+        if(tlast != CARRY_NONE) r1 = ref;
+        else if(wopen_first) r0 = ref;
+      }
+      if(g == 0) {
+        if(r1 != CARRY_NONE) {
+          if(cok) *(VT *) &s_wval[wave][1][c * V] = v;
+        } else if(r0 != CARRY_NONE) {
+          if(cok) *(VT *) &s_wval[wave][0][c * V] = v;
+        } else if(m != 0) {
+          vstore<T, V>(a.out, ref, R, col, cok, v);
+        }
+        s_wrow[wave][0] = r0;
+        s_wrow[wave][1] = r1;
+      }
+    } else {
+      s_crow[e0] = hrow;
+      s_crow[e0 + 1] = trow;
+      if(cok) *(VT *) &s_cval[e0 + 1][c * V] = tval;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if(g == 0) {
+        uint32_t const e_begin = wave * G * 2;
+        uint32_t const e_lastp1 = e_begin + (last_item - item0) * 2 + 1;
+        combine_runs<T, V>(s_crow, &s_cval[0][0], CW, e_begin, e_begin + 2 * G,
+            wn > 0 ? wopen_first : false, wn > 0 ? e_lastp1 : 0xFFFFFFFFu,
+            &s_wrow[wave][0], &s_wval[wave][0][0], CW, a.out, R, col, c, cok);
+      }
+    }
   }
   __syncthreads();
-  if(wave == 0 && g == 0) {
+  if(wave == 0) {
     uint32_t const bitem0 = (uint32_t) blockIdx.x * R3_WAVES * G;
     bool const block_head = (bitem0 < a.nitems) && a.head[bitem0] != 0;
     // the last wave of the block with nonzeros
     uint64_t const bw0 = (uint64_t) blockIdx.x * R3_WAVES;
     uint64_t const nw_total = (a.nnz + R3_WN - 1) / R3_WN;
     uint32_t const lastw = (uint32_t) (((bw0 + R3_WAVES <= nw_total) ? R3_WAVES : nw_total - bw0) - 1);
-    uint32_t rows[2];
     T * cv = a.carry + (uint64_t) (2 * blockIdx.x) * R;
-    combine_runs<T, V>(&s_wrow[0][0], &s_wval[0][0][0], CW, 0, 2 * R3_WAVES,
-        block_head, 2 * lastw + 1, rows, nullptr, 0, a.out, R, col, c, cok, cv);
-    if(c == 0 && blockIdx.y == 0) {
+    // the block's 2 * R3_WAVES wave slots, entry e = 2 * wave + slot; group g
+    // holds entries g, g + G, ...
+    constexpr int NB = 2 * R3_WAVES;
+    uint32_t x = CARRY_NONE;
+    #pragma unroll
+    for(int e=(int) g; e < NB; e += G) {
+      uint32_t const r = s_wrow[e >> 1][e & 1];
+      if(r != CARRY_NONE) x = r;
+    }
+    uint64_t const m = __ballot(x != CARRY_NONE);
+    uint32_t ref = CARRY_NONE;
+    bool same_g = true;
+    if(m != 0) {
+      ref = __shfl(x, __ffsll((unsigned long long) m) - 1, 64);
+      #pragma unroll
+      for(int e=(int) g; e < NB; e += G) {
+        uint32_t const r = s_wrow[e >> 1][e & 1];
+        if(r != CARRY_NONE && r != ref) same_g = false;
+      }
+    }
+    bool const same = __all(same_g);
+    uint32_t rows[2] = {CARRY_NONE, CARRY_NONE};
+    if(same) {
+      VT v = vzero<VT>();
+      #pragma unroll
+      for(int e=(int) g; e < NB; e += G) {
+        if(s_wrow[e >> 1][e & 1] != CARRY_NONE) v += *(VT const *) &s_wval[e >> 1][e & 1][c * V];
+      }
+      v = group_allsum<T, V, L>(v);
+      if(m != 0) {
+        if(s_wrow[lastw][1] != CARRY_NONE) rows[1] = ref;
+        else if(block_head) rows[0] = ref;
+      }
+      if(g == 0) {
+        if(rows[1] != CARRY_NONE) vstore<T, V>(cv, 1u, R, col, cok, v);
+        else if(rows[0] != CARRY_NONE) vstore<T, V>(cv, 0u, R, col, cok, v);
+        else if(m != 0) vstore<T, V>(a.out, ref, R, col, cok, v);
+      }
+    } else if(g == 0) {
+      combine_runs<T, V>(&s_wrow[0][0], &s_wval[0][0][0], CW, 0, 2 * R3_WAVES,
+          block_head, 2 * lastw + 1, rows, nullptr, 0, a.out, R, col, c, cok, cv);
+    }
+    if(g == 0 && c == 0 && blockIdx.y == 0) {
       a.carry_row[2 * blockIdx.x] = rows[0];
       a.carry_row[2 * blockIdx.x + 1] = rows[1];
     }
