@@ -129,9 +129,11 @@ k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
 //
 // A wave owns 256 consecutive nonzeros, split into G = 64/L items of 256/G
 // nonzeros (one per group of L lanes). Before walking, the wave stages into
-// LDS, with coalesced loads: the values and leaf ids of its nonzeros, the
-// end and id of every fiber and the end and output row of every slice that
-// overlaps its range. A group then walks its item in batches of R3_BATCH
+// LDS, with coalesced loads: the values and leaf ids of its nonzeros and the
+// id of every fiber and output row of every slice that overlaps its range.
+// Where fibers and slices start comes from two bitmaps built once per CSF
+// (one bit per nonzero: a fiber starts, a slice starts); a position's local
+// fiber and slice index is a population count over the wave's 256 bits. A group then walks its item in batches of R3_BATCH
 // nonzeros: the B row and the fiber's A row of every nonzero of a batch are
 // gathered together (2 * R3_BATCH gathers in flight), and every boundary
 // test reads LDS. k_root3h (spd_root3h.hpp) is the persistent variant that
@@ -170,6 +172,7 @@ struct Root3Args {
   T * out;
   uint32_t const * plan;           // per item: slice, fiber containing its first nnz
   uint8_t const * head;            // per item: first slice began earlier
+  uint64_t const * bits;           // per 64 nonzeros: fiber-start word, slice-start word
   T * carry;                       // 2 rows per block
   uint32_t * carry_row;
   uint64_t nnz;
@@ -282,19 +285,15 @@ k_root3(Root3Args<T> a)
   __shared__ T s_v64[sizeof(T) == 8 ? R3_WAVES : 1][sizeof(T) == 8 ? R3_WN + G : 1];
   __shared__ uint32_t s_srow[R3_WAVES][R3_WN + 1];   // output row of slice S0 + k
   __shared__ uint32_t s_crow[NE];
-  // staging-only arrays share their LDS with the walk's partial rows (the
-  // block barrier between staging and walking separates the two uses):
-  // s_arow (A row offset of local fiber k), s_fi / s_si (fiber- and
-  // slice-start marks), then s_cval (two partial rows per item). With the
-  // batch of 6 this keeps five blocks (20 waves) per CU.
+  // the staging-only array s_arow (A row offset of local fiber k) shares its
+  // LDS with the walk's partial rows s_cval (two per item; the block barrier
+  // between staging and walking separates the two uses). With the batch of
+  // 6 this keeps five blocks (20 waves) per CU.
   constexpr int SZ_AROW = R3_WAVES * (R3_WN + 1) * 4;
-  constexpr int SZ_MARK = R3_WAVES * R3_WN * 2;
-  constexpr int SZ_STAGE = SZ_AROW + 2 * SZ_MARK;
+  constexpr int SZ_STAGE = SZ_AROW;
   constexpr int SZ_WALK = NE * CW * (int) sizeof(T);
   __shared__ __align__(16) unsigned char s_u[SZ_STAGE > SZ_WALK ? SZ_STAGE : SZ_WALK];
   auto s_arow = reinterpret_cast<uint32_t (*)[R3_WN + 1]>(s_u);
-  auto s_fi = reinterpret_cast<uint16_t (*)[R3_WN]>(s_u + SZ_AROW);
-  auto s_si = reinterpret_cast<uint16_t (*)[R3_WN]>(s_u + SZ_AROW + SZ_MARK);
   auto s_cval = reinterpret_cast<T (*)[CW]>(s_u);
   __shared__ uint32_t s_wrow[R3_WAVES][2];
   __shared__ __align__(16) T s_wval[R3_WAVES][2][CW];
@@ -317,9 +316,11 @@ k_root3(Root3Args<T> a)
   uint32_t const wn = (W0 < a.nnz) ? (uint32_t) (W1 - W0) : 0u;
 
   // ---- stage this wave's range ----
-  // Each lane owns PER consecutive positions p = lane*PER + q. Fiber and
-  // slice starts inside the range are marked and turned into local indices
-  // with a wave prefix sum; each position then gets its record.
+  // Each lane owns PER consecutive positions p = lane*PER + q, inside word
+  // lane/16 of the wave's four 64-position bitmap words. The local index of
+  // a position's fiber (slice) is the number of fiber (slice) starts at
+  // positions 1..p; local index 0 is fiber F0 (slice S0), the one holding
+  // the wave's first nonzero. Each position then gets its record.
   uint32_t F0 = 0, S0 = 0;
   if(wn > 0) {
     S0 = a.plan[2ull * item0 + 0];
@@ -357,49 +358,50 @@ k_root3(Root3Args<T> a)
         lv[q] = (p < wn) ? a.vals[W0 + p] : (T) 0;
       }
     }
-    uint32_t const nfw = (a.nfibers - F0 < (uint32_t) (R3_WN + 1)) ? a.nfibers - F0 : (uint32_t) (R3_WN + 1);
-    uint32_t const nsw = (a.nslices - S0 < (uint32_t) (R3_WN + 1)) ? a.nslices - S0 : (uint32_t) (R3_WN + 1);
-    for(uint32_t k=lane; k < R3_WN; k += 64) {
-      s_fi[wave][k] = 0;
-      s_si[wave][k] = 0;
+    // this lane's fiber-start and slice-start words (16 lanes share one)
+    uint4 const bw = *(uint4 const *) (a.bits + 2 * (wid * 4 + (uint64_t) (lane >> 4)));
+    uint64_t fw = (uint64_t) bw.x | ((uint64_t) bw.y << 32);
+    uint64_t sw = (uint64_t) bw.z | ((uint64_t) bw.w << 32);
+    if(lane < 16) {
+      // position 0 belongs to F0 and S0 whether or not it starts them
+      fw &= ~1ull;
+      sw &= ~1ull;
     }
+    uint32_t const fcnt = (uint32_t) __popcll(fw);
+    uint32_t const scnt = (uint32_t) __popcll(sw);
+    uint32_t const fc0 = __builtin_amdgcn_readlane(fcnt, 0), fc1 = __builtin_amdgcn_readlane(fcnt, 16);
+    uint32_t const fc2 = __builtin_amdgcn_readlane(fcnt, 32), fc3 = __builtin_amdgcn_readlane(fcnt, 48);
+    uint32_t const sc0 = __builtin_amdgcn_readlane(scnt, 0), sc1 = __builtin_amdgcn_readlane(scnt, 16);
+    uint32_t const sc2 = __builtin_amdgcn_readlane(scnt, 32), sc3 = __builtin_amdgcn_readlane(scnt, 48);
+    uint32_t const wq = (uint32_t) lane >> 4;
+    uint32_t const sh = ((uint32_t) lane & 15u) * PER;
+    uint64_t const below = (1ull << sh) - 1ull;
+    // starts at positions before this lane's first one
+    uint32_t fpre = (wq > 0 ? fc0 : 0u) + (wq > 1 ? fc1 : 0u) + (wq > 2 ? fc2 : 0u) +
+                    (uint32_t) __popcll(fw & below);
+    uint32_t spre = (wq > 0 ? sc0 : 0u) + (wq > 1 ? sc1 : 0u) + (wq > 2 ? sc2 : 0u) +
+                    (uint32_t) __popcll(sw & below);
+    uint32_t const f4 = (uint32_t) (fw >> sh) & 15u;
+    uint32_t const s4 = (uint32_t) (sw >> sh) & 15u;
+    // fibers F0 .. F0+nfw-1 and slices S0 .. S0+nsw-1 overlap the range
+    uint32_t const nfw = 1u + fc0 + fc1 + fc2 + fc3;
+    uint32_t const nsw = 1u + sc0 + sc1 + sc2 + sc3;
     for(uint32_t k=lane; k < nsw; k += 64) {
       s_srow[wave][k] = a.rows_are_positions ? S0 + k : a.fids0[S0 + k];
     }
     for(uint32_t k=lane; k < nfw; k += 64) {
       s_arow[wave][k] = a.fids1[F0 + k] * rowb;
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    // fiber k (local, >= 1) starts at fptr1[F0+k]; slice q (local, >= 1)
-    // starts at fiber fptr0[S0+q], i.e. at nonzero fptr1[fptr0[S0+q]]
-    for(uint32_t k=1 + lane; k < nfw; k += 64) {
-      uint64_t const st = a.fptr1[F0 + k];
-      if(st >= W1) break;
-      s_fi[wave][st - W0] = 1;
-    }
-    for(uint32_t q=1 + lane; q < nsw; q += 64) {
-      uint32_t const fs = a.fptr0[S0 + q];
-      if(fs >= a.nfibers) break;
-      uint64_t const st = a.fptr1[fs];
-      if(st >= W1) break;
-      s_si[wave][st - W0] = 1;
-    }
-    __builtin_amdgcn_wave_barrier();
-    uint32_t fm[PER], sm[PER];
-    uint32_t fsum = 0, ssum = 0;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t fm[PER];
     #pragma unroll
-    for(int q=0; q < PER; ++q) {
-      fm[q] = s_fi[wave][lane * PER + q];
-      sm[q] = s_si[wave][lane * PER + q];
-      fsum += fm[q];
-      ssum += sm[q];
-    }
-    uint32_t fpre = wave_incl_scan(fsum) - fsum;
-    uint32_t spre = wave_incl_scan(ssum) - ssum;
+    for(int q=0; q < PER; ++q) fm[q] = (f4 >> q) & 1u;
     #pragma unroll
     for(int q=0; q < PER; ++q) {
       fpre += fm[q];
-      spre += sm[q];
+      spre += (s4 >> q) & 1u;
       uint32_t const p = lane * PER + q;
       uint32_t const leaf = lf[q];
       T const v = lv[q];
@@ -516,9 +518,8 @@ k_root3(Root3Args<T> a)
       complete = (s_rec[wave][k1 + k1 / S].z & ~R3_FST) != cur_s;
     } else if(!complete) {
       // the item ends at the wave's end: the slice is complete iff the
-      // next slice starts exactly there
-      uint32_t const fs = a.fptr0[S0 + cur_s + 1];
-      complete = (fs < a.nfibers) && ((uint64_t) a.fptr1[fs] == n1);
+      // next item's first slice starts there
+      complete = a.head[item + 1] == 0;
     }
     uint32_t const orow = s_srow[wave][cur_s];
     if(complete && !head_partial) {
@@ -1108,6 +1109,7 @@ struct MttkrpPlan {
   uint32_t nitems;
   uint32_t * nodes;     // nitems x (N-1)
   uint8_t * head;       // nitems
+  uint64_t * bits;      // 3-mode CSFs: per 64 nonzeros, fiber- and slice-start words
   MttkrpPlan * next;
 };
 
@@ -1154,6 +1156,40 @@ __global__ void k_plan(PlanArgs p)
   p.head[it] = (s < n0) ? 1 : 0;
 }
 
+// Start bitmaps of a 3-mode CSF: for the 64 nonzeros of word w, bit j of
+// bits[2w] is set when a fiber starts at nonzero 64w + j and bit j of
+// bits[2w + 1] when a slice starts there. One thread per word; fibers and
+// slices are never empty, so a word has at most 64 of each.
+__global__ void k_start_bits(uint32_t const * __restrict__ fptr0, uint32_t const * __restrict__ fptr1,
+    uint32_t nslices, uint32_t nfibers, uint64_t nwords, uint64_t * __restrict__ bits)
+{
+  uint64_t const w = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if(w >= nwords) return;
+  uint64_t const p0 = w * 64, p1 = p0 + 64;
+  uint64_t fb = 0, sb = 0;
+  // first fiber starting at or after p0
+  uint32_t lo = 0, hi = nfibers;
+  while(lo < hi) {
+    uint32_t const mid = lo + ((hi - lo) >> 1);
+    if((uint64_t) fptr1[mid] < p0) lo = mid + 1; else hi = mid;
+  }
+  for(uint32_t f=lo; f < nfibers && (uint64_t) fptr1[f] < p1; ++f) fb |= 1ull << (fptr1[f] - p0);
+  // first slice starting at or after p0 (slice s starts at fptr1[fptr0[s]])
+  lo = 0;
+  hi = nslices;
+  while(lo < hi) {
+    uint32_t const mid = lo + ((hi - lo) >> 1);
+    if((uint64_t) fptr1[fptr0[mid]] < p0) lo = mid + 1; else hi = mid;
+  }
+  for(uint32_t q=lo; q < nslices; ++q) {
+    uint64_t const st = fptr1[fptr0[q]];
+    if(st >= p1) break;
+    sb |= 1ull << (st - p0);
+  }
+  bits[2 * w] = fb;
+  bits[2 * w + 1] = sb;
+}
+
 // Rows of one output level ranked by how many nodes of that level carry
 // them (descending): rank[row] < nhot selects the rows a block privatises.
 struct HotPlan {
@@ -1196,6 +1232,7 @@ void spd_csf_drop_plans(spd_csf * c)
     MttkrpPlan * nx = p->next;
     spd_free(p->nodes);
     spd_free(p->head);
+    spd_free(p->bits);
     free(p);
     p = nx;
   }
@@ -1320,10 +1357,14 @@ static int get_plan(spd_csf * c, uint32_t S, MttkrpPlan ** out, hipStream_t st)
   p->S = S;
   p->nitems = (uint32_t) spd_cdiv(c->nnz, S);
   int const NL = c->nmodes - 1;
+  // 3-mode CSFs: start bitmaps padded to whole 256-nonzero waves of the root
+  // kernel (the words past the last nonzero are zero)
+  uint64_t const nwords = (c->nmodes == 3) ? 4 * spd_cdiv(c->nnz, 256) : 0;
   int rc = spd_malloc((void **) &p->nodes, (uint64_t) p->nitems * NL * sizeof(uint32_t));
   if(rc == SPD_OK) rc = spd_malloc((void **) &p->head, p->nitems);
+  if(rc == SPD_OK && nwords > 0) rc = spd_malloc((void **) &p->bits, 2 * nwords * sizeof(uint64_t));
   if(rc != SPD_OK) {
-    spd_free(p->nodes); spd_free(p->head); free(p);
+    spd_free(p->nodes); spd_free(p->head); spd_free(p->bits); free(p);
     return rc;
   }
   PlanArgs pa = {};
@@ -1338,8 +1379,13 @@ static int get_plan(spd_csf * c, uint32_t S, MttkrpPlan ** out, hipStream_t st)
   pa.nodes = p->nodes;
   pa.head = p->head;
   hipLaunchKernelGGL(k_plan, dim3((unsigned) spd_cdiv(p->nitems, 256)), dim3(256), 0, st, pa);
+  if(nwords > 0) {
+    hipLaunchKernelGGL(k_start_bits, dim3((unsigned) spd_cdiv(nwords, 256)), dim3(256), 0, st,
+        (uint32_t const *) c->u.fptr[0], (uint32_t const *) c->u.fptr[1], (uint32_t) c->u.nfibs[0],
+        (uint32_t) c->u.nfibs[1], nwords, p->bits);
+  }
   if(hipGetLastError() != hipSuccess) {
-    spd_free(p->nodes); spd_free(p->head); free(p);
+    spd_free(p->nodes); spd_free(p->head); spd_free(p->bits); free(p);
     spd_set_error(SPD_EHIP, "plan launch failed");
     return SPD_EHIP;
   }
@@ -1784,6 +1830,7 @@ static int run_root3h(spd_csf * c, uint64_t R, void * const * mats, void * out,
   a.out = (T *) out;
   a.plan = plan->nodes;
   a.head = plan->head;
+  a.bits = plan->bits;
   a.carry = (T *) cv;
   a.carry_row = (uint32_t *) cr;
   a.nnz = c->nnz;
@@ -1854,6 +1901,7 @@ static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
   a.out = (T *) out;
   a.plan = plan->nodes;
   a.head = plan->head;
+  a.bits = plan->bits;
   a.carry = (T *) cv;
   a.carry_row = (uint32_t *) cr;
   a.nnz = c->nnz;
