@@ -42,6 +42,7 @@ struct spd_csf {
   struct HotPlan * hot;            // output-row rankings, one per level
   struct RowCopy * copies;         // row-sorted copies, one per output level
   struct RootHot * roothot;        // hot-encoded leaf ids of the 3-mode root kernel
+  uint64_t * startbits;            // node starts: per 64 nonzeros, one word per level < N-1
 };
 
 void spd_csf_drop_plans(spd_csf * c);

@@ -172,7 +172,7 @@ struct Root3Args {
   T * out;
   uint32_t const * plan;           // per item: slice, fiber containing its first nnz
   uint8_t const * head;            // per item: first slice began earlier
-  uint64_t const * bits;           // per 64 nonzeros: fiber-start word, slice-start word
+  uint64_t const * bits;           // per 64 nonzeros: slice-start word, fiber-start word
   T * carry;                       // 2 rows per block
   uint32_t * carry_row;
   uint64_t nnz;
@@ -358,10 +358,10 @@ k_root3(Root3Args<T> a)
         lv[q] = (p < wn) ? a.vals[W0 + p] : (T) 0;
       }
     }
-    // this lane's fiber-start and slice-start words (16 lanes share one)
+    // this lane's slice-start and fiber-start words (16 lanes share them)
     uint4 const bw = *(uint4 const *) (a.bits + 2 * (wid * 4 + (uint64_t) (lane >> 4)));
-    uint64_t fw = (uint64_t) bw.x | ((uint64_t) bw.y << 32);
-    uint64_t sw = (uint64_t) bw.z | ((uint64_t) bw.w << 32);
+    uint64_t sw = (uint64_t) bw.x | ((uint64_t) bw.y << 32);
+    uint64_t fw = (uint64_t) bw.z | ((uint64_t) bw.w << 32);
     if(lane < 16) {
       // position 0 belongs to F0 and S0 whether or not it starts them
       fw &= ~1ull;
@@ -702,6 +702,7 @@ struct WsArgs {
   T const * gm[SPD_MAX_MODES];    // factor of the q-th gathered level, levels in order
   T * out;
   uint32_t const * plan;          // per chunk: node at levels 0..N-2 holding its first nonzero
+  uint64_t const * bits;          // node starts: per 64 nonzeros, one word per level < N-1
   int32_t const * hot_slot;       // internal/leaf: LDS slot of an output row (>= nhot: cold)
   uint32_t const * hot_rows;      // internal/leaf: output row of each LDS slot
   T * carry;                      // root/copy: 2 partial rows per range (R wide)
@@ -855,9 +856,8 @@ k_walks(WsArgs<T> a)
   constexpr int NLS = STRUCT ? NL : 1;
   constexpr int HS = PRIV ? WS_WN : 1;
   __shared__ uint32_t s_row[WS_WAVES][NLS][WS_WN];  // row id of local node k
-  // child pointers, then node start positions, then gathered row ids
+  // gathered row ids per position
   __shared__ __align__(16) uint32_t s_tmp[WS_WAVES][NL][WS_WN];
-  __shared__ uint8_t s_mk[WS_WAVES][NLS][WS_WN];    // node-start marks
   __shared__ __align__(16) uint32_t s_orow[WS_WAVES][WS_WN];   // output row per position
   __shared__ int32_t s_osl[WS_WAVES][HS];           // its LDS slot
   __shared__ int32_t s_nsl[WS_WAVES][HS];           // slot of output-level node k
@@ -911,16 +911,37 @@ k_walks(WsArgs<T> a)
         }
       }
     } else {
-      // nodes overlapping the chunk: from the node holding W0 to the node
-      // holding the next chunk's first nonzero (at most wn of them)
+      // Nodes overlapping the chunk, per level l: local node 0 is the node
+      // holding W0 (from the plan), local node k >= 1 the k-th node starting
+      // at a position 1..wn-1. A position's local node index is the number
+      // of starts at positions 1..p, counted in the level's bitmap words
+      // (lane owns positions 4*lane..4*lane+3, inside word lane/16).
       uint32_t a0[NL], cnt[NL];
-      #pragma unroll
-      for(int l=0; l < NL; ++l) {
-        a0[l] = a.plan[(uint64_t) ch * NL + l];
-        uint32_t const last = (ch + 1 < a.nchunks) ? a.plan[(uint64_t) (ch + 1) * NL + l]
-                                                   : a.nfibs[l] - 1;
-        uint32_t const n = last - a0[l] + 1;
-        cnt[l] = n < wn ? n : wn;
+      uint32_t idx[NL][PER];
+      {
+        uint32_t const wq = (uint32_t) lane >> 4;
+        uint32_t const sh = ((uint32_t) lane & 15u) * PER;
+        uint64_t const below = (1ull << sh) - 1ull;
+        uint64_t const wbase = ((uint64_t) ch * 4 + wq) * NL;
+        #pragma unroll
+        for(int l=0; l < NL; ++l) {
+          a0[l] = a.plan[(uint64_t) ch * NL + l];
+          uint2 const b2 = *(uint2 const *) (a.bits + wbase + l);
+          uint64_t bw = (uint64_t) b2.x | ((uint64_t) b2.y << 32);
+          if(lane < 16) bw &= ~1ull;     // position 0 belongs to local node 0
+          uint32_t const bc = (uint32_t) __popcll(bw);
+          uint32_t const c0 = __builtin_amdgcn_readlane(bc, 0), c1 = __builtin_amdgcn_readlane(bc, 16);
+          uint32_t const c2 = __builtin_amdgcn_readlane(bc, 32), c3 = __builtin_amdgcn_readlane(bc, 48);
+          uint32_t pre = (wq > 0 ? c0 : 0u) + (wq > 1 ? c1 : 0u) + (wq > 2 ? c2 : 0u) +
+                         (uint32_t) __popcll(bw & below);
+          uint32_t const b4 = (uint32_t) (bw >> sh) & 15u;
+          #pragma unroll
+          for(int q=0; q < PER; ++q) {
+            pre += (b4 >> q) & 1u;
+            idx[l][q] = pre;
+          }
+          cnt[l] = 1u + c0 + c1 + c2 + c3;
+        }
       }
       #pragma unroll
       for(int l=0; l < NL; ++l) {
@@ -928,56 +949,14 @@ k_walks(WsArgs<T> a)
           uint32_t const node = a0[l] + k;
           uint32_t const row = (l == 0 && a.rows_are_positions) ? node : a.fids[l][node];
           s_row[wave][l][k] = row;
-          s_tmp[wave][l][k] = a.fptr[l][node];
           if constexpr (KIND == KIND_INTL) {
             if(l == d) s_nsl[wave][k] = (nhot > 0) ? a.hot_slot[row] : -1;
           }
         }
-        for(uint32_t p=lane; p < WS_WN; p += 64) s_mk[wave][l][p] = 0;
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      // node starts, bottom-up; local node k >= 1 starts inside the chunk
-      // (node 0 holds W0) or at or after W1 (then WS_SENT)
-      for(uint32_t k=lane; k < cnt[NL - 1]; k += 64) {
-        uint32_t const st = s_tmp[wave][NL - 1][k];
-        uint32_t pos = WS_SENT;
-        if(k > 0 && (uint64_t) st > W0 && (uint64_t) st < W1) {
-          pos = (uint32_t) (st - W0);
-          s_mk[wave][NL - 1][pos] = 1;
-        }
-        s_tmp[wave][NL - 1][k] = pos;
-      }
-      __builtin_amdgcn_wave_barrier();
-      #pragma unroll
-      for(int l=NL - 2; l >= 0; --l) {
-        for(uint32_t k=lane; k < cnt[l]; k += 64) {
-          uint32_t const ci = s_tmp[wave][l][k] - a0[l + 1];   // first child, local
-          uint32_t pos = WS_SENT;
-          if(k > 0 && ci < cnt[l + 1]) pos = s_tmp[wave][l + 1][ci];
-          if(pos < wn) s_mk[wave][l][pos] = 1;
-          s_tmp[wave][l][k] = pos;
-        }
-        __builtin_amdgcn_wave_barrier();
-      }
-      // local node index of every position at every level
-      uint32_t idx[NL][PER];
-      #pragma unroll
-      for(int l=0; l < NL; ++l) {
-        uint32_t m[PER];
-        uint32_t sum = 0;
-        #pragma unroll
-        for(int q=0; q < PER; ++q) {
-          m[q] = s_mk[wave][l][lane * PER + q];
-          sum += m[q];
-        }
-        uint32_t pre = wave_incl_scan(sum) - sum;
-        #pragma unroll
-        for(int q=0; q < PER; ++q) {
-          pre += m[q];
-          idx[l][q] = pre;
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       // records; each lane's PER leaf ids (and values) with one 16-byte
       // load each when all of them lie in the chunk
       bool const leaf_out = (KIND == KIND_LEAF) || (KIND == KIND_EXPAND && d == NL);
@@ -1109,7 +1088,6 @@ struct MttkrpPlan {
   uint32_t nitems;
   uint32_t * nodes;     // nitems x (N-1)
   uint8_t * head;       // nitems
-  uint64_t * bits;      // 3-mode CSFs: per 64 nonzeros, fiber- and slice-start words
   MttkrpPlan * next;
 };
 
@@ -1156,38 +1134,65 @@ __global__ void k_plan(PlanArgs p)
   p.head[it] = (s < n0) ? 1 : 0;
 }
 
-// Start bitmaps of a 3-mode CSF: for the 64 nonzeros of word w, bit j of
-// bits[2w] is set when a fiber starts at nonzero 64w + j and bit j of
-// bits[2w + 1] when a slice starts there. One thread per word; fibers and
-// slices are never empty, so a word has at most 64 of each.
-__global__ void k_start_bits(uint32_t const * __restrict__ fptr0, uint32_t const * __restrict__ fptr1,
-    uint32_t nslices, uint32_t nfibers, uint64_t nwords, uint64_t * __restrict__ bits)
+// Node-start bitmaps of a CSF: for the 64 nonzeros of word w, bit j of
+// bits[w * (N-1) + l] is set when a node of level l starts at nonzero
+// 64w + j (its first nonzero: follow first children down the levels). One
+// thread per word; nodes are never empty, so a word holds at most 64 starts
+// per level, and the words past the last nonzero are zero.
+__global__ void k_start_bits(PlanArgs p, uint64_t nwords, uint64_t * __restrict__ bits)
 {
   uint64_t const w = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if(w >= nwords) return;
+  int const NL = p.N - 1;
   uint64_t const p0 = w * 64, p1 = p0 + 64;
-  uint64_t fb = 0, sb = 0;
-  // first fiber starting at or after p0
-  uint32_t lo = 0, hi = nfibers;
-  while(lo < hi) {
-    uint32_t const mid = lo + ((hi - lo) >> 1);
-    if((uint64_t) fptr1[mid] < p0) lo = mid + 1; else hi = mid;
+  for(int l=0; l < NL; ++l) {
+    auto start = [&](uint32_t n) -> uint64_t {
+      uint64_t x = n;
+      for(int m=l; m < NL; ++m) x = p.fptr[m][x];
+      return x;
+    };
+    // first node of level l starting at or after p0
+    uint32_t lo = 0, hi = p.nfibs[l];
+    while(lo < hi) {
+      uint32_t const mid = lo + ((hi - lo) >> 1);
+      if(start(mid) < p0) lo = mid + 1; else hi = mid;
+    }
+    uint64_t b = 0;
+    for(uint32_t n=lo; n < p.nfibs[l]; ++n) {
+      uint64_t const st = start(n);
+      if(st >= p1) break;
+      b |= 1ull << (st - p0);
+    }
+    bits[w * NL + l] = b;
   }
-  for(uint32_t f=lo; f < nfibers && (uint64_t) fptr1[f] < p1; ++f) fb |= 1ull << (fptr1[f] - p0);
-  // first slice starting at or after p0 (slice s starts at fptr1[fptr0[s]])
-  lo = 0;
-  hi = nslices;
-  while(lo < hi) {
-    uint32_t const mid = lo + ((hi - lo) >> 1);
-    if((uint64_t) fptr1[fptr0[mid]] < p0) lo = mid + 1; else hi = mid;
+}
+
+// The node-start bitmaps of c (built once, kept with the CSF), padded to
+// whole 256-nonzero chunks.
+static int get_startbits(spd_csf * c, hipStream_t st)
+{
+  if(c->startbits != nullptr) return SPD_OK;
+  int const NL = c->nmodes - 1;
+  uint64_t const nwords = 4 * spd_cdiv(c->nnz, 256);
+  if(nwords == 0) return SPD_OK;
+  uint64_t * b = nullptr;
+  SPD_TRY(spd_malloc((void **) &b, nwords * NL * sizeof(uint64_t)));
+  PlanArgs pa = {};
+  for(int l=0; l < NL; ++l) {
+    pa.fptr[l] = c->u.fptr[l];
+    pa.nfibs[l] = (uint32_t) c->u.nfibs[l];
   }
-  for(uint32_t q=lo; q < nslices; ++q) {
-    uint64_t const st = fptr1[fptr0[q]];
-    if(st >= p1) break;
-    sb |= 1ull << (st - p0);
+  pa.N = c->nmodes;
+  pa.nnz = c->nnz;
+  hipLaunchKernelGGL(k_start_bits, dim3((unsigned) spd_cdiv(nwords, 256)), dim3(256), 0, st,
+      pa, nwords, b);
+  if(hipGetLastError() != hipSuccess) {
+    spd_free(b);
+    spd_set_error(SPD_EHIP, "node-start bitmap launch failed");
+    return SPD_EHIP;
   }
-  bits[2 * w] = fb;
-  bits[2 * w + 1] = sb;
+  c->startbits = b;
+  return SPD_OK;
 }
 
 // Rows of one output level ranked by how many nodes of that level carry
@@ -1222,6 +1227,8 @@ struct RootHot {
 
 void spd_csf_drop_plans(spd_csf * c)
 {
+  spd_free(c->startbits);
+  c->startbits = nullptr;
   if(c->roothot != nullptr) {
     spd_free(c->roothot->leafx);
     delete c->roothot;
@@ -1232,7 +1239,6 @@ void spd_csf_drop_plans(spd_csf * c)
     MttkrpPlan * nx = p->next;
     spd_free(p->nodes);
     spd_free(p->head);
-    spd_free(p->bits);
     free(p);
     p = nx;
   }
@@ -1357,14 +1363,10 @@ static int get_plan(spd_csf * c, uint32_t S, MttkrpPlan ** out, hipStream_t st)
   p->S = S;
   p->nitems = (uint32_t) spd_cdiv(c->nnz, S);
   int const NL = c->nmodes - 1;
-  // 3-mode CSFs: start bitmaps padded to whole 256-nonzero waves of the root
-  // kernel (the words past the last nonzero are zero)
-  uint64_t const nwords = (c->nmodes == 3) ? 4 * spd_cdiv(c->nnz, 256) : 0;
   int rc = spd_malloc((void **) &p->nodes, (uint64_t) p->nitems * NL * sizeof(uint32_t));
   if(rc == SPD_OK) rc = spd_malloc((void **) &p->head, p->nitems);
-  if(rc == SPD_OK && nwords > 0) rc = spd_malloc((void **) &p->bits, 2 * nwords * sizeof(uint64_t));
   if(rc != SPD_OK) {
-    spd_free(p->nodes); spd_free(p->head); spd_free(p->bits); free(p);
+    spd_free(p->nodes); spd_free(p->head); free(p);
     return rc;
   }
   PlanArgs pa = {};
@@ -1379,13 +1381,8 @@ static int get_plan(spd_csf * c, uint32_t S, MttkrpPlan ** out, hipStream_t st)
   pa.nodes = p->nodes;
   pa.head = p->head;
   hipLaunchKernelGGL(k_plan, dim3((unsigned) spd_cdiv(p->nitems, 256)), dim3(256), 0, st, pa);
-  if(nwords > 0) {
-    hipLaunchKernelGGL(k_start_bits, dim3((unsigned) spd_cdiv(nwords, 256)), dim3(256), 0, st,
-        (uint32_t const *) c->u.fptr[0], (uint32_t const *) c->u.fptr[1], (uint32_t) c->u.nfibs[0],
-        (uint32_t) c->u.nfibs[1], nwords, p->bits);
-  }
   if(hipGetLastError() != hipSuccess) {
-    spd_free(p->nodes); spd_free(p->head); spd_free(p->bits); free(p);
+    spd_free(p->nodes); spd_free(p->head); free(p);
     spd_set_error(SPD_EHIP, "plan launch failed");
     return SPD_EHIP;
   }
@@ -1830,7 +1827,7 @@ static int run_root3h(spd_csf * c, uint64_t R, void * const * mats, void * out,
   a.out = (T *) out;
   a.plan = plan->nodes;
   a.head = plan->head;
-  a.bits = plan->bits;
+  a.bits = c->startbits;
   a.carry = (T *) cv;
   a.carry_row = (uint32_t *) cr;
   a.nnz = c->nnz;
@@ -1870,7 +1867,8 @@ static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
   MttkrpPlan * plan = nullptr;
   if(fx == nullptr) {
     SPD_TRY(get_plan(c, S, &plan, spd_hs(nullptr)));
-  } else if((plan = find_plan(c, S)) == nullptr) {
+    SPD_TRY(get_startbits(c, spd_hs(nullptr)));
+  } else if((plan = find_plan(c, S)) == nullptr || c->startbits == nullptr) {
     return not_prepared("launch plan");
   }
   if(st != spd_hs(nullptr)) {
@@ -1901,7 +1899,7 @@ static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
   a.out = (T *) out;
   a.plan = plan->nodes;
   a.head = plan->head;
-  a.bits = plan->bits;
+  a.bits = c->startbits;
   a.carry = (T *) cv;
   a.carry_row = (uint32_t *) cr;
   a.nnz = c->nnz;
@@ -1978,6 +1976,7 @@ static int get_copy(spd_csf * c, int level, int rows_pos, RowCopy ** out, hipStr
   if(fr < 2 * (keep + temp) + (1ull << 30)) return SPD_OK;
   MttkrpPlan * plan = nullptr;
   SPD_TRY(get_plan(c, WS_WN, &plan, st));
+  SPD_TRY(get_startbits(c, st));
   RowCopy * r = new RowCopy();
   r->level = level;
   r->rows_pos = rows_pos;
@@ -2006,6 +2005,7 @@ static int get_copy(spd_csf * c, int level, int rows_pos, RowCopy ** out, hipStr
     for(int q=0; q < NG; ++q) a.xg[q] = xg[q];
     a.xrow = xrow;
     a.plan = plan->nodes;
+    a.bits = c->startbits;
     a.nnz = n;
     a.nchunks = plan->nitems;
     a.depth = level;
@@ -2084,7 +2084,8 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   // plans are built on the library stream; make the caller's stream wait
   if(fx == nullptr) {
     SPD_TRY(get_plan(c, WS_WN, &plan, spd_hs(nullptr)));
-  } else if((plan = find_plan(c, WS_WN)) == nullptr) {
+    SPD_TRY(get_startbits(c, spd_hs(nullptr)));
+  } else if((plan = find_plan(c, WS_WN)) == nullptr || c->startbits == nullptr) {
     return not_prepared("launch plan");
   }
   WsArgs<T> a = {};
@@ -2100,6 +2101,7 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   a.vals = (sizeof(T) == 4) ? (T const *) c->vals32 : (T const *) c->u.vals;
   a.out = (T *) out;
   a.plan = plan->nodes;
+  a.bits = c->startbits;
   a.nnz = c->nnz;
   a.nchunks = plan->nitems;
   a.R = (uint32_t) R;
