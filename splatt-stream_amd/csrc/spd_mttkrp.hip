@@ -729,19 +729,22 @@ vload_c(T const * __restrict__ base, uint32_t row, uint32_t R, uint32_t colc)
 
 // Walk positions [k0, k1) of the staged chunk: s_gid[q][k] gathered rows,
 // s_orow[k] output rows, s_osl[k] LDS slots (internal/leaf), s_val[k]
-// values. KIND is KIND_ROOT (runs unique per row: stores and carries) or
-// KIND_INTL/KIND_LEAF (every run added).
+// values. KIND is KIND_ROOT (runs unique per row: stores, and the range's
+// first and last runs returned in (r0, v0) and (r1, v1), as they may
+// continue in the neighbouring ranges; a range with one run returns it as
+// the last) or KIND_INTL/KIND_LEAF (every run added).
 template<typename T, int NG, int KIND, int V, int L, int B>
 __device__ __forceinline__ void
 walk_range(WsArgs<T> const & a, uint32_t const (*s_gid)[WS_WN], uint32_t const * s_orow,
     int32_t const * s_osl, T const * s_val, T * hot, uint32_t nhot, uint32_t k0, uint32_t k1,
-    uint32_t item, uint32_t c, uint32_t col, uint32_t colc, bool cok)
+    uint32_t c, uint32_t col, uint32_t colc, bool cok,
+    uint32_t & r0, typename VecT<T, V>::type & v0, uint32_t & r1, typename VecT<T, V>::type & v1)
 {
   typedef typename VecT<T, V>::type VT;
   constexpr int CW = L * V;
   uint32_t const R = a.R;
-  uint32_t carry0 = CARRY_NONE;
-  uint32_t carry1 = CARRY_NONE;
+  r0 = CARRY_NONE;
+  r1 = CARRY_NONE;
   if(k0 < k1) {
     uint32_t cur = s_orow[k0];
     int32_t csl = -1;
@@ -752,8 +755,8 @@ walk_range(WsArgs<T> const & a, uint32_t const (*s_gid)[WS_WN], uint32_t const *
       if constexpr (KIND == KIND_ROOT) {
         if(first) {
           // may have begun in an earlier range
-          carry0 = cur;
-          vstore<T, V>(a.carry, 2 * item, R, col, cok, acc);
+          r0 = cur;
+          v0 = acc;
         } else {
           vstore<T, V>(a.out, cur, R, col, cok, acc);
         }
@@ -822,16 +825,10 @@ walk_range(WsArgs<T> const & a, uint32_t const (*s_gid)[WS_WN], uint32_t const *
     }
     if constexpr (KIND == KIND_ROOT) {
       // the last run may continue in the next range
-      carry1 = cur;
-      vstore<T, V>(a.carry, 2 * item + 1, R, col, cok, acc);
+      r1 = cur;
+      v1 = acc;
     } else {
       flush();
-    }
-  }
-  if constexpr (KIND == KIND_ROOT) {
-    if(c == 0 && blockIdx.y == 0) {
-      a.carry_row[2 * item] = carry0;
-      a.carry_row[2 * item + 1] = carry1;
     }
   }
 }
@@ -1034,13 +1031,79 @@ k_walks(WsArgs<T> a)
     }
     __builtin_amdgcn_wave_barrier();
     if constexpr (KIND != KIND_EXPAND) {
+      typedef typename VecT<T, V>::type VT;
       uint32_t const k0 = g * S;
       uint32_t const k1 = (k0 + S < wn) ? k0 + S : wn;
+      uint32_t r0, r1;
+      VT v0 = vzero<VT>(), v1 = vzero<VT>();
       walk_range<T, NG, WALK, V, L, B>(a, s_tmp[wave], s_orow[wave], s_osl[wave], s_val[wave],
-          hot, nhot, k0, k1, ch * G + g, c, col, colc, cok);
+          hot, nhot, k0, k1, c, col, colc, cok, r0, v0, r1, v1);
+      if constexpr (WALK == KIND_ROOT) {
+        // The chunk's edge runs, 2 per group in position order: runs of one
+        // row are summed; a run closed on both sides inside the chunk is
+        // complete and stored; the chunk's first run (it may have begun in
+        // the previous chunk) and its last run (it may continue) go to the
+        // chunk's two carry rows for k_root_fixup. Every group with
+        // positions ends with a last run, so the chunk's last run is always
+        // carried.
+        uint32_t const x = (r0 != CARRY_NONE) ? r0 : r1;
+        uint64_t const m = __ballot(x != CARRY_NONE);
+        uint32_t ref = CARRY_NONE;
+        bool same = true;
+        if(m != 0) {
+          ref = __shfl(x, __ffsll((unsigned long long) m) - 1, 64);
+          same = __all((r0 == CARRY_NONE || r0 == ref) && (r1 == CARRY_NONE || r1 == ref));
+        }
+        T * cv = a.carry + (uint64_t) (2 * ch) * a.R;
+        if(same) {
+          VT v = vzero<VT>();
+          if(r0 != CARRY_NONE) v += v0;
+          if(r1 != CARRY_NONE) v += v1;
+          v = group_allsum<T, V, L>(v);
+          if(g == 0) {
+            if(m != 0) vstore<T, V>(cv, 1u, a.R, col, cok, v);
+            if(c == 0 && blockIdx.y == 0) {
+              a.carry_row[2 * ch] = CARRY_NONE;
+              a.carry_row[2 * ch + 1] = ref;
+            }
+          }
+        } else {
+          // through LDS: the records are dead once every group has walked
+          constexpr int NE = 2 * G;
+          static_assert(NE * CW * sizeof(T) <= NL * WS_WN * sizeof(uint32_t),
+              "the edge runs fit in the record space");
+          uint32_t * s_erow = s_orow[wave];
+          T * s_eval = reinterpret_cast<T *>(&s_tmp[wave][0][0]);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          s_erow[2 * g] = r0;
+          s_erow[2 * g + 1] = r1;
+          if(cok) {
+            *(VT *) (s_eval + (2 * g) * CW + c * V) = v0;
+            *(VT *) (s_eval + (2 * g + 1) * CW + c * V) = v1;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          if(g == 0) {
+            // the last group with positions holds the chunk's last run
+            uint32_t const glast = (wn + S - 1) / S - 1;
+            uint32_t rows[2];
+            combine_runs<T, V>(s_erow, s_eval, CW, 0, NE, true, 2 * glast + 1, rows, nullptr, 0,
+                a.out, a.R, col, c, cok, cv);
+            if(c == 0 && blockIdx.y == 0) {
+              a.carry_row[2 * ch] = rows[0];
+              a.carry_row[2 * ch + 1] = rows[1];
+            }
+          }
+        }
+      }
     }
     // the next chunk's staging overwrites these records
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 
   if constexpr (PRIV) {
@@ -1414,7 +1477,7 @@ static int launch_walks(WsArgs<T> a, hipStream_t st)
   SPD_LAUNCH_CHECK();
   if constexpr (KIND == KIND_ROOT || KIND == KIND_COPY) {
     constexpr uint32_t G = 64 / L;
-    uint32_t const nent = 2 * a.nchunks * G;
+    uint32_t const nent = 2 * a.nchunks;
     uint32_t const chunks = (uint32_t) spd_cdiv(nent, FIX_CHUNK);
     uint32_t const groups_per_block = (MK_THREADS / 64) * G;
     dim3 g2((unsigned) spd_cdiv(chunks, groups_per_block), ny);
@@ -2126,10 +2189,8 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   // (everything above is built on the library stream and kept with the CSF)
   SpdCallScratch scratch(fx ? spd_hs(nullptr) : st);
   if(depth == 0 || cp != nullptr) {
-    // two partial rows per range of 256 / (64 / L) positions
-    int V = 1, L = 16;
-    lane_layout((uint32_t) R, dtype, &V, &L);
-    uint64_t const nent = 2ull * plan->nitems * (64 / L);
+    // two partial rows per chunk of 256 positions
+    uint64_t const nent = 2ull * plan->nitems;
     void * cv = nullptr;
     void * cr = nullptr;
     if(fx != nullptr) {
