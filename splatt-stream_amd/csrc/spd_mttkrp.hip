@@ -84,11 +84,13 @@ vadd_atomic(T * base, uint32_t row, uint32_t R, uint32_t col, bool ok,
 #define CARRY_NONE 0xFFFFFFFFu
 
 // Root fix-up: sums the partial rows of slices that spanned several items.
-// Each group of L lanes owns a chunk of FIX_CHUNK consecutive carry entries;
-// entries of one slice are consecutive, so a chunk flushes each of its runs
-// with one atomic row-add (a slice spanning P items costs ~P/FIX_CHUNK
-// atomics on its row).
+// Each group of L lanes owns a chunk of FIX_CHUNK consecutive carry entries,
+// loaded FIX_BATCH row ids and partial rows at a time; entries of one slice
+// are consecutive, so a chunk flushes each of its runs with one atomic
+// row-add (a slice spanning P entries costs ~P/FIX_CHUNK atomics on its
+// row: the chunk is long so that a heavy row sees few atomics).
 #define FIX_CHUNK 64
+#define FIX_BATCH 8
 
 template<typename T, int V, int L>
 __global__ void __launch_bounds__(MK_THREADS)
@@ -109,15 +111,26 @@ k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
   bool const cok = col < R;
   uint32_t cur = CARRY_NONE;
   VT acc = vzero<VT>();
-  for(uint64_t e=e0; e < e1; ++e) {
-    uint32_t const row = crow[e];
-    if(row == CARRY_NONE) continue;
-    if(row != cur) {
-      if(cur != CARRY_NONE) vadd_atomic<T, V>(out, cur, R, col, cok, acc);
-      cur = row;
-      acc = vzero<VT>();
+  for(uint64_t eb=e0; eb < e1; eb += FIX_BATCH) {
+    uint32_t rw[FIX_BATCH];
+    VT v[FIX_BATCH];
+    #pragma unroll
+    for(int j=0; j < FIX_BATCH; ++j) {
+      bool const in = eb + j < e1;
+      rw[j] = in ? crow[eb + j] : CARRY_NONE;
+      // entries past the chunk read entry eb's row (in bounds, unused)
+      v[j] = vload<T, V>(carry, (uint32_t) (in ? eb + j : eb), R, col, cok);
     }
-    acc += vload<T, V>(carry, (uint32_t) e, R, col, cok);
+    #pragma unroll
+    for(int j=0; j < FIX_BATCH; ++j) {
+      if(rw[j] == CARRY_NONE) continue;
+      if(rw[j] != cur) {
+        if(cur != CARRY_NONE) vadd_atomic<T, V>(out, cur, R, col, cok, acc);
+        cur = rw[j];
+        acc = vzero<VT>();
+      }
+      acc += v[j];
+    }
   }
   if(cur != CARRY_NONE) vadd_atomic<T, V>(out, cur, R, col, cok, acc);
 }
