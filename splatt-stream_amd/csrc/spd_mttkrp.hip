@@ -83,58 +83,6 @@ vadd_atomic(T * base, uint32_t row, uint32_t R, uint32_t col, bool ok,
 
 #define CARRY_NONE 0xFFFFFFFFu
 
-// Root fix-up: sums the partial rows of slices that spanned several items.
-// Each group of L lanes owns a chunk of FIX_CHUNK consecutive carry entries,
-// loaded FIX_BATCH row ids and partial rows at a time; entries of one slice
-// are consecutive, so a chunk flushes each of its runs with one atomic
-// row-add (a slice spanning P entries costs ~P/FIX_CHUNK atomics on its
-// row: the chunk is long so that a heavy row sees few atomics).
-#define FIX_CHUNK 64
-#define FIX_BATCH 8
-
-template<typename T, int V, int L>
-__global__ void __launch_bounds__(MK_THREADS)
-k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
-    uint32_t nentries, T * __restrict__ out, uint32_t R)
-{
-  typedef typename VecT<T, V>::type VT;
-  constexpr int G = 64 / L;
-  int const lane = threadIdx.x & 63;
-  uint32_t const c = lane % L;
-  uint32_t const g = lane / L;
-  uint32_t const chunk = ((uint32_t) blockIdx.x * (MK_THREADS / 64) + (threadIdx.x >> 6)) * G + g;
-  uint64_t const e0 = (uint64_t) chunk * FIX_CHUNK;
-  if(e0 >= nentries) return;
-  uint64_t e1 = e0 + FIX_CHUNK;
-  if(e1 > nentries) e1 = nentries;
-  uint32_t const col = blockIdx.y * (L * V) + c * V;
-  bool const cok = col < R;
-  uint32_t cur = CARRY_NONE;
-  VT acc = vzero<VT>();
-  for(uint64_t eb=e0; eb < e1; eb += FIX_BATCH) {
-    uint32_t rw[FIX_BATCH];
-    VT v[FIX_BATCH];
-    #pragma unroll
-    for(int j=0; j < FIX_BATCH; ++j) {
-      bool const in = eb + j < e1;
-      rw[j] = in ? crow[eb + j] : CARRY_NONE;
-      // entries past the chunk read entry eb's row (in bounds, unused)
-      v[j] = vload<T, V>(carry, (uint32_t) (in ? eb + j : eb), R, col, cok);
-    }
-    #pragma unroll
-    for(int j=0; j < FIX_BATCH; ++j) {
-      if(rw[j] == CARRY_NONE) continue;
-      if(rw[j] != cur) {
-        if(cur != CARRY_NONE) vadd_atomic<T, V>(out, cur, R, col, cok, acc);
-        cur = rw[j];
-        acc = vzero<VT>();
-      }
-      acc += v[j];
-    }
-  }
-  if(cur != CARRY_NONE) vadd_atomic<T, V>(out, cur, R, col, cok, acc);
-}
-
 // ---- 3-mode root kernel with LDS staging ---------------------------------
 //
 // The hot path of an all-mode (or two-mode) CSF: out[s] = sum over fibers f
@@ -275,6 +223,133 @@ group_allsum(typename VecT<T, V>::type v)
     }
   }
   return v;
+}
+
+// Root fix-up: sums the partial rows of slices that spanned several work
+// ranges (carry entries, in position order: the entries of one row are
+// consecutive). One wave owns a chunk of FIX_CHUNK entries; group g of L
+// lanes loads entries g*FIX_CHUNK/G .. (g+1)*FIX_CHUNK/G - 1 at once and sums
+// their runs. A run closed on both sides inside a group is added to its row;
+// the groups' first and last runs are combined across the wave as in the
+// MTTKRP kernels (registers when they all belong to one row, LDS otherwise),
+// and the runs that reach the chunk's edges are added with one atomic
+// row-add each (a slice spanning P entries costs ~2P/FIX_CHUNK atomics on
+// its row).
+#define FIX_CHUNK 64
+
+template<typename T, int V, int L>
+__global__ void __launch_bounds__(MK_THREADS)
+k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
+    uint32_t nentries, T * __restrict__ out, uint32_t R)
+{
+  typedef typename VecT<T, V>::type VT;
+  constexpr int G = 64 / L;
+  constexpr int EPG = FIX_CHUNK / G;           // entries per group
+  constexpr int CW = L * V;
+  constexpr int NWB = MK_THREADS / 64;
+  __shared__ uint32_t s_erow[NWB][2 * G];
+  __shared__ __align__(16) T s_eval[NWB][2 * G][CW];
+  __shared__ __align__(16) T s_slot[NWB][2][CW];
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  uint32_t const c = lane % L;
+  uint32_t const g = lane / L;
+  uint64_t const w0 = ((uint64_t) blockIdx.x * NWB + wave) * FIX_CHUNK;
+  if(w0 >= nentries) return;                  // the whole wave leaves together
+  uint32_t const col = blockIdx.y * CW + c * V;
+  bool const cok = col < R;
+  uint64_t const e0 = w0 + (uint64_t) g * EPG;
+  uint32_t rw[EPG];
+  VT v[EPG];
+  #pragma unroll
+  for(int j=0; j < EPG; ++j) {
+    bool const in = e0 + j < nentries;
+    rw[j] = in ? crow[e0 + j] : CARRY_NONE;
+    // entries past the end read entry w0's row (in bounds, unused)
+    v[j] = vload<T, V>(carry, (uint32_t) (in ? e0 + j : w0), R, col, cok);
+  }
+  // this group's runs: the first (r0, v0) and last (r1, v1) may continue
+  // in the neighbouring groups; the ones between are complete
+  uint32_t r0 = CARRY_NONE, r1 = CARRY_NONE;
+  VT v0 = vzero<VT>(), acc = vzero<VT>();
+  bool first = true;
+  #pragma unroll
+  for(int j=0; j < EPG; ++j) {
+    if(rw[j] == CARRY_NONE) continue;
+    if(rw[j] != r1) {
+      if(r1 != CARRY_NONE) {
+        if(first) {
+          r0 = r1;
+          v0 = acc;
+        } else {
+          vadd_atomic<T, V>(out, r1, R, col, cok, acc);
+        }
+        first = false;
+      }
+      r1 = rw[j];
+      acc = vzero<VT>();
+    }
+    acc += v[j];
+  }
+  VT const v1 = acc;
+  // the chunk's runs across the groups
+  uint32_t const x = (r0 != CARRY_NONE) ? r0 : r1;
+  uint64_t const m = __ballot(x != CARRY_NONE);
+  if(m == 0) return;                          // wave-uniform
+  uint32_t const ref = __shfl(x, __ffsll((unsigned long long) m) - 1, 64);
+  bool const same = __all((r0 == CARRY_NONE || r0 == ref) && (r1 == CARRY_NONE || r1 == ref));
+  if(same) {
+    VT t = vzero<VT>();
+    if(r0 != CARRY_NONE) t += v0;
+    if(r1 != CARRY_NONE) t += v1;
+    t = group_allsum<T, V, L>(t);
+    if(g == 0) vadd_atomic<T, V>(out, ref, R, col, cok, t);
+    return;
+  }
+  s_erow[wave][2 * g] = r0;
+  s_erow[wave][2 * g + 1] = r1;
+  if(cok) {
+    *(VT *) &s_eval[wave][2 * g][c * V] = v0;
+    *(VT *) &s_eval[wave][2 * g + 1][c * V] = v1;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if(g == 0) {
+    // runs closed inside the chunk are stored (they hold every partial of
+    // their row); the chunk's first run and the run of its last entry (it
+    // may continue past empty entries into the next chunk) go to the slots
+    uint32_t el = 0;
+    for(int e=2 * G - 1; e >= 0; --e) {
+      if(s_erow[wave][e] != CARRY_NONE) {
+        el = (uint32_t) e;
+        break;
+      }
+    }
+    uint32_t rows[2];
+    combine_runs<T, V>(&s_erow[wave][0], &s_eval[wave][0][0], CW, 0, 2 * G, true, el,
+        rows, &s_slot[wave][0][0], CW, out, R, col, c, cok);
+    #pragma unroll
+    for(int q=0; q < 2; ++q) {
+      if(rows[q] != CARRY_NONE && cok) {
+        vadd_atomic<T, V>(out, rows[q], R, col, cok, *(VT const *) &s_slot[wave][q][c * V]);
+      }
+    }
+  }
+}
+
+// One wave per FIX_CHUNK entries.
+template<typename T, int V, int L>
+static int launch_fixup(T const * carry, uint32_t const * crow, uint64_t n, T * out, uint32_t R,
+    uint32_t ny, hipStream_t st)
+{
+  if(n == 0) return SPD_OK;
+  uint64_t const waves = spd_cdiv(n, FIX_CHUNK);
+  dim3 grid((unsigned) spd_cdiv(waves, MK_THREADS / 64), ny);
+  hipLaunchKernelGGL((k_root_fixup<T, V, L>), grid, dim3(MK_THREADS), 0, st, carry, crow,
+      (uint32_t) n, out, R);
+  SPD_LAUNCH_CHECK();
+  return SPD_OK;
 }
 
 // HOIST: the fiber's A row is gathered once per fiber (and at the start of
@@ -1489,14 +1564,7 @@ static int launch_walks(WsArgs<T> a, hipStream_t st)
       dim3(WS_WAVES * 64), shmem, st, a);
   SPD_LAUNCH_CHECK();
   if constexpr (KIND == KIND_ROOT || KIND == KIND_COPY) {
-    constexpr uint32_t G = 64 / L;
-    uint32_t const nent = 2 * a.nchunks;
-    uint32_t const chunks = (uint32_t) spd_cdiv(nent, FIX_CHUNK);
-    uint32_t const groups_per_block = (MK_THREADS / 64) * G;
-    dim3 g2((unsigned) spd_cdiv(chunks, groups_per_block), ny);
-    hipLaunchKernelGGL((k_root_fixup<T, V, L>), g2, dim3(MK_THREADS), 0, st,
-        (T const *) a.carry, (uint32_t const *) a.carry_row, nent, a.out, a.R);
-    SPD_LAUNCH_CHECK();
+    SPD_TRY((launch_fixup<T, V, L>(a.carry, a.carry_row, 2ull * a.nchunks, a.out, a.R, ny, st)));
   }
   return SPD_OK;
 }
@@ -1535,14 +1603,7 @@ static int launch_root3(Root3Args<T> const & a, uint32_t nblocks, hipStream_t st
   hipLaunchKernelGGL((k_root3<T, V, L, HOIST>), grid, dim3(R3_WAVES * 64), 0, st, a);
   SPD_LAUNCH_CHECK();
   if(ev_main_end) SPD_HIP(hipEventRecord(ev_main_end, st));
-  uint32_t const nent = 2 * nblocks;
-  uint32_t const chunks = (uint32_t) spd_cdiv(nent, FIX_CHUNK);
-  uint32_t const groups_per_block = (MK_THREADS / 64) * (64 / L);
-  dim3 g2((unsigned) spd_cdiv(chunks, groups_per_block), grid.y);
-  hipLaunchKernelGGL((k_root_fixup<T, V, L>), g2, dim3(MK_THREADS), 0, st,
-      (T const *) a.carry, (uint32_t const *) a.carry_row, nent, a.out, a.R);
-  SPD_LAUNCH_CHECK();
-  return SPD_OK;
+  return launch_fixup<T, V, L>(a.carry, a.carry_row, 2ull * nblocks, a.out, a.R, grid.y, st);
 }
 
 // Lane layout (V columns per lane, L lanes per row chunk) for a rank R.
@@ -1800,14 +1861,8 @@ static int launch_root3h(Root3hArgs<T> h, hipStream_t st, hipEvent_t ev_main_end
   hipLaunchKernelGGL((k_root3h<T, V, L, NW, HOT>), grid, dim3(NW * 64), 0, st, h);
   SPD_LAUNCH_CHECK();
   if(ev_main_end) SPD_HIP(hipEventRecord(ev_main_end, st));
-  uint32_t const nent = (uint32_t) (2 * nb * NW);
-  uint32_t const chunks = (uint32_t) spd_cdiv(nent, FIX_CHUNK);
-  uint32_t const groups_per_block = (MK_THREADS / 64) * (64 / L);
-  dim3 g2((unsigned) spd_cdiv(chunks, groups_per_block), grid.y);
-  hipLaunchKernelGGL((k_root_fixup<T, V, L>), g2, dim3(MK_THREADS), 0, st,
-      (T const *) h.b.carry, (uint32_t const *) h.b.carry_row, nent, h.b.out, h.b.R);
-  SPD_LAUNCH_CHECK();
-  return SPD_OK;
+  return launch_fixup<T, V, L>(h.b.carry, h.b.carry_row, 2ull * nb * NW, h.b.out, h.b.R, grid.y,
+      st);
 }
 
 // waves per block of k_root3h: one block per CU, as many waves as its LDS

@@ -135,6 +135,69 @@ def test_heavy_zipf_slices(gpu_available, layout):
             assert rel(s.mttkrp(mode, mats), orc.mttkrp_coo(coo, mats, mode)) <= TOL64
 
 
+def _edge_tensor(seed):
+    """A 3-mode tensor whose mode-0 slices have lengths around the kernels'
+    work-item (32 nonzeros), wave (256) and block (1024) edges, long slices
+    spanning many waves next to runs of one-nonzero slices, and fibers of
+    1 to 40 nonzeros: every path of the partial-slice combines (one slice per
+    wave or block, several slices, empty items) and of the carry fix-up."""
+    rng = np.random.default_rng(seed)
+    pattern = [1, 31, 32, 33, 1, 1, 255, 256, 257, 2, 511, 512, 513, 1023, 1024, 1025,
+               7, 1, 20000, 3, 1, 1, 1, 64, 96, 300, 1, 45000, 5, 129]
+    flen = [1, 2, 7, 31, 33, 40, 3, 1, 1]
+    D1, D2 = 4999, 3001
+    i0, i1, i2 = [], [], []
+    s = 0
+    for rep in range(3):
+        for L in pattern:
+            f = 0
+            left = L
+            while left > 0:
+                n = min(left, flen[(s + f) % len(flen)])
+                j = (f * 37 + s) % D1
+                ks = (np.arange(n) * 17 + f + s) % D2
+                i0.append(np.full(n, s, np.uint64))
+                i1.append(np.full(n, j, np.uint64))
+                i2.append(ks.astype(np.uint64))
+                left -= n
+                f += 1
+            s += 1
+    inds = [np.concatenate(i0), np.concatenate(i1), np.concatenate(i2)]
+    # no empty index in any mode (an untiled CSF's root rows are then its
+    # slice positions, as in the reference)
+    dims = [s]
+    for m in (1, 2):
+        used, inds[m] = np.unique(inds[m], return_inverse=True)
+        inds[m] = inds[m].astype(np.uint64)
+        dims.append(len(used))
+    vals = rng.lognormal(0.0, 1.0, len(inds[0]))
+    perm = rng.permutation(len(vals))
+    return [a[perm] for a in inds], vals[perm], dims
+
+
+@pytest.mark.parametrize("prec", [ps.F64, ps.F32])
+@pytest.mark.parametrize("layout", [ps.LAYOUT_AUTO, ps.LAYOUT_CSF])
+def test_slice_edges(gpu_available, prec, layout):
+    print("case slice edges", prec, layout, flush=True)
+    inds, vals, dims = _edge_tensor(11)
+    coo = orc.Coo.from_arrays(inds, vals, dims)
+    mats = [ps.synth_matrix(d, 32, 70 + m) for m, d in enumerate(dims)]
+    m32 = [m.astype(np.float32) for m in mats]
+    _set_precision(prec)
+    _set_layout(layout)
+    for kind in (ps.CSF_ONEMODE, ps.CSF_TWOMODE, ps.CSF_ALLMODE):
+        s = ps.CsfSet.convert(inds, vals, ps.Opts(csf_alloc=kind))
+        try:
+            for mode in range(3):
+                got = s.mttkrp(mode, mats)
+                if prec == ps.F64:
+                    assert rel(got, orc.mttkrp_coo(coo, mats, mode)) <= TOL64, (kind, mode)
+                else:
+                    assert rel(got, orc.mttkrp_coo(coo, m32, mode, f32=True)) <= TOL32, (kind, mode)
+        finally:
+            s.free()
+
+
 def test_device_entry_point(gpu_available):
     inds, vals, dims = ps.synth_coord([500, 400, 300], 50000, zipf=[1.2] * 3, seed=9)
     coo = orc.Coo.from_arrays(inds, vals, dims)
