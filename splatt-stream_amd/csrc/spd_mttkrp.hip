@@ -100,8 +100,10 @@ vadd_atomic(T * base, uint32_t row, uint32_t R, uint32_t col, bool ok,
 // test reads LDS. k_root3h (spd_root3h.hpp) is the persistent variant that
 // gathers A once per fiber and keeps the hottest B rows in LDS.
 // Slices completed inside an item are stored; the partial sums at item
-// edges are combined per block in LDS, and only a block's first and last
-// slice (when they cross the block's edges) reach the carry buffer.
+// edges are combined per wave and then per block (across lane groups in
+// registers when they all belong to one slice, through LDS otherwise), and
+// only a block's first and last slice (when they cross the block's edges)
+// reach the carry buffer.
 
 #define R3_WAVES 4
 #define R3_WN 256                  // nonzeros per wave
@@ -760,9 +762,10 @@ k_root3(Root3Args<T> a)
 // are in flight at once, and no step waits on a dependent global load.
 //
 // Root outputs (rows unique per node): runs closed inside a group's range
-// are stored; the first and last run of a range go to a carry buffer (two
-// rows per range) that k_root_fixup sums in order, so a heavy row cut into
-// thousands of ranges costs a few dozen atomics. Internal and leaf outputs:
+// are stored; the first and last runs of the groups of a chunk are combined
+// across the wave, and the chunk's first and last run go to a carry buffer
+// (two rows per chunk of 256 nonzeros) that k_root_fixup sums in order, so a
+// heavy row cut into thousands of chunks costs a few dozen atomics. Internal and leaf outputs:
 // rows repeat across nodes, so every run is added, in LDS for the most
 // frequent rows (flushed once per block) and with global atomics for the
 // rest.
