@@ -980,7 +980,7 @@ k_admm_pass(Admm2Args<TM> a)
 // final so far: j, or j-1 on convergence, which sets ctl[0]). The code is
 // kept short: a single block runs it and its instruction fetches are cold.
 #define RD_THREADS 1024
-#define ADMM_MAX_BLOCKS 1536       // blocks of an ADMM pass, at most (a multiple of 64)
+#define ADMM_MAX_BLOCKS 1024       // blocks of an ADMM pass, at most (a multiple of 64)
 template<int E>
 __device__ __forceinline__ void
 b_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
@@ -1888,7 +1888,8 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
     uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
 {
   uint32_t const RP = dn_rpad(R);
-  // one block per 64 rows, at most six resident blocks per CU's worth
+  // one block per 64 rows, at most one resident round (four blocks per CU;
+  // a larger grid only lengthens the one-block reduction)
   uint32_t nb = (uint32_t) (spd_cdiv(I, 64) < ADMM_MAX_BLOCKS ? spd_cdiv(I, 64)
                                                              : ADMM_MAX_BLOCKS);
   if(nb < 1) nb = 1;
