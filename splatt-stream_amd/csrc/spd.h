@@ -162,6 +162,9 @@ int spd_mttkrp_set_copies(int enable);
  * layouts apply); 2 the persistent kernel with every row from HBM; 3
  * k_root3 gathering each fiber's A row once. */
 int spd_mttkrp_set_root_variant(int variant);
+/* Carry entries from which the root fix-up takes two passes (default
+ * 131072; 0: always two). */
+int spd_mttkrp_set_fix_two_level(uint64_t entries);
 
 /* ---- dense / CPD ------------------------------------------------------- */
 

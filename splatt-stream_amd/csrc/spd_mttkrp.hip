@@ -236,13 +236,26 @@ group_allsum(typename VecT<T, V>::type v)
 // MTTKRP kernels (registers when they all belong to one row, LDS otherwise),
 // and the runs that reach the chunk's edges are added with one atomic
 // row-add each (a slice spanning P entries costs ~2P/FIX_CHUNK atomics on
-// its row).
+// its row). With a second-level buffer (carry2, crow2 not null) the chunk's
+// first and last runs go to its two entries there instead (the first run,
+// or none, then the last), and a second pass over those entries adds them:
+// a heavy row then sees ~4P/FIX_CHUNK^2 atomics instead of ~2P/FIX_CHUNK,
+// which long carry buffers need (atomics on one row serialise).
 #define FIX_CHUNK 64
+// carry entries from which the fix-up takes two passes
+static uint64_t g_fix_two_level = 131072;
+
+extern "C" int spd_mttkrp_set_fix_two_level(uint64_t entries)
+{
+  g_fix_two_level = entries;
+  return SPD_OK;
+}
 
 template<typename T, int V, int L>
 __global__ void __launch_bounds__(MK_THREADS)
 k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
-    uint32_t nentries, T * __restrict__ out, uint32_t R)
+    uint32_t nentries, T * __restrict__ out, uint32_t R, T * __restrict__ carry2,
+    uint32_t * __restrict__ crow2)
 {
   typedef typename VecT<T, V>::type VT;
   constexpr int G = 64 / L;
@@ -256,8 +269,11 @@ k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
   int const wave = threadIdx.x >> 6;
   uint32_t const c = lane % L;
   uint32_t const g = lane / L;
-  uint64_t const w0 = ((uint64_t) blockIdx.x * NWB + wave) * FIX_CHUNK;
+  uint64_t const wc = (uint64_t) blockIdx.x * NWB + wave;
+  uint64_t const w0 = wc * FIX_CHUNK;
   if(w0 >= nentries) return;                  // the whole wave leaves together
+  bool const two = carry2 != nullptr;
+  bool const rowst = lane == 0 && blockIdx.y == 0;
   uint32_t const col = blockIdx.y * CW + c * V;
   bool const cok = col < R;
   uint64_t const e0 = w0 + (uint64_t) g * EPG;
@@ -297,7 +313,13 @@ k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
   // the chunk's runs across the groups
   uint32_t const x = (r0 != CARRY_NONE) ? r0 : r1;
   uint64_t const m = __ballot(x != CARRY_NONE);
-  if(m == 0) return;                          // wave-uniform
+  if(m == 0) {                                // wave-uniform
+    if(two && rowst) {
+      crow2[2 * wc] = CARRY_NONE;
+      crow2[2 * wc + 1] = CARRY_NONE;
+    }
+    return;
+  }
   uint32_t const ref = __shfl(x, __ffsll((unsigned long long) m) - 1, 64);
   bool const same = __all((r0 == CARRY_NONE || r0 == ref) && (r1 == CARRY_NONE || r1 == ref));
   if(same) {
@@ -305,7 +327,16 @@ k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
     if(r0 != CARRY_NONE) t += v0;
     if(r1 != CARRY_NONE) t += v1;
     t = group_allsum<T, V, L>(t);
-    if(g == 0) vadd_atomic<T, V>(out, ref, R, col, cok, t);
+    if(two) {
+      // one run: it may continue on both sides
+      if(g == 0) vstore<T, V>(carry2, (uint32_t) (2 * wc + 1), R, col, cok, t);
+      if(rowst) {
+        crow2[2 * wc] = CARRY_NONE;
+        crow2[2 * wc + 1] = ref;
+      }
+    } else if(g == 0) {
+      vadd_atomic<T, V>(out, ref, R, col, cok, t);
+    }
     return;
   }
   s_erow[wave][2 * g] = r0;
@@ -334,22 +365,50 @@ k_root_fixup(T const * __restrict__ carry, uint32_t const * __restrict__ crow,
     #pragma unroll
     for(int q=0; q < 2; ++q) {
       if(rows[q] != CARRY_NONE && cok) {
-        vadd_atomic<T, V>(out, rows[q], R, col, cok, *(VT const *) &s_slot[wave][q][c * V]);
+        VT const sv = *(VT const *) &s_slot[wave][q][c * V];
+        if(two) vstore<T, V>(carry2, (uint32_t) (2 * wc + q), R, col, cok, sv);
+        else vadd_atomic<T, V>(out, rows[q], R, col, cok, sv);
       }
+    }
+    if(two && rowst) {
+      crow2[2 * wc] = rows[0];
+      crow2[2 * wc + 1] = rows[1];
     }
   }
 }
 
-// One wave per FIX_CHUNK entries.
+// Entries of the fix-up's second-level buffer for n carry entries (kept
+// after the n entries in the carry buffers).
+static inline uint64_t fix_level2_entries(uint64_t n)
+{
+  return 2 * ((n + FIX_CHUNK - 1) / FIX_CHUNK);
+}
+
+// One wave per FIX_CHUNK entries; from g_fix_two_level entries on, the
+// chunks' edge runs pass through the second-level buffer (carry and crow then hold
+// n + fix_level2_entries(n) entries).
 template<typename T, int V, int L>
-static int launch_fixup(T const * carry, uint32_t const * crow, uint64_t n, T * out, uint32_t R,
+static int launch_fixup(T * carry, uint32_t * crow, uint64_t n, T * out, uint32_t R,
     uint32_t ny, hipStream_t st)
 {
   if(n == 0) return SPD_OK;
   uint64_t const waves = spd_cdiv(n, FIX_CHUNK);
   dim3 grid((unsigned) spd_cdiv(waves, MK_THREADS / 64), ny);
-  hipLaunchKernelGGL((k_root_fixup<T, V, L>), grid, dim3(MK_THREADS), 0, st, carry, crow,
-      (uint32_t) n, out, R);
+  if(n >= g_fix_two_level) {
+    T * const carry2 = carry + n * R;
+    uint32_t * const crow2 = crow + n;
+    hipLaunchKernelGGL((k_root_fixup<T, V, L>), grid, dim3(MK_THREADS), 0, st,
+        (T const *) carry, (uint32_t const *) crow, (uint32_t) n, out, R, carry2, crow2);
+    SPD_LAUNCH_CHECK();
+    carry = carry2;
+    crow = crow2;
+    n = fix_level2_entries(n);
+    dim3 g2((unsigned) spd_cdiv(spd_cdiv(n, FIX_CHUNK), MK_THREADS / 64), ny);
+    grid = g2;
+  }
+  hipLaunchKernelGGL((k_root_fixup<T, V, L>), grid, dim3(MK_THREADS), 0, st,
+      (T const *) carry, (uint32_t const *) crow, (uint32_t) n, out, R, (T *) nullptr,
+      (uint32_t *) nullptr);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
@@ -2012,14 +2071,15 @@ static int run_root3(spd_csf * c, uint64_t R, void * const * mats, void * out,
   SpdCallScratch scratch(fx ? spd_hs(nullptr) : st);
   uint64_t const nwaves = spd_cdiv(c->nnz, R3_WN);
   uint32_t const nblocks = (uint32_t) spd_cdiv(nwaves, R3_WAVES);
+  // two partial rows per block, then the fix-up's second level
+  uint64_t const nent = 2ull * nblocks + fix_level2_entries(2ull * nblocks);
   void * cv = nullptr;
   void * cr = nullptr;
   if(fx != nullptr) {
-    SPD_TRY(fixed_carve(fx, 2ull * nblocks * R * sizeof(T), 2ull * nblocks * sizeof(uint32_t),
-        &cv, &cr));
+    SPD_TRY(fixed_carve(fx, nent * R * sizeof(T), nent * sizeof(uint32_t), &cv, &cr));
   } else {
-    SPD_TRY(spd_scratch(2, 2ull * nblocks * R * sizeof(T), &cv));
-    SPD_TRY(spd_scratch(3, 2ull * nblocks * sizeof(uint32_t), &cr));
+    SPD_TRY(spd_scratch(2, nent * R * sizeof(T), &cv));
+    SPD_TRY(spd_scratch(3, nent * sizeof(uint32_t), &cr));
   }
   Root3Args<T> a = {};
   a.fptr0 = c->u.fptr[0];
@@ -2260,8 +2320,9 @@ static int run_mttkrp(spd_csf * c, int mode, uint64_t R, void * const * mats,
   // (everything above is built on the library stream and kept with the CSF)
   SpdCallScratch scratch(fx ? spd_hs(nullptr) : st);
   if(depth == 0 || cp != nullptr) {
-    // two partial rows per chunk of 256 positions
-    uint64_t const nent = 2ull * plan->nitems;
+    // two partial rows per chunk of 256 positions, then the fix-up's second
+    // level
+    uint64_t const nent = 2ull * plan->nitems + fix_level2_entries(2ull * plan->nitems);
     void * cv = nullptr;
     void * cr = nullptr;
     if(fx != nullptr) {

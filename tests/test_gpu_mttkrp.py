@@ -198,6 +198,23 @@ def test_slice_edges(gpu_available, prec, layout):
             s.free()
 
 
+def test_slice_edges_two_level_fixup(gpu_available):
+    # the carry fix-up's two-pass form (taken by long carry buffers, from
+    # 131072 entries) forced on every size
+    L = ps.lib()
+    L.spd_mttkrp_set_fix_two_level.argtypes = [ctypes.c_uint64]
+    assert L.spd_mttkrp_set_fix_two_level(0) == 0
+    try:
+        for prec in (ps.F64, ps.F32):
+            test_slice_edges(gpu_available, prec, ps.LAYOUT_AUTO)
+        _set_precision(ps.F64)
+        test_heavy_zipf_slices(gpu_available, ps.LAYOUT_AUTO)
+    finally:
+        L.spd_mttkrp_set_fix_two_level(131072)
+        _set_precision(ps.F64)
+        _set_layout(ps.LAYOUT_AUTO)
+
+
 def test_device_entry_point(gpu_available):
     inds, vals, dims = ps.synth_coord([500, 400, 300], 50000, zipf=[1.2] * 3, seed=9)
     coo = orc.Coo.from_arrays(inds, vals, dims)
