@@ -936,6 +936,20 @@ walk_range(WsArgs<T> const & a, uint32_t const (*s_gid)[WS_WN], uint32_t const *
       }
       acc += prod;
     };
+    // FP32 root walks of 4+ modes: the rows of the upper gathered levels
+    // repeat over consecutive positions; a group reuses the previous
+    // position's row when the id is unchanged, and the wave skips that
+    // level's gather when no group needs a new row (the FP64 walks, at two
+    // waves per SIMD, measured slower with it)
+    constexpr bool SKIP = (KIND == KIND_ROOT) && NG >= 3 && sizeof(T) == 4;
+    constexpr int NS = SKIP ? NG - 1 : 1;
+    uint32_t lgid[NS];
+    VT lb[NS];
+    #pragma unroll
+    for(int q=0; q < NS; ++q) {
+      lgid[q] = 0xFFFFFFFFu;
+      lb[q] = vzero<VT>();
+    }
     uint32_t k = k0;
     for(; k + B <= k1; k += B) {
       uint32_t gid[B][NG];
@@ -955,7 +969,41 @@ walk_range(WsArgs<T> const & a, uint32_t const (*s_gid)[WS_WN], uint32_t const *
       #pragma unroll
       for(int j=0; j < B; ++j) {
         #pragma unroll
-        for(int q=0; q < NG; ++q) b[j][q] = vload_c<T, V>(a.gm[q], gid[j][q], R, colc);
+        for(int q=0; q < NG; ++q) {
+          if constexpr (SKIP) {
+            if(q < NS) {
+              uint32_t const pg = (j == 0) ? lgid[q] : gid[j - 1][q];
+              bool const nd = gid[j][q] != pg;
+              b[j][q] = vzero<VT>();
+              if(__any(nd)) {
+                if(nd) b[j][q] = vload_c<T, V>(a.gm[q], gid[j][q], R, colc);
+              }
+              continue;
+            }
+          }
+          b[j][q] = vload_c<T, V>(a.gm[q], gid[j][q], R, colc);
+        }
+      }
+      if constexpr (SKIP) {
+        // resolve the reused rows once every gather of the batch is issued
+        #pragma unroll
+        for(int j=0; j < B; ++j) {
+          #pragma unroll
+          for(int q=0; q < NS; ++q) r3_opaque(b[j][q]);
+        }
+        #pragma unroll
+        for(int j=0; j < B; ++j) {
+          #pragma unroll
+          for(int q=0; q < NS; ++q) {
+            uint32_t const pg = (j == 0) ? lgid[q] : gid[j - 1][q];
+            if(gid[j][q] == pg) b[j][q] = (j == 0) ? lb[q] : b[j - 1][q];
+          }
+        }
+        #pragma unroll
+        for(int q=0; q < NS; ++q) {
+          lgid[q] = gid[B - 1][q];
+          lb[q] = b[B - 1][q];
+        }
       }
       #pragma unroll
       for(int j=0; j < B; ++j) {
