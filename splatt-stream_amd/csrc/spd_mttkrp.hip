@@ -936,8 +936,8 @@ walk_range(WsArgs<T> const & a, uint32_t const (*s_gid)[WS_WN], uint32_t const *
       }
       acc += prod;
     };
-    // FP32 root walks of 4+ modes: the rows of the upper gathered levels
-    // repeat over consecutive positions; a group reuses the previous
+    // FP32 root and row-sorted-copy walks of 4+ modes: the rows of the upper
+    // gathered levels repeat over consecutive positions; a group reuses the previous
     // position's row when the id is unchanged, and the wave skips that
     // level's gather when no group needs a new row (the FP64 walks, at two
     // waves per SIMD, measured slower with it)
