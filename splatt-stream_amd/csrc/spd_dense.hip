@@ -1053,6 +1053,63 @@ k_admm_pass_reduce(double const * __restrict__ part, uint32_t nb, uint32_t R,
   b_admm_pass_reduce<E>(part, nb, R, norms, sums, ctl, tol, check, j);
 }
 
+// The same reduction with one 64-lane block per entry, all entries at once:
+// block e's lane l sums blocks l, l + 64, ... of entry e in order, then the
+// butterfly (the sums of b_admm_pass_reduce, bit for bit) and writes
+// tot[e]; the last block to finish (a ticket counter, reset by it) runs the
+// norms and the test of iteration j-1.
+template<int E>
+__global__ void __launch_bounds__(64)
+k_admm_pass_reduce_e(double const * __restrict__ part, uint32_t nb, uint32_t R,
+    double * __restrict__ norms, double * __restrict__ sums, int * __restrict__ ctl,
+    double tol, int check, int j, double * __restrict__ tot, unsigned * __restrict__ ticket)
+{
+  if(ctl[0] != 0) return;                  // every block: converged earlier
+  constexpr int RP = E - 4;
+  constexpr int PB = ADMM_MAX_BLOCKS / 64;
+  int const lane = threadIdx.x;
+  int const e = blockIdx.x;
+  double const * const pe = part + (uint64_t) e * nb;
+  double x[PB];
+  #pragma unroll
+  for(int k=0; k < PB; ++k) {
+    uint32_t const b = lane + 64u * k;
+    x[k] = (b < nb) ? pe[b] : 0.;
+  }
+  double v = 0.;
+  #pragma unroll
+  for(int k=0; k < PB; ++k) {
+    if(lane + 64u * k < nb) v += x[k];
+  }
+  for(int off=32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __shared__ int s_last;
+  if(lane == 0) {
+    tot[e] = v;
+    __threadfence();
+    s_last = (atomicAdd(ticket, 1u) == (unsigned) (E - 1));
+  }
+  __syncthreads();
+  if(!s_last) return;
+  __threadfence();
+  for(int c=lane; c < RP; c += 64) {
+    if(c < (int) R) norms[c] = sqrt(tot[c]);
+  }
+  if(lane == 0) {
+    bool conv = false;
+    if(check) {
+      for(int k=0; k < 4; ++k) sums[k] = tot[RP + k];
+      conv = (tot[RP + 1] <= tol * tot[RP] && tot[RP + 2] <= tol * tot[RP + 3]);
+    }
+    if(conv) {
+      ctl[0] = 1;
+      ctl[1] = j - 1;
+    } else {
+      ctl[1] = j;
+    }
+    *ticket = 0u;
+  }
+}
+
 struct Bufs3 { double * b[3]; double * n[3]; };
 
 // P = prox(P'_k), D = P - P'_k for the final state k = ctl[1] (k >= 1).
@@ -1088,9 +1145,10 @@ k_admm_final(Bufs3 bf, double * __restrict__ P, double * __restrict__ D,
   b_admm_final(bf, P, D, n, R, prox, lam, rho, ctl, blockIdx.x, gridDim.x);
 }
 
-__global__ void k_ctl_reset(int * ctl)
+__global__ void k_ctl_reset(int * ctl, unsigned * ticket)
 {
   if(threadIdx.x < 2) ctl[threadIdx.x] = 0;
+  if(threadIdx.x == 2 && ticket != nullptr) *ticket = 0u;
 }
 
 // Chunked ADMM for a row-separable proximal operator (src/cpd/admm.c:355-400
@@ -1850,7 +1908,7 @@ static int admm_t(TM const * M, double const * ginv, double const * rho, uint64_
 
 template<int TT, typename TM>
 static int launch_admm2(Admm2Args<TM> const & a, int j, uint32_t nb, double * norms,
-    double * sums, int * ctl, double tol, hipStream_t st)
+    double * sums, int * ctl, double tol, double * tot, unsigned * ticket, hipStream_t st)
 {
   constexpr int E = TT * 16 + 4;
   // every matrix of the pass (I x R, 8 bytes at most) below 2 GiB
@@ -1873,8 +1931,8 @@ static int launch_admm2(Admm2Args<TM> const & a, int j, uint32_t nb, double * no
     }
   }
   SPD_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_admm_pass_reduce<E>), dim3(1), dim3(RD_THREADS), 0, st,
-      (double const *) a.part, nb, a.R, norms, sums, ctl, tol, j >= 2 ? 1 : 0, j);
+  hipLaunchKernelGGL((k_admm_pass_reduce_e<E>), dim3(E), dim3(64), 0, st,
+      (double const *) a.part, nb, a.R, norms, sums, ctl, tol, j >= 2 ? 1 : 0, j, tot, ticket);
   SPD_LAUNCH_CHECK();
   return SPD_OK;
 }
@@ -1895,14 +1953,18 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
   if(nb < 1) nb = 1;
   void * pc = nullptr;
   size_t const part_n = (size_t) nb * (RP + 4);
-  SPD_TRY(spd_scratch(SLOT_COL, (part_n + 3 * (size_t) RP) * sizeof(double), &pc));
+  // block partials, the three states' column norms, the entry totals of the
+  // reduction and its ticket counter
+  SPD_TRY(spd_scratch(SLOT_COL, (part_n + 3 * (size_t) RP + (RP + 4) + 1) * sizeof(double), &pc));
   double * const part = (double *) pc;
+  double * const tot = part + part_n + 3 * (size_t) RP;
+  unsigned * const ticket = (unsigned *) (tot + RP + 4);
   Bufs3 bf;
   bf.b[0] = P;
   bf.b[1] = aux;
   bf.b[2] = init;
   for(int k=0; k < 3; ++k) bf.n[k] = part + part_n + (size_t) k * RP;
-  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, st, ctl);
+  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, st, ctl, ticket);
   SPD_LAUNCH_CHECK();
   Admm2Args<TM> a = {};
   a.M = M;
@@ -1924,10 +1986,10 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
     a.out = bf.b[j % 3];
     double * const nrm = bf.n[j % 3];
     switch(RP / 16) {
-    case 1: SPD_TRY((launch_admm2<1, TM>(a, (int) j, nb, nrm, sums, ctl, tol, st))); break;
-    case 2: SPD_TRY((launch_admm2<2, TM>(a, (int) j, nb, nrm, sums, ctl, tol, st))); break;
-    case 3: SPD_TRY((launch_admm2<3, TM>(a, (int) j, nb, nrm, sums, ctl, tol, st))); break;
-    default: SPD_TRY((launch_admm2<4, TM>(a, (int) j, nb, nrm, sums, ctl, tol, st))); break;
+    case 1: SPD_TRY((launch_admm2<1, TM>(a, (int) j, nb, nrm, sums, ctl, tol, tot, ticket, st))); break;
+    case 2: SPD_TRY((launch_admm2<2, TM>(a, (int) j, nb, nrm, sums, ctl, tol, tot, ticket, st))); break;
+    case 3: SPD_TRY((launch_admm2<3, TM>(a, (int) j, nb, nrm, sums, ctl, tol, tot, ticket, st))); break;
+    default: SPD_TRY((launch_admm2<4, TM>(a, (int) j, nb, nrm, sums, ctl, tol, tot, ticket, st))); break;
     }
   }
   uint64_t const n = I * R;
