@@ -741,6 +741,7 @@ struct Admm2Args {
   double lam;               // lasso multiplier
   double const * P0;        // explicit state (P_0, D_0)
   double const * D0;
+  int d0zero;               // D_0 is zero: not read
   double const * cur;       // P'_{j-1} (states >= 1)
   double const * cur_n;     // its prox column norms
   double const * prev;      // P'_{j-2} (test against a state >= 1)
@@ -831,11 +832,11 @@ b_admm_pass(Admm2Args<TM> a,
         mv[q] = ok ? (double) *(TM const *) ((char const *) a.M + bm) : 0.;
         if(MODE == 0) {
           cv[q] = ok ? ld(a.P0) : 0.;
-          xv[q] = ok ? ld(a.D0) : 0.;
+          xv[q] = (ok && !a.d0zero) ? ld(a.D0) : 0.;
         } else if(MODE == 1) {
           cv[q] = ok ? ld(a.cur) : 0.;
           xv[q] = ok ? ld(a.P0) : 0.;
-          yv[q] = ok ? ld(a.D0) : 0.;
+          yv[q] = (ok && !a.d0zero) ? ld(a.D0) : 0.;
         } else {
           cv[q] = ok ? ld(a.cur) : 0.;
           xv[q] = ok ? ld(a.prev) : 0.;
@@ -844,11 +845,11 @@ b_admm_pass(Admm2Args<TM> a,
         mv[q] = ok ? (double) a.M[idx] : 0.;
         if(MODE == 0) {
           cv[q] = ok ? a.P0[idx] : 0.;
-          xv[q] = ok ? a.D0[idx] : 0.;
+          xv[q] = (ok && !a.d0zero) ? a.D0[idx] : 0.;
         } else if(MODE == 1) {
           cv[q] = ok ? a.cur[idx] : 0.;
           xv[q] = ok ? a.P0[idx] : 0.;
-          yv[q] = ok ? a.D0[idx] : 0.;
+          yv[q] = (ok && !a.d0zero) ? a.D0[idx] : 0.;
         } else {
           cv[q] = ok ? a.cur[idx] : 0.;
           xv[q] = ok ? a.prev[idx] : 0.;
@@ -1943,7 +1944,7 @@ static int launch_admm2(Admm2Args<TM> const & a, int j, uint32_t nb, double * no
 template<typename TM>
 static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, uint64_t I,
     uint32_t R, double * P, double * D, double * aux, double * init, int prox, double lam,
-    uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
+    uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st, int d_zero)
 {
   uint32_t const RP = dn_rpad(R);
   // one block per 64 rows, at most one resident round (four blocks per CU;
@@ -1976,6 +1977,7 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
   a.lam = lam;
   a.P0 = P;
   a.D0 = D;
+  a.d0zero = d_zero;
   a.part = part;
   a.ctl = ctl;
   for(uint64_t j=1; j <= max_inner; ++j) {
@@ -2001,23 +2003,28 @@ static int admm_solve2_t(TM const * M, double const * ginv, double const * rho, 
 
 int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const * rho,
     uint64_t I, uint32_t R, double * P, double * D, double * aux, double * init, int prox,
-    double lam, uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st)
+    double lam, uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st,
+    int d_zero)
 {
   if(prox == DN_PROX_ROWSIMP) {
     spd_set_error(SPD_EINPUT, "the row-simplex prox runs in chunked ADMM only");
     return SPD_EINPUT;
   }
-  if(I == 0 || max_inner == 0) return SPD_OK;
+  if(I == 0) return SPD_OK;
+  if(max_inner == 0) return d_zero ? spd_zero_async(D, I * R * sizeof(double), st) : SPD_OK;
   double * sf = nullptr;
   if(prox == DN_PROX_SMOOTH) SPD_TRY(smooth_factor(I, lam, rho, &sf, st));
   if(R <= 64 && prox != DN_PROX_SMOOTH && prox != DN_PROX_COLSIMP) {
+    // the pass form reads D only as the starting dual, in its first two
+    // passes, and writes all of it at the end
     if(mdtype == SPD_F32) {
       return admm_solve2_t<float>((float const *) M, ginv, rho, I, R, P, D, aux, init, prox,
-          lam, max_inner, tol, sums, ctl, st);
+          lam, max_inner, tol, sums, ctl, st, d_zero);
     }
     return admm_solve2_t<double>((double const *) M, ginv, rho, I, R, P, D, aux, init, prox,
-        lam, max_inner, tol, sums, ctl, st);
+        lam, max_inner, tol, sums, ctl, st, d_zero);
   }
+  if(d_zero) SPD_TRY(spd_zero_async(D, I * R * sizeof(double), st));
   // ranks above 64 and the column-coupled smoothness and simplex proxes: one
   // step at a time, the test on the host
   int hctl[2] = {0, 0};

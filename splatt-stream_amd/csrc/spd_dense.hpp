@@ -88,7 +88,8 @@ int dn_admm_step(void const * M, int mdtype, double const * ginv,
 // used only above rank 64.
 int dn_admm_solve(void const * M, int mdtype, double const * ginv, double const * rho,
     uint64_t I, uint32_t R, double * P, double * D, double * aux, double * init, int prox,
-    double lam, uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st);
+    double lam, uint64_t max_inner, double tol, double * sums, int * ctl, hipStream_t st,
+    int d_zero = 0);
 
 // ADMM in independent chunks of `chunk` rows for a row-separable prox
 // (DN_PROX_NONNEG, DN_PROX_LASSO, DN_PROX_ROWSIMP or none; src/cpd/admm.c:

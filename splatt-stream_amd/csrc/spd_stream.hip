@@ -683,12 +683,13 @@ int StreamState::slice(uint32_t const * const * ind_in, double const * vals_b, u
           hs, nc, R, RP, H);
       SPD_LAUNCH_CHECK();
       SPD_TRY(dn_apply_acc(Uold[m], H, I, R, Mbuf, st));
-      SPD_TRY(spd_zero_async(Dual[m], I * R * sizeof(double), st));
       spd_con const & con = cfg.con[m];
       if(con.kind == SPD_CON_ADMM) {
+        // the duals restart from zero for every mode's solve (the solve
+        // takes that as given instead of reading a cleared matrix)
         SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 1, ginv, d_rho, info, st));
         SPD_TRY(dn_admm_solve(Mbuf, SPD_F64, ginv, d_rho, I, R, Unew[m], Dual[m], aux,
-            init_buf, con.prox, con.lam, cfg.max_inner, cfg.inner_tol, d_sums, ctl, st));
+            init_buf, con.prox, con.lam, cfg.max_inner, cfg.inner_tol, d_sums, ctl, st, 1));
       } else {
         SPD_TRY(dn_chol_inv(aTa_c, N, m, R, con.diag_add, 0, ginv, d_rho, info, st));
         SPD_TRY(dn_apply(Mbuf, SPD_F64, ginv, I, R, Unew[m], nullptr, st));
