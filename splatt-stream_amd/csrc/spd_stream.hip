@@ -536,7 +536,7 @@ int StreamState::init(int nmodes, spd_stream_cfg const & c, uint64_t const * cap
   SPD_TRY(mem.get((void **) &H, (size_t) RP * RP * sizeof(double)));
   SPD_TRY(mem.get((void **) &cross, (size_t) SPD_MAX_MODES * R * R * sizeof(double)));
   SPD_TRY(mem.get((void **) &tgram, (size_t) R * R * sizeof(double)));
-  SPD_TRY(mem.get((void **) &sc, (16 + 4 + R) * sizeof(double)));
+  SPD_TRY(mem.get((void **) &sc, (16 + 4 + R + 2 * SPD_MAX_MODES) * sizeof(double)));
   SPD_TRY(mem.get((void **) &info, sizeof(int)));
   SPD_TRY(spd_zero_async(info, sizeof(int), st));
   SPD_TRY(mem.get((void **) &ctl, 2 * sizeof(int)));
@@ -553,8 +553,9 @@ int StreamState::slice(uint32_t const * const * ind_in, double const * vals_b, u
   SPD_HIP(hipStreamSynchronize(st));
   double const t0 = sst_wall_s();
   double * const d_rho = sc + 0;
-  double * const d_diff = sc + 2;
   double * const d_inner = sc + 4;
+  // change sums of every mode of an outer iteration, read back together
+  double * const d_diffm = sc + 16 + 4 + R;
   double * const d_sums = sc + 8;
   // the streamed mode of a slice is all zeros
   if(bn > zero_cap) {
@@ -695,10 +696,14 @@ int StreamState::slice(uint32_t const * const * ind_in, double const * vals_b, u
         SPD_TRY(dn_apply(Mbuf, SPD_F64, ginv, I, R, Unew[m], nullptr, st));
       }
       SPD_TRY(dn_gram(Unew[m], I, R, nullptr, nullptr, aTa[m], st));
-      SPD_TRY(diff2(Uold[m], Unew[m], I * R, d_diff, st));
-      double dd[2];
-      SPD_TRY(st_d2h(dd, d_diff, sizeof(dd), st));
-      delta += sqrt(dd[0]) / sqrt(dd[1]);
+      SPD_TRY(diff2(Uold[m], Unew[m], I * R, d_diffm + 2 * m, st));
+    }
+    // one readback per outer iteration; the modes' terms added in mode order
+    double dd[2 * SPD_MAX_MODES];
+    SPD_TRY(st_d2h(dd, d_diffm, 2 * (size_t) N * sizeof(double), st));
+    for(int m=0; m < N; ++m) {
+      if(m == t) continue;
+      delta += sqrt(dd[2 * m]) / sqrt(dd[2 * m + 1]);
     }
     if(cfg.verbosity > 1) {
       printf("  delta: %e prev_delta: %e (%e diff)\n", delta, prev_delta,

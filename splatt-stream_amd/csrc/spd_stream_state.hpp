@@ -54,7 +54,7 @@ struct StreamState {
   double * H = nullptr;
   double * cross = nullptr;
   double * tgram = nullptr;
-  double * sc = nullptr;       // [0] rho, [2..3] change sums, [4..5] error terms, [8..] ADMM
+  double * sc = nullptr;       // [0] rho, [4..5] error terms, [8..] ADMM, [20 + R ..] change sums per mode
   double * lam1 = nullptr;
   double * tmpv = nullptr;
   int * info = nullptr;
